@@ -1,0 +1,322 @@
+"""helmholtzbem_amd -- MI355X-native sigma_min(k) path of HelmholtzBEM.
+
+Host-side mirror of the reference's C++ interface for the path (names,
+argument meaning, defaults and error messages), over the C-ABI of
+include/hbem.h:
+
+* ``BuilderData`` / ``GalerkinBuilder`` / ``LayerType`` (galerkin_builder.hpp)
+* ``SolutionsOperator`` (gen_sol_op.hpp)
+* ``randomized_svd.{sv, sv_der, sv_der2, randGaussian}`` (randsvd.hpp)
+* ``sigma_sweep`` -- the batched k-sweep the reference drivers run with
+  ``std::transform(par_unseq, ...)`` (examples/roots_brent_square_rsvd.cpp:167-171)
+
+Matrices are numpy complex128 arrays; results are Fortran-ordered like
+Eigen::MatrixXcd.  Everything runs on the GPU through libhbem.so; nothing here
+computes operator entries on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import HbemError, check, lib
+from .geometry import (ParametrizedCircularArc, ParametrizedLine, ParametrizedMesh, ParametrizedTrigCurve,
+                       Xoshiro256ss, circle_mesh, gaussian_test_matrix, polygon_mesh, random_polygon_mesh,
+                       square_mesh, star_curve, star_mesh)
+
+__all__ = ["LayerType", "ContinuousSpace1", "BuilderData", "GalerkinBuilder", "SolutionsOperator",
+           "randomized_svd", "sigma_sweep", "HbemError", "ParametrizedLine", "ParametrizedCircularArc",
+           "ParametrizedTrigCurve", "ParametrizedMesh", "square_mesh", "star_mesh", "circle_mesh",
+           "polygon_mesh", "random_polygon_mesh", "gaussian_test_matrix", "Xoshiro256ss", "star_curve"]
+
+FLAG_QUIRK_SLP_COINC_DER = 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _vptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class LayerType(enum.IntFlag):
+    """galerkin_builder.hpp:100-106."""
+    NONE = 0
+    DOUBLE = 1
+    SINGLE = 2
+    HYPERSINGULAR = 4
+    ALL = 7
+
+
+class ContinuousSpace1:
+    """ContinuousSpace<1> (continuous_space.hpp:33-152): P1 on a closed curve."""
+
+    q_ = 2
+
+    def getQ(self):
+        return 2
+
+    def getSpaceDim(self, numpanels):
+        return numpanels * (self.q_ - 1)
+
+    def LocGlobMap(self, q, n, N):
+        if not (q <= self.q_ and n <= N):
+            raise IndexError("Panel/RSF index out of range!")
+        return n if q == 2 else (1 if n % N == 0 else n + 1)
+
+
+class _Geometry:
+    """Owns one device context (hbem_geometry_t)."""
+
+    def __init__(self, mesh, order, proj, flags):
+        self.mesh = mesh
+        pan = np.ascontiguousarray(mesh.packed, dtype=np.float64)
+        coefs = np.ascontiguousarray(mesh.coefs if mesh.coefs.size else np.zeros((1, 3)), dtype=np.float64)
+        self._keep = (pan, coefs)
+        h = ctypes.c_void_p()
+        check(lib().hbem_geometry_create(_dptr(pan), pan.shape[0], _dptr(coefs), int(mesh.coefs.shape[0]), int(order),
+                                         1 if proj else 0, int(flags), ctypes.byref(h)))
+        self.h = h
+        self.N = pan.shape[0]
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                lib().hbem_geometry_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+
+class BuilderData:
+    """BuilderData(mesh, test_space, trial_space, order) (galerkin_builder.hpp:43-46).
+
+    Test and trial space must be the same ContinuousSpace1 object (the only
+    configuration of the reference's sigma_min drivers).  ``flags`` selects
+    reference quirks (FLAG_QUIRK_SLP_COINC_DER reproduces the coinciding
+    single-layer derivative copy of galerkin_builder.cpp:744-746).
+    """
+
+    def __init__(self, mesh, test_space, trial_space, order, flags=0):
+        if test_space is not trial_space or not isinstance(test_space, ContinuousSpace1):
+            raise ValueError("test and trial space must be the same ContinuousSpace1 object")
+        self.mesh = mesh
+        self.test_space = test_space
+        self.trial_space = trial_space
+        self.order = int(order)
+        self.flags = int(flags)
+        self._geo = _Geometry(mesh, order, True, flags)
+        self._geo_noproj = None
+
+    def _ctx(self, proj=True):
+        if proj:
+            return self._geo
+        if self._geo_noproj is None:
+            self._geo_noproj = _Geometry(self.mesh, self.order, False, self.flags)
+        return self._geo_noproj
+
+    def getTestSpaceDimension(self, n_panels=0):
+        return self.test_space.getSpaceDim(n_panels or self.getNumPanels())
+
+    def getTrialSpaceDimension(self, n_panels=0):
+        return self.trial_space.getSpaceDim(n_panels or self.getNumPanels())
+
+    def testTrialSpacesAreEqual(self):
+        return self.test_space is self.trial_space
+
+    def getQtest(self):
+        return 2
+
+    def getQtrial(self):
+        return 2
+
+    def getGaussQROrder(self):
+        return self.order
+
+    def getCGaussQROrder(self):
+        return self.order * (self.order + 1) // 2 - 1
+
+    def getNumPanels(self):
+        return self.mesh.getNumPanels()
+
+    def getPanelsAreLines(self):
+        return self.mesh.isPolygonal()
+
+
+class GalerkinBuilder:
+    """GalerkinBuilder (galerkin_builder.hpp:117-277)."""
+
+    def __init__(self, builder_data: BuilderData):
+        self.data = builder_data
+        self._dense = {}
+
+    def getData(self):
+        return self.data
+
+    def assembleDense(self, k, c, layers=LayerType.ALL, d=0, row=0, col=0, rows=0, cols=0):
+        k = complex(k)
+        if rows or cols or row or col:
+            raise NotImplementedError("submatrix assembly is not provided by the device path")
+        N = self.data.getNumPanels()
+        layers = LayerType(int(layers))
+        outs = {}
+        args = []
+        for lt in (LayerType.DOUBLE, LayerType.HYPERSINGULAR, LayerType.SINGLE):
+            if layers & lt:
+                outs[lt] = np.zeros((d + 1, N, N), dtype=np.complex128)   # slices written column-major below
+                args.append(outs[lt])
+            else:
+                args.append(None)
+        check(lib().hbem_assemble_dense(self.data._geo.h, k.real, k.imag, float(c), int(layers), int(d),
+                                        *[_vptr(a) for a in args]))
+        self._dense = {}
+        for lt, a in outs.items():
+            # C-ABI slices are column-major N x N: reinterpret as (d+1, N, N) transposed per slice
+            self._dense[lt] = [np.asfortranarray(a[t].T) for t in range(d + 1)]
+        self._d = d
+
+    def _get(self, layer, d):
+        if d > 2:
+            raise RuntimeError("Invalid order of derivative")
+        if layer not in self._dense or d >= len(self._dense[layer]):
+            raise RuntimeError("Requested layer is not computed, so cannot be retrieved")
+        return self._dense[layer][d]
+
+    def getDoubleLayer(self, d=0):
+        return self._get(LayerType.DOUBLE, d)
+
+    def getHypersingular(self, d=0):
+        return self._get(LayerType.HYPERSINGULAR, d)
+
+    def getSingleLayer(self, d=0):
+        return self._get(LayerType.SINGLE, d)
+
+
+class SolutionsOperator:
+    """SolutionsOperator (gen_sol_op.hpp:40-127).
+
+    The derivative variants return (T, T') and (T, T', T'') -- the intended
+    semantics; the reference writes every assemble_operator call into T
+    (gen_sol_op.cpp:150-151, 193-195).
+    """
+
+    def __init__(self, builder_data: BuilderData, proj=True):
+        self.builder_data = builder_data
+        self.do_projection = bool(proj)
+        self._g = builder_data._ctx(self.do_projection)
+
+    def _gen(self, k, c_o, c_i, d):
+        k = complex(k)
+        n = 2 * self.builder_data.getNumPanels()
+        outs = [np.zeros((n, n), dtype=np.complex128, order="F") for _ in range(d + 1)]
+        ptrs = [_vptr(o) for o in outs] + [None] * (2 - d)
+        check(lib().hbem_gen_sol_op(self._g.h, k.real, k.imag, float(c_o), float(c_i), int(d), *ptrs))
+        return outs
+
+    def gen_sol_op(self, k, c_o, c_i, builder=None):
+        return self._gen(k, c_o, c_i, 0)[0]
+
+    def gen_sol_op_1st_der(self, k, c_o, c_i, builder=None):
+        return tuple(self._gen(k, c_o, c_i, 1))
+
+    def gen_sol_op_2nd_der(self, k, c_o, c_i, builder=None):
+        return tuple(self._gen(k, c_o, c_i, 2))
+
+    def mass_matrix(self):
+        N = self.builder_data.getNumPanels()
+        M = np.zeros((N, N), order="F")
+        check(lib().hbem_mass_matrix(self._g.h, _dptr(M)))
+        return M
+
+    def projection_factor(self):
+        n = 2 * self.builder_data.getNumPanels()
+        F = np.zeros((n, n), order="F")
+        tr = np.zeros(n, dtype=np.int32)
+        check(lib().hbem_projection_factor(self._g.h, _dptr(F), tr.ctypes.data_as(_ip)))
+        return F, tr
+
+    def projection(self, T):
+        if not self.do_projection:
+            raise RuntimeError("projection is disabled for this SolutionsOperator")
+        T = np.asfortranarray(T, dtype=np.complex128)
+        out = np.zeros_like(T, order="F")
+        check(lib().hbem_projection(self._g.h, _vptr(T), _vptr(out)))
+        return out
+
+    def getBuilderData(self):
+        return self.builder_data
+
+
+class randomized_svd:
+    """namespace randomized_svd (randsvd.hpp:23-77)."""
+
+    @staticmethod
+    def randGaussian(nr, nc=None, seed=0):
+        """Complex Gaussian test matrix, reproducible from ``seed`` (the
+        reference's draws are unseeded, randsvd.cpp:49-69)."""
+        if nc is None:
+            return gaussian_test_matrix(nr, 1, seed)[:, 0]
+        return gaussian_test_matrix(nr, nc, seed)
+
+    @staticmethod
+    def sv(T, R, q=0):
+        T = np.asfortranarray(T, dtype=np.complex128)
+        R = np.asfortranarray(R, dtype=np.complex128)
+        s = ctypes.c_double()
+        check(lib().hbem_rsvd_sv(_vptr(T), T.shape[0], _vptr(R), R.shape[1], int(q), ctypes.byref(s)))
+        return s.value
+
+    @staticmethod
+    def sv_der(T, T_der, R, q=0):
+        T = np.asfortranarray(T, dtype=np.complex128)
+        Td = np.asfortranarray(T_der, dtype=np.complex128)
+        R = np.asfortranarray(R, dtype=np.complex128)
+        out = np.zeros(2)
+        check(lib().hbem_rsvd_sv_der(_vptr(T), _vptr(Td), T.shape[0], _vptr(R), R.shape[1], int(q), _dptr(out)))
+        return out
+
+    @staticmethod
+    def sv_der2(T, T_der, T_der2, R, q):
+        T = np.asfortranarray(T, dtype=np.complex128)
+        Td = np.asfortranarray(T_der, dtype=np.complex128)
+        Tdd = np.asfortranarray(T_der2, dtype=np.complex128)
+        R = np.asfortranarray(R, dtype=np.complex128)
+        out = np.zeros(3)
+        check(lib().hbem_rsvd_sv_der2(_vptr(T), _vptr(Td), _vptr(Tdd), T.shape[0], _vptr(R), R.shape[1], int(q),
+                                      _dptr(out)))
+        return out
+
+
+def sigma_sweep(builder_data: BuilderData, ks, c_o, c_i, R, q=2, der=0, proj=True):
+    """sigma_min(k) for every k (and dsigma/dk when der >= 1; der = 2 assembles
+    the operators with their first and second k-derivatives)."""
+    g = builder_data._ctx(proj)
+    ks = np.ascontiguousarray(ks, dtype=np.float64)
+    R = np.asfortranarray(R, dtype=np.complex128)
+    sig = np.zeros(ks.size)
+    dsig = np.zeros(ks.size) if der else None
+    check(lib().hbem_sigma_sweep(g.h, _dptr(ks), ks.size, float(c_o), float(c_i), _vptr(R), R.shape[1], int(q),
+                                 int(der), _dptr(sig), _dptr(dsig) if der else None))
+    return (sig, dsig) if der else sig
+
+
+def profile_enable(builder_data: BuilderData, on=True, proj=True):
+    check(lib().hbem_profile_enable(builder_data._ctx(proj).h, 1 if on else 0))
+
+
+def profile_query(builder_data: BuilderData, which, proj=True):
+    ms = ctypes.c_double()
+    n = ctypes.c_longlong()
+    fl = ctypes.c_double()
+    check(lib().hbem_profile_query(builder_data._ctx(proj).h, int(which), ctypes.byref(ms), ctypes.byref(n),
+                                   ctypes.byref(fl)))
+    return ms.value, n.value, fl.value

@@ -1,0 +1,499 @@
+// hbem_assembly.hip -- Galerkin assembly of V, K, W (and d/dk, d2/dk2) on gfx950.
+//
+// Restates GalerkinBuilder (galerkin_builder.cpp:17-1018) for continuous P1
+// test = trial spaces:
+//   * every unordered panel pair (i >= j) is one of three classes
+//     (coinciding i == j, adjacent = shared endpoint, general), assembled by
+//     its own kernel launch;
+//   * per pair and wavenumber the kernel produces the reference's local
+//     interaction matrices (double layer direct + transposed block,
+//     hypersingular, single layer; derivative orders 0..D) into a pair-indexed
+//     buffer;
+//   * a gather kernel sums, for each global entry, the (at most four) pair
+//     blocks that touch it, in a fixed order -- deterministic, no atomics --
+//     and applies the factor i/4 (galerkin_builder.cpp:1014-1017).
+//
+// Work item = (pair, wavenumber).  G lanes of a wavefront cooperate on one
+// item, striding over its quadrature nodes, and reduce with cross-lane
+// shuffles; 64/G items share a wavefront.  G = 1 (one lane per wavenumber,
+// no reduction) serves batched k-sweeps, G = 16/64 single wavenumbers.
+//
+// Local slot layout per derivative order t (16 complex):
+//   [0..3] K direct (test on panel i, trial on j), [4..7] K transposed block,
+//   [8..11] W, [12..15] V; entry (m, n) at m + 2n, m/n = P1 local index
+//   (b0(t) = t, b1(t) = 1 - t; continuous_space.hpp:115-120).
+#include "hbem_device.hpp"
+#include "hbem_internal.hpp"
+
+namespace hbem {
+
+// ---------------------------------------------------------------------------
+// Setup kernels (once per geometry)
+// ---------------------------------------------------------------------------
+
+// Per-panel point / tangent / |tangent| at the Gauss nodes (BuilderData's
+// op_sg, op_tg, Derivative_01_sg/tg; galerkin_builder.cpp:167-186).
+__global__ void k_setup_gauss_table(const double* __restrict__ panels, const double* __restrict__ coefs, int npan,
+                                    const double* __restrict__ xg, int ns, double* __restrict__ gtab) {
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= npan * ns) return;
+    int p = id / ns, a = id % ns;
+    cdbl z, tau;
+    panel_eval(panels + 8 * p, coefs, xg[a], false, z, tau);
+    double* o = gtab + (size_t)id * 5;
+    o[0] = z.x; o[1] = z.y; o[2] = tau.x; o[3] = tau.y; o[4] = hypot(tau.x, tau.y);
+}
+
+// General pairs: skip flags (galerkin_builder.cpp:470-495, 852-870).
+// bit0: all |v.N_j| < eps (direct K skipped), bit1: transposed, bit2: all |tau_i.tau_j| < eps.
+__global__ void k_setup_general_flags(const int2* __restrict__ pairs, int npairs, const double* __restrict__ gtab,
+                                      int ns, unsigned char* __restrict__ flags) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= npairs) return;
+    int2 ij = pairs[e];
+    bool zd = true, zt = true, tz = true;
+    for (int a = 0; a < ns; ++a) {
+        const double* gi = gtab + ((size_t)ij.x * ns + a) * 5;
+        for (int b = 0; b < ns; ++b) {
+            const double* gj = gtab + ((size_t)ij.y * ns + b) * 5;
+            double vx = gi[0] - gj[0], vy = gi[1] - gj[1];
+            double vd = vx * gj[3] - vy * gj[2];
+            double vt = vy * gi[2] - vx * gi[3];
+            double td = gi[3] * gj[3] + gi[2] * gj[2];
+            zd = zd && (fabs(vd) < kDblEps);
+            zt = zt && (fabs(vt) < kDblEps);
+            tz = tz && (fabs(td) < kDblEps);
+        }
+    }
+    flags[e] = (zd ? 1 : 0) | (zt ? 2 : 0) | (tz ? 4 : 0);
+}
+
+// Singular pairs: node geometry {r, r2, vd, vt, tdt, ttp} per node and skip flags.
+// Coinciding (cls 0): one node set of nc^2 nodes; vd/vt hold the two
+// triangles' double-layer factors (galerkin_builder.cpp:376-383).
+// Adjacent (cls 1): two node sets (halves K = 0, 1) of nc^2 nodes each
+// (galerkin_builder.cpp:267-306, 406-426).
+// One workgroup per pair.
+__global__ void k_setup_singular(int cls, const int* __restrict__ ent, int nent, const double* __restrict__ panels,
+                                 const double* __restrict__ coefs, const double* __restrict__ xc,
+                                 const double* __restrict__ wc, int nc, double* __restrict__ ngeo,
+                                 unsigned char* __restrict__ flags) {
+    const int e = blockIdx.x;
+    if (e >= nent) return;
+    const int i = ent[3 * e], j = ent[3 * e + 1];
+    const bool swap = ent[3 * e + 2] != 0;
+    const int nn = nc * nc;
+    const int nhalf = cls == 0 ? 1 : 2;
+    __shared__ int nz[6];
+    if (threadIdx.x < 6) nz[threadIdx.x] = 0;
+    __syncthreads();
+    int loc[6] = {0, 0, 0, 0, 0, 0};
+    for (int nd = threadIdx.x; nd < nhalf * nn; nd += blockDim.x) {
+        const int h = nd / nn, g = nd % nn, a = g / nc, b = g % nc;
+        cdbl zi, zj, ti, tj;
+        if (cls == 0) {
+            double t = xc[b], s = t * (1.0 - xc[a]);
+            panel_eval(panels + 8 * i, coefs, s, false, zi, ti);
+            panel_eval(panels + 8 * i, coefs, t, false, zj, tj);
+        } else {
+            double t = xc[a], s = t * xc[b];
+            double si = h == 0 ? s : t, sj = h == 0 ? t : s;
+            panel_eval(panels + 8 * i, coefs, si, !swap, zi, ti);
+            panel_eval(panels + 8 * j, coefs, sj, swap, zj, tj);
+        }
+        const cdbl v = zi - zj;
+        const double r2 = v.x * v.x + v.y * v.y;
+        const double ni = hypot(ti.x, ti.y), nj = hypot(tj.x, tj.y);
+        double vd = (v.x * tj.y - v.y * tj.x) * ni;
+        double vt = (v.y * ti.x - v.x * ti.y) * nj;
+        const double td = ti.y * tj.y + ti.x * tj.x;
+        double* o = ngeo + ((size_t)e * nhalf * nn + nd) * 6;
+        o[0] = sqrt(r2); o[1] = r2; o[2] = vd; o[3] = vt; o[4] = td; o[5] = ni * nj;
+        if (!(fabs(vd) < kDblEps)) loc[3 * h + 0] = 1;
+        if (!(fabs(vt) < kDblEps)) loc[3 * h + 1] = 1;
+        if (!(fabs(td) < kDblEps)) loc[3 * h + 2] = 1;
+    }
+    for (int q = 0; q < 6; ++q)
+        if (loc[q]) atomicOr(&nz[q], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned char f = 0;
+        if (cls == 0) {
+            // coinciding: bit0 = triangle 0 K skipped, bit3 = triangle 1 K skipped, bit2 = tdottp zero
+            if (!nz[0]) f |= 1;
+            if (!nz[1]) f |= 8;
+            if (!nz[2]) f |= 4;
+        } else {
+            for (int h = 0; h < 2; ++h) {
+                if (!nz[3 * h + 0]) f |= (unsigned char)(1 << (3 * h));
+                if (!nz[3 * h + 1]) f |= (unsigned char)(2 << (3 * h));
+                if (!nz[3 * h + 2]) f |= (unsigned char)(4 << (3 * h));
+            }
+        }
+        flags[e] = f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Assembly kernels
+// ---------------------------------------------------------------------------
+template <int D>
+struct Acc {
+    cdbl kd[D + 1][4];   // sum cfd * {h1/r, h0, h1 r}
+    cdbl kt[D + 1][4];   // sum cft * {h1/r, h0, h1 r}
+    cdbl wa[D + 1];      // sum w * {h0, h1 r, h0 r^2}
+    cdbl wb[D + 1][4];   // sum Bc * {h0, h1 r, h0 r^2}
+    cdbl v[D + 1][4];    // sum g * {h0, h1 r, h0 r^2}
+};
+
+template <int D>
+__device__ __forceinline__ void acc_zero(Acc<D>& A) {
+#pragma unroll
+    for (int t = 0; t <= D; ++t) {
+        A.wa[t] = mk(0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { A.kd[t][q] = mk(0, 0); A.kt[t][q] = mk(0, 0); A.wb[t][q] = mk(0, 0); A.v[t][q] = mk(0, 0); }
+    }
+}
+
+template <int G>
+__device__ __forceinline__ double group_sum(double x) {
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+template <int D, int G>
+__device__ __forceinline__ void acc_reduce(Acc<D>& A) {
+    if (G == 1) return;
+#pragma unroll
+    for (int t = 0; t <= D; ++t) {
+        A.wa[t].x = group_sum<G>(A.wa[t].x); A.wa[t].y = group_sum<G>(A.wa[t].y);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            A.kd[t][q].x = group_sum<G>(A.kd[t][q].x); A.kd[t][q].y = group_sum<G>(A.kd[t][q].y);
+            A.kt[t][q].x = group_sum<G>(A.kt[t][q].x); A.kt[t][q].y = group_sum<G>(A.kt[t][q].y);
+            A.wb[t][q].x = group_sum<G>(A.wb[t][q].x); A.wb[t][q].y = group_sum<G>(A.wb[t][q].y);
+            A.v[t][q].x = group_sum<G>(A.v[t][q].x);   A.v[t][q].y = group_sum<G>(A.v[t][q].y);
+        }
+    }
+}
+
+// One node's contribution.  Geometry: r, r2 (|v|, |v|^2), w (weight),
+// bi/bj (P1 shape values of the test panel i / trial panel j), cd (K direct
+// factor v.N_j |tau_i|, 0 when skipped), ct (transposed factor, 0 when
+// skipped), bt (2 tau_i.tau_j, 0 when tdottp is zero), ttp (|tau_i||tau_j|).
+// Per lane: kappa.  `sym` selects the coinciding form, where the two
+// triangles of [0,1]^2 share the node (roles of s and t exchanged).
+template <int D>
+__device__ __forceinline__ void node_accum(Acc<D>& A, double kap, double r, double r2, double w, const double bi[2],
+                                           const double bj[2], double cd, double ct, double bt, double ttp,
+                                           const SeriesTables& ST, bool sym, double cd2) {
+    double j0, y0, j1, y1;
+    bessel_jy01(kap * r, ST, j0, y0, j1, y1);
+    const cdbl h0 = mk(j0, y0);
+    const cdbl hr = mk(j1 * r, y1 * r);
+    const double rinv = r > 0.0 ? 1.0 / r : 0.0;
+    const cdbl h1r = mk(j1 * rinv, y1 * rinv);
+    const cdbl h0r2 = mk(j0 * r2, y0 * r2);
+    double pbb[4], pbt[4];
+#pragma unroll
+    for (int n = 0; n < 2; ++n)
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            pbb[m + 2 * n] = bi[m] * bj[n];   // test m at panel-i point, trial n at panel-j point
+            pbt[m + 2 * n] = bj[m] * bi[n];   // exchanged roles
+        }
+    const double wcd = w * cd, wct = w * ct, wbt = w * bt, wtt = w * ttp;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double ckd, ckt, cb, cg;
+        if (sym) {
+            ckd = wcd * pbb[q] + w * cd2 * pbt[q];
+            ckt = 0.0;
+            cb = wbt * (pbb[q] + pbt[q]);
+            cg = wtt * (pbb[q] + pbt[q]);
+        } else {
+            ckd = wcd * pbb[q];
+            ckt = wct * pbt[q];
+            cb = wbt * pbb[q];
+            cg = wtt * pbb[q];
+        }
+        axpy(A.kd[0][q], ckd, h1r);
+        if (!sym) axpy(A.kt[0][q], ckt, h1r);
+        axpy(A.wb[0][q], cb, h0);
+        axpy(A.v[0][q], cg, h0);
+        if (D >= 1) {
+            axpy(A.kd[1][q], ckd, h0);
+            if (!sym) axpy(A.kt[1][q], ckt, h0);
+            axpy(A.wb[1][q], cb, hr);
+            axpy(A.v[1][q], cg, hr);
+        }
+        if (D >= 2) {
+            axpy(A.kd[2][q], ckd, hr);
+            if (!sym) axpy(A.kt[2][q], ckt, hr);
+            axpy(A.wb[2][q], cb, h0r2);
+            axpy(A.v[2][q], cg, h0r2);
+        }
+    }
+    const double wa = sym ? 2.0 * w : w;
+    axpy(A.wa[0], wa, h0);
+    if (D >= 1) axpy(A.wa[1], wa, hr);
+    if (D >= 2) axpy(A.wa[2], wa, h0r2);
+}
+
+// Accumulators -> the reference's local matrices (scaled as at the end of
+// double_layer_*, hypersingular_*, single_layer_*; before the factor i/4).
+template <int D>
+__device__ __forceinline__ void acc_store(const Acc<D>& A, double kap, double kr, double c, bool coinc, int qflags,
+                                          cdbl* __restrict__ out) {
+    const double sqc = sqrt(c), k2h = 0.5 * kap * kap, kinv = 1.0 / kap;
+    const double dd[4] = {1.0, -1.0, -1.0, 1.0};   // b'_m b'_n for P1 (+1, -1)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        cdbl K0 = kap * A.kd[0][q], Kt0 = kap * A.kt[0][q];
+        cdbl W0 = dd[q] * A.wa[0] - k2h * A.wb[0][q];
+        cdbl V0 = A.v[0][q];
+        out[q] = K0; out[4 + q] = Kt0; out[8 + q] = W0; out[12 + q] = V0;
+        if (D >= 1) {
+            cdbl K1 = (kr * c) * A.kd[1][q], Kt1 = (kr * c) * A.kt[1][q];
+            cdbl W1 = (-sqc) * (dd[q] * A.wa[1] - k2h * A.wb[1][q] + kap * A.wb[0][q]);
+            cdbl V1 = (-sqc) * A.v[1][q];
+            out[16 + q] = K1; out[20 + q] = Kt1; out[24 + q] = W1; out[28 + q] = V1;
+        }
+        if (D >= 2) {
+            cdbl K2 = c * (A.kd[1][q] - kap * A.kd[2][q]), Kt2 = c * (A.kt[1][q] - kap * A.kt[2][q]);
+            cdbl W2 = (-c) * (dd[q] * (A.wa[2] - kinv * A.wa[1]) - k2h * (A.wb[2][q] - kinv * A.wb[1][q]) -
+                              (2.0 * kap) * A.wb[1][q] + A.wb[0][q]);
+            cdbl V2 = c * (kinv * A.v[1][q] - A.v[2][q]);
+            out[32 + q] = K2; out[36 + q] = Kt2; out[40 + q] = W2; out[44 + q] = V2;
+        }
+    }
+    if (coinc) {
+        // entry (0,1) = entry (1,0) (galerkin_builder.cpp:571-573, 744-746)
+#pragma unroll
+        for (int t = 0; t <= D; ++t) {
+            out[16 * t + 8 + 2] = out[16 * t + 8 + 1];
+            out[16 * t + 12 + 2] = out[16 * t + 12 + 1];
+            out[16 * t + 4 + 0] = mk(0, 0); out[16 * t + 4 + 1] = mk(0, 0);
+            out[16 * t + 4 + 2] = mk(0, 0); out[16 * t + 4 + 3] = mk(0, 0);
+        }
+        if (qflags & 1) {
+            // reference quirk: the d = 0 sum of entry (1,0), scaled for order t
+            if (D >= 1) out[16 + 12 + 2] = (-sqc) * out[12 + 1];
+            if (D >= 2) out[32 + 12 + 2] = c * out[12 + 1];
+        }
+    }
+}
+
+// cls: 0 coinciding, 1 adjacent, 2 general
+template <int CLS, int D, int G>
+__global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int g = lane % G;
+    const long long nitems = (long long)P.nent * P.nk;
+    long long item = wave * (64 / G) + lane / G;
+    const bool active = item < nitems;
+    if (!active) item = nitems - 1;
+    const int e = (int)(item / P.nk), q = (int)(item % P.nk);
+    const double kap = P.kap[q];
+    const unsigned f = P.flags[e];
+
+    Acc<D> A;
+    acc_zero<D>(A);
+
+    if (CLS == 2) {
+        const int2 ij = P.pairs[e];
+        const double fd = (f & 1) ? 0.0 : 1.0, ft = (f & 2) ? 0.0 : 1.0, fb = (f & 4) ? 0.0 : 2.0;
+        const int ns = P.ns, nn = ns * ns;
+        for (int nd = g; nd < nn; nd += G) {
+            const int a = nd / ns, b = nd - a * ns;
+            const double* gi = P.gtab + ((size_t)ij.x * ns + a) * 5;
+            const double* gj = P.gtab + ((size_t)ij.y * ns + b) * 5;
+            const double s = P.xg[a], t = P.xg[b];
+            const double w = P.wg[a] * P.wg[b];
+            const double vx = gi[0] - gj[0], vy = gi[1] - gj[1];
+            const double r2 = vx * vx + vy * vy, r = sqrt(r2);
+            const double cdv = fd * ((vx * gj[3] - vy * gj[2]) * gi[4]);
+            const double ctv = ft * ((vy * gi[2] - vx * gi[3]) * gj[4]);
+            const double bt = fb * (gi[3] * gj[3] + gi[2] * gj[2]);
+            const double bi[2] = {s, 1.0 - s}, bj[2] = {t, 1.0 - t};
+            node_accum<D>(A, kap, r, r2, w, bi, bj, cdv, ctv, bt, gi[4] * gj[4], P.st, false, 0.0);
+        }
+    } else if (CLS == 1) {
+        const bool swap = P.ent[3 * e + 2] != 0;
+        const int nc = P.nc, nn = nc * nc;
+        for (int nd = g; nd < 2 * nn; nd += G) {
+            const int h = nd >= nn ? 1 : 0, gg = nd - h * nn, a = gg / nc, b = gg - a * nc;
+            const double t = P.xc[a], s = t * P.xc[b];
+            const double w = t * P.wc[a] * P.wc[b];
+            const double si = h == 0 ? s : t, sj = h == 0 ? t : s;
+            // test panel i: forward when start(i) is the shared vertex (swap), else reversed; trial j opposite
+            const double bi[2] = {swap ? si : 1.0 - si, swap ? 1.0 - si : si};
+            const double bj[2] = {swap ? 1.0 - sj : sj, swap ? sj : 1.0 - sj};
+            const double* o = P.ngeo + ((size_t)e * 2 * nn + nd) * 6;
+            const unsigned fh = f >> (3 * h);
+            const double cdv = (fh & 1) ? 0.0 : o[2];
+            const double ctv = (fh & 2) ? 0.0 : o[3];
+            const double bt = (fh & 4) ? 0.0 : 2.0 * o[4];
+            node_accum<D>(A, kap, o[0], o[1], w, bi, bj, cdv, ctv, bt, o[5], P.st, false, 0.0);
+        }
+    } else {
+        const int nc = P.nc, nn = nc * nc;
+        for (int nd = g; nd < nn; nd += G) {
+            const int a = nd / nc, b = nd - a * nc;
+            const double t = P.xc[b], s = t * (1.0 - P.xc[a]);
+            const double w = t * P.wc[a] * P.wc[b];
+            const double bi[2] = {s, 1.0 - s}, bj[2] = {t, 1.0 - t};
+            const double* o = P.ngeo + ((size_t)e * nn + nd) * 6;
+            const double cd0 = (f & 1) ? 0.0 : o[2];
+            const double cd1 = (f & 8) ? 0.0 : o[3];
+            const double bt = (f & 4) ? 0.0 : 2.0 * o[4];
+            node_accum<D>(A, kap, o[0], o[1], w, bi, bj, cd0, 0.0, bt, o[5], P.st, true, cd1);
+        }
+    }
+
+    acc_reduce<D, G>(A);
+    if (active && g == 0) {
+        cdbl* out = P.out + ((size_t)q * P.npairs_total + P.pidx[e]) * (16 * (D + 1));
+        acc_store<D>(A, kap, P.kr[q], P.cv[q], CLS == 0, P.qflags, out);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gather: global entry (m, n) of an operator = sum of the local blocks of the
+// panel pairs (P, Q), P in {m-1, m}, Q in {n-1, n} (P1 dofs: dof(0,P) = P+1,
+// dof(1,P) = P; continuous_space.hpp:36-45), scatter rule of
+// galerkin_builder.cpp:992-1009.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long long tri(int i, int j) { return (long long)i * (i + 1) / 2 + j; }
+
+// layer: 0 K, 1 W, 2 V
+__device__ __forceinline__ cdbl op_entry(const cdbl* __restrict__ L, long long npt, int nslot, int q, int layer,
+                                         int t, int m, int n, int N) {
+    cdbl acc = mk(0, 0);
+    const cdbl* base = L + (size_t)q * npt * nslot + 16 * t;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int P = a == 0 ? (m == 0 ? N - 1 : m - 1) : m;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int Q = b == 0 ? (n == 0 ? N - 1 : n - 1) : n;
+            long long p;
+            int slot;
+            if (P >= Q) {
+                p = tri(P, Q);
+                slot = (layer == 0 ? 0 : layer == 1 ? 8 : 12) + a + 2 * b;
+            } else {
+                p = tri(Q, P);
+                slot = layer == 0 ? 4 + a + 2 * b : (layer == 1 ? 8 : 12) + b + 2 * a;
+            }
+            acc = acc + base[p * nslot + slot];
+        }
+    }
+    return mk(-0.25 * acc.y, 0.25 * acc.x);   // * i/4
+}
+
+// Builder matrices for wavenumber slot q: out[layer][t] (N x N column-major)
+__global__ void k_gather_layers(const cdbl* __restrict__ L, long long npt, int nslot, int q, int N, int D, int layers,
+                                cdbl* __restrict__ Kout, cdbl* __restrict__ Wout, cdbl* __restrict__ Vout) {
+    const long long NN = (long long)N * N;
+    long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= NN * (D + 1)) return;
+    const int t = (int)(id / NN);
+    const long long e = id - t * NN;
+    const int m = (int)(e % N), n = (int)(e / N);
+    if ((layers & 1) && Kout) Kout[id] = op_entry(L, npt, nslot, q, 0, t, m, n, N);
+    if ((layers & 4) && Wout) Wout[id] = op_entry(L, npt, nslot, q, 1, t, m, n, N);
+    if ((layers & 2) && Vout) Vout[id] = op_entry(L, npt, nslot, q, 2, t, m, n, N);
+}
+
+// Transmission matrix (assemble_operator, gen_sol_op.cpp:59-74, unprojected):
+// T = [[K_i - K_o + M, V_o - V_i], [W_o - W_i, K_o - K_i + M^T]], M only for t = 0.
+// Output b uses wavenumber slots qi[b] (inner c_i), qo[b] (outer c_o); T_t of
+// output b is written at Tt + b (2N)^2.
+__global__ void k_form_T(const cdbl* __restrict__ L, long long npt, int nslot, const int* __restrict__ qi,
+                         const int* __restrict__ qo, int nb, int N, int D, const double* __restrict__ M,
+                         cdbl* __restrict__ T0, cdbl* __restrict__ T1, cdbl* __restrict__ T2) {
+    const int n2 = 2 * N;
+    const long long NN = (long long)n2 * n2;
+    long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = NN * (D + 1) * nb;
+    if (id >= total) return;
+    const long long mat = id / NN;
+    const long long e = id - mat * NN;
+    const int b = (int)(mat / (D + 1)), t = (int)(mat % (D + 1));
+    const int r = (int)(e % n2), s = (int)(e / n2);
+    const int br = r >= N, bs = s >= N;
+    const int m = r - br * N, n = s - bs * N;
+    const int layer = (br == bs) ? 0 : (br ? 1 : 2);
+    const cdbl xi = op_entry(L, npt, nslot, qi[b], layer, t, m, n, N);
+    const cdbl xo = op_entry(L, npt, nslot, qo[b], layer, t, m, n, N);
+    cdbl v = (br == 0 && bs == 0) ? (xi - xo) : (xo - xi);
+    if (t == 0 && br == bs) v.x += br == 0 ? M[m + (size_t)n * N] : M[n + (size_t)m * N];
+    cdbl* Tt = t == 0 ? T0 : (t == 1 ? T1 : T2);
+    Tt[(size_t)b * NN + e] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+void launch_setup_gauss_table(const double* panels, const double* coefs, int npan, const double* xg, int ns,
+                              double* gtab, hipStream_t st) {
+    int n = npan * ns;
+    hipLaunchKernelGGL(k_setup_gauss_table, dim3((n + 255) / 256), dim3(256), 0, st, panels, coefs, npan, xg, ns, gtab);
+}
+
+void launch_setup_general_flags(const int2* pairs, int npairs, const double* gtab, int ns, unsigned char* flags,
+                                hipStream_t st) {
+    if (npairs <= 0) return;
+    hipLaunchKernelGGL(k_setup_general_flags, dim3((npairs + 255) / 256), dim3(256), 0, st, pairs, npairs, gtab, ns, flags);
+}
+
+void launch_setup_singular(int cls, const int* ent, int nent, const double* panels, const double* coefs,
+                           const double* xc, const double* wc, int nc, double* ngeo, unsigned char* flags,
+                           hipStream_t st) {
+    if (nent <= 0) return;
+    hipLaunchKernelGGL(k_setup_singular, dim3(nent), dim3(256), 0, st, cls, ent, nent, panels, coefs, xc, wc, nc, ngeo, flags);
+}
+
+template <int CLS, int D>
+static void launch_asm_g(const AsmParams& P, int G, hipStream_t st) {
+    const long long nitems = (long long)P.nent * P.nk;
+    if (nitems <= 0) return;
+    const long long waves = (nitems * G + 63) / 64;
+    const int blocks = (int)((waves + 3) / 4);
+    switch (G) {
+    case 1: hipLaunchKernelGGL((k_assemble<CLS, D, 1>), dim3(blocks), dim3(256), 0, st, P); break;
+    case 4: hipLaunchKernelGGL((k_assemble<CLS, D, 4>), dim3(blocks), dim3(256), 0, st, P); break;
+    case 16: hipLaunchKernelGGL((k_assemble<CLS, D, 16>), dim3(blocks), dim3(256), 0, st, P); break;
+    default: hipLaunchKernelGGL((k_assemble<CLS, D, 64>), dim3(blocks), dim3(256), 0, st, P); break;
+    }
+}
+
+void launch_assemble(int cls, int D, const AsmParams& P, int G, hipStream_t st) {
+    if (cls == 0) {
+        if (D == 0) launch_asm_g<0, 0>(P, G, st); else if (D == 1) launch_asm_g<0, 1>(P, G, st); else launch_asm_g<0, 2>(P, G, st);
+    } else if (cls == 1) {
+        if (D == 0) launch_asm_g<1, 0>(P, G, st); else if (D == 1) launch_asm_g<1, 1>(P, G, st); else launch_asm_g<1, 2>(P, G, st);
+    } else {
+        if (D == 0) launch_asm_g<2, 0>(P, G, st); else if (D == 1) launch_asm_g<2, 1>(P, G, st); else launch_asm_g<2, 2>(P, G, st);
+    }
+}
+
+void launch_gather_layers(const cdbl* L, long long npt, int nslot, int q, int N, int D, int layers, cdbl* K, cdbl* W,
+                          cdbl* V, hipStream_t st) {
+    long long n = (long long)N * N * (D + 1);
+    hipLaunchKernelGGL(k_gather_layers, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, npt, nslot, q, N, D,
+                       layers, K, W, V);
+}
+
+void launch_form_T(const cdbl* L, long long npt, int nslot, const int* qi, const int* qo, int nb, int N, int D,
+                   const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st) {
+    long long n = 4LL * N * N * (D + 1) * nb;
+    hipLaunchKernelGGL(k_form_T, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, npt, nslot, qi, qo, nb, N, D, M,
+                       T0, T1, T2);
+}
+
+}  // namespace hbem

@@ -1,0 +1,770 @@
+// hbem_capi.hip -- geometry contexts and the C-ABI of include/hbem.h.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hbem.h"
+#include "hbem_internal.hpp"
+
+using hbem::cdbl;
+
+namespace {
+
+thread_local std::string g_err;
+int g_device = 0;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(HBEM_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPC(x) hip_check((x), #x)
+
+// Growable device buffer.
+struct DBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    void* get(size_t bytes) {
+        if (bytes > n) {
+            if (p) hipFree(p);
+            p = nullptr;
+            n = 0;
+            if (bytes > 0) {
+                if (hipMalloc(&p, bytes) != hipSuccess) { p = nullptr; throw Error(HBEM_ENOMEM, "device allocation failed"); }
+            }
+            n = bytes;
+        }
+        return p;
+    }
+    template <class T> T* as(size_t count) { return (T*)get(count * sizeof(T)); }
+    ~DBuf() { if (p) hipFree(p); }
+};
+
+template <class T>
+T* upload(DBuf& b, const std::vector<T>& v, hipStream_t st) {
+    T* d = b.as<T>(std::max<size_t>(v.size(), 1));
+    if (!v.empty()) HIPC(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return d;
+}
+
+// Algorithmic FP64 flops per node (frozen model): Hankel node (13 geometry + 171
+// Bessel) and accumulation for P1 x P1, three layers, derivative order d.
+double flops_per_node(int d) { return 184.0 + (d == 0 ? 108.0 : d == 1 ? 192.0 : 314.0); }
+
+}  // namespace
+
+struct hbem_geometry {
+    int dev = 0;
+    hipStream_t st = nullptr;
+    int N = 0, order = 0, ns = 0, nc = 0, flags = 0;
+    bool lines = true, proj = true;
+    long long npairs = 0;
+    std::vector<double> panels, coefs, xg, wg, xc, wc;
+    std::vector<double> M, F, Finv;
+    std::vector<int> tr;
+    hbem::SeriesTables series;
+    // device geometry
+    DBuf d_panels, d_coefs, d_xg, d_wg, d_xc, d_wc, d_gtab;
+    DBuf d_gpairs, d_gpidx, d_gflags;
+    DBuf d_aent, d_apidx, d_aflags, d_angeo;
+    DBuf d_cent, d_cpidx, d_cflags, d_cngeo;
+    int ngen = 0, nadj = 0, ncoi = 0;
+    DBuf d_M, d_Frm, d_Fcm, d_Finvcm, d_Finvc;
+    // work space
+    DBuf w_L, w_T0, w_T1, w_T2, w_piv, w_perm, w_X, w_Y, w_work, w_R, w_u, w_v, w_a, w_b, w_sig, w_dsig, w_kap,
+        w_kr, w_cv, w_qi, w_qo, w_tmp, w_out;
+    // profiling
+    bool prof = false;
+    double prof_ms[4] = {0, 0, 0, 0};
+    long long prof_n[4] = {0, 0, 0, 0};
+    double prof_flops[4] = {0, 0, 0, 0};
+    std::mutex mu;
+};
+
+namespace {
+
+int fail(const std::exception& e) {
+    g_err = e.what();
+    if (auto* he = dynamic_cast<const Error*>(&e)) return he->code;
+    return HBEM_EINVAL;
+}
+
+#define API_BEGIN try {
+#define API_END \
+    }                                              \
+    catch (const std::exception& e) { return fail(e); } \
+    return HBEM_OK;
+
+void check_k(double k_re, double k_im, double c) {
+    if (c < 1.0) throw Error(HBEM_EINVAL, "Refraction index must not be smaller than 1");
+    if (k_im < 0.0)
+        throw Error(HBEM_EINVAL, "Wavenumber does not meet conditions for uniqueness of the solution (Im(k) >= 0)");
+    if (k_im != 0.0 || !(k_re > 0.0))
+        throw Error(HBEM_ENOTSUP, "only real positive wavenumbers are supported by the device path");
+}
+
+// Assemble the local matrices of all pairs for wavenumber slots (kap, kr, cv);
+// returns the local buffer.  Uses G = 1 (lanes over wavenumbers) when the
+// batch is wide, node-parallel groups otherwise.
+cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const std::vector<double>& kr,
+                      const std::vector<double>& cv, int D) {
+    const int nk = (int)kap.size();
+    const int nslot = 16 * (D + 1);
+    cdbl* L = g->w_L.as<cdbl>((size_t)nk * g->npairs * nslot);
+    hbem::AsmParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.gtab = (const double*)g->d_gtab.p;
+    P.xg = (const double*)g->d_xg.p;
+    P.wg = (const double*)g->d_wg.p;
+    P.xc = (const double*)g->d_xc.p;
+    P.wc = (const double*)g->d_wc.p;
+    P.nk = nk;
+    P.kap = upload(g->w_kap, kap, g->st);
+    P.kr = upload(g->w_kr, kr, g->st);
+    P.cv = upload(g->w_cv, cv, g->st);
+    P.npairs_total = g->npairs;
+    P.out = L;
+    P.ns = g->ns;
+    P.nc = g->nc;
+    P.qflags = g->flags;
+    P.st = g->series;
+    const int Gg = nk >= 64 ? 1 : (nk >= 16 ? 4 : 16);
+    const int Gs = nk >= 64 ? 1 : 64;
+    struct Launch { int cls, which, G, nent; const int* ent; const int2* pairs; const double* ngeo; const unsigned char* flags; const long long* pidx; double nodes; };
+    Launch ls[3] = {
+        {2, 0, Gg, g->ngen, nullptr, (const int2*)g->d_gpairs.p, nullptr, (const unsigned char*)g->d_gflags.p,
+         (const long long*)g->d_gpidx.p, (double)g->ns * g->ns},
+        {1, 1, Gs, g->nadj, (const int*)g->d_aent.p, nullptr, (const double*)g->d_angeo.p,
+         (const unsigned char*)g->d_aflags.p, (const long long*)g->d_apidx.p, 2.0 * g->nc * g->nc},
+        {0, 2, Gs, g->ncoi, (const int*)g->d_cent.p, nullptr, (const double*)g->d_cngeo.p,
+         (const unsigned char*)g->d_cflags.p, (const long long*)g->d_cpidx.p, (double)g->nc * g->nc},
+    };
+    for (auto& l : ls) {
+        if (l.nent == 0) continue;
+        P.nent = l.nent;
+        P.ent = l.ent;
+        P.pairs = l.pairs;
+        P.ngeo = l.ngeo;
+        P.flags = l.flags;
+        P.pidx = l.pidx;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (g->prof) {
+            HIPC(hipEventCreate(&e0));
+            HIPC(hipEventCreate(&e1));
+            HIPC(hipEventRecord(e0, g->st));
+        }
+        hbem::launch_assemble(l.cls, D, P, l.G, g->st);
+        HIPC(hipGetLastError());
+        if (g->prof) {
+            HIPC(hipEventRecord(e1, g->st));
+            HIPC(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, e0, e1));
+            g->prof_ms[l.which] += ms;
+            g->prof_n[l.which] += 1;
+            g->prof_flops[l.which] += (double)l.nent * nk * l.nodes * flops_per_node(D);
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+        }
+    }
+    return L;
+}
+
+// Dense stage shared by sv, sv_der and the sweep: given LU factors of the
+// unprojected T (batch matrices), the projection factor (F row-major /
+// column-major, or null for no projection) and R, returns sigma and the
+// singular pair (u, v) of T_p = F^{-1} T F^{-T} in d_u / d_v.
+struct DenseCtx {
+    hipStream_t st;
+    int n, batch, nc, q;
+    const cdbl* LU;
+    const int* perm;
+    const double* Frm;   // F row-major (apply F), null = identity
+    const double* Fcm;   // F column-major (apply F^T)
+    cdbl* X;             // n x nc x batch
+    cdbl* Y;
+    cdbl* work;
+};
+
+// Y <- T_p^{-1} X  or  T_p^{-H} X  (T_p^{-1} = F^T T^{-1} F, T_p^{-H} = F^T T^{-H} F)
+void tp_solve(DenseCtx& c, cdbl* X, cdbl* Y, bool adj) {
+    if (c.Frm) {
+        hbem::real_apply_thin_batched(c.Frm, c.n, c.batch, X, Y, c.nc, c.st);
+        hbem::lu_solve_thin_batched(c.LU, c.perm, c.n, c.batch, Y, c.nc, adj, c.work, c.st);
+        hbem::real_apply_thin_batched(c.Fcm, c.n, c.batch, Y, X, c.nc, c.st);
+        HIPC(hipMemcpyAsync(Y, X, sizeof(cdbl) * (size_t)c.n * c.nc * c.batch, hipMemcpyDeviceToDevice, c.st));
+    } else {
+        if (Y != X) HIPC(hipMemcpyAsync(Y, X, sizeof(cdbl) * (size_t)c.n * c.nc * c.batch, hipMemcpyDeviceToDevice, c.st));
+        hbem::lu_solve_thin_batched(c.LU, c.perm, c.n, c.batch, Y, c.nc, adj, c.work, c.st);
+    }
+}
+
+// randomized_svd::sv (randsvd.cpp:99-146) on the batch; X holds R replicated per matrix.
+void rsvd_core(DenseCtx& c, double* d_sigma, cdbl* d_u, cdbl* d_v) {
+    tp_solve(c, c.X, c.Y, false);                              // Y = T_p^{-1} R
+    hbem::orthonormalize_thin_batched(c.Y, c.n, c.batch, c.nc, c.st);   // Q
+    for (int it = 0; it < c.q; ++it) {
+        HIPC(hipMemcpyAsync(c.X, c.Y, sizeof(cdbl) * (size_t)c.n * c.nc * c.batch, hipMemcpyDeviceToDevice, c.st));
+        tp_solve(c, c.X, c.Y, true);
+        hbem::orthonormalize_thin_batched(c.Y, c.n, c.batch, c.nc, c.st);
+        HIPC(hipMemcpyAsync(c.X, c.Y, sizeof(cdbl) * (size_t)c.n * c.nc * c.batch, hipMemcpyDeviceToDevice, c.st));
+        tp_solve(c, c.X, c.Y, false);
+        hbem::orthonormalize_thin_batched(c.Y, c.n, c.batch, c.nc, c.st);
+    }
+    // X = Q, Y = T_p^{-H} Q
+    HIPC(hipMemcpyAsync(c.X, c.Y, sizeof(cdbl) * (size_t)c.n * c.nc * c.batch, hipMemcpyDeviceToDevice, c.st));
+    cdbl* Q = c.work + (size_t)c.n * c.nc * c.batch;   // second half of work holds Q
+    HIPC(hipMemcpyAsync(Q, c.X, sizeof(cdbl) * (size_t)c.n * c.nc * c.batch, hipMemcpyDeviceToDevice, c.st));
+    tp_solve(c, c.X, c.Y, true);
+    hbem::sigma_from_thin_batched(c.Y, Q, c.n, c.batch, c.nc, d_sigma, d_u, d_v, c.st);
+}
+
+void copy_out_square(const cdbl* d, void* h, size_t count, hipStream_t st) {
+    if (h) HIPC(hipMemcpyAsync(h, d, count * sizeof(cdbl), hipMemcpyDeviceToHost, st));
+}
+
+// T_p = Finv T Finv^T for `count` matrices (in place through tmp)
+void project_inplace(hbem_geometry* g, cdbl* T, int count) {
+    const int n = 2 * g->N;
+    const long long s = (long long)n * n;
+    cdbl* tmp = g->w_tmp.as<cdbl>((size_t)s * count);
+    const cdbl* Fi = (const cdbl*)g->d_Finvc.p;
+    hbem::zgemm_batched(n, n, n, hbem::mk(1, 0), Fi, n, 0, T, n, s, false, hbem::mk(0, 0), tmp, n, s, count, g->st);
+    hbem::zgemm_batched(n, n, n, hbem::mk(1, 0), tmp, n, s, Fi, n, 0, true, hbem::mk(0, 0), T, n, s, count, g->st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int hbem_last_error(char* buf, size_t len) {
+    if (buf && len > 0) {
+        std::strncpy(buf, g_err.c_str(), len - 1);
+        buf[len - 1] = 0;
+    }
+    return (int)g_err.size();
+}
+
+int hbem_init(int device) {
+    API_BEGIN
+    int n = 0;
+    HIPC(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw Error(HBEM_EINVAL, "invalid device index");
+    g_device = device;
+    HIPC(hipSetDevice(device));
+    API_END
+}
+
+int hbem_geometry_create(const double* panels, int npanels, const double* coefs, int ncoefs, int order, int proj,
+                         int flags, hbem_geometry_t* out) {
+    hbem_geometry* g = nullptr;
+    API_BEGIN
+    if (!out) throw Error(HBEM_EINVAL, "null output handle");
+    if (npanels < 3) throw Error(HBEM_EINVAL, "a closed P1 mesh needs at least 3 panels");
+    if (order < 1 || order > 40) throw Error(HBEM_EINVAL, "quadrature order out of range");
+    for (int i = 0; i < npanels; ++i) {
+        int t = (int)panels[8 * i];
+        if (t < 0 || t > 2 || panels[8 * i] != (double)t) throw Error(HBEM_EINVAL, "invalid panel type");
+        if (t == 2) {
+            int c0 = (int)panels[8 * i + 3], nc = (int)panels[8 * i + 4];
+            if (c0 < 0 || nc < 1 || c0 + nc > ncoefs) throw Error(HBEM_EINVAL, "trig-curve panel coefficients out of range");
+        }
+    }
+    g = new hbem_geometry;
+    g->dev = g_device;
+    HIPC(hipSetDevice(g->dev));
+    HIPC(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
+    g->N = npanels;
+    g->order = order;
+    g->ns = order;
+    g->nc = order * (order + 1) / 2 - 1;
+    g->flags = flags;
+    g->proj = proj != 0;
+    g->panels.assign(panels, panels + 8 * (size_t)npanels);
+    if (ncoefs > 0) g->coefs.assign(coefs, coefs + 3 * (size_t)ncoefs);
+    g->lines = true;
+    for (int i = 0; i < npanels; ++i) if ((int)panels[8 * i] != 0) g->lines = false;
+    g->xg.resize(g->ns); g->wg.resize(g->ns); g->xc.resize(g->nc); g->wc.resize(g->nc);
+    hbem::gauss_rule(0.0, 1.0, g->ns, g->xg.data(), g->wg.data());
+    hbem::composite_rule(order, g->xc.data(), g->wc.data());
+    g->series = hbem::make_series_tables();
+    const int N = npanels;
+    g->npairs = (long long)N * (N + 1) / 2;
+    // classify pairs (galerkin_builder.cpp:872-897)
+    std::vector<int2> gp;
+    std::vector<long long> gpi, api, cpi;
+    std::vector<int> aent, cent;
+    gp.reserve((size_t)g->npairs);
+    gpi.reserve((size_t)g->npairs);
+    const double* cptr = g->coefs.empty() ? nullptr : g->coefs.data();
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j <= i; ++j) {
+            long long p = (long long)i * (i + 1) / 2 + j;
+            bool sw = false;
+            if (i == j) { cent.push_back(i); cent.push_back(i); cent.push_back(0); cpi.push_back(p); }
+            else if (hbem::panels_adjacent(g->panels.data(), cptr, i, j, sw)) {
+                aent.push_back(i); aent.push_back(j); aent.push_back(sw ? 1 : 0); api.push_back(p);
+            } else { gp.push_back(make_int2(i, j)); gpi.push_back(p); }
+        }
+    g->ngen = (int)gp.size();
+    g->nadj = (int)api.size();
+    g->ncoi = (int)cpi.size();
+    hipStream_t st = g->st;
+    const double* dpan = upload(g->d_panels, g->panels, st);
+    std::vector<double> cf = g->coefs.empty() ? std::vector<double>(3, 0.0) : g->coefs;
+    const double* dcoef = upload(g->d_coefs, cf, st);
+    const double* dxg = upload(g->d_xg, g->xg, st);
+    upload(g->d_wg, g->wg, st);
+    const double* dxc = upload(g->d_xc, g->xc, st);
+    const double* dwc = upload(g->d_wc, g->wc, st);
+    double* gtab = g->d_gtab.as<double>((size_t)N * g->ns * 5);
+    hbem::launch_setup_gauss_table(dpan, dcoef, N, dxg, g->ns, gtab, st);
+    upload(g->d_gpairs, gp, st);
+    upload(g->d_gpidx, gpi, st);
+    g->d_gflags.get(std::max(1, g->ngen));
+    hbem::launch_setup_general_flags((const int2*)g->d_gpairs.p, g->ngen, gtab, g->ns, (unsigned char*)g->d_gflags.p, st);
+    upload(g->d_aent, aent, st);
+    upload(g->d_apidx, api, st);
+    g->d_aflags.get(std::max(1, g->nadj));
+    double* angeo = g->d_angeo.as<double>(std::max<size_t>(1, (size_t)g->nadj * 2 * g->nc * g->nc * 6));
+    hbem::launch_setup_singular(1, (const int*)g->d_aent.p, g->nadj, dpan, dcoef, dxc, dwc, g->nc, angeo,
+                                (unsigned char*)g->d_aflags.p, st);
+    upload(g->d_cent, cent, st);
+    upload(g->d_cpidx, cpi, st);
+    g->d_cflags.get(std::max(1, g->ncoi));
+    double* cngeo = g->d_cngeo.as<double>(std::max<size_t>(1, (size_t)g->ncoi * g->nc * g->nc * 6));
+    hbem::launch_setup_singular(0, (const int*)g->d_cent.p, g->ncoi, dpan, dcoef, dxc, dwc, g->nc, cngeo,
+                                (unsigned char*)g->d_cflags.p, st);
+    HIPC(hipGetLastError());
+    // mass matrix and projection factor (gen_sol_op.cpp:23-43)
+    g->M.resize((size_t)N * N);
+    hbem::mass_matrix_host(g->panels.data(), N, cptr, g->lines, order, g->M.data());
+    upload(g->d_M, g->M, st);
+    if (g->proj) {
+        const int n = 2 * N;
+        std::vector<double> A((size_t)n * n, 0.0);
+        for (int j = 0; j < N; ++j)
+            for (int i = 0; i < N; ++i) {
+                A[i + (size_t)j * n] = g->M[i + (size_t)j * N];
+                A[(N + i) + (size_t)(N + j) * n] = g->M[j + (size_t)i * N];
+            }
+        g->F.resize((size_t)n * n);
+        g->tr.resize(n);
+        hbem::ldlt_projection_factor(A.data(), n, g->F.data(), g->tr.data());
+        g->Finv.resize((size_t)n * n);
+        hbem::invert_projection_factor(g->F.data(), g->tr.data(), n, g->Finv.data());
+        std::vector<double> Frm((size_t)n * n);
+        for (int j = 0; j < n; ++j) for (int i = 0; i < n; ++i) Frm[(size_t)i * n + j] = g->F[i + (size_t)j * n];
+        upload(g->d_Frm, Frm, st);
+        upload(g->d_Fcm, g->F, st);
+        upload(g->d_Finvcm, g->Finv, st);
+        std::vector<cdbl> Fic((size_t)n * n);
+        for (size_t e = 0; e < Fic.size(); ++e) Fic[e] = hbem::mk(g->Finv[e], 0.0);
+        upload(g->d_Finvc, Fic, st);
+    }
+    HIPC(hipStreamSynchronize(st));
+    *out = g;
+    g = nullptr;
+    }
+    catch (const std::exception& e) {
+        if (g) { if (g->st) hipStreamDestroy(g->st); delete g; }
+        return fail(e);
+    }
+    return HBEM_OK;
+}
+
+int hbem_geometry_destroy(hbem_geometry_t g) {
+    API_BEGIN
+    if (!g) return HBEM_OK;
+    hipStreamSynchronize(g->st);
+    hipStream_t st = g->st;
+    delete g;
+    hipStreamDestroy(st);
+    API_END
+}
+
+int hbem_geometry_info(hbem_geometry_t g, int* npanels, int* order, int* corder, int* lines) {
+    API_BEGIN
+    if (!g) throw Error(HBEM_EINVAL, "null geometry");
+    if (npanels) *npanels = g->N;
+    if (order) *order = g->ns;
+    if (corder) *corder = g->nc;
+    if (lines) *lines = g->lines ? 1 : 0;
+    API_END
+}
+
+int hbem_assemble_dense(hbem_geometry_t g, double k_re, double k_im, double c, int layers, int d, void* K, void* W,
+                        void* V) {
+    API_BEGIN
+    if (!g) throw Error(HBEM_EINVAL, "null geometry");
+    if (d < 0 || d > 2) throw Error(HBEM_EINVAL, "Invalid order of derivative");
+    check_k(k_re, k_im, c);
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPC(hipSetDevice(g->dev));
+    const double sc = std::sqrt(c);
+    cdbl* L = assemble_locals(g, {k_re * sc}, {k_re}, {c}, d);
+    const int N = g->N;
+    const size_t NN = (size_t)N * N * (d + 1);
+    cdbl* out = g->w_out.as<cdbl>(3 * NN);
+    hbem::launch_gather_layers(L, g->npairs, 16 * (d + 1), 0, N, d, layers, out, out + NN, out + 2 * NN, g->st);
+    HIPC(hipGetLastError());
+    if (layers & HBEM_LAYER_DOUBLE) copy_out_square(out, K, NN, g->st);
+    if (layers & HBEM_LAYER_HYPERSINGULAR) copy_out_square(out + NN, W, NN, g->st);
+    if (layers & HBEM_LAYER_SINGLE) copy_out_square(out + 2 * NN, V, NN, g->st);
+    HIPC(hipStreamSynchronize(g->st));
+    API_END
+}
+
+int hbem_mass_matrix(hbem_geometry_t g, double* M) {
+    API_BEGIN
+    if (!g || !M) throw Error(HBEM_EINVAL, "null argument");
+    std::memcpy(M, g->M.data(), sizeof(double) * g->M.size());
+    API_END
+}
+
+int hbem_projection_factor(hbem_geometry_t g, double* F, int* transpositions) {
+    API_BEGIN
+    if (!g) throw Error(HBEM_EINVAL, "null geometry");
+    if (!g->proj) throw Error(HBEM_EINVAL, "geometry was created without projection");
+    if (F) std::memcpy(F, g->F.data(), sizeof(double) * g->F.size());
+    if (transpositions) std::memcpy(transpositions, g->tr.data(), sizeof(int) * g->tr.size());
+    API_END
+}
+
+int hbem_gen_sol_op(hbem_geometry_t g, double k_re, double k_im, double c_o, double c_i, int d, void* T, void* Td,
+                    void* Tdd) {
+    API_BEGIN
+    if (!g) throw Error(HBEM_EINVAL, "null geometry");
+    if (d < 0 || d > 2) throw Error(HBEM_EINVAL, "Invalid order of derivative");
+    check_k(k_re, k_im, c_i);
+    check_k(k_re, k_im, c_o);
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPC(hipSetDevice(g->dev));
+    cdbl* L = assemble_locals(g, {k_re * std::sqrt(c_i), k_re * std::sqrt(c_o)}, {k_re, k_re}, {c_i, c_o}, d);
+    const int n = 2 * g->N;
+    const size_t nn = (size_t)n * n;
+    cdbl* T0 = g->w_T0.as<cdbl>(nn);
+    cdbl* T1 = d >= 1 ? g->w_T1.as<cdbl>(nn) : nullptr;
+    cdbl* T2 = d >= 2 ? g->w_T2.as<cdbl>(nn) : nullptr;
+    std::vector<int> qi = {0}, qo = {1};
+    const int* dqi = upload(g->w_qi, qi, g->st);
+    const int* dqo = upload(g->w_qo, qo, g->st);
+    hbem::launch_form_T(L, g->npairs, 16 * (d + 1), dqi, dqo, 1, g->N, d, (const double*)g->d_M.p, T0, T1, T2, g->st);
+    HIPC(hipGetLastError());
+    cdbl* Ts[3] = {T0, T1, T2};
+    void* hs[3] = {T, Td, Tdd};
+    for (int t = 0; t <= d; ++t) {
+        if (!hs[t]) continue;
+        if (g->proj) project_inplace(g, Ts[t], 1);
+        copy_out_square(Ts[t], hs[t], nn, g->st);
+    }
+    HIPC(hipStreamSynchronize(g->st));
+    API_END
+}
+
+int hbem_projection(hbem_geometry_t g, const void* T, void* Tp) {
+    API_BEGIN
+    if (!g || !T || !Tp) throw Error(HBEM_EINVAL, "null argument");
+    if (!g->proj) throw Error(HBEM_EINVAL, "geometry was created without projection");
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPC(hipSetDevice(g->dev));
+    const int n = 2 * g->N;
+    const size_t nn = (size_t)n * n;
+    cdbl* d = g->w_T0.as<cdbl>(nn);
+    HIPC(hipMemcpyAsync(d, T, nn * sizeof(cdbl), hipMemcpyHostToDevice, g->st));
+    project_inplace(g, d, 1);
+    copy_out_square(d, Tp, nn, g->st);
+    HIPC(hipStreamSynchronize(g->st));
+    API_END
+}
+
+}  // extern "C"
+
+namespace {
+
+// Standalone dense context for randomized_svd::* on caller matrices.
+struct Standalone {
+    std::mutex mu;
+    hipStream_t st = nullptr;
+    DBuf T, T1, T2, piv, perm, X, Y, work, u, v, sig, out, W, Wpiv, Wperm, p, p2, ud, tmp;
+    void init() {
+        if (!st) {
+            HIPC(hipSetDevice(g_device));
+            HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        }
+    }
+};
+Standalone g_sa;
+
+void check_dense_args(const void* T, int n, const void* R, int nc, int q) {
+    if (!T || !R) throw Error(HBEM_EINVAL, "null matrix");
+    if (n < 1) throw Error(HBEM_EINVAL, "matrix order must be positive");
+    if (nc < 1 || nc > 8 || nc > n) throw Error(HBEM_EINVAL, "number of random columns must be in [1, min(8, n)]");
+    if (q < 0) throw Error(HBEM_EINVAL, "number of subspace iterations must be >= 0");
+}
+
+// Runs LU + rsvd on one host matrix; leaves LU/u/v on the device.
+double standalone_sv(const void* T, int n, const void* R, int nc, int q, bool want_vectors) {
+    Standalone& s = g_sa;
+    s.init();
+    const size_t nn = (size_t)n * n;
+    cdbl* dT = s.T.as<cdbl>(nn);
+    HIPC(hipMemcpyAsync(dT, T, nn * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
+    int* piv = s.piv.as<int>(n);
+    int* perm = s.perm.as<int>(n);
+    hbem::lu_factor_batched(dT, n, 1, piv, perm, s.st);
+    DenseCtx c;
+    c.st = s.st; c.n = n; c.batch = 1; c.nc = nc; c.q = q; c.LU = dT; c.perm = perm; c.Frm = nullptr; c.Fcm = nullptr;
+    c.X = s.X.as<cdbl>((size_t)n * nc);
+    c.Y = s.Y.as<cdbl>((size_t)n * nc);
+    c.work = s.work.as<cdbl>(2 * (size_t)n * nc);
+    HIPC(hipMemcpyAsync(c.X, R, (size_t)n * nc * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
+    double* dsig = s.sig.as<double>(1);
+    cdbl* du = want_vectors ? s.u.as<cdbl>(n) : nullptr;
+    cdbl* dv = want_vectors ? s.v.as<cdbl>(n) : nullptr;
+    rsvd_core(c, dsig, du, dv);
+    HIPC(hipGetLastError());
+    double sigma = 0.0;
+    HIPC(hipMemcpyAsync(&sigma, dsig, sizeof(double), hipMemcpyDeviceToHost, s.st));
+    HIPC(hipStreamSynchronize(s.st));
+    return sigma;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hbem_rsvd_sv(const void* T, int n, const void* R, int nc, int q, double* sigma) {
+    API_BEGIN
+    check_dense_args(T, n, R, nc, q);
+    if (!sigma) throw Error(HBEM_EINVAL, "null output");
+    std::lock_guard<std::mutex> lk(g_sa.mu);
+    *sigma = standalone_sv(T, n, R, nc, q, false);
+    API_END
+}
+
+int hbem_rsvd_sv_der(const void* T, const void* Td, int n, const void* R, int nc, int q, double* out2) {
+    API_BEGIN
+    check_dense_args(T, n, R, nc, q);
+    if (!Td || !out2) throw Error(HBEM_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(g_sa.mu);
+    Standalone& s = g_sa;
+    double sigma = standalone_sv(T, n, R, nc, q, true);
+    const size_t nn = (size_t)n * n;
+    cdbl* dT1 = s.T1.as<cdbl>(nn);
+    HIPC(hipMemcpyAsync(dT1, Td, nn * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
+    double* dout = s.out.as<double>(1);
+    hbem::bilinear_re_batched(dT1, (long long)nn, (const cdbl*)s.u.p, (const cdbl*)s.v.p, n, 1, dout, s.st);
+    HIPC(hipGetLastError());
+    double der = 0.0;
+    HIPC(hipMemcpyAsync(&der, dout, sizeof(double), hipMemcpyDeviceToHost, s.st));
+    HIPC(hipStreamSynchronize(s.st));
+    out2[0] = sigma;
+    out2[1] = der;
+    API_END
+}
+
+// randomized_svd::sv_der2 (randsvd.cpp:171-216): the eigenvector x of the
+// Wielandt matrix W = [[0,T],[T^H,0]] for sigma is [u; v]/sqrt2 from the
+// randomized subspace; the bordered system follows the reference.
+int hbem_rsvd_sv_der2(const void* T, const void* Td, const void* Tdd, int n, const void* R, int nc, int q,
+                      double* out3) {
+    API_BEGIN
+    check_dense_args(T, n, R, nc, q);
+    if (!Td || !Tdd || !out3) throw Error(HBEM_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(g_sa.mu);
+    Standalone& s = g_sa;
+    double sigma = standalone_sv(T, n, R, nc, q, true);
+    const size_t nn = (size_t)n * n;
+    const int m2 = 2 * n;
+    cdbl* dT1 = s.T1.as<cdbl>(nn);
+    cdbl* dT2 = s.T2.as<cdbl>(nn);
+    HIPC(hipMemcpyAsync(dT1, Td, nn * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
+    HIPC(hipMemcpyAsync(dT2, Tdd, nn * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
+    double* dout = s.out.as<double>(1);
+    hbem::bilinear_re_batched(dT1, (long long)nn, (const cdbl*)s.u.p, (const cdbl*)s.v.p, n, 1, dout, s.st);
+    double der = 0.0;
+    HIPC(hipMemcpyAsync(&der, dout, sizeof(double), hipMemcpyDeviceToHost, s.st));
+    // x = [u; v] / sqrt2, normalised by its largest-modulus entry
+    std::vector<cdbl> x(m2);
+    HIPC(hipMemcpyAsync(x.data(), s.u.p, n * sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
+    HIPC(hipMemcpyAsync(x.data() + n, s.v.p, n * sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
+    HIPC(hipStreamSynchronize(s.st));
+    double best = 0.0;
+    int m = 5 < m2 ? 5 : 0;
+    for (int l = 0; l < m2; ++l) {
+        double a = std::sqrt(x[l].x * x[l].x + x[l].y * x[l].y);
+        if (a > best) { best = a; m = l; }
+    }
+    const std::complex<double> xm(x[m].x, x[m].y);
+    std::vector<cdbl> u(m2);
+    for (int l = 0; l < m2; ++l) {
+        std::complex<double> z = std::complex<double>(x[l].x, x[l].y) / xm;
+        u[l] = hbem::mk(z.real(), z.imag());
+    }
+    cdbl* W = s.W.as<cdbl>((size_t)m2 * m2);
+    // s.T holds the LU factors of T; W is built from the caller's T
+    HIPC(hipMemcpyAsync(s.T.p, T, nn * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
+    hbem::wielandt_build((const cdbl*)s.T.p, n, sigma, W, s.st);
+    cdbl* du = s.p.as<cdbl>(m2);
+    HIPC(hipMemcpyAsync(du, u.data(), m2 * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
+    std::vector<cdbl> negu(m2);
+    for (int l = 0; l < m2; ++l) negu[l] = hbem::mk(-u[l].x, -u[l].y);
+    HIPC(hipMemcpyAsync(W + (size_t)m * m2, negu.data(), m2 * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
+    int* wpiv = s.Wpiv.as<int>(m2);
+    int* wperm = s.Wperm.as<int>(m2);
+    hbem::lu_factor_batched(W, m2, 1, wpiv, wperm, s.st);
+    // p = [T' u2; T'^H u1]
+    cdbl* p = s.p2.as<cdbl>(m2);
+    hbem::gemv_batched(dT1, 0, du + n, p, n, 1, false, s.st);
+    hbem::gemv_batched(dT1, 0, du, p + n, n, 1, true, s.st);
+    cdbl* wk = s.work.as<cdbl>(2 * (size_t)m2);
+    hbem::lu_solve_thin_batched(W, wperm, m2, 1, p, 1, false, wk, s.st);   // p <- W^{-1} p
+    std::vector<cdbl> ud(m2);
+    HIPC(hipMemcpyAsync(ud.data(), p, m2 * sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
+    HIPC(hipStreamSynchronize(s.st));
+    for (auto& z : ud) z = hbem::mk(-z.x, -z.y);   // u_der = -W^{-1} p
+    const std::complex<double> t(ud[m].x, ud[m].y);
+    ud[m] = hbem::mk(0, 0);
+    cdbl* dud = s.ud.as<cdbl>(m2);
+    HIPC(hipMemcpyAsync(dud, ud.data(), m2 * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
+    cdbl* pp = s.tmp.as<cdbl>(2 * (size_t)m2);
+    cdbl* p2 = pp + m2;
+    hbem::gemv_batched(dT1, 0, dud + n, pp, n, 1, false, s.st);
+    hbem::gemv_batched(dT1, 0, dud, pp + n, n, 1, true, s.st);
+    hbem::gemv_batched(dT2, 0, du + n, p2, n, 1, false, s.st);
+    hbem::gemv_batched(dT2, 0, du, p2 + n, n, 1, true, s.st);
+    std::vector<cdbl> hp(m2), hp2(m2);
+    HIPC(hipMemcpyAsync(hp.data(), pp, m2 * sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
+    HIPC(hipMemcpyAsync(hp2.data(), p2, m2 * sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
+    HIPC(hipStreamSynchronize(s.st));
+    std::vector<cdbl> rhs(m2);
+    for (int l = 0; l < m2; ++l) {
+        std::complex<double> v = std::complex<double>(hp2[l].x, hp2[l].y) +
+                                 2.0 * (std::complex<double>(hp[l].x, hp[l].y) - t * std::complex<double>(ud[l].x, ud[l].y));
+        rhs[l] = hbem::mk(v.real(), v.imag());
+    }
+    HIPC(hipMemcpyAsync(p, rhs.data(), m2 * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
+    hbem::lu_solve_thin_batched(W, wperm, m2, 1, p, 1, false, wk, s.st);
+    cdbl sol;
+    HIPC(hipMemcpyAsync(&sol, p + m, sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
+    HIPC(hipStreamSynchronize(s.st));
+    out3[0] = sigma;
+    out3[1] = der;
+    out3[2] = -sol.x;
+    API_END
+}
+
+int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i, const void* R, int nc, int q,
+                     int want_der, double* sigma, double* dsigma) {
+    API_BEGIN
+    if (!g || !ks || !R || !sigma) throw Error(HBEM_EINVAL, "null argument");
+    if (nk < 0) throw Error(HBEM_EINVAL, "negative number of wavenumbers");
+    const int n = 2 * g->N;
+    check_dense_args(ks, n, R, nc, q);
+    if (want_der && !dsigma) throw Error(HBEM_EINVAL, "null dsigma");
+    for (int i = 0; i < nk; ++i) { check_k(ks[i], 0.0, c_i); check_k(ks[i], 0.0, c_o); }
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPC(hipSetDevice(g->dev));
+    const int D = want_der ? (want_der >= 2 ? 2 : 1) : 0;
+    const size_t nn = (size_t)n * n;
+    // batch size: bounded by the dense working set (T, T', T'' per wavenumber)
+    int B = 32;
+    const size_t per = nn * sizeof(cdbl) * (D + 1) + (size_t)2 * g->npairs * 16 * (D + 1) * sizeof(cdbl);
+    while (B > 1 && (size_t)B * per > ((size_t)48 << 30)) B /= 2;
+    cdbl* dR = g->w_R.as<cdbl>((size_t)n * nc);
+    HIPC(hipMemcpyAsync(dR, R, (size_t)n * nc * sizeof(cdbl), hipMemcpyHostToDevice, g->st));
+    for (int k0 = 0; k0 < nk; k0 += B) {
+        const int nb = std::min(B, nk - k0);
+        std::vector<double> kap(2 * nb), kr(2 * nb), cv(2 * nb);
+        std::vector<int> qi(nb), qo(nb);
+        for (int b = 0; b < nb; ++b) {
+            const double k = ks[k0 + b];
+            kap[2 * b] = k * std::sqrt(c_i); kr[2 * b] = k; cv[2 * b] = c_i;
+            kap[2 * b + 1] = k * std::sqrt(c_o); kr[2 * b + 1] = k; cv[2 * b + 1] = c_o;
+            qi[b] = 2 * b;
+            qo[b] = 2 * b + 1;
+        }
+        cdbl* L = assemble_locals(g, kap, kr, cv, D);
+        cdbl* T0 = g->w_T0.as<cdbl>(nn * nb);
+        cdbl* T1 = D >= 1 ? g->w_T1.as<cdbl>(nn * nb) : nullptr;
+        cdbl* T2 = D >= 2 ? g->w_T2.as<cdbl>(nn * nb) : nullptr;
+        const int* dqi = upload(g->w_qi, qi, g->st);
+        const int* dqo = upload(g->w_qo, qo, g->st);
+        hbem::launch_form_T(L, g->npairs, 16 * (D + 1), dqi, dqo, nb, g->N, D, (const double*)g->d_M.p, T0, T1, T2, g->st);
+        int* piv = g->w_piv.as<int>((size_t)n * nb);
+        int* perm = g->w_perm.as<int>((size_t)n * nb);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (g->prof) { HIPC(hipEventCreate(&e0)); HIPC(hipEventCreate(&e1)); HIPC(hipEventRecord(e0, g->st)); }
+        hbem::lu_factor_batched(T0, n, nb, piv, perm, g->st);
+        if (g->prof) {
+            HIPC(hipEventRecord(e1, g->st));
+            HIPC(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, e0, e1));
+            g->prof_ms[3] += ms;
+            g->prof_n[3] += 1;
+            g->prof_flops[3] += (8.0 / 3.0) * (double)n * n * n * nb;
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+        }
+        DenseCtx c;
+        c.st = g->st; c.n = n; c.batch = nb; c.nc = nc; c.q = q; c.LU = T0; c.perm = perm;
+        c.Frm = g->proj ? (const double*)g->d_Frm.p : nullptr;
+        c.Fcm = g->proj ? (const double*)g->d_Fcm.p : nullptr;
+        c.X = g->w_X.as<cdbl>((size_t)n * nc * nb);
+        c.Y = g->w_Y.as<cdbl>((size_t)n * nc * nb);
+        c.work = g->w_work.as<cdbl>(2 * (size_t)n * nc * nb);
+        for (int b = 0; b < nb; ++b)
+            HIPC(hipMemcpyAsync(c.X + (size_t)b * n * nc, dR, (size_t)n * nc * sizeof(cdbl), hipMemcpyDeviceToDevice, g->st));
+        double* dsig = g->w_sig.as<double>(nb);
+        cdbl* du = D ? g->w_u.as<cdbl>((size_t)n * nb) : nullptr;
+        cdbl* dv = D ? g->w_v.as<cdbl>((size_t)n * nb) : nullptr;
+        rsvd_core(c, dsig, du, dv);
+        HIPC(hipMemcpyAsync(sigma + k0, dsig, nb * sizeof(double), hipMemcpyDeviceToHost, g->st));
+        if (D) {
+            cdbl* a = g->w_a.as<cdbl>((size_t)n * nb);
+            cdbl* bb = g->w_b.as<cdbl>((size_t)n * nb);
+            if (g->proj) {
+                hbem::real_apply_thin_batched((const double*)g->d_Finvcm.p, n, nb, du, a, 1, g->st);
+                hbem::real_apply_thin_batched((const double*)g->d_Finvcm.p, n, nb, dv, bb, 1, g->st);
+            } else {
+                HIPC(hipMemcpyAsync(a, du, (size_t)n * nb * sizeof(cdbl), hipMemcpyDeviceToDevice, g->st));
+                HIPC(hipMemcpyAsync(bb, dv, (size_t)n * nb * sizeof(cdbl), hipMemcpyDeviceToDevice, g->st));
+            }
+            double* dd = g->w_dsig.as<double>(nb);
+            hbem::bilinear_re_batched(T1, (long long)nn, a, bb, n, nb, dd, g->st);
+            HIPC(hipMemcpyAsync(dsigma + k0, dd, nb * sizeof(double), hipMemcpyDeviceToHost, g->st));
+        }
+        HIPC(hipGetLastError());
+    }
+    HIPC(hipStreamSynchronize(g->st));
+    API_END
+}
+
+int hbem_profile_enable(hbem_geometry_t g, int enable) {
+    API_BEGIN
+    if (!g) throw Error(HBEM_EINVAL, "null geometry");
+    g->prof = enable != 0;
+    for (int i = 0; i < 4; ++i) { g->prof_ms[i] = 0; g->prof_n[i] = 0; g->prof_flops[i] = 0; }
+    API_END
+}
+
+int hbem_profile_query(hbem_geometry_t g, int which, double* ms, long long* launches, double* flops) {
+    API_BEGIN
+    if (!g || which < 0 || which > 3) throw Error(HBEM_EINVAL, "invalid argument");
+    if (ms) *ms = g->prof_ms[which];
+    if (launches) *launches = g->prof_n[which];
+    if (flops) *flops = g->prof_flops[which];
+    API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,576 @@
+// hbem_dense.hip -- dense complex FP64 kernels of the randomized-SVD stage.
+//
+// Restates the Eigen calls of randomized_svd::sv (randsvd.cpp:99-146) for the
+// GPU, batched over independent matrices (one per wavenumber):
+//   * PartialPivLU (randsvd.cpp:104): right-looking blocked LU with partial
+//     pivoting (largest modulus, first index on ties); panel factorisation by
+//     one workgroup per matrix, row interchanges, triangular solve of the
+//     U12 block, trailing update by a tiled complex GEMM;
+//   * lu.solve / lu.adjoint().solve on thin right-hand sides (n x nc): blocked
+//     substitution, the 64 x 64 diagonal blocks by one wavefront with
+//     cross-lane broadcasts, the off-diagonal updates by the whole workgroup;
+//   * HouseholderQR's role (an orthonormal basis of the span): modified
+//     Gram-Schmidt applied twice;
+//   * JacobiSVD's role (largest singular value of the thin n x nc matrix):
+//     Gram matrix plus a Hermitian Jacobi eigensolver on one lane.
+#include "hbem_internal.hpp"
+
+namespace hbem {
+
+__device__ __forceinline__ cdbl crecip(cdbl d) {
+    const double s = 1.0 / (d.x * d.x + d.y * d.y);
+    return mk(d.x * s, -d.y * s);
+}
+__device__ __forceinline__ void cfms(cdbl& acc, cdbl a, cdbl b) {   // acc -= a * b
+    acc.x = fma(-a.x, b.x, fma(a.y, b.y, acc.x));
+    acc.y = fma(-a.x, b.y, fma(-a.y, b.x, acc.y));
+}
+__device__ __forceinline__ void cfma(cdbl& acc, cdbl a, cdbl b) {   // acc += a * b
+    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
+}
+__device__ __forceinline__ cdbl shfl_c(cdbl v, int src) {
+    return mk(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+
+// ---------------------------------------------------------------------------
+// LU with partial pivoting
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_lu_panel(cdbl* __restrict__ A, int n, int j0, int jb, int* __restrict__ ipiv) {
+    cdbl* a = A + (size_t)blockIdx.x * n * n;
+    int* ip = ipiv + (size_t)blockIdx.x * n;
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    const int tid = threadIdx.x;
+    for (int c = 0; c < jb; ++c) {
+        const int col = j0 + c;
+        double best = -1.0;
+        int bi = n;
+        for (int i = col + tid; i < n; i += 256) {
+            double v = cabs2(a[i + (size_t)col * n]);
+            if (v > best) { best = v; bi = i; }
+        }
+        sv[tid] = best;
+        si[tid] = bi;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (tid < s) {
+                double v2 = sv[tid + s];
+                int i2 = si[tid + s];
+                if (v2 > sv[tid] || (v2 == sv[tid] && i2 < si[tid])) { sv[tid] = v2; si[tid] = i2; }
+            }
+            __syncthreads();
+        }
+        const int piv = si[0];
+        __syncthreads();
+        if (tid == 0) ip[col] = piv;
+        if (piv != col)
+            for (int cc = tid; cc < jb; cc += 256) {
+                cdbl t = a[col + (size_t)(j0 + cc) * n];
+                a[col + (size_t)(j0 + cc) * n] = a[piv + (size_t)(j0 + cc) * n];
+                a[piv + (size_t)(j0 + cc) * n] = t;
+            }
+        __syncthreads();
+        const cdbl d = a[col + (size_t)col * n];
+        const cdbl inv = (d.x == 0.0 && d.y == 0.0) ? mk(0, 0) : crecip(d);
+        for (int i = col + 1 + tid; i < n; i += 256) {
+            const cdbl l = a[i + (size_t)col * n] * inv;
+            a[i + (size_t)col * n] = l;
+            for (int cc = c + 1; cc < jb; ++cc) cfms(a[i + (size_t)(j0 + cc) * n], l, a[col + (size_t)(j0 + cc) * n]);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_lu_swap(cdbl* __restrict__ A, int n, int j0, int jb, const int* __restrict__ ipiv) {
+    const int b = blockIdx.y;
+    cdbl* a = A + (size_t)b * n * n;
+    const int* ip = ipiv + (size_t)b * n;
+    int co = blockIdx.x * blockDim.x + threadIdx.x;
+    if (co >= n - jb) return;
+    const int c = co < j0 ? co : co + jb;
+    for (int k = j0; k < j0 + jb; ++k) {
+        const int p = ip[k];
+        if (p != k) {
+            cdbl t = a[k + (size_t)c * n];
+            a[k + (size_t)c * n] = a[p + (size_t)c * n];
+            a[p + (size_t)c * n] = t;
+        }
+    }
+}
+
+// A12 <- L11^{-1} A12 (L11 unit lower jb x jb)
+__global__ __launch_bounds__(256) void k_lu_trsm(cdbl* __restrict__ A, int n, int j0, int jb) {
+    const int b = blockIdx.y;
+    cdbl* a = A + (size_t)b * n * n;
+    __shared__ cdbl L[32][33];
+    for (int e = threadIdx.x; e < jb * jb; e += 256) {
+        const int r = e % jb, c = e / jb;
+        L[r][c] = a[(j0 + r) + (size_t)(j0 + c) * n];
+    }
+    __syncthreads();
+    const int col = j0 + jb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= n) return;
+    cdbl x[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) if (k < jb) x[k] = a[(j0 + k) + (size_t)col * n];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+        if (k >= jb) break;
+#pragma unroll
+        for (int i = k + 1; i < 32; ++i) if (i < jb) cfms(x[i], L[i][k], x[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 32; ++k) if (k < jb) a[(j0 + k) + (size_t)col * n] = x[k];
+}
+
+// final row permutation: (P A)(i, :) = A(perm[i], :)
+__global__ void k_lu_perm(int n, const int* __restrict__ ipiv, int* __restrict__ perm) {
+    if (threadIdx.x != 0) return;
+    const int* ip = ipiv + (size_t)blockIdx.x * n;
+    int* p = perm + (size_t)blockIdx.x * n;
+    for (int i = 0; i < n; ++i) p[i] = i;
+    for (int k = 0; k < n; ++k) {
+        int j = ip[k];
+        if (j != k) { int t = p[k]; p[k] = p[j]; p[j] = t; }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled complex GEMM: C = alpha A op(B) + beta C
+// ---------------------------------------------------------------------------
+template <bool TB>
+__global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cdbl alpha, const cdbl* __restrict__ A, int lda,
+                                               long long sa, const cdbl* __restrict__ B, int ldb, long long sb,
+                                               cdbl beta, cdbl* __restrict__ C, int ldc, long long sc) {
+    A += blockIdx.z * sa;
+    B += blockIdx.z * sb;
+    C += blockIdx.z * sc;
+    __shared__ cdbl As[16][65];
+    __shared__ cdbl Bs[16][65];
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int row0 = blockIdx.x * 64, col0 = blockIdx.y * 64;
+    cdbl acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = mk(0, 0);
+    for (int kb = 0; kb < k; kb += 16) {
+        for (int e = tid; e < 1024; e += 256) {
+            const int r = e & 63, kk = e >> 6;
+            As[kk][r] = (row0 + r < m && kb + kk < k) ? A[(row0 + r) + (size_t)(kb + kk) * lda] : mk(0, 0);
+        }
+        for (int e = tid; e < 1024; e += 256) {
+            int kk, cc;
+            if (TB) { cc = e & 63; kk = e >> 6; } else { kk = e & 15; cc = e >> 4; }
+            cdbl v = mk(0, 0);
+            if (col0 + cc < n && kb + kk < k)
+                v = TB ? B[(col0 + cc) + (size_t)(kb + kk) * ldb] : B[(kb + kk) + (size_t)(col0 + cc) * ldb];
+            Bs[kk][cc] = v;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < 16; ++kk) {
+            cdbl av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) av[i] = As[kk][tx + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cfma(acc[i][j], av[i], bv[j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = row0 + tx + 16 * i, c = col0 + ty + 16 * j;
+            if (r < m && c < n) {
+                cdbl* p = C + r + (size_t)c * ldc;
+                cdbl v = alpha * acc[i][j];
+                if (beta.x != 0.0 || beta.y != 0.0) v = v + beta * (*p);
+                *p = v;
+            }
+        }
+}
+
+void zgemm_batched(int m, int n, int k, cdbl alpha, const cdbl* A, int lda, long long sa, const cdbl* B, int ldb,
+                   long long sb, bool transB, cdbl beta, cdbl* C, int ldc, long long sc, int batch, hipStream_t st) {
+    if (m <= 0 || n <= 0 || batch <= 0) return;
+    dim3 grid((m + 63) / 64, (n + 63) / 64, batch);
+    if (transB)
+        hipLaunchKernelGGL(k_zgemm<true>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc);
+    else
+        hipLaunchKernelGGL(k_zgemm<false>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc);
+}
+
+void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStream_t st) {
+    const int nb = 32;
+    const long long s = (long long)n * n;
+    for (int j0 = 0; j0 < n; j0 += nb) {
+        const int jb = n - j0 < nb ? n - j0 : nb;
+        hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(256), 0, st, A, n, j0, jb, ipiv);
+        const int nout = n - jb;
+        if (nout > 0) hipLaunchKernelGGL(k_lu_swap, dim3((nout + 255) / 256, batch), dim3(256), 0, st, A, n, j0, jb, ipiv);
+        const int nr = n - j0 - jb;
+        if (nr > 0) {
+            hipLaunchKernelGGL(k_lu_trsm, dim3((nr + 255) / 256, batch), dim3(256), 0, st, A, n, j0, jb);
+            zgemm_batched(nr, nr, jb, mk(-1, 0), A + (j0 + jb) + (size_t)j0 * n, n, s, A + j0 + (size_t)(j0 + jb) * n, n,
+                          s, false, mk(1, 0), A + (j0 + jb) + (size_t)(j0 + jb) * n, n, s, batch, st);
+        }
+    }
+    hipLaunchKernelGGL(k_lu_perm, dim3(batch), dim3(64), 0, st, n, ipiv, perm);
+}
+
+// ---------------------------------------------------------------------------
+// Thin triangular solves
+// ---------------------------------------------------------------------------
+// Triangular operand of the current pass.
+//   kind 0: L  (unit lower)      L(i,j) = LU(i,j)
+//   kind 1: U  (upper)           U(i,j) = LU(i,j)
+//   kind 2: U^H (lower)          (U^H)(i,j) = conj(LU(j,i))
+//   kind 3: L^H (unit upper)     (L^H)(i,j) = conj(LU(j,i))
+__device__ __forceinline__ cdbl tri_el(const cdbl* lu, int n, int kind, int i, int j) {
+    if (kind <= 1) return lu[i + (size_t)j * n];
+    return cconj(lu[j + (size_t)i * n]);
+}
+
+constexpr int kMaxNc = 8;
+
+// forward (lower) substitution, unit or not, on w (n x nc, ld n)
+__device__ void fwd_subst(const cdbl* lu, int n, int kind, bool unit, cdbl* w, int nc) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int r0 = 0; r0 < n; r0 += 64) {
+        const int bs = n - r0 < 64 ? n - r0 : 64;
+        if (wv == 0) {
+            cdbl x[kMaxNc];
+            for (int c = 0; c < nc; ++c) x[c] = lane < bs ? w[r0 + lane + (size_t)c * n] : mk(0, 0);
+            for (int j = 0; j < bs; ++j) {
+                if (!unit && lane == j) {
+                    const cdbl inv = crecip(tri_el(lu, n, kind, r0 + j, r0 + j));
+                    for (int c = 0; c < nc; ++c) x[c] = x[c] * inv;
+                }
+                const cdbl l = (lane > j && lane < bs) ? tri_el(lu, n, kind, r0 + lane, r0 + j) : mk(0, 0);
+                for (int c = 0; c < nc; ++c) {
+                    const cdbl xj = shfl_c(x[c], j);
+                    if (lane > j) cfms(x[c], l, xj);
+                }
+            }
+            if (lane < bs) for (int c = 0; c < nc; ++c) w[r0 + lane + (size_t)c * n] = x[c];
+        }
+        __syncthreads();
+        for (int i = r0 + bs + tid; i < n; i += blockDim.x) {
+            cdbl s[kMaxNc];
+            for (int c = 0; c < nc; ++c) s[c] = w[i + (size_t)c * n];
+            for (int j = 0; j < bs; ++j) {
+                const cdbl l = tri_el(lu, n, kind, i, r0 + j);
+                for (int c = 0; c < nc; ++c) cfms(s[c], l, w[r0 + j + (size_t)c * n]);
+            }
+            for (int c = 0; c < nc; ++c) w[i + (size_t)c * n] = s[c];
+        }
+        __syncthreads();
+    }
+}
+
+// backward (upper) substitution
+__device__ void bwd_subst(const cdbl* lu, int n, int kind, bool unit, cdbl* w, int nc) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int r1 = n; r1 > 0; r1 -= 64) {
+        const int r0 = r1 - 64 > 0 ? r1 - 64 : 0;
+        const int bs = r1 - r0;
+        if (wv == 0) {
+            cdbl x[kMaxNc];
+            for (int c = 0; c < nc; ++c) x[c] = lane < bs ? w[r0 + lane + (size_t)c * n] : mk(0, 0);
+            for (int j = bs - 1; j >= 0; --j) {
+                if (!unit && lane == j) {
+                    const cdbl inv = crecip(tri_el(lu, n, kind, r0 + j, r0 + j));
+                    for (int c = 0; c < nc; ++c) x[c] = x[c] * inv;
+                }
+                const cdbl u = (lane < j) ? tri_el(lu, n, kind, r0 + lane, r0 + j) : mk(0, 0);
+                for (int c = 0; c < nc; ++c) {
+                    const cdbl xj = shfl_c(x[c], j);
+                    if (lane < j) cfms(x[c], u, xj);
+                }
+            }
+            if (lane < bs) for (int c = 0; c < nc; ++c) w[r0 + lane + (size_t)c * n] = x[c];
+        }
+        __syncthreads();
+        for (int i = tid; i < r0; i += blockDim.x) {
+            cdbl s[kMaxNc];
+            for (int c = 0; c < nc; ++c) s[c] = w[i + (size_t)c * n];
+            for (int j = 0; j < bs; ++j) {
+                const cdbl u = tri_el(lu, n, kind, i, r0 + j);
+                for (int c = 0; c < nc; ++c) cfms(s[c], u, w[r0 + j + (size_t)c * n]);
+            }
+            for (int c = 0; c < nc; ++c) w[i + (size_t)c * n] = s[c];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_solve_thin(const cdbl* __restrict__ LU, const int* __restrict__ perm, int n,
+                                                    cdbl* __restrict__ X, int nc, int adj, cdbl* __restrict__ work) {
+    const int b = blockIdx.x;
+    const cdbl* lu = LU + (size_t)b * n * n;
+    const int* p = perm + (size_t)b * n;
+    cdbl* x = X + (size_t)b * n * nc;
+    cdbl* w = work + (size_t)b * n * nc;
+    const int tid = threadIdx.x;
+    if (!adj) {
+        // A = P^T L U:  w = P x, L y = w, U z = y
+        for (int e = tid; e < n * nc; e += blockDim.x) {
+            const int i = e % n, c = e / n;
+            w[e] = x[p[i] + (size_t)c * n];
+        }
+        __syncthreads();
+        fwd_subst(lu, n, 0, true, w, nc);
+        bwd_subst(lu, n, 1, false, w, nc);
+        for (int e = tid; e < n * nc; e += blockDim.x) x[e] = w[e];
+    } else {
+        // A^H = U^H L^H P:  U^H y = x, L^H z = y, x = P^T z
+        for (int e = tid; e < n * nc; e += blockDim.x) w[e] = x[e];
+        __syncthreads();
+        fwd_subst(lu, n, 2, false, w, nc);
+        bwd_subst(lu, n, 3, true, w, nc);
+        for (int e = tid; e < n * nc; e += blockDim.x) {
+            const int i = e % n, c = e / n;
+            x[p[i] + (size_t)c * n] = w[e];
+        }
+    }
+}
+
+void lu_solve_thin_batched(const cdbl* LU, const int* perm, int n, int batch, cdbl* X, int nc, bool adj, cdbl* work,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_solve_thin, dim3(batch), dim3(256), 0, st, LU, perm, n, X, nc, adj ? 1 : 0, work);
+}
+
+// ---------------------------------------------------------------------------
+// Y = F X for a real dense F given row-major; one wavefront per row.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_real_apply(const double* __restrict__ Frm, int n, const cdbl* __restrict__ X,
+                                                    cdbl* __restrict__ Y, int nc) {
+    const int b = blockIdx.y;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const cdbl* x = X + (size_t)b * n * nc;
+    const double* f = Frm + (size_t)row * n;
+    cdbl s[kMaxNc];
+    for (int c = 0; c < nc; ++c) s[c] = mk(0, 0);
+    for (int j = lane; j < n; j += 64) {
+        const double fv = f[j];
+        if (fv != 0.0)
+            for (int c = 0; c < nc; ++c) axpy(s[c], fv, x[j + (size_t)c * n]);
+    }
+    for (int c = 0; c < nc; ++c) {
+        double re = s[c].x, im = s[c].y;
+        for (int o = 32; o > 0; o >>= 1) { re += __shfl_xor(re, o, 64); im += __shfl_xor(im, o, 64); }
+        if (lane == 0) Y[(size_t)b * n * nc + row + (size_t)c * n] = mk(re, im);
+    }
+}
+
+void real_apply_thin_batched(const double* Frm, int n, int batch, const cdbl* X, cdbl* Y, int nc, hipStream_t st) {
+    hipLaunchKernelGGL(k_real_apply, dim3((n + 3) / 4, batch), dim3(256), 0, st, Frm, n, X, Y, nc);
+}
+
+// ---------------------------------------------------------------------------
+// Block reductions
+// ---------------------------------------------------------------------------
+__device__ cdbl block_sum(cdbl v, cdbl* sh) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int o = 32; o > 0; o >>= 1) { v.x += __shfl_xor(v.x, o, 64); v.y += __shfl_xor(v.y, o, 64); }
+    __syncthreads();
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    cdbl r = mk(0, 0);
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) r = r + sh[q];
+    return r;
+}
+
+// Modified Gram-Schmidt, applied twice per column.
+__global__ __launch_bounds__(256) void k_orth(cdbl* __restrict__ X, int n, int nc) {
+    __shared__ cdbl sh[4];
+    cdbl* x = X + (size_t)blockIdx.x * n * nc;
+    const int tid = threadIdx.x;
+    for (int c = 0; c < nc; ++c) {
+        cdbl* xc = x + (size_t)c * n;
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int c2 = 0; c2 < c; ++c2) {
+                const cdbl* q = x + (size_t)c2 * n;
+                cdbl part = mk(0, 0);
+                for (int i = tid; i < n; i += blockDim.x) cfma(part, cconj(q[i]), xc[i]);
+                const cdbl r = block_sum(part, sh);
+                for (int i = tid; i < n; i += blockDim.x) cfms(xc[i], r, q[i]);
+                __syncthreads();
+            }
+        }
+        double part = 0.0;
+        for (int i = tid; i < n; i += blockDim.x) part += cabs2(xc[i]);
+        const cdbl nr = block_sum(mk(part, 0.0), sh);
+        const double inv = nr.x > 0.0 ? 1.0 / sqrt(nr.x) : 0.0;
+        for (int i = tid; i < n; i += blockDim.x) xc[i] = inv * xc[i];
+        __syncthreads();
+    }
+}
+
+void orthonormalize_thin_batched(cdbl* X, int n, int batch, int nc, hipStream_t st) {
+    hipLaunchKernelGGL(k_orth, dim3(batch), dim3(256), 0, st, X, n, nc);
+}
+
+// Hermitian Jacobi eigensolver (cyclic, complex rotations) for nc x nc, one lane.
+__device__ void herm_eig_max(cdbl* G, int m, double& lmax, cdbl* y) {
+    cdbl Z[kMaxNc * kMaxNc];
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) Z[i + j * m] = mk(i == j ? 1.0 : 0.0, 0.0);
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int p = 0; p < m; ++p) {
+            diag += G[p + p * m].x * G[p + p * m].x;
+            for (int q = p + 1; q < m; ++q) off += cabs2(G[p + q * m]);
+        }
+        if (off <= 1e-32 * diag) break;
+        for (int p = 0; p < m; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                const cdbl apq = G[p + q * m];
+                const double g = sqrt(cabs2(apq));
+                if (g == 0.0) continue;
+                const cdbl e = mk(apq.x / g, apq.y / g);
+                const double app = G[p + p * m].x, aqq = G[q + q * m].x;
+                const double tau = (aqq - app) / (2.0 * g);
+                const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
+                // V = [[cs, sn e], [-sn conj(e), cs]] on columns p, q:  G <- V^H G V, Z <- Z V
+                for (int k = 0; k < m; ++k) {   // G <- G V
+                    const cdbl gp = G[k + p * m], gq = G[k + q * m];
+                    G[k + p * m] = cs * gp - sn * (gq * cconj(e));
+                    G[k + q * m] = sn * (gp * e) + cs * gq;
+                    const cdbl zp = Z[k + p * m], zq = Z[k + q * m];
+                    Z[k + p * m] = cs * zp - sn * (zq * cconj(e));
+                    Z[k + q * m] = sn * (zp * e) + cs * zq;
+                }
+                for (int k = 0; k < m; ++k) {   // G <- V^H G
+                    const cdbl gp = G[p + k * m], gq = G[q + k * m];
+                    G[p + k * m] = cs * gp - sn * (e * gq);
+                    G[q + k * m] = sn * (cconj(e) * gp) + cs * gq;
+                }
+                G[p + q * m] = mk(0, 0);
+                G[q + p * m] = mk(0, 0);
+            }
+    }
+    int best = 0;
+    for (int i = 1; i < m; ++i) if (G[i + i * m].x > G[best + best * m].x) best = i;
+    lmax = G[best + best * m].x;
+    for (int i = 0; i < m; ++i) y[i] = Z[i + best * m];
+}
+
+// sigma = 1 / sigma_max(B); u = B y / sigma_max(B), v = Q y (y: top right singular vector of B)
+__global__ __launch_bounds__(256) void k_sigma(const cdbl* __restrict__ Bm, const cdbl* __restrict__ Qm, int n, int nc,
+                                               double* __restrict__ sigma, cdbl* __restrict__ U, cdbl* __restrict__ V) {
+    __shared__ cdbl sh[4];
+    __shared__ cdbl Gs[kMaxNc * kMaxNc];
+    __shared__ cdbl ys[kMaxNc];
+    __shared__ double smax;
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const cdbl* B = Bm + (size_t)b * n * nc;
+    for (int p = 0; p < nc; ++p)
+        for (int q = p; q < nc; ++q) {
+            cdbl part = mk(0, 0);
+            for (int i = tid; i < n; i += blockDim.x) cfma(part, cconj(B[i + (size_t)p * n]), B[i + (size_t)q * n]);
+            const cdbl r = block_sum(part, sh);
+            if (tid == 0) { Gs[p + q * nc] = r; Gs[q + p * nc] = cconj(r); }
+        }
+    __syncthreads();
+    if (tid == 0) {
+        cdbl G[kMaxNc * kMaxNc], y[kMaxNc];
+        for (int e = 0; e < nc * nc; ++e) G[e] = Gs[e];
+        for (int p = 0; p < nc; ++p) G[p + p * nc].y = 0.0;
+        double lmax;
+        herm_eig_max(G, nc, lmax, y);
+        const double s1 = sqrt(lmax);
+        smax = s1;
+        for (int i = 0; i < nc; ++i) ys[i] = y[i];
+        sigma[b] = 1.0 / s1;
+    }
+    __syncthreads();
+    if (U == nullptr) return;
+    const cdbl* Q = Qm + (size_t)b * n * nc;
+    const double inv = 1.0 / smax;
+    for (int i = tid; i < n; i += blockDim.x) {
+        cdbl u = mk(0, 0), v = mk(0, 0);
+        for (int c = 0; c < nc; ++c) { cfma(u, B[i + (size_t)c * n], ys[c]); cfma(v, Q[i + (size_t)c * n], ys[c]); }
+        U[(size_t)b * n + i] = inv * u;
+        V[(size_t)b * n + i] = v;
+    }
+}
+
+void sigma_from_thin_batched(const cdbl* B, const cdbl* Q, int n, int batch, int nc, double* sigma, cdbl* u, cdbl* v,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_sigma, dim3(batch), dim3(256), 0, st, B, Q, n, nc, sigma, u, v);
+}
+
+// out = Re(a^H T b): one wavefront per column j computes (a^H T)_j
+__global__ __launch_bounds__(256) void k_bilinear(const cdbl* __restrict__ T, long long tstride, const cdbl* __restrict__ a,
+                                                  const cdbl* __restrict__ bv, int n, double* __restrict__ out) {
+    __shared__ cdbl sh[4];
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const cdbl* Tm = T + (size_t)b * tstride;
+    const cdbl* av = a + (size_t)b * n;
+    const cdbl* bb = bv + (size_t)b * n;
+    double acc = 0.0;
+    for (int j = wv; j < n; j += 4) {
+        cdbl s = mk(0, 0);
+        for (int i = lane; i < n; i += 64) cfma(s, cconj(av[i]), Tm[i + (size_t)j * n]);
+        for (int o = 32; o > 0; o >>= 1) { s.x += __shfl_xor(s.x, o, 64); s.y += __shfl_xor(s.y, o, 64); }
+        const cdbl t = s * bb[j];
+        acc += t.x;
+    }
+    const cdbl r = block_sum(mk(lane == 0 ? acc : 0.0, 0.0), sh);
+    if (tid == 0) out[b] = r.x;
+}
+
+void bilinear_re_batched(const cdbl* T, long long tstride, const cdbl* a, const cdbl* bvec, int n, int batch,
+                         double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_bilinear, dim3(batch), dim3(256), 0, st, T, tstride, a, bvec, n, out);
+}
+
+// y = T x or T^H x; one wavefront per output row (T^H: row i of T^H = column i of T)
+__global__ __launch_bounds__(256) void k_gemv(const cdbl* __restrict__ T, long long tstride, const cdbl* __restrict__ x,
+                                              cdbl* __restrict__ y, int n, int adj) {
+    const int b = blockIdx.y;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const cdbl* Tm = T + (size_t)b * tstride;
+    const cdbl* xv = x + (size_t)b * n;
+    cdbl s = mk(0, 0);
+    if (!adj) { for (int j = lane; j < n; j += 64) cfma(s, Tm[row + (size_t)j * n], xv[j]); }
+    else { for (int j = lane; j < n; j += 64) cfma(s, cconj(Tm[j + (size_t)row * n]), xv[j]); }
+    for (int o = 32; o > 0; o >>= 1) { s.x += __shfl_xor(s.x, o, 64); s.y += __shfl_xor(s.y, o, 64); }
+    if (lane == 0) y[(size_t)b * n + row] = s;
+}
+
+void gemv_batched(const cdbl* T, long long tstride, const cdbl* x, cdbl* y, int n, int batch, bool adj, hipStream_t st) {
+    hipLaunchKernelGGL(k_gemv, dim3((n + 3) / 4, batch), dim3(256), 0, st, T, tstride, x, y, n, adj ? 1 : 0);
+}
+
+__global__ void k_wielandt(const cdbl* __restrict__ T, int n, double s, cdbl* __restrict__ W) {
+    const int m2 = 2 * n;
+    long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)m2 * m2) return;
+    const int i = (int)(id % m2), j = (int)(id / m2);
+    cdbl v = mk(0, 0);
+    if (i < n && j >= n) v = T[i + (size_t)(j - n) * n];
+    else if (i >= n && j < n) v = cconj(T[j + (size_t)(i - n) * n]);
+    if (i == j) v.x -= s;
+    W[id] = v;
+}
+
+void wielandt_build(const cdbl* T, int n, double s, cdbl* W, hipStream_t st) {
+    long long tot = 4LL * n * n;
+    hipLaunchKernelGGL(k_wielandt, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, T, n, s, W);
+}
+
+}  // namespace hbem

@@ -1,0 +1,148 @@
+// hbem_device.hpp -- device-side math shared by the assembly kernels.
+//
+//  * complex double helpers
+//  * J0, Y0, J1, Y1 of real x >= 0 (Hankel H0 = J0 + iY0, H1 = J1 + iY1) with the
+//    branch points of the reference's Hankel1Real01::jy_01
+//    (cbessel.cpp:1896-1980): x <= 0, x < 1e-8, x < 8, x <= 1e6, x > 1e6
+//  * panel evaluation for line / arc / trig-curve panels
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "hankel_tables.inc"
+
+namespace hbem {
+
+struct cdbl {
+    double x, y;
+};
+
+__host__ __device__ __forceinline__ cdbl mk(double a, double b) { return cdbl{a, b}; }
+__host__ __device__ __forceinline__ cdbl operator+(cdbl a, cdbl b) { return cdbl{a.x + b.x, a.y + b.y}; }
+__host__ __device__ __forceinline__ cdbl operator-(cdbl a, cdbl b) { return cdbl{a.x - b.x, a.y - b.y}; }
+__host__ __device__ __forceinline__ cdbl operator*(double s, cdbl a) { return cdbl{s * a.x, s * a.y}; }
+__host__ __device__ __forceinline__ cdbl operator*(cdbl a, cdbl b) { return cdbl{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__host__ __device__ __forceinline__ cdbl cconj(cdbl a) { return cdbl{a.x, -a.y}; }
+__host__ __device__ __forceinline__ double cabs2(cdbl a) { return a.x * a.x + a.y * a.y; }
+// acc += s * a  (s real)
+__device__ __forceinline__ void axpy(cdbl& acc, double s, cdbl a) { acc.x = fma(s, a.x, acc.x); acc.y = fma(s, a.y, acc.y); }
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoOverPi = 0.63661977236758134308;
+constexpr double kOneOverPi = 0.31830988618379067154;
+constexpr double kEuler = 0.57721566490153286061;
+constexpr double kDblEps = 2.220446049250313080847e-16;
+
+// 1/k^2, 1/(k+1) and harmonic numbers H_k for the ascending series (k <= 40).
+struct SeriesTables {
+    double inv_k2[41];
+    double inv_kp1[41];
+    double harm[41];
+};
+
+__device__ __forceinline__ double horner_pq(const double* c, double z) {
+    double v = c[HBEM_PQ_DEG];
+#pragma unroll
+    for (int i = HBEM_PQ_DEG - 1; i >= 0; --i) v = fma(v, z, c[i]);
+    return v;
+}
+
+// J0, Y0, J1, Y1 of x.  Branches:
+//   x <= 0       J0 = 1, J1 = 0, Y0 = Y1 = 0 (the reference zeroes Y there, cbessel.cpp:1976-1980)
+//   x < 1e-8     limiting forms (cbessel.cpp:1771-1776)
+//   x < 8        joint ascending series of J0, J1, Y0, Y1 (A&S 9.1.10-9.1.13),
+//                summed until H_k |t_k| <= eps (cbessel.cpp:1778-1805)
+//   x <= 1e6     Hankel asymptotic form with P_nu, x Q_nu from the polynomial
+//                fits of hankel_tables.inc (A&S 9.2.5-9.2.6)
+//   x > 1e6      leading asymptotic term (cbessel.cpp:1847-1856)
+__device__ __forceinline__ void bessel_jy01(double x, const SeriesTables& T, double& j0, double& y0, double& j1,
+                                            double& y1) {
+    if (!(x > 0.0)) {
+        j0 = 1.0; j1 = 0.0; y0 = 0.0; y1 = 0.0;
+    } else if (x < 1e-8) {
+        j0 = 1.0;
+        j1 = 0.5 * x;
+        y0 = kTwoOverPi * (log(j1) + kEuler);
+        y1 = -kTwoOverPi / x;
+    } else if (x < 8.0) {
+        const double mq = -0.25 * x * x;
+        double t = 1.0, sj0 = 1.0, sy0 = 0.0, sj1 = 1.0, sy1 = 0.0;
+        for (int k = 1; k <= 40; ++k) {
+            t = (t * mq) * T.inv_k2[k];
+            const double H = T.harm[k];
+            sj0 += t;
+            sy0 = fma(H, t, sy0);
+            const double tk = t * T.inv_kp1[k];
+            sj1 += tk;
+            sy1 = fma(H, tk, sy1);
+            if (!(H * fabs(t) > kDblEps)) break;
+        }
+        const double L = log(0.5 * x) + kEuler;
+        const double xinv = 1.0 / x;
+        j0 = sj0;
+        j1 = 0.5 * x * sj1;
+        y0 = kTwoOverPi * fma(L, j0, -sy0);
+        y1 = kOneOverPi * fma(-x, sy1, 2.0 * fma(L, j1, (j0 - 2.0) * xinv));
+    } else {
+        double s, c;
+        sincos(x, &s, &c);
+        const double b = 1.0 / sqrt(kPi * x);      // sqrt(2/(pi x)) / sqrt(2)
+        const double cs = c + s, sc = s - c;       // sqrt2 cos(x - pi/4), sqrt2 sin(x - pi/4)
+        if (x <= 1e6) {
+            const double xinv = 1.0 / x;
+            const double z = 64.0 * xinv * xinv;
+            const double P0 = horner_pq(kP0, z), Q0 = horner_pq(kQ0, z) * xinv;
+            const double P1 = horner_pq(kP1, z), Q1 = horner_pq(kQ1, z) * xinv;
+            j0 = b * fma(P0, cs, -Q0 * sc);
+            y0 = b * fma(P0, sc, Q0 * cs);
+            j1 = b * fma(P1, sc, Q1 * cs);
+            y1 = b * fma(Q1, sc, -P1 * cs);
+        } else {
+            j0 = b * cs; y0 = b * sc; j1 = b * sc; y1 = -b * cs;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Panels (8 doubles {type, a0..a6}).  Returns the point and the forward
+// tangent gamma'(param) at param = s (or at 1 - s when `swapped`), i.e.
+// operator[] / Derivative_01, resp. swapped_op / Derivative_01_swapped(neg=true)
+// (parametrized_line.cpp:43-107, parametrized_circular_arc.cpp:54-121).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void panel_eval(const double* __restrict__ p, const double* __restrict__ coefs, double s,
+                                           bool swapped, cdbl& z, cdbl& tau) {
+    const int type = (int)p[0];
+    if (type == 0) {
+        const double dx = p[3] - p[1], dy = p[4] - p[2];
+        if (!swapped) { z = mk(fma(dx, s, p[1]), fma(dy, s, p[2])); }
+        else { z = mk(fma(-dx, s, p[3]), fma(-dy, s, p[4])); }
+        tau = mk(dx, dy);
+    } else if (type == 1) {
+        const double d = p[5] - p[4];
+        const double th = swapped ? fma(s, -d, p[5]) : fma(s, d, p[4]);
+        double sn, cs;
+        sincos(th, &sn, &cs);
+        z = mk(fma(p[3], cs, p[1]), fma(p[3], sn, p[2]));
+        const double a = p[3] * d;
+        tau = mk(-a * sn, a * cs);
+    } else {
+        const double dt = p[2] - p[1];
+        const double t = swapped ? fma(s, -dt, p[2]) : fma(s, dt, p[1]);
+        const int c0 = (int)p[3], nc = (int)p[4];
+        cdbl zz = mk(0.0, 0.0), dz = mk(0.0, 0.0);
+        for (int k = 0; k < nc; ++k) {
+            const double m = coefs[3 * (c0 + k)];
+            const cdbl cm = mk(coefs[3 * (c0 + k) + 1], coefs[3 * (c0 + k) + 2]);
+            double sn, cs;
+            sincos(m * t, &sn, &cs);
+            const cdbl e = cm * mk(cs, sn);
+            zz = zz + e;
+            dz = dz + mk(-m * e.y, m * e.x);
+        }
+        z = zz;
+        tau = dt * dz;
+    }
+}
+
+}  // namespace hbem

@@ -1,0 +1,87 @@
+// hbem_internal.hpp -- declarations shared by the host and kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "hbem_device.hpp"
+
+namespace hbem {
+
+struct AsmParams {
+    const double* gtab;          // general class: per-panel Gauss tables [N][ns][5]
+    const double* xg;            // Gauss rule on [0,1]
+    const double* wg;
+    const double* xc;            // composite rule
+    const double* wc;
+    const int2* pairs;           // general class entries (i, j)
+    const int* ent;              // singular class entries (i, j, swap)
+    const double* ngeo;          // singular class node geometry [entry][nodes][6]
+    const unsigned char* flags;  // per entry skip flags
+    const long long* pidx;       // triangular pair index per entry
+    int nent;                    // entries of this class
+    int nk;                      // wavenumber slots in the batch
+    const double* kap;           // kappa = k sqrt(c) per slot
+    const double* kr;            // k per slot
+    const double* cv;            // c per slot
+    long long npairs_total;      // N (N + 1) / 2
+    cdbl* out;                   // local matrices [nk][npairs_total][16 (D+1)]
+    int ns, nc;                  // Gauss / composite rule sizes
+    int qflags;                  // HBEM_FLAG_* quirk bits
+    SeriesTables st;
+};
+
+// assembly (hbem_assembly.hip)
+void launch_setup_gauss_table(const double* panels, const double* coefs, int npan, const double* xg, int ns,
+                              double* gtab, hipStream_t st);
+void launch_setup_general_flags(const int2* pairs, int npairs, const double* gtab, int ns, unsigned char* flags,
+                                hipStream_t st);
+void launch_setup_singular(int cls, const int* ent, int nent, const double* panels, const double* coefs,
+                           const double* xc, const double* wc, int nc, double* ngeo, unsigned char* flags,
+                           hipStream_t st);
+void launch_assemble(int cls, int D, const AsmParams& P, int G, hipStream_t st);
+void launch_gather_layers(const cdbl* L, long long npt, int nslot, int q, int N, int D, int layers, cdbl* K, cdbl* W,
+                          cdbl* V, hipStream_t st);
+// T_t for output b is written at T[t] + b * (2N)^2 (t = 0..D)
+void launch_form_T(const cdbl* L, long long npt, int nslot, const int* qi, const int* qo, int nb, int N, int D,
+                   const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st);
+
+// dense (hbem_dense.hip); all batched over `batch` matrices of order n with
+// stride n*n (column-major, lda = n) unless noted.
+void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStream_t st);
+// X (n x nc, stride n*nc per matrix) <- op(A)^{-1} X with A = P^T L U from lu_factor_batched;
+// adj = false solves A X = B, adj = true solves A^H X = B.
+void lu_solve_thin_batched(const cdbl* LU, const int* perm, int n, int batch, cdbl* X, int nc, bool adj,
+                           cdbl* work, hipStream_t st);
+// Y = op(F) X for a real dense n x n F given row-major (so op = F) -- pass the
+// column-major array to apply F^T.
+void real_apply_thin_batched(const double* Frm, int n, int batch, const cdbl* X, cdbl* Y, int nc, hipStream_t st);
+void orthonormalize_thin_batched(cdbl* X, int n, int batch, int nc, hipStream_t st);
+// sigma = 1 / sigma_max(B); u = left singular vector of B for sigma_max,
+// y = right singular vector (nc) -- v = Q y is formed by the caller.
+void sigma_from_thin_batched(const cdbl* B, const cdbl* Q, int n, int batch, int nc, double* sigma, cdbl* u, cdbl* v,
+                             hipStream_t st);
+// out[b] = Re(a_b^H T_b b_b) for T_b column-major n x n (stride tstride)
+void bilinear_re_batched(const cdbl* T, long long tstride, const cdbl* a, const cdbl* bvec, int n, int batch,
+                         double* out, hipStream_t st);
+// y_b = T_b x_b (adj = false) or T_b^H x_b (adj = true), T_b n x n with stride tstride
+void gemv_batched(const cdbl* T, long long tstride, const cdbl* x, cdbl* y, int n, int batch, bool adj, hipStream_t st);
+// W = [[0, T], [T^H, 0]] - s I  (2n x 2n) for a single matrix
+void wielandt_build(const cdbl* T, int n, double s, cdbl* W, hipStream_t st);
+// C = alpha * A * op(B) + beta * C   (complex, column-major, batched with strides)
+void zgemm_batched(int m, int n, int k, cdbl alpha, const cdbl* A, int lda, long long sa, const cdbl* B, int ldb,
+                   long long sb, bool transB, cdbl beta, cdbl* C, int ldc, long long sc, int batch, hipStream_t st);
+
+// host setup (hbem_setup.cpp)
+void gauss_rule(double a, double b, int n, double* x, double* w);
+void composite_rule(int n, double* x, double* w);
+void mass_matrix_host(const double* panels, int N, const double* coefs, bool lines, int order, double* M);
+// Eigen-style LDLT with diagonal pivoting; F = P^T L sqrt(D) (n x n, column-major)
+void ldlt_projection_factor(const double* A, int n, double* F, int* transpositions);
+void invert_projection_factor(const double* F, const int* tr, int n, double* Finv);
+SeriesTables make_series_tables();
+bool panels_adjacent(const double* panels, const double* coefs, int i, int j, bool& swap);
+
+}  // namespace hbem

@@ -1,0 +1,148 @@
+/*
+ * hbem.h -- C-ABI of the MI355X-native HelmholtzBEM sigma_min(k) path.
+ *
+ * This is the drop-in boundary.  The reference exposes the path as a C++
+ * library API (GalerkinBuilder / SolutionsOperator / randomized_svd); this ABI
+ * is the layer underneath a C++ facade that keeps those names
+ * (include/hbem/*.hpp), and is what ctypes / cgo / JNI style bindings call.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative code on error; the
+ *     message of the last error on the calling thread is returned by
+ *     hbem_last_error().  The messages of the reference's runtime_errors are
+ *     kept verbatim (galerkin_builder.cpp:901-906, 1050-1056).
+ *   - Matrices are column-major arrays of complex double ("double _Complex" /
+ *     std::complex<double>, interleaved re,im), leading dimension = rows, the
+ *     layout of Eigen::MatrixXcd.  Host pointers unless the name says _dev.
+ *   - Panels: 8 doubles each {type, a0..a6}:
+ *       type 0  line        {x1, y1, x2, y2}            ParametrizedLine
+ *       type 1  arc         {cx, cy, r, phi0, phi1}     ParametrizedCircularArc
+ *       type 2  trig curve  {t0, t1, first, count}      z(t) = sum c_m e^{imt}
+ *     with a coefficient table of {m, re, im} triples for type 2.
+ *   - Spaces: test = trial = continuous P1 (ContinuousSpace<1>), the space of
+ *     every sigma_min driver in the reference.
+ *   - One GPU; all work of a geometry context runs on the context's stream.
+ */
+#ifndef HBEM_H
+#define HBEM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define HBEM_OK 0
+#define HBEM_EINVAL (-1)     /* invalid argument (reference: std::runtime_error / out_of_range) */
+#define HBEM_ENOTSUP (-2)    /* not supported by this build (e.g. complex wavenumber) */
+#define HBEM_EDEVICE (-3)    /* HIP runtime error */
+#define HBEM_ENOMEM (-4)     /* allocation failed */
+
+/* LayerType bits (galerkin_builder.hpp:100-106) */
+#define HBEM_LAYER_DOUBLE 1
+#define HBEM_LAYER_SINGLE 2
+#define HBEM_LAYER_HYPERSINGULAR 4
+#define HBEM_LAYER_ALL 7
+
+/* geometry flags */
+/* Reproduce the reference's coinciding single-layer symmetry copy for d >= 1
+ * (galerkin_builder.cpp:744-746 copies the d = 0 entry into the d = 1, 2
+ * entries).  Off by default: the default is the symmetric copy of the same
+ * derivative order, which is what the k-derivative of V requires. */
+#define HBEM_FLAG_QUIRK_SLP_COINC_DER 1
+
+typedef struct hbem_geometry* hbem_geometry_t;
+
+/* Last error message of the calling thread; returns its length. */
+int hbem_last_error(char* buf, size_t len);
+
+/* Selects the HIP device used by contexts created afterwards. */
+int hbem_init(int device);
+
+/*
+ * BuilderData(mesh, test_space, trial_space, order) + SolutionsOperator's ctor
+ * (galerkin_builder.hpp:43-46, galerkin_builder.cpp:17-187;
+ *  gen_sol_op.hpp:28, gen_sol_op.cpp:23-43).
+ * Uploads the panels, builds the quadrature grids, classifies panel pairs
+ * (coinciding / adjacent / general, galerkin_builder.cpp:189-201, 872-897),
+ * precomputes the pair-node geometry on the device and, when proj != 0, the
+ * mass matrix and the projection factor F = P^T L sqrt(D) of
+ * blockdiag(M, M^T).
+ */
+int hbem_geometry_create(const double* panels, int npanels, const double* coefs, int ncoefs,
+                         int order, int proj, int flags, hbem_geometry_t* out);
+int hbem_geometry_destroy(hbem_geometry_t g);
+/* getNumPanels, getGaussQROrder, getCGaussQROrder, getPanelsAreLines (galerkin_builder.hpp:94-97) */
+int hbem_geometry_info(hbem_geometry_t g, int* npanels, int* order, int* corder, int* lines);
+
+/*
+ * GalerkinBuilder::assembleDense(k, c, layers, d) followed by the getters
+ * getDoubleLayer(t) / getHypersingular(t) / getSingleLayer(t), t = 0..d
+ * (galerkin_builder.hpp:221-276, galerkin_builder.cpp:918-1018, 1050-1068).
+ * K, W, V: host arrays of (d+1) N x N matrices each (slice t at offset t*N*N),
+ * or NULL to skip copying that layer.  Only real k > 0 is supported
+ * (HBEM_ENOTSUP otherwise).
+ */
+int hbem_assemble_dense(hbem_geometry_t g, double k_re, double k_im, double c, int layers, int d,
+                        void* K, void* W, void* V);
+
+/* mass_matrix::GalerkinMatrix as built by SolutionsOperator (mass_matrix.cpp:6-47,
+ * gen_sol_op.cpp:30-33).  M: N x N real, column-major. */
+int hbem_mass_matrix(hbem_geometry_t g, double* M);
+
+/* The projection factor F (2N x 2N, real) and the LDLT transpositions. */
+int hbem_projection_factor(hbem_geometry_t g, double* F, int* transpositions);
+
+/*
+ * SolutionsOperator::gen_sol_op / gen_sol_op_1st_der / gen_sol_op_2nd_der
+ * (gen_sol_op.hpp:32-46, gen_sol_op.cpp:50-196).  Writes T and, for d >= 1,
+ * T' (Td) and for d = 2 T'' (Tdd), each 2N x 2N, projected when the context
+ * was created with proj != 0.  The derivative outputs follow the intended
+ * semantics (the reference writes every assemble_operator call into T,
+ * gen_sol_op.cpp:150-151, 193-195).
+ */
+int hbem_gen_sol_op(hbem_geometry_t g, double k_re, double k_im, double c_o, double c_i, int d,
+                    void* T, void* Td, void* Tdd);
+
+/* SolutionsOperator::projection(T) = F^{-1} T F^{-T} (gen_sol_op.cpp:45-48). */
+int hbem_projection(hbem_geometry_t g, const void* T, void* Tp);
+
+/*
+ * randomized_svd::sv / sv_der / sv_der2 (randsvd.hpp:23-30,
+ * randsvd.cpp:99-216) on host matrices; the factorisations run on the GPU.
+ * out: sigma (sv), {sigma, sigma'} (sv_der), {sigma, sigma', sigma''} (sv_der2).
+ * sv_der computes sigma' = Re(u^H T' v) from the singular pair of the
+ * randomized subspace; sv_der2 solves the reference's bordered system
+ * (randsvd.cpp:190-214) for sigma''.
+ */
+int hbem_rsvd_sv(const void* T, int n, const void* R, int nc, int q, double* sigma);
+int hbem_rsvd_sv_der(const void* T, const void* Td, int n, const void* R, int nc, int q, double* out2);
+int hbem_rsvd_sv_der2(const void* T, const void* Td, const void* Tdd, int n, const void* R, int nc, int q,
+                      double* out3);
+
+/*
+ * Batched sigma_min(k) sweep -- the caller pattern of the reference drivers
+ * (std::transform(par_unseq, ks) { gen_sol_op(k, c_o, c_i, T); sv(T, R, q); },
+ * examples/roots_brent_square_rsvd.cpp:167-171).  Everything stays on the
+ * device; only the nk results come back.  With want_der != 0 the operators
+ * are assembled with their first and second k-derivatives (d = 2) and
+ * dsigma/dk = Re(u^H T' v) is returned in dsigma (may be NULL otherwise).
+ * R: n x nc test matrix (n = 2N), uploaded once per call.
+ */
+int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i,
+                     const void* R, int nc, int q, int want_der, double* sigma, double* dsigma);
+
+/* Kernel timing (HIP events on the context's stream) of the dominant
+ * assembly kernel, for the benchmark's roofline figure.  enable != 0 starts
+ * recording; query returns the accumulated milliseconds and launch count of
+ * kernel `which` (0 = general-pair assembly, 1 = adjacent, 2 = coinciding,
+ * 3 = dense LU update) and the algorithmic FP64 flop count they performed. */
+int hbem_profile_enable(hbem_geometry_t g, int enable);
+int hbem_profile_query(hbem_geometry_t g, int which, double* ms, long long* launches, double* flops);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HBEM_H */

@@ -1,0 +1,1187 @@
+// hbem_oracle.cpp -- CPU FP64 restatement of HelmholtzBEM's sigma_min(k) hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the parity oracle: only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg load it, and only as
+// the checker (or as the timed CPU baseline).  The product library
+// (helmholtzbem_amd/libhbem.so) never links, loads or calls anything here.
+//
+// Every function cites the reference file:line it restates (paths relative to
+// the HelmholtzBEM source tree).  The restatement is formula-for-formula; where
+// it deliberately deviates the comment says so:
+//   * sin/cos come from libm instead of the reference's private 12-term sine
+//     fit (cbessel.cpp:1732-1769); the difference is O(1e-15) absolute.
+//   * for 8 <= x <= 1e6 J0/Y0/J1/Y1 come from glibc's j0/y0/j1/y1, which
+//     implement the same published fdlibm P/Q rational forms (pR8/pS8/...) that
+//     cbessel.cpp:1807-1845 restates.
+//   * the coinciding single-layer symmetry copy (galerkin_builder.cpp:744-746)
+//     is reproduced literally only when HBEM_QUIRK_SLP_COINC_DER is set; by
+//     default the mathematically symmetric copy is used (see flags below).
+//
+// Layout conventions (shared with the product's C-ABI):
+//   panels: 8 doubles each  {type, a0..a6}
+//     type 0  line       a = {x1, y1, x2, y2}                 (parametrized_line.cpp)
+//     type 1  arc        a = {cx, cy, r, phi0, phi1}           (parametrized_circular_arc.cpp)
+//     type 2  Fourier    a = {t0, t1, first_coef, n_coefs}     z(t) = sum c_m e^{i m t}
+//   Fourier coefficient table: 3 doubles each {m, re, im}.
+//   Dense matrices: column-major, std::complex<double>.
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+typedef std::complex<double> cd;
+static const double kEps = 2.220446049250313080847e-16;   // DBL_EPSILON
+static const double kPi = 3.14159265358979323846;
+static const double kEuler = 0.57721566490153286060651209;
+
+enum { LAYER_DOUBLE = 1, LAYER_SINGLE = 2, LAYER_HYPER = 4 };
+enum { HBEM_QUIRK_SLP_COINC_DER = 1 };
+
+static thread_local std::string g_err;
+
+// ------------------------------------------------------------------------
+// Quadrature.  gauleg.hpp:35-79 (Gauss-Legendre by Newton on the Legendre
+// three-term recurrence) and gauleg.hpp:89-121 (reduced geometrically graded
+// composite rule on [sigma^(n-1), 1], sigma = (sqrt2-1)^2).
+// ------------------------------------------------------------------------
+static void gl_rule(double a, double b, int n, double tol, double* x, double* w) {
+    const int half = (n + 1) / 2;
+    const double mid = 0.5 * (a + b), hw = 0.5 * (b - a);
+    for (int i = 0; i < half; ++i) {
+        double z = std::cos(kPi * (i + 1 - 0.25) / (n + 0.5)), zold, pn, pn1, dpn;
+        do {
+            pn = 1.0; pn1 = 0.0;
+            for (int j = 1; j <= n; ++j) {
+                double pn2 = pn1;
+                pn1 = pn;
+                pn = ((2.0 * j - 1.0) * z * pn1 - (j - 1.0) * pn2) / j;
+            }
+            dpn = n * (z * pn - pn1) / (z * z - 1.0);
+            zold = z;
+            z = zold - pn / dpn;
+        } while (std::fabs(z - zold) > tol);
+        x[i] = mid - hw * z;
+        x[n - 1 - i] = mid + hw * z;
+        double wi = 2.0 * hw / ((1.0 - z * z) * dpn * dpn);
+        w[i] = wi;
+        w[n - 1 - i] = wi;
+    }
+}
+
+static void composite_rule(int n, double tol, double* x, double* w) {
+    const int total = n * (n + 1) / 2 - 1;
+    const double sig = (std::sqrt(2.0) - 1.0) * (std::sqrt(2.0) - 1.0);
+    double lo = sig, hi = 1.0;
+    int pos = total - 1;
+    std::vector<double> gx(n), gw(n);
+    for (int lev = 1, npt = n; lev < n; ++lev, --npt) {
+        gl_rule(0.0, 1.0, npt, tol, gx.data(), gw.data());
+        for (int j = npt - 1; j >= 0; --j, --pos) {
+            w[pos] = (hi - lo) * gw[j];
+            x[pos] = (hi - lo) * gx[j] + lo;
+        }
+        hi = lo;
+        lo = lo * sig;
+    }
+}
+
+// ------------------------------------------------------------------------
+// J0, Y0, J1, Y1 for real x.  Branch structure of Hankel1Real01::jy_01
+// (cbessel.cpp:1896-1980): x<=0 -> Y forced to 0; x<1e-8 limiting forms
+// (1771-1776); x<8 joint ascending series (1778-1805, A&S 9.1.10-9.1.13);
+// 8<=x<=1e6 fdlibm P/Q forms (1807-1845); x>1e6 leading asymptotics (1847-1856).
+// ------------------------------------------------------------------------
+static void jy01(double x, double& j0, double& y0, double& j1, double& y1) {
+    if (!(x > 0.0)) { j0 = 1.0; j1 = 0.0; y0 = 0.0; y1 = 0.0; return; }
+    if (x < 1e-8) {
+        j0 = 1.0;
+        j1 = 0.5 * x;
+        y0 = (2.0 / kPi) * (std::log(j1) + kEuler);
+        y1 = -(2.0 / kPi) / x;
+        return;
+    }
+    if (x < 8.0) {
+        // J0 = sum t_k, t_k = (-x^2/4)^k/(k!)^2; Y0 = (2/pi)[(ln(x/2)+g) J0 - sum H_k t_k];
+        // J1 = (x/2) sum t_k/(k+1); Y1 = (1/pi)[-x sum H_k t_k/(k+1) + 2((ln(x/2)+g) J1 + (J0-2)/x)]
+        const double q = 0.25 * x * x;
+        double t = 1.0, H = 0.0, sj0 = 1.0, sy0 = 0.0, sj1 = 1.0, sy1 = 0.0;
+        for (int k = 1; k <= 1000; ++k) {
+            t *= -q / ((double)k * k);
+            H += 1.0 / k;
+            sj0 += t;
+            sy0 += H * t;
+            sj1 += t / (k + 1);
+            sy1 += (H / (k + 1)) * t;
+            if (!(H * std::fabs(t) > kEps)) break;
+        }
+        const double L = std::log(0.5 * x) + kEuler;
+        j0 = sj0;
+        j1 = 0.5 * x * sj1;
+        y0 = (2.0 / kPi) * (L * j0 - sy0);
+        y1 = (1.0 / kPi) * (-x * sy1 + 2.0 * (L * j1 + (j0 - 2.0) / x));
+        return;
+    }
+    if (x > 1e6) {
+        const double a = std::sqrt((2.0 / kPi) / x);
+        j0 = a * std::cos(x - 0.25 * kPi);
+        y0 = a * std::sin(x - 0.25 * kPi);
+        j1 = a * std::cos(x - 0.75 * kPi);
+        y1 = a * std::sin(x - 0.75 * kPi);
+        return;
+    }
+    j0 = ::j0(x); y0 = ::y0(x); j1 = ::j1(x); y1 = ::y1(x);
+}
+
+// ------------------------------------------------------------------------
+// Panels.  ParametrizedLine (parametrized_line.cpp:43-107),
+// ParametrizedCircularArc (parametrized_circular_arc.cpp:54-121) and an
+// analytic Fourier-sum curve segment z(t) = sum c_m e^{imt}, t in [t0,t1].
+// point = operator[], swapped = swapped_op, tangent = Derivative_01,
+// tangent_sw = Derivative_01_swapped(neg=true) (forward tangent at 1-s).
+// ------------------------------------------------------------------------
+struct Geo {
+    int N;
+    std::vector<double> pan;     // 8*N
+    std::vector<double> coefs;   // 3*ncoef
+    bool lines;
+    const double* P(int i) const { return &pan[8 * i]; }
+};
+
+static void fourier_eval(const Geo& g, const double* p, double t, cd& z, cd& dz) {
+    int c0 = (int)p[3], nc = (int)p[4];
+    z = 0.0; dz = 0.0;
+    for (int k = 0; k < nc; ++k) {
+        const double* c = &g.coefs[3 * (c0 + k)];
+        double m = c[0];
+        cd e = std::polar(1.0, m * t);
+        cd cm(c[1], c[2]);
+        z += cm * e;
+        dz += cd(0.0, m) * cm * e;
+    }
+}
+
+static cd pan_point(const Geo& g, int i, double s) {
+    const double* p = g.P(i);
+    switch ((int)p[0]) {
+    case 0: return cd(p[3] - p[1], p[4] - p[2]) * s + cd(p[1], p[2]);
+    case 1: { double d = p[5] - p[4]; return cd(p[1], p[2]) + p[3] * std::exp(cd(0.0, s * d + p[4])); }
+    default: { cd z, dz; fourier_eval(g, p, p[1] + s * (p[2] - p[1]), z, dz); return z; }
+    }
+}
+static cd pan_swapped(const Geo& g, int i, double s) {
+    const double* p = g.P(i);
+    switch ((int)p[0]) {
+    case 0: return cd(p[1] - p[3], p[2] - p[4]) * s + cd(p[3], p[4]);
+    case 1: { double d = p[4] - p[5]; return cd(p[1], p[2]) + p[3] * std::exp(cd(0.0, s * d + p[5])); }
+    default: { cd z, dz; fourier_eval(g, p, p[2] + s * (p[1] - p[2]), z, dz); return z; }
+    }
+}
+static cd pan_tangent(const Geo& g, int i, double s) {
+    const double* p = g.P(i);
+    switch ((int)p[0]) {
+    case 0: return cd(p[3] - p[1], p[4] - p[2]);
+    case 1: { double d = p[5] - p[4]; return cd(0.0, p[3] * d) * std::exp(cd(0.0, s * d + p[4])); }
+    default: { cd z, dz; fourier_eval(g, p, p[1] + s * (p[2] - p[1]), z, dz); return (p[2] - p[1]) * dz; }
+    }
+}
+static cd pan_tangent_sw(const Geo& g, int i, double s) {
+    const double* p = g.P(i);
+    switch ((int)p[0]) {
+    case 0: return -cd(p[1] - p[3], p[2] - p[4]);
+    case 1: { double d = p[4] - p[5]; return -(cd(0.0, p[3] * d) * std::exp(cd(0.0, s * d + p[5]))); }
+    default: { cd z, dz; fourier_eval(g, p, p[2] + s * (p[1] - p[2]), z, dz); return -((p[1] - p[2]) * dz); }
+    }
+}
+// Start / end point through operator() (parameter range [-1,1]) as
+// BuilderData::is_adjacent uses them (galerkin_builder.cpp:189-201).
+static cd pan_end(const Geo& g, int i, bool end) {
+    const double* p = g.P(i);
+    if ((int)p[0] == 0) {
+        double t = end ? 1.0 : -1.0;
+        return 0.5 * cd(t * (p[3] - p[1]) + (p[3] + p[1]), t * (p[4] - p[2]) + (p[4] + p[2]));
+    }
+    return pan_point(g, i, end ? 1.0 : 0.0);
+}
+static bool is_adjacent(const Geo& g, int i, int j, bool& swap) {
+    const double tol = kEps * 100.0;
+    if (std::abs(pan_end(g, i, true) - pan_end(g, j, false)) < tol) { swap = false; return true; }
+    if (std::abs(pan_end(g, i, false) - pan_end(g, j, true)) < tol) { swap = true; return true; }
+    return false;
+}
+
+// P1 continuous space (continuous_space.hpp:33-152): shape b0(t)=t, b1(t)=1-t;
+// swapped b0~(t)=1-t, b1~(t)=t; derivatives (+1,-1) for both orientations;
+// local-to-global map dof(0,p)=(p+1)%N, dof(1,p)=p.
+static inline double shp(int m, double t) { return m == 0 ? t : 1.0 - t; }
+static inline double shp_sw(int m, double t) { return m == 0 ? 1.0 - t : t; }
+static inline double dshp(int m) { return m == 0 ? 1.0 : -1.0; }
+static inline int dof(int m, int p, int N) { return m == 0 ? (p + 1) % N : p; }
+
+// ------------------------------------------------------------------------
+// Local interaction matrices of one panel pair (i >= j).
+// K[d][o][m][n]: o=0 direct block (test on i, trial on j), o=1 transposed
+// block (test on j, trial on i).  W[d][m][n], V[d][m][n] direct blocks.
+// Scalars follow GalerkinBuilder::initialize_constants (899-916).
+// ------------------------------------------------------------------------
+struct Local {
+    cd K[3][2][2][2];
+    cd W[3][2][2];
+    cd V[3][2][2];
+    void zero() { std::memset((void*)this, 0, sizeof(*this)); }
+};
+
+struct Consts {
+    double kr, c, sqrtc, kap, kap_inv, kap2h;   // k (real), c, sqrt(c), kappa=k sqrt c, 1/kappa, kappa^2/2
+};
+
+struct Quad {
+    int ns, nc;
+    std::vector<double> xg, wg, xc, wc;
+};
+
+// Per-node record shared by the three layers.
+struct Node {
+    double r, r2, w;
+    cd v, ti, tj;       // v = x_i - x_j, tangents at the two points (forward)
+    double ni, nj;      // |tangents|
+    double bi[2], bj[2];// test/trial shape values at the panel-i / panel-j points
+    cd h0, h1;
+};
+
+static void hankel_node(Node& nd, const Consts& C) {
+    double j0, y0, j1, y1;
+    jy01(C.kap * nd.r, j0, y0, j1, y1);
+    nd.h0 = cd(j0, y0);
+    nd.h1 = cd(j1, y1);
+}
+
+static bool all_small(const std::vector<double>& a) {
+    for (double v : a) if (!(std::fabs(v) < kEps)) return false;
+    return true;
+}
+
+// Accumulate the double-layer sums of one node set (direct or transposed).
+// galerkin_builder.cpp:376-398 / 406-458 / 514-537.
+static void dl_accum(const std::vector<Node>& nodes, const std::vector<double>& vdn, bool transposed,
+                     int d, const Consts& C, cd out[3][2][2]) {
+    for (size_t q = 0; q < nodes.size(); ++q) {
+        const Node& nd = nodes[q];
+        cd m1 = (nd.r <= 0.0) ? cd(0.0) : C.kap * nd.h1 / nd.r;
+        cd m3 = nd.h0 - C.kap * nd.r * nd.h1;
+        for (int n = 0; n < 2; ++n)
+            for (int m = 0; m < 2; ++m) {
+                double fg = transposed ? nd.bj[m] * nd.bi[n] : nd.bi[m] * nd.bj[n];
+                double cfr = vdn[q] * (fg * nd.w);
+                out[0][m][n] += cfr * m1;
+                if (d > 0) out[1][m][n] += cfr * nd.h0;
+                if (d > 1) out[2][m][n] += cfr * m3;
+            }
+    }
+}
+
+// Hypersingular sums (galerkin_builder.cpp:566-602, 613-655, 681-713);
+// tdt already carries the factor 2 of compute_tdottp (845-849).
+static void hyp_accum(const std::vector<Node>& nodes, const std::vector<double>& tdt2, bool tz,
+                      int d, const Consts& C, cd out[3][2][2], bool lower_only) {
+    for (size_t q = 0; q < nodes.size(); ++q) {
+        const Node& nd = nodes[q];
+        cd hr = nd.h1 * nd.r;
+        cd a2 = nd.h0 * nd.r2 - hr * C.kap_inv;
+        cd a3 = 2.0 * C.kap * hr - nd.h0;
+        for (int n = 0; n < 2; ++n)
+            for (int m = 0; m < 2; ++m) {
+                if (lower_only && m < n) continue;
+                double A = dshp(m) * dshp(n) * nd.w;
+                double B = tz ? 0.0 : (nd.bi[m] * nd.bj[n] * nd.w) * tdt2[q];
+                double cf = A - C.kap2h * B;
+                out[0][m][n] += cf * nd.h0;
+                if (d > 0) out[1][m][n] += hr * cf + C.kap * nd.h0 * B;
+                if (d > 1) out[2][m][n] += a2 * cf - B * a3;
+            }
+    }
+}
+
+// Single-layer sums (galerkin_builder.cpp:741-756, 767-789, 818-827).
+static void slp_accum(const std::vector<Node>& nodes, int d, const Consts& C, cd out[3][2][2], bool lower_only) {
+    for (size_t q = 0; q < nodes.size(); ++q) {
+        const Node& nd = nodes[q];
+        cd m2 = nd.h1 * nd.r;
+        cd m3 = m2 * C.kap_inv - nd.h0 * nd.r2;
+        double ttp = nd.ni * nd.nj;
+        for (int n = 0; n < 2; ++n)
+            for (int m = 0; m < 2; ++m) {
+                if (lower_only && m < n) continue;
+                double cfr = ttp * (nd.bi[m] * nd.bj[n] * nd.w);
+                out[0][m][n] += cfr * nd.h0;
+                if (d > 0) out[1][m][n] += cfr * m2;
+                if (d > 1) out[2][m][n] += cfr * m3;
+            }
+    }
+}
+
+static void finish_node(Node& nd, const Consts& C) {
+    nd.r2 = std::norm(nd.v);
+    nd.r = std::sqrt(nd.r2);
+    nd.ni = std::abs(nd.ti);
+    nd.nj = std::abs(nd.tj);
+    hankel_node(nd, C);
+}
+
+// Pair classes.  Coinciding (237-265, 346-399, 543-607, 720-761),
+// adjacent (267-306, 401-459, 609-660, 763-794), general (308-327, 461-541,
+// 662-718, 796-832).  "i"/"j" are the test/trial panels of the direct block.
+static void pair_local(const Geo& g, const Quad& Q, const Consts& C, int i, int j, int layers, int d,
+                       int flags, Local& L) {
+    L.zero();
+    std::vector<std::vector<Node>> halves;
+    int cls;
+    bool swap = false;
+    if (i == j) {
+        cls = 0;
+        std::vector<Node> nodes;
+        for (int a = 0; a < Q.nc; ++a)
+            for (int b = 0; b < Q.nc; ++b) {
+                Node nd;
+                double t = Q.xc[b], s = t * (1.0 - Q.xc[a]);
+                nd.w = t * Q.wc[a] * Q.wc[b];
+                nd.v = pan_point(g, i, s) - pan_point(g, i, t);
+                nd.ti = pan_tangent(g, i, s);
+                nd.tj = pan_tangent(g, i, t);
+                nd.bi[0] = shp(0, s); nd.bi[1] = shp(1, s);
+                nd.bj[0] = shp(0, t); nd.bj[1] = shp(1, t);
+                finish_node(nd, C);
+                nodes.push_back(nd);
+            }
+        halves.push_back(nodes);
+        // second triangle: roles of s and t exchanged, v unchanged (Hankel reused)
+        for (auto& nd : nodes) {
+            std::swap(nd.ti, nd.tj); std::swap(nd.ni, nd.nj);
+            std::swap(nd.bi[0], nd.bj[0]); std::swap(nd.bi[1], nd.bj[1]);
+            nd.v = -nd.v;
+        }
+        halves.push_back(nodes);
+    } else if (is_adjacent(g, i, j, swap)) {
+        cls = 1;
+        for (int K = 0; K < 2; ++K) {
+            std::vector<Node> nodes;
+            for (int a = 0; a < Q.nc; ++a)
+                for (int b = 0; b < Q.nc; ++b) {
+                    Node nd;
+                    double t = Q.xc[a], s = t * Q.xc[b];
+                    nd.w = t * Q.wc[a] * Q.wc[b];
+                    double si = (K == 0) ? s : t, sj = (K == 0) ? t : s;   // distances from the shared vertex
+                    if (swap) {   // start(i) == end(j)
+                        nd.v = pan_point(g, i, si) - pan_swapped(g, j, sj);
+                        nd.ti = pan_tangent(g, i, si);
+                        nd.tj = pan_tangent_sw(g, j, sj);
+                        nd.bi[0] = shp(0, si); nd.bi[1] = shp(1, si);
+                        nd.bj[0] = shp_sw(0, sj); nd.bj[1] = shp_sw(1, sj);
+                    } else {      // end(i) == start(j)
+                        nd.v = pan_swapped(g, i, si) - pan_point(g, j, sj);
+                        nd.ti = pan_tangent_sw(g, i, si);
+                        nd.tj = pan_tangent(g, j, sj);
+                        nd.bi[0] = shp_sw(0, si); nd.bi[1] = shp_sw(1, si);
+                        nd.bj[0] = shp(0, sj); nd.bj[1] = shp(1, sj);
+                    }
+                    finish_node(nd, C);
+                    nodes.push_back(nd);
+                }
+            halves.push_back(nodes);
+        }
+    } else {
+        cls = 2;
+        std::vector<Node> nodes;
+        for (int a = 0; a < Q.ns; ++a)
+            for (int b = 0; b < Q.ns; ++b) {
+                Node nd;
+                double s = Q.xg[a], t = Q.xg[b];
+                nd.w = Q.wg[a] * Q.wg[b];
+                nd.v = pan_point(g, i, s) - pan_point(g, j, t);
+                nd.ti = pan_tangent(g, i, s);
+                nd.tj = pan_tangent(g, j, t);
+                nd.bi[0] = shp(0, s); nd.bi[1] = shp(1, s);
+                nd.bj[0] = shp(0, t); nd.bj[1] = shp(1, t);
+                finish_node(nd, C);
+                nodes.push_back(nd);
+            }
+        halves.push_back(nodes);
+    }
+
+    // ---------------- double layer ----------------
+    if (layers & LAYER_DOUBLE) {
+        cd acc[2][3][2][2];
+        std::memset((void*)acc, 0, sizeof(acc));
+        for (auto& nodes : halves) {
+            size_t n = nodes.size();
+            std::vector<double> vd(n), vt(n), vd_raw(n), vt_raw(n);
+            for (size_t q = 0; q < n; ++q) {
+                const Node& nd = nodes[q];
+                // v . N_j with N_j = (Im tau_j, -Re tau_j), times |tau_i|
+                vd_raw[q] = nd.v.real() * nd.tj.imag() - nd.v.imag() * nd.tj.real();
+                vd[q] = vd_raw[q] * nd.ni;
+                // transposed: (-v) . N_i times |tau_j|
+                vt_raw[q] = nd.v.imag() * nd.ti.real() - nd.v.real() * nd.ti.imag();
+                vt[q] = vt_raw[q] * nd.nj;
+            }
+            if (cls == 2) {
+                if (!all_small(vd_raw)) dl_accum(nodes, vd, false, d, C, acc[0]);
+                if (!all_small(vt_raw)) dl_accum(nodes, vt, true, d, C, acc[1]);
+            } else if (cls == 1) {
+                if (!all_small(vd)) dl_accum(nodes, vd, false, d, C, acc[0]);
+                if (!all_small(vt)) dl_accum(nodes, vt, true, d, C, acc[1]);
+            } else {
+                // coinciding: each triangle contributes only to the direct block
+                if (!all_small(vd)) dl_accum(nodes, vd, false, d, C, acc[0]);
+            }
+        }
+        for (int o = 0; o < 2; ++o)
+            for (int m = 0; m < 2; ++m)
+                for (int n = 0; n < 2; ++n) {
+                    L.K[0][o][m][n] = acc[o][0][m][n];
+                    L.K[1][o][m][n] = acc[o][1][m][n] * (C.kr * C.c);
+                    L.K[2][o][m][n] = acc[o][2][m][n] * C.c;
+                }
+    }
+    // ---------------- hypersingular ----------------
+    if (layers & LAYER_HYPER) {
+        cd acc[3][2][2];
+        std::memset((void*)acc, 0, sizeof(acc));
+        for (auto& nodes : halves) {
+            size_t n = nodes.size();
+            std::vector<double> tdt(n);
+            for (size_t q = 0; q < n; ++q)
+                tdt[q] = nodes[q].ti.imag() * nodes[q].tj.imag() + nodes[q].ti.real() * nodes[q].tj.real();
+            bool tz = all_small(tdt);
+            for (auto& v : tdt) v *= 2.0;
+            hyp_accum(nodes, tdt, tz, d, C, acc, cls == 0);
+        }
+        if (cls == 0)
+            for (int l = 0; l < 3; ++l) acc[l][0][1] = acc[l][1][0];
+        for (int m = 0; m < 2; ++m)
+            for (int n = 0; n < 2; ++n) {
+                L.W[0][m][n] = acc[0][m][n];
+                L.W[1][m][n] = acc[1][m][n] * (-C.sqrtc);
+                L.W[2][m][n] = acc[2][m][n] * (-C.c);
+            }
+    }
+    // ---------------- single layer ----------------
+    if (layers & LAYER_SINGLE) {
+        cd acc[3][2][2];
+        std::memset((void*)acc, 0, sizeof(acc));
+        for (auto& nodes : halves) slp_accum(nodes, d, C, acc, cls == 0);
+        if (cls == 0) {
+            for (int l = 0; l < 3; ++l)
+                acc[l][0][1] = (flags & HBEM_QUIRK_SLP_COINC_DER) ? acc[0][1][0] : acc[l][1][0];
+        }
+        for (int m = 0; m < 2; ++m)
+            for (int n = 0; n < 2; ++n) {
+                L.V[0][m][n] = acc[0][m][n];
+                L.V[1][m][n] = acc[1][m][n] * (-C.sqrtc);
+                L.V[2][m][n] = acc[2][m][n] * C.c;
+            }
+    }
+}
+
+static void parallel_for(int n, int nthreads, const std::function<void(int)>& f) {
+    if (nthreads <= 1 || n <= 1) { for (int i = 0; i < n; ++i) f(i); return; }
+    std::atomic<int> next(0);
+    std::vector<std::thread> th;
+    int nt = std::min(nthreads, n);
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&]() { for (int i; (i = next.fetch_add(1)) < n;) f(i); });
+    for (auto& x : th) x.join();
+}
+
+static Geo make_geo(const double* panels, int N, const double* coefs, int ncoef) {
+    Geo g;
+    g.N = N;
+    g.pan.assign(panels, panels + 8 * N);
+    if (ncoef > 0) g.coefs.assign(coefs, coefs + 3 * ncoef);
+    g.lines = true;
+    for (int i = 0; i < N; ++i) if ((int)g.pan[8 * i] != 0) g.lines = false;
+    return g;
+}
+
+static Quad make_quad(int order) {
+    Quad Q;
+    Q.ns = order;
+    Q.nc = order * (order + 1) / 2 - 1;
+    Q.xg.resize(Q.ns); Q.wg.resize(Q.ns); Q.xc.resize(Q.nc); Q.wc.resize(Q.nc);
+    gl_rule(0.0, 1.0, order, kEps, Q.xg.data(), Q.wg.data());
+    composite_rule(order, kEps, Q.xc.data(), Q.wc.data());
+    return Q;
+}
+
+// GalerkinBuilder::assembleDense (galerkin_builder.cpp:918-1018) for equal
+// test/trial P1 spaces: all pairs i >= j, local matrices, local-to-global
+// scatter with transposed writes, final factor i/4.
+static void assemble(const Geo& g, const Quad& Q, double k, double c, int layers, int d, int flags,
+                     int nthreads, cd* K, cd* W, cd* V) {
+    if (c < 1.0) throw std::runtime_error("Refraction index must not be smaller than 1");
+    if (!(k > 0.0)) throw std::runtime_error("oracle supports real positive k only");
+    Consts C;
+    C.kr = k; C.c = c; C.sqrtc = std::sqrt(c); C.kap = k * C.sqrtc; C.kap_inv = 1.0 / C.kap;
+    C.kap2h = C.kap * C.kap * 0.5;
+    const int N = g.N;
+    const size_t NN = (size_t)N * N;
+    int npairs = N * (N + 1) / 2;
+    std::vector<Local> loc(npairs);
+    std::vector<int> pi(npairs), pj(npairs);
+    for (int i = 0, p = 0; i < N; ++i)
+        for (int j = 0; j <= i; ++j, ++p) { pi[p] = i; pj[p] = j; }
+    parallel_for(npairs, nthreads, [&](int p) { pair_local(g, Q, C, pi[p], pj[p], layers, d, flags, loc[p]); });
+    for (int t = 0; t <= d; ++t) {
+        if (K) std::fill(K + t * NN, K + (t + 1) * NN, cd(0.0));
+        if (W) std::fill(W + t * NN, W + (t + 1) * NN, cd(0.0));
+        if (V) std::fill(V + t * NN, V + (t + 1) * NN, cd(0.0));
+    }
+    for (int p = 0; p < npairs; ++p) {
+        int i = pi[p], j = pj[p];
+        const Local& L = loc[p];
+        for (int t = 0; t <= d; ++t)
+            for (int n = 0; n < 2; ++n)
+                for (int m = 0; m < 2; ++m) {
+                    size_t dd = t * NN + (size_t)dof(m, i, N) + (size_t)dof(n, j, N) * N;
+                    size_t tt = t * NN + (size_t)dof(m, j, N) + (size_t)dof(n, i, N) * N;
+                    if (K && (layers & LAYER_DOUBLE)) {
+                        K[dd] += L.K[t][0][m][n];
+                        if (i != j) K[tt] += L.K[t][1][m][n];
+                    }
+                    if (W && (layers & LAYER_HYPER)) {
+                        W[dd] += L.W[t][m][n];
+                        if (i != j) W[tt] += L.W[t][n][m];
+                    }
+                    if (V && (layers & LAYER_SINGLE)) {
+                        V[dd] += L.V[t][m][n];
+                        if (i != j) V[tt] += L.V[t][n][m];
+                    }
+                }
+    }
+    const cd quarter_i(0.0, 0.25);
+    for (size_t e = 0; e < (d + 1) * NN; ++e) {
+        if (K) K[e] *= quarter_i;
+        if (W) W[e] *= quarter_i;
+        if (V) V[e] *= quarter_i;
+    }
+}
+
+// mass_matrix::GalerkinMatrix (mass_matrix.cpp:6-47) with the rule choice of
+// SolutionsOperator's ctor (gen_sol_op.cpp:30-33): Gauss order 4 for
+// polygonal meshes, the builder's order otherwise.  |gamma'| is the Euclidean
+// norm of the 2-vector, sqrt(x^2+y^2), as Eigen::Vector2d::norm computes it.
+static void mass_matrix(const Geo& g, int order, double* M) {
+    const int N = g.N;
+    int n = g.lines ? 4 : order;
+    std::vector<double> x(n), w(n);
+    gl_rule(0.0, 1.0, n, kEps, x.data(), w.data());
+    std::fill(M, M + (size_t)N * N, 0.0);
+    for (int p = 0; p < N; ++p)
+        for (int ii = 0; ii < 2; ++ii)
+            for (int jj = 0; jj < 2; ++jj) {
+                double sum = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    cd tg = pan_tangent(g, p, x[k]);
+                    double nrm = std::sqrt(tg.real() * tg.real() + tg.imag() * tg.imag());
+                    double F = shp(jj, x[k]) * nrm;
+                    double G = shp(ii, x[k]);
+                    double wf = w[k] * F;
+                    double term = wf * G;
+                    sum = sum + term;
+                }
+                M[dof(ii, p, N) + (size_t)dof(jj, p, N) * N] += sum;
+            }
+}
+
+// Eigen's LDLT with diagonal pivoting (left-looking; the pivot is the largest
+// |diagonal| of the not-yet-factored part, first index on ties), as used by
+// SolutionsOperator's ctor (gen_sol_op.cpp:34-41).  Returns F = P^T L sqrt(D).
+static void ldlt_factor(const double* Ain, int n, double* F, int* perm_out) {
+    std::vector<double> A(Ain, Ain + (size_t)n * n);
+    auto a = [&](int i, int j) -> double& { return A[i + (size_t)j * n]; };
+    std::vector<int> tr(n);
+    std::vector<double> temp(n);
+    for (int k = 0; k < n; ++k) {
+        int big = k;
+        double bv = std::fabs(a(k, k));
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(a(i, i)) > bv) { bv = std::fabs(a(i, i)); big = i; }
+        tr[k] = big;
+        if (big != k) {
+            for (int j = 0; j < k; ++j) std::swap(a(k, j), a(big, j));
+            for (int i = big + 1; i < n; ++i) std::swap(a(i, k), a(i, big));
+            std::swap(a(k, k), a(big, big));
+            for (int i = k + 1; i < big; ++i) {
+                double tmp = a(i, k);
+                a(i, k) = a(big, i);
+                a(big, i) = tmp;
+            }
+        }
+        int rs = n - k - 1;
+        if (k > 0) {
+            for (int j = 0; j < k; ++j) temp[j] = a(j, j) * a(k, j);
+            double s = 0.0;
+            for (int j = 0; j < k; ++j) s += a(k, j) * temp[j];
+            a(k, k) -= s;
+            for (int i = k + 1; i < n; ++i) {
+                double u = 0.0;
+                for (int j = 0; j < k; ++j) u += a(i, j) * temp[j];
+                a(i, k) -= u;
+            }
+        }
+        double akk = a(k, k);
+        if (rs > 0 && std::fabs(akk) > 0.0)
+            for (int i = k + 1; i < n; ++i) a(i, k) /= akk;
+    }
+    // F = P^T L sqrt(D):  L unit lower, D = diag(A); P = T_{n-1}..T_0.
+    std::vector<double> Lm((size_t)n * n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        double sd = std::sqrt(a(j, j));
+        Lm[j + (size_t)j * n] = sd;
+        for (int i = j + 1; i < n; ++i) Lm[i + (size_t)j * n] = a(i, j) * sd;
+    }
+    for (int k = n - 1; k >= 0; --k)
+        if (tr[k] != k)
+            for (int j = 0; j < n; ++j) std::swap(Lm[k + (size_t)j * n], Lm[tr[k] + (size_t)j * n]);
+    std::memcpy(F, Lm.data(), sizeof(double) * (size_t)n * n);
+    if (perm_out) for (int k = 0; k < n; ++k) perm_out[k] = tr[k];
+}
+
+// ------------------------------------------------------------------------
+// Dense complex linear algebra (Eigen PartialPivLU / HouseholderQR /
+// ColPivHouseholderQR / JacobiSVD roles in randsvd.cpp).
+// ------------------------------------------------------------------------
+struct LU {
+    int n;
+    std::vector<cd> a;
+    std::vector<int> piv;   // row interchanges
+};
+
+static void lu_factor(const cd* A, int n, LU& f) {
+    f.n = n;
+    f.a.assign(A, A + (size_t)n * n);
+    f.piv.resize(n);
+    auto a = [&](int i, int j) -> cd& { return f.a[i + (size_t)j * n]; };
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = std::abs(a(k, k));
+        for (int i = k + 1; i < n; ++i) { double v = std::abs(a(i, k)); if (v > best) { best = v; p = i; } }
+        f.piv[k] = p;
+        if (p != k) for (int j = 0; j < n; ++j) std::swap(a(k, j), a(p, j));
+        cd d = a(k, k);
+        if (d != 0.0) { cd inv = 1.0 / d; for (int i = k + 1; i < n; ++i) a(i, k) *= inv; }
+        for (int j = k + 1; j < n; ++j) {
+            cd u = a(k, j);
+            if (u == 0.0) continue;
+            cd* col = &f.a[(size_t)j * n];
+            const cd* lc = &f.a[(size_t)k * n];
+            for (int i = k + 1; i < n; ++i) col[i] -= lc[i] * u;
+        }
+    }
+}
+
+// solve A X = B (adj=false) or A^H X = B (adj=true), B is n x m column-major, in place
+static void lu_solve(const LU& f, cd* B, int m, bool adj) {
+    const int n = f.n;
+    auto a = [&](int i, int j) -> cd { return f.a[i + (size_t)j * n]; };
+    for (int c = 0; c < m; ++c) {
+        cd* b = B + (size_t)c * n;
+        if (!adj) {
+            for (int k = 0; k < n; ++k) if (f.piv[k] != k) std::swap(b[k], b[f.piv[k]]);
+            for (int j = 0; j < n; ++j) { cd bj = b[j]; for (int i = j + 1; i < n; ++i) b[i] -= a(i, j) * bj; }
+            for (int j = n - 1; j >= 0; --j) { b[j] /= a(j, j); cd bj = b[j]; for (int i = 0; i < j; ++i) b[i] -= a(i, j) * bj; }
+        } else {
+            // A = P^T L U  =>  A^H = U^H L^H P
+            for (int i = 0; i < n; ++i) { cd s = b[i]; for (int j = 0; j < i; ++j) s -= std::conj(a(j, i)) * b[j]; b[i] = s / std::conj(a(i, i)); }
+            for (int i = n - 1; i >= 0; --i) { cd s = b[i]; for (int j = i + 1; j < n; ++j) s -= std::conj(a(j, i)) * b[j]; b[i] = s; }
+            for (int k = n - 1; k >= 0; --k) if (f.piv[k] != k) std::swap(b[k], b[f.piv[k]]);
+        }
+    }
+}
+
+// Orthonormal basis of the column span of Y (n x m, full column rank) by
+// Householder QR; returns the first m columns of Q in place.
+static void thin_q(cd* Y, int n, int m) {
+    std::vector<cd> A(Y, Y + (size_t)n * m);
+    std::vector<std::vector<cd>> vs;
+    std::vector<cd> taus;
+    for (int k = 0; k < m; ++k) {
+        cd* col = &A[(size_t)k * n];
+        double xn = 0.0;
+        for (int i = k; i < n; ++i) xn += std::norm(col[i]);
+        xn = std::sqrt(xn);
+        cd alpha = col[k];
+        double aa = std::abs(alpha);
+        cd beta = (aa == 0.0) ? cd(-xn) : -(alpha / aa) * xn;
+        std::vector<cd> v(n, 0.0);
+        v[k] = alpha - beta;
+        for (int i = k + 1; i < n; ++i) v[i] = col[i];
+        double vn = 0.0;
+        for (int i = k; i < n; ++i) vn += std::norm(v[i]);
+        cd tau = (vn == 0.0) ? 0.0 : 2.0 / vn;
+        for (int j = k; j < m; ++j) {
+            cd* cj = &A[(size_t)j * n];
+            cd s = 0.0;
+            for (int i = k; i < n; ++i) s += std::conj(v[i]) * cj[i];
+            s *= tau;
+            for (int i = k; i < n; ++i) cj[i] -= s * v[i];
+        }
+        vs.push_back(v);
+        taus.push_back(tau);
+    }
+    for (int j = 0; j < m; ++j) {
+        std::vector<cd> e(n, 0.0);
+        e[j] = 1.0;
+        for (int k = m - 1; k >= 0; --k) {
+            cd s = 0.0;
+            for (int i = k; i < n; ++i) s += std::conj(vs[k][i]) * e[i];
+            s *= taus[k];
+            for (int i = k; i < n; ++i) e[i] -= s * vs[k][i];
+        }
+        std::copy(e.begin(), e.end(), Y + (size_t)j * n);
+    }
+}
+
+// Singular values of an n x m matrix (m small) by one-sided Jacobi (Hestenes).
+static void jacobi_sv(const cd* Bin, int n, int m, double* sv) {
+    std::vector<cd> B(Bin, Bin + (size_t)n * m);
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0;
+        for (int p = 0; p < m; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                cd* bp = &B[(size_t)p * n];
+                cd* bq = &B[(size_t)q * n];
+                double app = 0.0, aqq = 0.0;
+                cd apq = 0.0;
+                for (int i = 0; i < n; ++i) { app += std::norm(bp[i]); aqq += std::norm(bq[i]); apq += std::conj(bp[i]) * bq[i]; }
+                double g = std::abs(apq);
+                if (g == 0.0 || g <= kEps * std::sqrt(app * aqq)) continue;
+                off = std::max(off, g / std::sqrt(app * aqq));
+                cd ph = apq / g;
+                double zeta = (aqq - app) / (2.0 * g);
+                double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+                double cs = 1.0 / std::sqrt(1.0 + t * t), sn = cs * t;
+                for (int i = 0; i < n; ++i) {
+                    cd x = bp[i], y = bq[i];
+                    bp[i] = cs * x - sn * std::conj(ph) * y;
+                    bq[i] = sn * ph * x + cs * y;
+                }
+            }
+        if (off < kEps) break;
+    }
+    for (int p = 0; p < m; ++p) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += std::norm(B[(size_t)p * n + i]);
+        sv[p] = std::sqrt(s);
+    }
+    std::sort(sv, sv + m, [](double x, double y) { return x > y; });
+}
+
+// randomized_svd::sv (randsvd.cpp:99-146).
+static double rsvd_sv(const LU& f, const cd* R, int n, int nc, int q) {
+    std::vector<cd> Y(R, R + (size_t)n * nc);
+    lu_solve(f, Y.data(), nc, false);
+    thin_q(Y.data(), n, nc);
+    for (int it = 0; it < q; ++it) {
+        lu_solve(f, Y.data(), nc, true);
+        thin_q(Y.data(), n, nc);
+        lu_solve(f, Y.data(), nc, false);
+        thin_q(Y.data(), n, nc);
+    }
+    lu_solve(f, Y.data(), nc, true);
+    std::vector<double> s(nc);
+    jacobi_sv(Y.data(), n, nc, s.data());
+    return 1.0 / s[0];
+}
+
+// Complex Householder QR with column pivoting (LAPACK xGEQP3 pivot rule:
+// largest remaining column norm, first on ties); returns column `col` of Q.
+static void colpiv_qr_qcol(const cd* Ain, int n, int col, cd* out) {
+    std::vector<cd> A(Ain, Ain + (size_t)n * n);
+    std::vector<double> nrm(n);
+    for (int j = 0; j < n; ++j) { double s = 0; for (int i = 0; i < n; ++i) s += std::norm(A[i + (size_t)j * n]); nrm[j] = s; }
+    std::vector<std::vector<cd>> vs;
+    std::vector<cd> taus;
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int j = k + 1; j < n; ++j) if (nrm[j] > nrm[p]) p = j;
+        if (p != k) { for (int i = 0; i < n; ++i) std::swap(A[i + (size_t)k * n], A[i + (size_t)p * n]); std::swap(nrm[k], nrm[p]); }
+        cd* ck = &A[(size_t)k * n];
+        double xn = 0.0;
+        for (int i = k; i < n; ++i) xn += std::norm(ck[i]);
+        xn = std::sqrt(xn);
+        cd alpha = ck[k];
+        double aa = std::abs(alpha);
+        cd beta = (aa == 0.0) ? cd(-xn) : -(alpha / aa) * xn;
+        std::vector<cd> v(n, 0.0);
+        v[k] = alpha - beta;
+        for (int i = k + 1; i < n; ++i) v[i] = ck[i];
+        double vn = 0.0;
+        for (int i = k; i < n; ++i) vn += std::norm(v[i]);
+        cd tau = (vn == 0.0) ? 0.0 : 2.0 / vn;
+        for (int j = k; j < n; ++j) {
+            cd* cj = &A[(size_t)j * n];
+            cd s = 0.0;
+            for (int i = k; i < n; ++i) s += std::conj(v[i]) * cj[i];
+            s *= tau;
+            for (int i = k; i < n; ++i) cj[i] -= s * v[i];
+        }
+        for (int j = k + 1; j < n; ++j) { double s = 0; for (int i = k + 1; i < n; ++i) s += std::norm(A[i + (size_t)j * n]); nrm[j] = s; }
+        vs.push_back(v);
+        taus.push_back(tau);
+    }
+    std::vector<cd> e(n, 0.0);
+    e[col] = 1.0;
+    for (int k = n - 1; k >= 0; --k) {
+        cd s = 0.0;
+        for (int i = k; i < n; ++i) s += std::conj(vs[k][i]) * e[i];
+        s *= taus[k];
+        for (int i = k; i < n; ++i) e[i] -= s * vs[k][i];
+    }
+    std::copy(e.begin(), e.end(), out);
+}
+
+// Wielandt matrix W = [[0,T],[T^H,0]] - s I (randsvd.cpp:151-160).
+static std::vector<cd> wielandt(const cd* T, int n, double s) {
+    int m = 2 * n;
+    std::vector<cd> W((size_t)m * m, 0.0);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            W[i + (size_t)(n + j) * m] = T[i + (size_t)j * n];
+            W[(n + j) + (size_t)i * m] = std::conj(T[i + (size_t)j * n]);
+        }
+    for (int i = 0; i < m; ++i) W[i + (size_t)i * m] -= s;
+    return W;
+}
+
+static void matvec(const cd* A, int n, const cd* x, cd* y, bool adj) {
+    for (int i = 0; i < n; ++i) y[i] = 0.0;
+    if (!adj) { for (int j = 0; j < n; ++j) for (int i = 0; i < n; ++i) y[i] += A[i + (size_t)j * n] * x[j]; }
+    else { for (int i = 0; i < n; ++i) { cd s = 0; for (int j = 0; j < n; ++j) s += std::conj(A[j + (size_t)i * n]) * x[j]; y[i] = s; } }
+}
+
+// randomized_svd::sv_der (randsvd.cpp:148-169).
+static void rsvd_sv_der(const cd* T, const cd* Td, const cd* R, int n, int nc, int q, double* out) {
+    LU f;
+    lu_factor(T, n, f);
+    double s = rsvd_sv(f, R, n, nc, q);
+    std::vector<cd> W = wielandt(T, n, s);
+    std::vector<cd> x(2 * n), p(2 * n);
+    colpiv_qr_qcol(W.data(), 2 * n, 2 * n - 1, x.data());
+    matvec(Td, n, x.data() + n, p.data(), false);
+    matvec(Td, n, x.data(), p.data() + n, true);
+    cd dot = 0.0;
+    for (int i = 0; i < 2 * n; ++i) dot += std::conj(x[i]) * p[i];
+    out[0] = s;
+    out[1] = dot.real();
+}
+
+// randomized_svd::sv_der2 (randsvd.cpp:171-216).
+static void rsvd_sv_der2(const cd* T, const cd* Td, const cd* Tdd, const cd* R, int n, int nc, int q, double* out) {
+    LU f;
+    lu_factor(T, n, f);
+    double s = rsvd_sv(f, R, n, nc, q);
+    const int m2 = 2 * n;
+    std::vector<cd> W = wielandt(T, n, s);
+    std::vector<cd> x(m2), u, p(m2), p2(m2);
+    colpiv_qr_qcol(W.data(), m2, m2 - 1, x.data());
+    u = x;
+    double xn = 0; for (auto& v : x) xn += std::norm(v); xn = std::sqrt(xn);
+    for (auto& v : x) v /= xn;
+    matvec(Td, n, x.data() + n, p.data(), false);
+    matvec(Td, n, x.data(), p.data() + n, true);
+    cd dot = 0.0;
+    for (int i = 0; i < m2; ++i) dot += std::conj(x[i]) * p[i];
+    out[0] = s;
+    out[1] = dot.real();
+    double temp = 0; int m = 5;
+    for (int l = 0; l < m2; ++l) if (std::abs(u[l]) > temp) { temp = std::abs(u[l]); m = l; }
+    m += 1;
+    cd um = u[m - 1];
+    for (auto& v : u) v /= um;
+    for (int i = 0; i < m2; ++i) W[i + (size_t)(m - 1) * m2] = -u[i];
+    LU fb;
+    lu_factor(W.data(), m2, fb);
+    matvec(Td, n, u.data() + n, p.data(), false);
+    matvec(Td, n, u.data(), p.data() + n, true);
+    std::vector<cd> ud(p);
+    lu_solve(fb, ud.data(), 1, false);
+    for (auto& v : ud) v = -v;
+    cd t = ud[m - 1];
+    ud[m - 1] = 0.0;
+    matvec(Td, n, ud.data() + n, p.data(), false);
+    matvec(Td, n, ud.data(), p.data() + n, true);
+    matvec(Tdd, n, u.data() + n, p2.data(), false);
+    matvec(Tdd, n, u.data(), p2.data() + n, true);
+    std::vector<cd> rhs(m2);
+    for (int i = 0; i < m2; ++i) rhs[i] = p2[i] + 2.0 * (p[i] - t * ud[i]);
+    lu_solve(fb, rhs.data(), 1, false);
+    out[2] = -rhs[m - 1].real();
+}
+
+// ------------------------------------------------------------------------
+// SolutionsOperator (gen_sol_op.cpp:23-117 and 119-196, intended semantics:
+// T, T', T'' each from their own assemble_operator call).
+// ------------------------------------------------------------------------
+struct SolOp {
+    Geo g;
+    Quad Q;
+    int order, flags;
+    int N;
+    std::vector<double> M;     // N x N
+    std::vector<double> F;     // 2N x 2N
+    std::vector<int> perm;
+    LU Flu;                    // LU of F (as complex) for projection()
+};
+
+static void solop_init(SolOp& S, bool proj) {
+    S.N = S.g.N;
+    S.M.resize((size_t)S.N * S.N);
+    mass_matrix(S.g, S.order, S.M.data());
+    if (proj) {
+        const int n = 2 * S.N;
+        std::vector<double> A((size_t)n * n, 0.0);
+        for (int j = 0; j < S.N; ++j)
+            for (int i = 0; i < S.N; ++i) {
+                A[i + (size_t)j * n] = S.M[i + (size_t)j * S.N];
+                A[(S.N + i) + (size_t)(S.N + j) * n] = S.M[j + (size_t)i * S.N];
+            }
+        S.F.resize((size_t)n * n);
+        S.perm.resize(n);
+        ldlt_factor(A.data(), n, S.F.data(), S.perm.data());
+        std::vector<cd> Fc(S.F.begin(), S.F.end());
+        lu_factor(Fc.data(), n, S.Flu);
+    }
+}
+
+// T = F^{-1} T F^{-T} (gen_sol_op.cpp:45-48)
+static void projection(const SolOp& S, cd* T) {
+    const int n = 2 * S.N;
+    std::vector<cd> X(T, T + (size_t)n * n);
+    lu_solve(S.Flu, X.data(), n, false);
+    std::vector<cd> Xt((size_t)n * n);
+    for (int j = 0; j < n; ++j) for (int i = 0; i < n; ++i) Xt[j + (size_t)i * n] = X[i + (size_t)j * n];
+    lu_solve(S.Flu, Xt.data(), n, false);
+    for (int j = 0; j < n; ++j) for (int i = 0; i < n; ++i) T[i + (size_t)j * n] = Xt[j + (size_t)i * n];
+}
+
+// assemble_operator (gen_sol_op.cpp:59-74)
+static void assemble_operator(const SolOp& S, cd* T, bool mass, bool proj, const cd* Ki, const cd* Wi, const cd* Vi,
+                              const cd* Ko, const cd* Wo, const cd* Vo) {
+    const int N = S.N, n = 2 * N;
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < N; ++i) {
+            size_t e = i + (size_t)j * N;
+            T[i + (size_t)j * n] = -Ko[e] + Ki[e];
+            T[(N + i) + (size_t)j * n] = Wo[e] - Wi[e];
+            T[i + (size_t)(N + j) * n] = Vo[e] - Vi[e];
+            T[(N + i) + (size_t)(N + j) * n] = Ko[e] - Ki[e];
+            if (mass) {
+                T[i + (size_t)j * n] += S.M[e];
+                T[(N + i) + (size_t)(N + j) * n] += S.M[j + (size_t)i * N];
+            }
+        }
+    if (proj) projection(S, T);
+}
+
+static void gen_sol_op(const SolOp& S, double k, double c_o, double c_i, int d, bool proj, int nthreads,
+                       cd* T, cd* Td, cd* Tdd) {
+    const int N = S.N;
+    const size_t NN = (size_t)N * N;
+    std::vector<cd> Ki((d + 1) * NN), Wi((d + 1) * NN), Vi((d + 1) * NN);
+    std::vector<cd> Ko((d + 1) * NN), Wo((d + 1) * NN), Vo((d + 1) * NN);
+    assemble(S.g, S.Q, k, c_i, 7, d, S.flags, nthreads, Ki.data(), Wi.data(), Vi.data());
+    assemble(S.g, S.Q, k, c_o, 7, d, S.flags, nthreads, Ko.data(), Wo.data(), Vo.data());
+    cd* outs[3] = {T, Td, Tdd};
+    for (int t = 0; t <= d; ++t)
+        if (outs[t])
+            assemble_operator(S, outs[t], t == 0, proj, Ki.data() + t * NN, Wi.data() + t * NN, Vi.data() + t * NN,
+                              Ko.data() + t * NN, Wo.data() + t * NN, Vo.data() + t * NN);
+}
+
+// ------------------------------------------------------------------------
+// C ABI for ctypes.  All return 0 on success, -1 on error (message via
+// orc_last_error).
+// ------------------------------------------------------------------------
+#define ORC_TRY try {
+#define ORC_CATCH } catch (const std::exception& e) { g_err = e.what(); return -1; } return 0;
+
+extern "C" {
+
+int orc_last_error(char* buf, int len) {
+    if (len > 0) { std::strncpy(buf, g_err.c_str(), len - 1); buf[len - 1] = 0; }
+    return (int)g_err.size();
+}
+
+int orc_gauss_rule(double a, double b, int n, double* x, double* w) {
+    ORC_TRY gl_rule(a, b, n, kEps, x, w); ORC_CATCH
+}
+
+int orc_composite_rule(int n, double* x, double* w) {
+    ORC_TRY composite_rule(n, kEps, x, w); ORC_CATCH
+}
+
+int orc_jy01(const double* x, int n, double* out) {
+    ORC_TRY
+    for (int i = 0; i < n; ++i) jy01(x[i], out[4 * i], out[4 * i + 1], out[4 * i + 2], out[4 * i + 3]);
+    ORC_CATCH
+}
+
+// panel evaluation helper for tests: what = 0 point, 1 swapped, 2 tangent, 3 tangent_sw
+int orc_panel_eval(const double* panels, int npanels, const double* coefs, int ncoef, int i, int what,
+                   const double* s, int n, double* out) {
+    ORC_TRY
+    Geo g = make_geo(panels, npanels, coefs, ncoef);
+    for (int q = 0; q < n; ++q) {
+        cd z = what == 0 ? pan_point(g, i, s[q]) : what == 1 ? pan_swapped(g, i, s[q])
+             : what == 2 ? pan_tangent(g, i, s[q]) : pan_tangent_sw(g, i, s[q]);
+        out[2 * q] = z.real(); out[2 * q + 1] = z.imag();
+    }
+    ORC_CATCH
+}
+
+// pair classification: cls[p] for pairs (i>=j) in triangular order: 0 coinciding, 1 adjacent (swap=false),
+// 2 adjacent (swap=true), 3 general
+int orc_classify(const double* panels, int npanels, const double* coefs, int ncoef, int* cls) {
+    ORC_TRY
+    Geo g = make_geo(panels, npanels, coefs, ncoef);
+    for (int i = 0, p = 0; i < npanels; ++i)
+        for (int j = 0; j <= i; ++j, ++p) {
+            bool sw = false;
+            cls[p] = (i == j) ? 0 : is_adjacent(g, i, j, sw) ? (sw ? 2 : 1) : 3;
+        }
+    ORC_CATCH
+}
+
+// local matrices of pair (i, j): out = 48 complex (K[3][2][2][2], W[3][2][2], V[3][2][2]) as in struct Local
+int orc_pair_local(const double* panels, int npanels, const double* coefs, int ncoef, int order,
+                   double k, double c, int i, int j, int d, int flags, double* out) {
+    ORC_TRY
+    Geo g = make_geo(panels, npanels, coefs, ncoef);
+    Quad Q = make_quad(order);
+    Consts C;
+    C.kr = k; C.c = c; C.sqrtc = std::sqrt(c); C.kap = k * C.sqrtc; C.kap_inv = 1.0 / C.kap; C.kap2h = C.kap * C.kap * 0.5;
+    Local L;
+    pair_local(g, Q, C, i, j, 7, d, flags, L);
+    std::memcpy(out, &L, sizeof(L));
+    ORC_CATCH
+}
+
+int orc_assemble(const double* panels, int npanels, const double* coefs, int ncoef, int order,
+                 double k, double c, int layers, int d, int flags, int nthreads,
+                 double* K, double* W, double* V) {
+    ORC_TRY
+    if (d > 2) throw std::runtime_error("Invalid order of derivative");
+    Geo g = make_geo(panels, npanels, coefs, ncoef);
+    Quad Q = make_quad(order);
+    assemble(g, Q, k, c, layers, d, flags, nthreads, (cd*)K, (cd*)W, (cd*)V);
+    ORC_CATCH
+}
+
+int orc_mass_matrix(const double* panels, int npanels, const double* coefs, int ncoef, int order, double* M) {
+    ORC_TRY
+    Geo g = make_geo(panels, npanels, coefs, ncoef);
+    mass_matrix(g, order, M);
+    ORC_CATCH
+}
+
+int orc_ldlt_factor(const double* A, int n, double* F, int* perm) {
+    ORC_TRY ldlt_factor(A, n, F, perm); ORC_CATCH
+}
+
+struct OrcSolOp { SolOp S; bool proj; };
+
+void* orc_solop_create(const double* panels, int npanels, const double* coefs, int ncoef, int order, int flags, int proj) {
+    try {
+        OrcSolOp* h = new OrcSolOp;
+        h->S.g = make_geo(panels, npanels, coefs, ncoef);
+        h->S.Q = make_quad(order);
+        h->S.order = order;
+        h->S.flags = flags;
+        h->proj = proj != 0;
+        solop_init(h->S, h->proj);
+        return h;
+    } catch (const std::exception& e) { g_err = e.what(); return nullptr; }
+}
+
+void orc_solop_destroy(void* h) { delete (OrcSolOp*)h; }
+
+int orc_solop_factor(void* h, double* F, int* perm) {
+    ORC_TRY
+    OrcSolOp* s = (OrcSolOp*)h;
+    if (!s->proj) throw std::runtime_error("no projection");
+    size_t n = 2 * s->S.N;
+    if (F) std::memcpy(F, s->S.F.data(), sizeof(double) * n * n);
+    if (perm) std::memcpy(perm, s->S.perm.data(), sizeof(int) * n);
+    ORC_CATCH
+}
+
+int orc_solop_mass(void* h, double* M) {
+    ORC_TRY
+    OrcSolOp* s = (OrcSolOp*)h;
+    std::memcpy(M, s->S.M.data(), sizeof(double) * s->S.N * s->S.N);
+    ORC_CATCH
+}
+
+int orc_gen_sol_op(void* h, double k, double c_o, double c_i, int d, int nthreads, double* T, double* Td, double* Tdd) {
+    ORC_TRY
+    OrcSolOp* s = (OrcSolOp*)h;
+    if (c_o < 1.0 || c_i < 1.0) throw std::runtime_error("Refraction index must not be smaller than 1");
+    gen_sol_op(s->S, k, c_o, c_i, d, s->proj, nthreads, (cd*)T, (cd*)Td, (cd*)Tdd);
+    ORC_CATCH
+}
+
+int orc_projection(void* h, double* T) {
+    ORC_TRY projection(((OrcSolOp*)h)->S, (cd*)T); ORC_CATCH
+}
+
+int orc_sv(const double* T, int n, const double* R, int nc, int q, double* sigma) {
+    ORC_TRY
+    LU f;
+    lu_factor((const cd*)T, n, f);
+    *sigma = rsvd_sv(f, (const cd*)R, n, nc, q);
+    ORC_CATCH
+}
+
+int orc_sv_der(const double* T, const double* Td, int n, const double* R, int nc, int q, double* out) {
+    ORC_TRY rsvd_sv_der((const cd*)T, (const cd*)Td, (const cd*)R, n, nc, q, out); ORC_CATCH
+}
+
+int orc_sv_der2(const double* T, const double* Td, const double* Tdd, int n, const double* R, int nc, int q, double* out) {
+    ORC_TRY rsvd_sv_der2((const cd*)T, (const cd*)Td, (const cd*)Tdd, (const cd*)R, n, nc, q, out); ORC_CATCH
+}
+
+int orc_svd_values(const double* B, int n, int m, double* sv) {
+    ORC_TRY jacobi_sv((const cd*)B, n, m, sv); ORC_CATCH
+}
+
+// sigma(k) sweep: for each k, gen_sol_op (d=0, projected) then sv.  Parallel
+// over k (the reference drivers' outer std::transform(par_unseq), e.g.
+// roots_brent_square_rsvd.cpp:167-171), single-threaded per k.
+int orc_sigma_sweep(void* h, const double* ks, int nk, double c_o, double c_i, const double* R, int nc, int q,
+                    int nthreads, double* sigma) {
+    ORC_TRY
+    OrcSolOp* s = (OrcSolOp*)h;
+    const int n = 2 * s->S.N;
+    std::vector<std::string> errs(nk);
+    parallel_for(nk, nthreads, [&](int i) {
+        try {
+            std::vector<cd> T((size_t)n * n);
+            gen_sol_op(s->S, ks[i], c_o, c_i, 0, s->proj, 1, T.data(), nullptr, nullptr);
+            LU f;
+            lu_factor(T.data(), n, f);
+            sigma[i] = rsvd_sv(f, (const cd*)R, n, nc, q);
+        } catch (const std::exception& e) { errs[i] = e.what(); }
+    });
+    for (auto& e : errs) if (!e.empty()) throw std::runtime_error(e);
+    ORC_CATCH
+}
+
+}  // extern "C"
