@@ -1,0 +1,203 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle.
+
+Bars (stated per assertion): operator matrices <= 1e-12 relative Frobenius,
+sigma_min <= 1e-10 relative with the same test matrix R; derivative
+estimates at converged subspace iteration <= 1e-8.  Full-size (400-panel)
+checks use size-independent properties.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MAT_TOL = 1e-12
+SIG_TOL = 1e-10
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+CASES = {
+    # square of side 0.5 centred at the origin, 12 line panels per side, order 11
+    "square48_o11": lambda H: (H.square_mesh(0.5, 12), 11),
+    "square_odd_N": lambda H: (H.polygon_mesh([(0, 0), (1, 0), (0.3, 0.8)], 13), 7),
+    "circle_arcs": lambda H: (H.circle_mesh(16, 0.7, (0.2, -0.1)), 6),
+    "star_trig": lambda H: (H.star_mesh(40), 8),
+    "random_polygon": lambda H: (H.random_polygon_mesh(2, 9, 60), 12),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def case(request, H, O):
+    mesh, order = CASES[request.param](H)
+    sp = H.ContinuousSpace1()
+    bd = H.BuilderData(mesh, sp, sp, order)
+    return request.param, mesh, order, bd
+
+
+def test_assemble_dense_parity(case, H, O):
+    name, mesh, order, bd = case
+    gb = H.GalerkinBuilder(bd)
+    for (k, c) in ((2.0, 20.0), (4.5, 1.0)):
+        gb.assembleDense(k, c, H.LayerType.ALL, 2)
+        K, W, V = O.assemble(mesh, order, k, c, d=2)
+        for t in range(3):
+            assert rel(gb.getDoubleLayer(t), K[t]) <= MAT_TOL, (name, k, c, "K", t)
+            assert rel(gb.getHypersingular(t), W[t]) <= MAT_TOL, (name, k, c, "W", t)
+            assert rel(gb.getSingleLayer(t), V[t]) <= MAT_TOL, (name, k, c, "V", t)
+
+
+def test_layer_subsets_and_orders(case, H, O):
+    name, mesh, order, bd = case
+    gb = H.GalerkinBuilder(bd)
+    gb.assembleDense(3.0, 2.0, H.LayerType.SINGLE | H.LayerType.HYPERSINGULAR, 1)
+    K, W, V = O.assemble(mesh, order, 3.0, 2.0, d=1)
+    assert rel(gb.getSingleLayer(1), V[1]) <= MAT_TOL
+    assert rel(gb.getHypersingular(0), W[0]) <= MAT_TOL
+    with pytest.raises(RuntimeError, match="Requested layer is not computed"):
+        gb.getDoubleLayer(0)
+    with pytest.raises(RuntimeError, match="Invalid order of derivative"):
+        gb.getSingleLayer(3)
+
+
+def test_gen_sol_op_parity(case, H, O):
+    name, mesh, order, bd = case
+    for proj in (True, False):
+        so = H.SolutionsOperator(bd, proj)
+        T, Td, Tdd = so.gen_sol_op_2nd_der(2.0, 1.0, 20.0)
+        S = O.SolOp(mesh, order, proj=proj)
+        To = S.gen_sol_op(2.0, 1.0, 20.0, d=2)
+        for a, b, lab in zip((T, Td, Tdd), To, ("T", "T'", "T''")):
+            assert rel(a, b) <= MAT_TOL, (name, proj, lab, rel(a, b))
+        T0 = so.gen_sol_op(2.0, 1.0, 20.0)
+        assert np.array_equal(T0, T)
+
+
+def test_mass_and_projection_factor_identical(case, H, O):
+    name, mesh, order, bd = case
+    so = H.SolutionsOperator(bd)
+    S = O.SolOp(mesh, order)
+    assert np.array_equal(so.mass_matrix(), S.mass_matrix())
+    F, tr = so.projection_factor()
+    Fo, tro = S.factor()
+    assert np.array_equal(tr, tro)
+    assert np.abs(F - Fo).max() <= 1e-15 * np.abs(Fo).max()
+    T = S.gen_sol_op(1.5, 1.0, 4.0, d=0)[0]
+    assert rel(so.projection(T), S.projection(T)) <= MAT_TOL
+
+
+def test_rsvd_parity(case, H, O):
+    name, mesh, order, bd = case
+    S = O.SolOp(mesh, order)
+    T, Td, Tdd = S.gen_sol_op(2.0, 1.0, 20.0, d=2)
+    n = T.shape[0]
+    for nc, q in ((2, 2), (1, 0), (4, 3)):
+        R = H.gaussian_test_matrix(n, nc, 7 + nc)
+        s = H.randomized_svd.sv(T, R, q)
+        so = O.sv(T, R, q)
+        assert abs(s - so) <= SIG_TOL * so, (name, nc, q, s, so)
+    # derivatives: at converged subspace iteration the singular pair is unique
+    R = H.gaussian_test_matrix(n, 2, 3)
+    d1 = H.randomized_svd.sv_der(T, Td, R, 30)
+    d1o = O.sv_der(T, Td, R, 30)
+    assert abs(d1[0] - d1o[0]) <= SIG_TOL * d1o[0]
+    assert abs(d1[1] - d1o[1]) <= 1e-8 * max(1.0, abs(d1o[1])), (name, d1, d1o)
+    d2 = H.randomized_svd.sv_der2(T, Td, Tdd, R, 30)
+    d2o = O.sv_der2(T, Td, Tdd, R, 30)
+    assert abs(d2[1] - d2o[1]) <= 1e-8 * max(1.0, abs(d2o[1]))
+    assert abs(d2[2] - d2o[2]) <= 1e-6 * max(1.0, abs(d2o[2])), (name, d2, d2o)
+
+
+def test_sigma_sweep_parity(case, H, O):
+    name, mesh, order, bd = case
+    n = 2 * mesh.getNumPanels()
+    R = H.gaussian_test_matrix(n, 2, 0)
+    ks = np.linspace(1.0, 4.0, 7)
+    sig, dsig = H.sigma_sweep(bd, ks, 1.0, 20.0, R, 2, der=2)
+    so = O.SolOp(mesh, order).sigma_sweep(ks, 1.0, 20.0, R, 2)
+    assert np.max(np.abs(sig - so) / so) <= SIG_TOL, (name, sig, so)
+    sig1, dsig1 = H.sigma_sweep(bd, ks, 1.0, 20.0, R, 2, der=1)
+    assert np.array_equal(sig1, sig) and np.allclose(dsig1, dsig, rtol=1e-12, atol=0)
+    sig0 = H.sigma_sweep(bd, ks, 1.0, 20.0, R, 2)
+    assert np.array_equal(sig0, sig)
+    # dsigma/dk against a converged-subspace oracle evaluation at one k
+    S = O.SolOp(mesh, order)
+    T, Td = S.gen_sol_op(ks[3], 1.0, 20.0, d=1)
+    so_d = O.sv_der(T, Td, R, 40)
+    sig_c, dsig_c = H.sigma_sweep(bd, ks[3:4], 1.0, 20.0, R, 40, der=1)
+    assert abs(dsig_c[0] - so_d[1]) <= 1e-8 * max(1.0, abs(so_d[1]))
+
+
+def test_config0_sigma_exact(H, O):
+    """48-panel square, k=2, c_i=20, c_o=1, order 11, 2 random
+    columns, 2 power iterations: sigma_min vs the oracle with the same R."""
+    mesh = H.square_mesh(0.5, 12)
+    sp = H.ContinuousSpace1()
+    bd = H.BuilderData(mesh, sp, sp, 11)
+    R = H.gaussian_test_matrix(96, 2, 0)
+    s = H.sigma_sweep(bd, np.array([2.0]), 1.0, 20.0, R, 2)[0]
+    T = O.SolOp(mesh, 11).gen_sol_op(2.0, 1.0, 20.0, d=0)[0]
+    assert abs(s - O.sv(T, R, 2)) <= SIG_TOL * s
+    assert s >= np.linalg.svd(T, compute_uv=False)[-1] * (1 - 1e-12)
+
+
+def test_quirk_flag_parity(H, O):
+    mesh = H.star_mesh(24)
+    sp = H.ContinuousSpace1()
+    bd = H.BuilderData(mesh, sp, sp, 8, flags=H.FLAG_QUIRK_SLP_COINC_DER)
+    gb = H.GalerkinBuilder(bd)
+    gb.assembleDense(2.0, 3.0, H.LayerType.ALL, 2)
+    K, W, V = O.assemble(mesh, 8, 2.0, 3.0, d=2, flags=O.QUIRK_SLP_COINC_DER)
+    for t in range(3):
+        assert rel(gb.getSingleLayer(t), V[t]) <= MAT_TOL
+
+
+def test_error_paths(H):
+    mesh = H.square_mesh(0.5, 3)
+    sp = H.ContinuousSpace1()
+    bd = H.BuilderData(mesh, sp, sp, 6)
+    gb = H.GalerkinBuilder(bd)
+    with pytest.raises(H.HbemError, match="Refraction index must not be smaller than 1"):
+        gb.assembleDense(1.0, 0.5)
+    with pytest.raises(H.HbemError, match=r"Im\(k\) >= 0"):
+        gb.assembleDense(1.0 - 0.5j, 2.0)
+    with pytest.raises(H.HbemError) as e:
+        gb.assembleDense(1.0 + 0.5j, 2.0)
+    assert e.value.code == -2
+    with pytest.raises(H.HbemError, match="Invalid order of derivative"):
+        gb.assembleDense(1.0, 2.0, H.LayerType.ALL, 3)
+    R = H.gaussian_test_matrix(24, 2, 0)
+    with pytest.raises(H.HbemError):
+        H.randomized_svd.sv(np.eye(24), R[:, :0].copy(), 2)
+
+
+@pytest.fixture(scope="module")
+def big(H):
+    mesh = H.star_mesh(400)
+    sp = H.ContinuousSpace1()
+    return mesh, H.BuilderData(mesh, sp, sp, 16)
+
+
+def test_full_size_properties(big, H):
+    """400-panel star at order 16 (the benchmark geometry): exact symmetry of V
+    and W, the batched sweep equals the per-k API path, sigma is an upper bound
+    of sigma_min, and repeated sweeps are bitwise reproducible."""
+    mesh, bd = big
+    gb = H.GalerkinBuilder(bd)
+    gb.assembleDense(7.0, 16.0, H.LayerType.ALL, 1)
+    for t in range(2):
+        assert np.array_equal(gb.getSingleLayer(t), gb.getSingleLayer(t).T)
+        assert np.array_equal(gb.getHypersingular(t), gb.getHypersingular(t).T)
+    R = H.gaussian_test_matrix(800, 2, 1)
+    ks = np.array([1.0, 5.5, 10.0])
+    sig, dsig = H.sigma_sweep(bd, ks, 1.0, 16.0, R, 2, der=2)
+    sig2, dsig2 = H.sigma_sweep(bd, ks, 1.0, 16.0, R, 2, der=2)
+    assert np.array_equal(sig, sig2) and np.array_equal(dsig, dsig2)
+    so = H.SolutionsOperator(bd)
+    for i, k in enumerate(ks):
+        T = so.gen_sol_op(k, 1.0, 16.0)
+        s_api = H.randomized_svd.sv(T, R, 2)
+        assert abs(s_api - sig[i]) <= SIG_TOL * sig[i]
+        smin = np.linalg.svd(T, compute_uv=False)[-1]
+        assert sig[i] >= smin * (1 - 1e-10)
