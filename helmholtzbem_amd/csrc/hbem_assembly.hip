@@ -13,10 +13,17 @@
 //     blocks that touch it, in a fixed order -- deterministic, no atomics --
 //     and applies the factor i/4 (galerkin_builder.cpp:1014-1017).
 //
-// Work item = (pair, wavenumber).  G lanes of a wavefront cooperate on one
-// item, striding over its quadrature nodes, and reduce with cross-lane
-// shuffles; 64/G items share a wavefront.  G = 1 (one lane per wavenumber,
-// no reduction) serves batched k-sweeps, G = 16/64 single wavenumbers.
+// Geometry is precomputed once per mesh into per-pair node tables
+// {r, r^2, v.N_j |tau_i|, -v.N_i |tau_j|, tau_i.tau_j, |tau_i||tau_j|}, so the
+// per-wavenumber kernels only evaluate the Hankel functions and accumulate.
+//
+// Work item = (pair, node chunk, wavenumber).  G lanes of a wavefront
+// cooperate on one item, striding over its nodes, and reduce with cross-lane
+// shuffles; 64/G items share a wavefront.  G = 1 (one lane per wavenumber, no
+// reduction; the pair and chunk are then wavefront-uniform) serves batched
+// k-sweeps, G = 16/64 single wavenumbers.  The singular classes split their
+// Nc^2 / 2 Nc^2 nodes into chunks whose partial local matrices are summed in
+// chunk order by k_chunk_sum.
 //
 // Local slot layout per derivative order t (16 complex):
 //   [0..3] K direct (test on panel i, trial on j), [4..7] K transposed block,
@@ -44,28 +51,39 @@ __global__ void k_setup_gauss_table(const double* __restrict__ panels, const dou
     o[0] = z.x; o[1] = z.y; o[2] = tau.x; o[3] = tau.y; o[4] = hypot(tau.x, tau.y);
 }
 
-// General pairs: skip flags (galerkin_builder.cpp:470-495, 852-870).
-// bit0: all |v.N_j| < eps (direct K skipped), bit1: transposed, bit2: all |tau_i.tau_j| < eps.
-__global__ void k_setup_general_flags(const int2* __restrict__ pairs, int npairs, const double* __restrict__ gtab,
-                                      int ns, unsigned char* __restrict__ flags) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= npairs) return;
-    int2 ij = pairs[e];
-    bool zd = true, zt = true, tz = true;
-    for (int a = 0; a < ns; ++a) {
-        const double* gi = gtab + ((size_t)ij.x * ns + a) * 5;
-        for (int b = 0; b < ns; ++b) {
-            const double* gj = gtab + ((size_t)ij.y * ns + b) * 5;
-            double vx = gi[0] - gj[0], vy = gi[1] - gj[1];
-            double vd = vx * gj[3] - vy * gj[2];
-            double vt = vy * gi[2] - vx * gi[3];
-            double td = gi[3] * gj[3] + gi[2] * gj[2];
-            zd = zd && (fabs(vd) < kDblEps);
-            zt = zt && (fabs(vt) < kDblEps);
-            tz = tz && (fabs(td) < kDblEps);
-        }
+// General pairs: node geometry on the Ns x Ns tensor grid (galerkin_builder.cpp:
+// 308-327, 461-541) and the skip flags of the reference's isArrayZero tests
+// (470-495, 852-870): bit0 all |v.N_j| < eps (direct K skipped), bit1 the
+// transposed block, bit2 all |tau_i.tau_j| < eps.  One workgroup per pair.
+__global__ void k_setup_general(const int* __restrict__ ent, int nent, const double* __restrict__ gtab, int ns,
+                                double* __restrict__ ngeo, unsigned char* __restrict__ flags) {
+    const int e = blockIdx.x;
+    if (e >= nent) return;
+    const int i = ent[3 * e], j = ent[3 * e + 1];
+    const int nn = ns * ns;
+    __shared__ int nz[3];
+    if (threadIdx.x < 3) nz[threadIdx.x] = 0;
+    __syncthreads();
+    int loc[3] = {0, 0, 0};
+    for (int nd = threadIdx.x; nd < nn; nd += blockDim.x) {
+        const int a = nd / ns, b = nd % ns;
+        const double* gi = gtab + ((size_t)i * ns + a) * 5;
+        const double* gj = gtab + ((size_t)j * ns + b) * 5;
+        const double vx = gi[0] - gj[0], vy = gi[1] - gj[1];
+        const double r2 = vx * vx + vy * vy;
+        const double vd = vx * gj[3] - vy * gj[2];
+        const double vt = vy * gi[2] - vx * gi[3];
+        const double td = gi[3] * gj[3] + gi[2] * gj[2];
+        double* o = ngeo + ((size_t)e * nn + nd) * 6;
+        o[0] = sqrt(r2); o[1] = r2; o[2] = vd * gi[4]; o[3] = vt * gj[4]; o[4] = td; o[5] = gi[4] * gj[4];
+        if (!(fabs(vd) < kDblEps)) loc[0] = 1;
+        if (!(fabs(vt) < kDblEps)) loc[1] = 1;
+        if (!(fabs(td) < kDblEps)) loc[2] = 1;
     }
-    flags[e] = (zd ? 1 : 0) | (zt ? 2 : 0) | (tz ? 4 : 0);
+    for (int q = 0; q < 3; ++q)
+        if (loc[q]) atomicOr(&nz[q], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) flags[e] = (nz[0] ? 0 : 1) | (nz[1] ? 0 : 2) | (nz[2] ? 0 : 4);
 }
 
 // Singular pairs: node geometry {r, r2, vd, vt, tdt, ttp} per node and skip flags.
@@ -75,9 +93,8 @@ __global__ void k_setup_general_flags(const int2* __restrict__ pairs, int npairs
 // (galerkin_builder.cpp:267-306, 406-426).
 // One workgroup per pair.
 __global__ void k_setup_singular(int cls, const int* __restrict__ ent, int nent, const double* __restrict__ panels,
-                                 const double* __restrict__ coefs, const double* __restrict__ xc,
-                                 const double* __restrict__ wc, int nc, double* __restrict__ ngeo,
-                                 unsigned char* __restrict__ flags) {
+                                 const double* __restrict__ coefs, const double* __restrict__ xc, int nc,
+                                 double* __restrict__ ngeo, unsigned char* __restrict__ flags) {
     const int e = blockIdx.x;
     if (e >= nent) return;
     const int i = ent[3 * e], j = ent[3 * e + 1];
@@ -184,13 +201,14 @@ __device__ __forceinline__ void acc_reduce(Acc<D>& A) {
 // factor v.N_j |tau_i|, 0 when skipped), ct (transposed factor, 0 when
 // skipped), bt (2 tau_i.tau_j, 0 when tdottp is zero), ttp (|tau_i||tau_j|).
 // Per lane: kappa.  `sym` selects the coinciding form, where the two
-// triangles of [0,1]^2 share the node (roles of s and t exchanged).
+// triangles of [0,1]^2 share the node (roles of s and t exchanged; cd2 is the
+// second triangle's K factor) and only the entries m >= n of W and V are kept.
 template <int D>
 __device__ __forceinline__ void node_accum(Acc<D>& A, double kap, double r, double r2, double w, const double bi[2],
-                                           const double bj[2], double cd, double ct, double bt, double ttp,
-                                           const SeriesTables& ST, bool sym, double cd2) {
+                                           const double bj[2], double cd, double ct, double bt, double ttp, bool sym,
+                                           double cd2) {
     double j0, y0, j1, y1;
-    bessel_jy01(kap * r, ST, j0, y0, j1, y1);
+    bessel_jy01(kap * r, j0, y0, j1, y1);
     const cdbl h0 = mk(j0, y0);
     const cdbl hr = mk(j1 * r, y1 * r);
     const double rinv = r > 0.0 ? 1.0 / r : 0.0;
@@ -219,21 +237,19 @@ __device__ __forceinline__ void node_accum(Acc<D>& A, double kap, double r, doub
             cb = wbt * pbb[q];
             cg = wtt * pbb[q];
         }
+        const bool wv = !sym || q != 2;      // coinciding: entry (0,1) of W, V is copied afterwards
         axpy(A.kd[0][q], ckd, h1r);
         if (!sym) axpy(A.kt[0][q], ckt, h1r);
-        axpy(A.wb[0][q], cb, h0);
-        axpy(A.v[0][q], cg, h0);
+        if (wv) { axpy(A.wb[0][q], cb, h0); axpy(A.v[0][q], cg, h0); }
         if (D >= 1) {
             axpy(A.kd[1][q], ckd, h0);
             if (!sym) axpy(A.kt[1][q], ckt, h0);
-            axpy(A.wb[1][q], cb, hr);
-            axpy(A.v[1][q], cg, hr);
+            if (wv) { axpy(A.wb[1][q], cb, hr); axpy(A.v[1][q], cg, hr); }
         }
         if (D >= 2) {
             axpy(A.kd[2][q], ckd, hr);
             if (!sym) axpy(A.kt[2][q], ckt, hr);
-            axpy(A.wb[2][q], cb, h0r2);
-            axpy(A.v[2][q], cg, h0r2);
+            if (wv) { axpy(A.wb[2][q], cb, h0r2); axpy(A.v[2][q], cg, h0r2); }
         }
     }
     const double wa = sym ? 2.0 * w : w;
@@ -244,6 +260,8 @@ __device__ __forceinline__ void node_accum(Acc<D>& A, double kap, double r, doub
 
 // Accumulators -> the reference's local matrices (scaled as at the end of
 // double_layer_*, hypersingular_*, single_layer_*; before the factor i/4).
+// Linear in the accumulators, so partial sums over node chunks may be summed
+// afterwards.
 template <int D>
 __device__ __forceinline__ void acc_store(const Acc<D>& A, double kap, double kr, double c, bool coinc, int qflags,
                                           cdbl* __restrict__ out) {
@@ -286,79 +304,100 @@ __device__ __forceinline__ void acc_store(const Acc<D>& A, double kap, double kr
     }
 }
 
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // cls: 0 coinciding, 1 adjacent, 2 general
 template <int CLS, int D, int G>
 __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
     const int lane = threadIdx.x & 63;
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int g = lane % G;
-    const long long nitems = (long long)P.nent * P.nk;
+    const long long nitems = (long long)P.nent * P.nchunk * P.nk;
     long long item = wave * (64 / G) + lane / G;
     const bool active = item < nitems;
     if (!active) item = nitems - 1;
-    const int e = (int)(item / P.nk), q = (int)(item % P.nk);
+    const int q = (int)(item % P.nk);
+    const long long rest = item / P.nk;
+    int chunk = (int)(rest % P.nchunk);
+    int e = (int)(rest / P.nchunk);
+    if (G == 1 && P.wave_uniform) {
+        // nk is a multiple of 64: the wavefront shares (pair, chunk)
+        e = wave_uniform(e);
+        chunk = wave_uniform(chunk);
+    }
     const double kap = P.kap[q];
     const unsigned f = P.flags[e];
+    const int nn = CLS == 2 ? P.ns * P.ns : (CLS == 1 ? 2 * P.nc * P.nc : P.nc * P.nc);
+    const int lo = chunk * P.chunk;
+    const int hi = min(nn, lo + P.chunk);
+    const double* __restrict__ geo = P.ngeo + (size_t)e * nn * 6;
 
     Acc<D> A;
     acc_zero<D>(A);
 
     if (CLS == 2) {
-        const int2 ij = P.pairs[e];
         const double fd = (f & 1) ? 0.0 : 1.0, ft = (f & 2) ? 0.0 : 1.0, fb = (f & 4) ? 0.0 : 2.0;
-        const int ns = P.ns, nn = ns * ns;
-        for (int nd = g; nd < nn; nd += G) {
+        const int ns = P.ns;
+        for (int nd = lo + g; nd < hi; nd += G) {
             const int a = nd / ns, b = nd - a * ns;
-            const double* gi = P.gtab + ((size_t)ij.x * ns + a) * 5;
-            const double* gj = P.gtab + ((size_t)ij.y * ns + b) * 5;
+            const double* o = geo + (size_t)nd * 6;
             const double s = P.xg[a], t = P.xg[b];
             const double w = P.wg[a] * P.wg[b];
-            const double vx = gi[0] - gj[0], vy = gi[1] - gj[1];
-            const double r2 = vx * vx + vy * vy, r = sqrt(r2);
-            const double cdv = fd * ((vx * gj[3] - vy * gj[2]) * gi[4]);
-            const double ctv = ft * ((vy * gi[2] - vx * gi[3]) * gj[4]);
-            const double bt = fb * (gi[3] * gj[3] + gi[2] * gj[2]);
             const double bi[2] = {s, 1.0 - s}, bj[2] = {t, 1.0 - t};
-            node_accum<D>(A, kap, r, r2, w, bi, bj, cdv, ctv, bt, gi[4] * gj[4], P.st, false, 0.0);
+            node_accum<D>(A, kap, o[0], o[1], w, bi, bj, fd * o[2], ft * o[3], fb * o[4], o[5], false, 0.0);
         }
     } else if (CLS == 1) {
         const bool swap = P.ent[3 * e + 2] != 0;
-        const int nc = P.nc, nn = nc * nc;
-        for (int nd = g; nd < 2 * nn; nd += G) {
-            const int h = nd >= nn ? 1 : 0, gg = nd - h * nn, a = gg / nc, b = gg - a * nc;
+        const int nc = P.nc, nh = nc * nc;
+        for (int nd = lo + g; nd < hi; nd += G) {
+            const int h = nd >= nh ? 1 : 0, gg = nd - h * nh, a = gg / nc, b = gg - a * nc;
             const double t = P.xc[a], s = t * P.xc[b];
             const double w = t * P.wc[a] * P.wc[b];
             const double si = h == 0 ? s : t, sj = h == 0 ? t : s;
             // test panel i: forward when start(i) is the shared vertex (swap), else reversed; trial j opposite
             const double bi[2] = {swap ? si : 1.0 - si, swap ? 1.0 - si : si};
             const double bj[2] = {swap ? 1.0 - sj : sj, swap ? sj : 1.0 - sj};
-            const double* o = P.ngeo + ((size_t)e * 2 * nn + nd) * 6;
+            const double* o = geo + (size_t)nd * 6;
             const unsigned fh = f >> (3 * h);
             const double cdv = (fh & 1) ? 0.0 : o[2];
             const double ctv = (fh & 2) ? 0.0 : o[3];
             const double bt = (fh & 4) ? 0.0 : 2.0 * o[4];
-            node_accum<D>(A, kap, o[0], o[1], w, bi, bj, cdv, ctv, bt, o[5], P.st, false, 0.0);
+            node_accum<D>(A, kap, o[0], o[1], w, bi, bj, cdv, ctv, bt, o[5], false, 0.0);
         }
     } else {
-        const int nc = P.nc, nn = nc * nc;
-        for (int nd = g; nd < nn; nd += G) {
+        const int nc = P.nc;
+        const double f0 = (f & 1) ? 0.0 : 1.0, f1 = (f & 8) ? 0.0 : 1.0, fb = (f & 4) ? 0.0 : 2.0;
+        for (int nd = lo + g; nd < hi; nd += G) {
             const int a = nd / nc, b = nd - a * nc;
             const double t = P.xc[b], s = t * (1.0 - P.xc[a]);
             const double w = t * P.wc[a] * P.wc[b];
             const double bi[2] = {s, 1.0 - s}, bj[2] = {t, 1.0 - t};
-            const double* o = P.ngeo + ((size_t)e * nn + nd) * 6;
-            const double cd0 = (f & 1) ? 0.0 : o[2];
-            const double cd1 = (f & 8) ? 0.0 : o[3];
-            const double bt = (f & 4) ? 0.0 : 2.0 * o[4];
-            node_accum<D>(A, kap, o[0], o[1], w, bi, bj, cd0, 0.0, bt, o[5], P.st, true, cd1);
+            const double* o = geo + (size_t)nd * 6;
+            node_accum<D>(A, kap, o[0], o[1], w, bi, bj, f0 * o[2], 0.0, fb * o[4], o[5], true, f1 * o[3]);
         }
     }
 
     acc_reduce<D, G>(A);
     if (active && g == 0) {
-        cdbl* out = P.out + ((size_t)q * P.npairs_total + P.pidx[e]) * (16 * (D + 1));
+        cdbl* out = P.nchunk == 1 ? P.out + ((size_t)q * P.npairs_total + P.pidx[e]) * (16 * (D + 1))
+                                  : P.out + (((size_t)q * P.nent + e) * P.nchunk + chunk) * (16 * (D + 1));
         acc_store<D>(A, kap, P.kr[q], P.cv[q], CLS == 0, P.qflags, out);
     }
+}
+
+// Sums the node-chunk partials of one class in chunk order into the pair buffer.
+__global__ void k_chunk_sum(const cdbl* __restrict__ part, int nent, int nchunk, int nk, int nslot,
+                            const long long* __restrict__ pidx, long long npairs, cdbl* __restrict__ L) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)nk * nent * nslot;
+    if (id >= total) return;
+    const int s = (int)(id % nslot);
+    const long long qe = id / nslot;
+    const int e = (int)(qe % nent), q = (int)(qe / nent);
+    const cdbl* p = part + ((size_t)qe * nchunk) * nslot + s;
+    cdbl acc = mk(0, 0);
+    for (int c = 0; c < nchunk; ++c) acc = acc + p[(size_t)c * nslot];
+    L[((size_t)q * npairs + pidx[e]) * nslot + s] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -445,22 +484,21 @@ void launch_setup_gauss_table(const double* panels, const double* coefs, int npa
     hipLaunchKernelGGL(k_setup_gauss_table, dim3((n + 255) / 256), dim3(256), 0, st, panels, coefs, npan, xg, ns, gtab);
 }
 
-void launch_setup_general_flags(const int2* pairs, int npairs, const double* gtab, int ns, unsigned char* flags,
-                                hipStream_t st) {
-    if (npairs <= 0) return;
-    hipLaunchKernelGGL(k_setup_general_flags, dim3((npairs + 255) / 256), dim3(256), 0, st, pairs, npairs, gtab, ns, flags);
+void launch_setup_general(const int* ent, int nent, const double* gtab, int ns, double* ngeo, unsigned char* flags,
+                          hipStream_t st) {
+    if (nent <= 0) return;
+    hipLaunchKernelGGL(k_setup_general, dim3(nent), dim3(256), 0, st, ent, nent, gtab, ns, ngeo, flags);
 }
 
 void launch_setup_singular(int cls, const int* ent, int nent, const double* panels, const double* coefs,
-                           const double* xc, const double* wc, int nc, double* ngeo, unsigned char* flags,
-                           hipStream_t st) {
+                           const double* xc, int nc, double* ngeo, unsigned char* flags, hipStream_t st) {
     if (nent <= 0) return;
-    hipLaunchKernelGGL(k_setup_singular, dim3(nent), dim3(256), 0, st, cls, ent, nent, panels, coefs, xc, wc, nc, ngeo, flags);
+    hipLaunchKernelGGL(k_setup_singular, dim3(nent), dim3(256), 0, st, cls, ent, nent, panels, coefs, xc, nc, ngeo, flags);
 }
 
 template <int CLS, int D>
 static void launch_asm_g(const AsmParams& P, int G, hipStream_t st) {
-    const long long nitems = (long long)P.nent * P.nk;
+    const long long nitems = (long long)P.nent * P.nchunk * P.nk;
     if (nitems <= 0) return;
     const long long waves = (nitems * G + 63) / 64;
     const int blocks = (int)((waves + 3) / 4);
@@ -480,6 +518,14 @@ void launch_assemble(int cls, int D, const AsmParams& P, int G, hipStream_t st) 
     } else {
         if (D == 0) launch_asm_g<2, 0>(P, G, st); else if (D == 1) launch_asm_g<2, 1>(P, G, st); else launch_asm_g<2, 2>(P, G, st);
     }
+}
+
+void launch_chunk_sum(const cdbl* part, int nent, int nchunk, int nk, int nslot, const long long* pidx, long long npairs,
+                      cdbl* L, hipStream_t st) {
+    const long long total = (long long)nk * nent * nslot;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_chunk_sum, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, part, nent, nchunk, nk,
+                       nslot, pidx, npairs, L);
 }
 
 void launch_gather_layers(const cdbl* L, long long npt, int nslot, int q, int N, int D, int layers, cdbl* K, cdbl* W,

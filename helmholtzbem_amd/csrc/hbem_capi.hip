@@ -34,7 +34,7 @@ struct DBuf {
     size_t n = 0;
     void* get(size_t bytes) {
         if (bytes > n) {
-            if (p) hipFree(p);
+            if (p) (void)hipFree(p);
             p = nullptr;
             n = 0;
             if (bytes > 0) {
@@ -45,7 +45,7 @@ struct DBuf {
         return p;
     }
     template <class T> T* as(size_t count) { return (T*)get(count * sizeof(T)); }
-    ~DBuf() { if (p) hipFree(p); }
+    ~DBuf() { if (p) (void)hipFree(p); }
 };
 
 template <class T>
@@ -70,17 +70,16 @@ struct hbem_geometry {
     std::vector<double> panels, coefs, xg, wg, xc, wc;
     std::vector<double> M, F, Finv;
     std::vector<int> tr;
-    hbem::SeriesTables series;
     // device geometry
     DBuf d_panels, d_coefs, d_xg, d_wg, d_xc, d_wc, d_gtab;
-    DBuf d_gpairs, d_gpidx, d_gflags;
+    DBuf d_gent, d_gpidx, d_gflags, d_gngeo;
     DBuf d_aent, d_apidx, d_aflags, d_angeo;
     DBuf d_cent, d_cpidx, d_cflags, d_cngeo;
     int ngen = 0, nadj = 0, ncoi = 0;
     DBuf d_M, d_Frm, d_Fcm, d_Finvcm, d_Finvc;
     // work space
     DBuf w_L, w_T0, w_T1, w_T2, w_piv, w_perm, w_X, w_Y, w_work, w_R, w_u, w_v, w_a, w_b, w_sig, w_dsig, w_kap,
-        w_kr, w_cv, w_qi, w_qo, w_tmp, w_out;
+        w_kr, w_cv, w_qi, w_qo, w_tmp, w_out, w_part;
     // profiling
     bool prof = false;
     double prof_ms[4] = {0, 0, 0, 0};
@@ -121,7 +120,6 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
     cdbl* L = g->w_L.as<cdbl>((size_t)nk * g->npairs * nslot);
     hbem::AsmParams P;
     std::memset(&P, 0, sizeof(P));
-    P.gtab = (const double*)g->d_gtab.p;
     P.xg = (const double*)g->d_xg.p;
     P.wg = (const double*)g->d_wg.p;
     P.xc = (const double*)g->d_xc.p;
@@ -131,30 +129,42 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
     P.kr = upload(g->w_kr, kr, g->st);
     P.cv = upload(g->w_cv, cv, g->st);
     P.npairs_total = g->npairs;
-    P.out = L;
     P.ns = g->ns;
     P.nc = g->nc;
     P.qflags = g->flags;
-    P.st = g->series;
+    // lanes per item: one lane per wavenumber for wide batches, node groups otherwise
     const int Gg = nk >= 64 ? 1 : (nk >= 16 ? 4 : 16);
     const int Gs = nk >= 64 ? 1 : 64;
-    struct Launch { int cls, which, G, nent; const int* ent; const int2* pairs; const double* ngeo; const unsigned char* flags; const long long* pidx; double nodes; };
+    struct Launch {
+        int cls, which, G, nent, nodes;
+        const int* ent;
+        const double* ngeo;
+        const unsigned char* flags;
+        const long long* pidx;
+    };
     Launch ls[3] = {
-        {2, 0, Gg, g->ngen, nullptr, (const int2*)g->d_gpairs.p, nullptr, (const unsigned char*)g->d_gflags.p,
-         (const long long*)g->d_gpidx.p, (double)g->ns * g->ns},
-        {1, 1, Gs, g->nadj, (const int*)g->d_aent.p, nullptr, (const double*)g->d_angeo.p,
-         (const unsigned char*)g->d_aflags.p, (const long long*)g->d_apidx.p, 2.0 * g->nc * g->nc},
-        {0, 2, Gs, g->ncoi, (const int*)g->d_cent.p, nullptr, (const double*)g->d_cngeo.p,
-         (const unsigned char*)g->d_cflags.p, (const long long*)g->d_cpidx.p, (double)g->nc * g->nc},
+        {2, 0, Gg, g->ngen, g->ns * g->ns, (const int*)g->d_gent.p, (const double*)g->d_gngeo.p,
+         (const unsigned char*)g->d_gflags.p, (const long long*)g->d_gpidx.p},
+        {1, 1, Gs, g->nadj, 2 * g->nc * g->nc, (const int*)g->d_aent.p, (const double*)g->d_angeo.p,
+         (const unsigned char*)g->d_aflags.p, (const long long*)g->d_apidx.p},
+        {0, 2, Gs, g->ncoi, g->nc * g->nc, (const int*)g->d_cent.p, (const double*)g->d_cngeo.p,
+         (const unsigned char*)g->d_cflags.p, (const long long*)g->d_cpidx.p},
     };
     for (auto& l : ls) {
         if (l.nent == 0) continue;
         P.nent = l.nent;
         P.ent = l.ent;
-        P.pairs = l.pairs;
         P.ngeo = l.ngeo;
         P.flags = l.flags;
         P.pidx = l.pidx;
+        P.wave_uniform = (l.G == 1 && nk % 64 == 0) ? 1 : 0;
+        // general pairs: one chunk; singular pairs: chunks of ~512 nodes per lane-group
+        const int chunk_nodes = l.cls == 2 ? l.nodes : (l.G == 1 ? 512 : 64 * 48);
+        P.chunk = chunk_nodes;
+        P.nchunk = (l.nodes + chunk_nodes - 1) / chunk_nodes;
+        cdbl* part = nullptr;
+        if (P.nchunk > 1) part = g->w_part.as<cdbl>((size_t)nk * l.nent * P.nchunk * nslot);
+        P.out = P.nchunk > 1 ? part : L;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (g->prof) {
             HIPC(hipEventCreate(&e0));
@@ -163,6 +173,10 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
         }
         hbem::launch_assemble(l.cls, D, P, l.G, g->st);
         HIPC(hipGetLastError());
+        if (P.nchunk > 1) {
+            hbem::launch_chunk_sum(part, l.nent, P.nchunk, nk, nslot, l.pidx, g->npairs, L, g->st);
+            HIPC(hipGetLastError());
+        }
         if (g->prof) {
             HIPC(hipEventRecord(e1, g->st));
             HIPC(hipEventSynchronize(e1));
@@ -171,8 +185,8 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
             g->prof_ms[l.which] += ms;
             g->prof_n[l.which] += 1;
             g->prof_flops[l.which] += (double)l.nent * nk * l.nodes * flops_per_node(D);
-            hipEventDestroy(e0);
-            hipEventDestroy(e1);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
         }
     }
     return L;
@@ -295,26 +309,30 @@ int hbem_geometry_create(const double* panels, int npanels, const double* coefs,
     g->xg.resize(g->ns); g->wg.resize(g->ns); g->xc.resize(g->nc); g->wc.resize(g->nc);
     hbem::gauss_rule(0.0, 1.0, g->ns, g->xg.data(), g->wg.data());
     hbem::composite_rule(order, g->xc.data(), g->wc.data());
-    g->series = hbem::make_series_tables();
     const int N = npanels;
     g->npairs = (long long)N * (N + 1) / 2;
-    // classify pairs (galerkin_builder.cpp:872-897)
-    std::vector<int2> gp;
+    // classify pairs (galerkin_builder.cpp:872-897; adjacency test 189-201)
     std::vector<long long> gpi, api, cpi;
-    std::vector<int> aent, cent;
-    gp.reserve((size_t)g->npairs);
+    std::vector<int> gent, aent, cent;
+    gent.reserve(3 * (size_t)g->npairs);
     gpi.reserve((size_t)g->npairs);
     const double* cptr = g->coefs.empty() ? nullptr : g->coefs.data();
+    std::vector<double> ends(4 * (size_t)N);
+    hbem::panel_endpoints(g->panels.data(), cptr, N, ends.data());
+    const double tol = 100.0 * hbem::kDblEps;
     for (int i = 0; i < N; ++i)
         for (int j = 0; j <= i; ++j) {
             long long p = (long long)i * (i + 1) / 2 + j;
-            bool sw = false;
+            const double* ei = &ends[4 * (size_t)i];
+            const double* ej = &ends[4 * (size_t)j];
             if (i == j) { cent.push_back(i); cent.push_back(i); cent.push_back(0); cpi.push_back(p); }
-            else if (hbem::panels_adjacent(g->panels.data(), cptr, i, j, sw)) {
-                aent.push_back(i); aent.push_back(j); aent.push_back(sw ? 1 : 0); api.push_back(p);
-            } else { gp.push_back(make_int2(i, j)); gpi.push_back(p); }
+            else if (std::hypot(ei[2] - ej[0], ei[3] - ej[1]) < tol) {            // end(i) == start(j)
+                aent.push_back(i); aent.push_back(j); aent.push_back(0); api.push_back(p);
+            } else if (std::hypot(ei[0] - ej[2], ei[1] - ej[3]) < tol) {          // start(i) == end(j)
+                aent.push_back(i); aent.push_back(j); aent.push_back(1); api.push_back(p);
+            } else { gent.push_back(i); gent.push_back(j); gent.push_back(0); gpi.push_back(p); }
         }
-    g->ngen = (int)gp.size();
+    g->ngen = (int)gpi.size();
     g->nadj = (int)api.size();
     g->ncoi = (int)cpi.size();
     hipStream_t st = g->st;
@@ -324,24 +342,25 @@ int hbem_geometry_create(const double* panels, int npanels, const double* coefs,
     const double* dxg = upload(g->d_xg, g->xg, st);
     upload(g->d_wg, g->wg, st);
     const double* dxc = upload(g->d_xc, g->xc, st);
-    const double* dwc = upload(g->d_wc, g->wc, st);
+    upload(g->d_wc, g->wc, st);
     double* gtab = g->d_gtab.as<double>((size_t)N * g->ns * 5);
     hbem::launch_setup_gauss_table(dpan, dcoef, N, dxg, g->ns, gtab, st);
-    upload(g->d_gpairs, gp, st);
+    upload(g->d_gent, gent, st);
     upload(g->d_gpidx, gpi, st);
     g->d_gflags.get(std::max(1, g->ngen));
-    hbem::launch_setup_general_flags((const int2*)g->d_gpairs.p, g->ngen, gtab, g->ns, (unsigned char*)g->d_gflags.p, st);
+    double* gngeo = g->d_gngeo.as<double>(std::max<size_t>(1, (size_t)g->ngen * g->ns * g->ns * 6));
+    hbem::launch_setup_general((const int*)g->d_gent.p, g->ngen, gtab, g->ns, gngeo, (unsigned char*)g->d_gflags.p, st);
     upload(g->d_aent, aent, st);
     upload(g->d_apidx, api, st);
     g->d_aflags.get(std::max(1, g->nadj));
     double* angeo = g->d_angeo.as<double>(std::max<size_t>(1, (size_t)g->nadj * 2 * g->nc * g->nc * 6));
-    hbem::launch_setup_singular(1, (const int*)g->d_aent.p, g->nadj, dpan, dcoef, dxc, dwc, g->nc, angeo,
+    hbem::launch_setup_singular(1, (const int*)g->d_aent.p, g->nadj, dpan, dcoef, dxc, g->nc, angeo,
                                 (unsigned char*)g->d_aflags.p, st);
     upload(g->d_cent, cent, st);
     upload(g->d_cpidx, cpi, st);
     g->d_cflags.get(std::max(1, g->ncoi));
     double* cngeo = g->d_cngeo.as<double>(std::max<size_t>(1, (size_t)g->ncoi * g->nc * g->nc * 6));
-    hbem::launch_setup_singular(0, (const int*)g->d_cent.p, g->ncoi, dpan, dcoef, dxc, dwc, g->nc, cngeo,
+    hbem::launch_setup_singular(0, (const int*)g->d_cent.p, g->ncoi, dpan, dcoef, dxc, g->nc, cngeo,
                                 (unsigned char*)g->d_cflags.p, st);
     HIPC(hipGetLastError());
     // mass matrix and projection factor (gen_sol_op.cpp:23-43)
@@ -375,7 +394,7 @@ int hbem_geometry_create(const double* panels, int npanels, const double* coefs,
     g = nullptr;
     }
     catch (const std::exception& e) {
-        if (g) { if (g->st) hipStreamDestroy(g->st); delete g; }
+        if (g) { if (g->st) (void)hipStreamDestroy(g->st); delete g; }
         return fail(e);
     }
     return HBEM_OK;
@@ -384,10 +403,10 @@ int hbem_geometry_create(const double* panels, int npanels, const double* coefs,
 int hbem_geometry_destroy(hbem_geometry_t g) {
     API_BEGIN
     if (!g) return HBEM_OK;
-    hipStreamSynchronize(g->st);
+    (void)hipStreamSynchronize(g->st);
     hipStream_t st = g->st;
     delete g;
-    hipStreamDestroy(st);
+    (void)hipStreamDestroy(st);
     API_END
 }
 
@@ -676,10 +695,17 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
     HIPC(hipSetDevice(g->dev));
     const int D = want_der ? (want_der >= 2 ? 2 : 1) : 0;
     const size_t nn = (size_t)n * n;
-    // batch size: bounded by the dense working set (T, T', T'' per wavenumber)
-    int B = 32;
-    const size_t per = nn * sizeof(cdbl) * (D + 1) + (size_t)2 * g->npairs * 16 * (D + 1) * sizeof(cdbl);
-    while (B > 1 && (size_t)B * per > ((size_t)48 << 30)) B /= 2;
+    // Batch of B wavenumbers -> 2B slots ordered [c_i for every k | c_o for every k]:
+    // with B = 64 each wavefront of the assembly sees a single refraction index
+    // and 64 consecutive k, so the arguments kappa r of its lanes stay close
+    // (fewer divergent Bessel branches).  B is bounded by the working set:
+    // local matrices, chunk partials and T, T', T'' per wavenumber.
+    int B = 64;
+    const size_t nslot = 16 * (size_t)(D + 1);
+    const size_t maxchunks = (size_t)g->nadj * ((2 * (size_t)g->nc * g->nc + 511) / 512);
+    const size_t per = nn * sizeof(cdbl) * (D + 1) +
+                       (size_t)2 * ((size_t)g->npairs + maxchunks) * nslot * sizeof(cdbl);
+    while (B > 1 && (size_t)B * per > ((size_t)64 << 30)) B /= 2;
     cdbl* dR = g->w_R.as<cdbl>((size_t)n * nc);
     HIPC(hipMemcpyAsync(dR, R, (size_t)n * nc * sizeof(cdbl), hipMemcpyHostToDevice, g->st));
     for (int k0 = 0; k0 < nk; k0 += B) {
@@ -688,10 +714,10 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
         std::vector<int> qi(nb), qo(nb);
         for (int b = 0; b < nb; ++b) {
             const double k = ks[k0 + b];
-            kap[2 * b] = k * std::sqrt(c_i); kr[2 * b] = k; cv[2 * b] = c_i;
-            kap[2 * b + 1] = k * std::sqrt(c_o); kr[2 * b + 1] = k; cv[2 * b + 1] = c_o;
-            qi[b] = 2 * b;
-            qo[b] = 2 * b + 1;
+            kap[b] = k * std::sqrt(c_i); kr[b] = k; cv[b] = c_i;
+            kap[nb + b] = k * std::sqrt(c_o); kr[nb + b] = k; cv[nb + b] = c_o;
+            qi[b] = b;
+            qo[b] = nb + b;
         }
         cdbl* L = assemble_locals(g, kap, kr, cv, D);
         cdbl* T0 = g->w_T0.as<cdbl>(nn * nb);
@@ -713,8 +739,8 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
             g->prof_ms[3] += ms;
             g->prof_n[3] += 1;
             g->prof_flops[3] += (8.0 / 3.0) * (double)n * n * n * nb;
-            hipEventDestroy(e0);
-            hipEventDestroy(e1);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
         }
         DenseCtx c;
         c.st = g->st; c.n = n; c.batch = nb; c.nc = nc; c.q = q; c.LU = T0; c.perm = perm;

@@ -34,12 +34,19 @@ constexpr double kOneOverPi = 0.31830988618379067154;
 constexpr double kEuler = 0.57721566490153286061;
 constexpr double kDblEps = 2.220446049250313080847e-16;
 
-// 1/k^2, 1/(k+1) and harmonic numbers H_k for the ascending series (k <= 40).
-struct SeriesTables {
-    double inv_k2[41];
-    double inv_kp1[41];
-    double harm[41];
-};
+// The 128 coefficients of hankel_tables.inc live in constant memory and are
+// read (scalar loads, uniform addresses) at the point of use.  As immediates
+// they would be loop-invariant, and the compiler keeps loop invariants in
+// registers: the coefficients alone would then take every VGPR of the node
+// loop.  bessel_table() hides the table address behind an empty asm so the
+// loads stay inside the loop.
+static __constant__ double kBesselTable[HBEM_TAB_SIZE] = HBEM_BESSEL_TABLE_INIT;
+
+__device__ __forceinline__ const double* bessel_table() {
+    const double* t = kBesselTable;
+    asm volatile("" : "+s"(t));
+    return t;
+}
 
 __device__ __forceinline__ double horner_pq(const double* c, double z) {
     double v = c[HBEM_PQ_DEG];
@@ -48,42 +55,40 @@ __device__ __forceinline__ double horner_pq(const double* c, double z) {
     return v;
 }
 
-// J0, Y0, J1, Y1 of x.  Branches:
+__device__ __forceinline__ double horner_small(const double* c, double y) {
+    double v = c[HBEM_SMALL_DEG];
+#pragma unroll
+    for (int i = HBEM_SMALL_DEG - 1; i >= 0; --i) v = fma(v, y, c[i]);
+    return v;
+}
+
+// J0, Y0, J1, Y1 of x.  The branch points are the reference's
+// (Hankel1Real01::jy_01, cbessel.cpp:1896-1980):
 //   x <= 0       J0 = 1, J1 = 0, Y0 = Y1 = 0 (the reference zeroes Y there, cbessel.cpp:1976-1980)
-//   x < 1e-8     limiting forms (cbessel.cpp:1771-1776)
-//   x < 8        joint ascending series of J0, J1, Y0, Y1 (A&S 9.1.10-9.1.13),
-//                summed until H_k |t_k| <= eps (cbessel.cpp:1778-1805)
+//   x < 8        ascending-series form (A&S 9.1.10-9.1.13) with its four entire
+//                parts A, B, C, D from the fits of hankel_tables.inc:
+//                J0 = A, J1 = x B, Y0 = (2/pi) ln(x/2) J0 + C,
+//                Y1 = (2/pi) ln(x/2) J1 - 2/(pi x) + x D.  Below 1e-8 this
+//                agrees with the reference's limiting forms to rounding.
 //   x <= 1e6     Hankel asymptotic form with P_nu, x Q_nu from the polynomial
 //                fits of hankel_tables.inc (A&S 9.2.5-9.2.6)
 //   x > 1e6      leading asymptotic term (cbessel.cpp:1847-1856)
-__device__ __forceinline__ void bessel_jy01(double x, const SeriesTables& T, double& j0, double& y0, double& j1,
-                                            double& y1) {
-    if (!(x > 0.0)) {
-        j0 = 1.0; j1 = 0.0; y0 = 0.0; y1 = 0.0;
-    } else if (x < 1e-8) {
-        j0 = 1.0;
-        j1 = 0.5 * x;
-        y0 = kTwoOverPi * (log(j1) + kEuler);
-        y1 = -kTwoOverPi / x;
-    } else if (x < 8.0) {
-        const double mq = -0.25 * x * x;
-        double t = 1.0, sj0 = 1.0, sy0 = 0.0, sj1 = 1.0, sy1 = 0.0;
-        for (int k = 1; k <= 40; ++k) {
-            t = (t * mq) * T.inv_k2[k];
-            const double H = T.harm[k];
-            sj0 += t;
-            sy0 = fma(H, t, sy0);
-            const double tk = t * T.inv_kp1[k];
-            sj1 += tk;
-            sy1 = fma(H, tk, sy1);
-            if (!(H * fabs(t) > kDblEps)) break;
-        }
-        const double L = log(0.5 * x) + kEuler;
-        const double xinv = 1.0 / x;
-        j0 = sj0;
-        j1 = 0.5 * x * sj1;
-        y0 = kTwoOverPi * fma(L, j0, -sy0);
-        y1 = kOneOverPi * fma(-x, sy1, 2.0 * fma(L, j1, (j0 - 2.0) * xinv));
+__device__ __forceinline__ void bessel_jy01(double x, double& j0, double& y0, double& j1, double& y1) {
+    if (x < 8.0) {
+        const double xs = x > 0.0 ? x : 1.0;
+        const double h = 0.125 * xs;
+        const double y = h * h;
+        const double* tab = bessel_table();
+        const double A = horner_small(tab + HBEM_TAB_A, y), B = horner_small(tab + HBEM_TAB_B, y);
+        const double C = horner_small(tab + HBEM_TAB_C, y), D = horner_small(tab + HBEM_TAB_D, y);
+        const double L2 = kTwoOverPi * log(0.5 * xs);
+        const double xinv = 1.0 / xs;
+        const double J1 = xs * B;
+        const bool pos = x > 0.0;
+        j0 = pos ? A : 1.0;
+        j1 = pos ? J1 : 0.0;
+        y0 = pos ? fma(L2, A, C) : 0.0;
+        y1 = pos ? fma(L2, J1, fma(xs, D, -kTwoOverPi * xinv)) : 0.0;
     } else {
         double s, c;
         sincos(x, &s, &c);
@@ -92,8 +97,9 @@ __device__ __forceinline__ void bessel_jy01(double x, const SeriesTables& T, dou
         if (x <= 1e6) {
             const double xinv = 1.0 / x;
             const double z = 64.0 * xinv * xinv;
-            const double P0 = horner_pq(kP0, z), Q0 = horner_pq(kQ0, z) * xinv;
-            const double P1 = horner_pq(kP1, z), Q1 = horner_pq(kQ1, z) * xinv;
+            const double* tab = bessel_table();
+            const double P0 = horner_pq(tab + HBEM_TAB_P0, z), Q0 = horner_pq(tab + HBEM_TAB_Q0, z) * xinv;
+            const double P1 = horner_pq(tab + HBEM_TAB_P1, z), Q1 = horner_pq(tab + HBEM_TAB_Q1, z) * xinv;
             j0 = b * fma(P0, cs, -Q0 * sc);
             y0 = b * fma(P0, sc, Q0 * cs);
             j1 = b * fma(P1, sc, Q1 * cs);

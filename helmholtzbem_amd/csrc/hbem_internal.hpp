@@ -11,37 +11,38 @@
 namespace hbem {
 
 struct AsmParams {
-    const double* gtab;          // general class: per-panel Gauss tables [N][ns][5]
     const double* xg;            // Gauss rule on [0,1]
     const double* wg;
     const double* xc;            // composite rule
     const double* wc;
-    const int2* pairs;           // general class entries (i, j)
-    const int* ent;              // singular class entries (i, j, swap)
-    const double* ngeo;          // singular class node geometry [entry][nodes][6]
+    const int* ent;              // class entries (i, j, swap)
+    const double* ngeo;          // node geometry [entry][nodes][6]
     const unsigned char* flags;  // per entry skip flags
     const long long* pidx;       // triangular pair index per entry
     int nent;                    // entries of this class
     int nk;                      // wavenumber slots in the batch
+    int nchunk, chunk;           // node chunks per entry, nodes per chunk
+    int wave_uniform;            // 1: nk % 64 == 0, a G = 1 wavefront shares (entry, chunk)
     const double* kap;           // kappa = k sqrt(c) per slot
     const double* kr;            // k per slot
     const double* cv;            // c per slot
     long long npairs_total;      // N (N + 1) / 2
-    cdbl* out;                   // local matrices [nk][npairs_total][16 (D+1)]
+    cdbl* out;                   // nchunk == 1: local matrices [nk][npairs_total][16 (D+1)];
+                                 // else partials [nk][nent][nchunk][16 (D+1)]
     int ns, nc;                  // Gauss / composite rule sizes
     int qflags;                  // HBEM_FLAG_* quirk bits
-    SeriesTables st;
 };
 
 // assembly (hbem_assembly.hip)
 void launch_setup_gauss_table(const double* panels, const double* coefs, int npan, const double* xg, int ns,
                               double* gtab, hipStream_t st);
-void launch_setup_general_flags(const int2* pairs, int npairs, const double* gtab, int ns, unsigned char* flags,
-                                hipStream_t st);
+void launch_setup_general(const int* ent, int nent, const double* gtab, int ns, double* ngeo, unsigned char* flags,
+                          hipStream_t st);
 void launch_setup_singular(int cls, const int* ent, int nent, const double* panels, const double* coefs,
-                           const double* xc, const double* wc, int nc, double* ngeo, unsigned char* flags,
-                           hipStream_t st);
+                           const double* xc, int nc, double* ngeo, unsigned char* flags, hipStream_t st);
 void launch_assemble(int cls, int D, const AsmParams& P, int G, hipStream_t st);
+void launch_chunk_sum(const cdbl* part, int nent, int nchunk, int nk, int nslot, const long long* pidx, long long npairs,
+                      cdbl* L, hipStream_t st);
 void launch_gather_layers(const cdbl* L, long long npt, int nslot, int q, int N, int D, int layers, cdbl* K, cdbl* W,
                           cdbl* V, hipStream_t st);
 // T_t for output b is written at T[t] + b * (2N)^2 (t = 0..D)
@@ -81,7 +82,8 @@ void mass_matrix_host(const double* panels, int N, const double* coefs, bool lin
 // Eigen-style LDLT with diagonal pivoting; F = P^T L sqrt(D) (n x n, column-major)
 void ldlt_projection_factor(const double* A, int n, double* F, int* transpositions);
 void invert_projection_factor(const double* F, const int* tr, int n, double* Finv);
-SeriesTables make_series_tables();
-bool panels_adjacent(const double* panels, const double* coefs, int i, int j, bool& swap);
+// start/end points of every panel as BuilderData::is_adjacent reads them
+// (operator()(-1 / +1), galerkin_builder.cpp:189-201): 4 doubles per panel
+void panel_endpoints(const double* panels, const double* coefs, int N, double* ends);
 
 }  // namespace hbem

@@ -11,12 +11,6 @@
 
 namespace hbem {
 
-struct SeriesTables {
-    double inv_k2[41];
-    double inv_kp1[41];
-    double harm[41];
-};
-
 static const double kEpsH = 2.220446049250313080847e-16;
 static const double kPiH = 3.14159265358979323846;
 
@@ -66,21 +60,6 @@ void composite_rule(int n, double* x, double* w) {
         hi = lo;
         lo = lo * sig;
     }
-}
-
-SeriesTables make_series_tables() {
-    SeriesTables T;
-    double H = 0.0;
-    T.inv_k2[0] = 1.0;
-    T.inv_kp1[0] = 1.0;
-    T.harm[0] = 0.0;
-    for (int k = 1; k <= 40; ++k) {
-        H += 1.0 / k;
-        T.inv_k2[k] = 1.0 / ((double)k * k);
-        T.inv_kp1[k] = 1.0 / (k + 1);
-        T.harm[k] = H;
-    }
-    return T;
 }
 
 // Host panel evaluation (same parametrizations as the device panel_eval).
@@ -136,13 +115,13 @@ static std::complex<double> host_endpoint(const double* p, const double* coefs, 
     return host_point(p, coefs, end ? 1.0 : 0.0);
 }
 
-bool panels_adjacent(const double* panels, const double* coefs, int i, int j, bool& swap) {
-    const double tol = kEpsH * 100.0;
-    const double* pi = panels + 8 * i;
-    const double* pj = panels + 8 * j;
-    if (std::abs(host_endpoint(pi, coefs, true) - host_endpoint(pj, coefs, false)) < tol) { swap = false; return true; }
-    if (std::abs(host_endpoint(pi, coefs, false) - host_endpoint(pj, coefs, true)) < tol) { swap = true; return true; }
-    return false;
+// ends[4i..4i+3] = start(i).x, start(i).y, end(i).x, end(i).y
+void panel_endpoints(const double* panels, const double* coefs, int N, double* ends) {
+    for (int i = 0; i < N; ++i) {
+        std::complex<double> a = host_endpoint(panels + 8 * i, coefs, false);
+        std::complex<double> b = host_endpoint(panels + 8 * i, coefs, true);
+        ends[4 * i] = a.real(); ends[4 * i + 1] = a.imag(); ends[4 * i + 2] = b.real(); ends[4 * i + 3] = b.imag();
+    }
 }
 
 // mass_matrix::GalerkinMatrix with the rule chosen by SolutionsOperator

@@ -4,7 +4,7 @@
  * This is the drop-in boundary.  The reference exposes the path as a C++
  * library API (GalerkinBuilder / SolutionsOperator / randomized_svd); this ABI
  * is the layer underneath a C++ facade that keeps those names
- * (include/hbem/*.hpp), and is what ctypes / cgo / JNI style bindings call.
+ * (include/hbem/hbem.hpp), and is what ctypes / cgo / JNI style bindings call.
  *
  * Conventions
  *   - Every function returns 0 on success and a negative code on error; the
@@ -125,9 +125,10 @@ int hbem_rsvd_sv_der2(const void* T, const void* Td, const void* Tdd, int n, con
  * Batched sigma_min(k) sweep -- the caller pattern of the reference drivers
  * (std::transform(par_unseq, ks) { gen_sol_op(k, c_o, c_i, T); sv(T, R, q); },
  * examples/roots_brent_square_rsvd.cpp:167-171).  Everything stays on the
- * device; only the nk results come back.  With want_der != 0 the operators
- * are assembled with their first and second k-derivatives (d = 2) and
- * dsigma/dk = Re(u^H T' v) is returned in dsigma (may be NULL otherwise).
+ * device; only the nk results come back.  want_der = 1 assembles the
+ * operators with their first k-derivatives, want_der = 2 with the first and
+ * second (gen_sol_op_2nd_der's operator set); in both cases
+ * dsigma/dk = Re(u^H T' v) is returned in dsigma (may be NULL when want_der = 0).
  * R: n x nc test matrix (n = 2N), uploaded once per call.
  */
 int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i,
