@@ -306,24 +306,52 @@ __device__ __forceinline__ void acc_store(const Acc<D>& A, double kap, double kr
 
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Inputs of one quadrature node: the six geometry values of the node table
+// and the two rule points / weights it combines.
+struct NodeIn {
+    double o[6];
+    double xa, xb, wa, wb;
+};
+
+__device__ __forceinline__ void fetch_node(NodeIn& n, const double* __restrict__ geo, int nd,
+                                           const double* __restrict__ x, const double* __restrict__ w, int a, int b) {
+    const double* p = geo + (size_t)nd * 6;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) n.o[i] = p[i];
+    n.xa = x[a]; n.xb = x[b]; n.wa = w[a]; n.wb = w[b];
+}
+
 // cls: 0 coinciding, 1 adjacent, 2 general
 template <int CLS, int D, int G>
 __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
     const int lane = threadIdx.x & 63;
-    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int g = lane % G;
-    const long long nitems = (long long)P.nent * P.nchunk * P.nk;
-    long long item = wave * (64 / G) + lane / G;
-    const bool active = item < nitems;
-    if (!active) item = nitems - 1;
-    const int q = (int)(item % P.nk);
-    const long long rest = item / P.nk;
-    int chunk = (int)(rest % P.nchunk);
-    int e = (int)(rest / P.nchunk);
-    if (G == 1 && P.wave_uniform) {
-        // nk is a multiple of 64: the wavefront shares (pair, chunk)
-        e = wave_uniform(e);
-        chunk = wave_uniform(chunk);
+    int e, chunk, q;
+    bool active;
+    if (G == 1) {
+        // Waves enumerate (pair, chunk, block of 64 slots); lanes run over the
+        // slots.  The wavefront shares (pair, chunk), so the node geometry is
+        // read with wave-uniform addresses.
+        const int wave = wave_uniform((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+        const int nqb = (P.nk + 63) >> 6;
+        const int qb = wave % nqb;
+        const int rest = wave / nqb;
+        chunk = rest % P.nchunk;
+        e = rest / P.nchunk;
+        q = qb * 64 + lane;
+        active = e < P.nent && q < P.nk;
+        if (e >= P.nent) { e = P.nent - 1; chunk = 0; }
+        if (q >= P.nk) q = P.nk - 1;
+    } else {
+        const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+        const long long nitems = (long long)P.nent * P.nchunk * P.nk;
+        long long item = wave * (64 / G) + lane / G;
+        active = item < nitems;
+        if (!active) item = nitems - 1;
+        q = (int)(item % P.nk);
+        const long long rest = item / P.nk;
+        chunk = (int)(rest % P.nchunk);
+        e = (int)(rest / P.nchunk);
     }
     const double kap = P.kap[q];
     const unsigned f = P.flags[e];
@@ -335,45 +363,61 @@ __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
     Acc<D> A;
     acc_zero<D>(A);
 
+    // The node loop is software-pipelined: the tables of node nd + G are
+    // loaded before node nd is evaluated, so their latency hides behind one
+    // node's arithmetic.
     if (CLS == 2) {
         const double fd = (f & 1) ? 0.0 : 1.0, ft = (f & 2) ? 0.0 : 1.0, fb = (f & 4) ? 0.0 : 2.0;
         const int ns = P.ns;
-        for (int nd = lo + g; nd < hi; nd += G) {
-            const int a = nd / ns, b = nd - a * ns;
-            const double* o = geo + (size_t)nd * 6;
-            const double s = P.xg[a], t = P.xg[b];
-            const double w = P.wg[a] * P.wg[b];
+        NodeIn cur, nxt;
+        int nd = lo + g;
+        if (nd < hi) fetch_node(cur, geo, nd, P.xg, P.wg, nd / ns, nd % ns);
+        for (; nd < hi; nd += G) {
+            if (nd + G < hi) fetch_node(nxt, geo, nd + G, P.xg, P.wg, (nd + G) / ns, (nd + G) % ns);
+            const double s = cur.xa, t = cur.xb;
+            const double w = cur.wa * cur.wb;
             const double bi[2] = {s, 1.0 - s}, bj[2] = {t, 1.0 - t};
-            node_accum<D>(A, kap, o[0], o[1], w, bi, bj, fd * o[2], ft * o[3], fb * o[4], o[5], false, 0.0);
+            node_accum<D>(A, kap, cur.o[0], cur.o[1], w, bi, bj, fd * cur.o[2], ft * cur.o[3], fb * cur.o[4],
+                          cur.o[5], false, 0.0);
+            cur = nxt;
         }
     } else if (CLS == 1) {
         const bool swap = P.ent[3 * e + 2] != 0;
         const int nc = P.nc, nh = nc * nc;
-        for (int nd = lo + g; nd < hi; nd += G) {
-            const int h = nd >= nh ? 1 : 0, gg = nd - h * nh, a = gg / nc, b = gg - a * nc;
-            const double t = P.xc[a], s = t * P.xc[b];
-            const double w = t * P.wc[a] * P.wc[b];
+        NodeIn cur, nxt;
+        int nd = lo + g;
+        if (nd < hi) { const int gg = nd % nh; fetch_node(cur, geo, nd, P.xc, P.wc, gg / nc, gg % nc); }
+        for (; nd < hi; nd += G) {
+            if (nd + G < hi) { const int gg = (nd + G) % nh; fetch_node(nxt, geo, nd + G, P.xc, P.wc, gg / nc, gg % nc); }
+            const int h = nd >= nh ? 1 : 0;
+            const double t = cur.xa, s = t * cur.xb;
+            const double w = t * cur.wa * cur.wb;
             const double si = h == 0 ? s : t, sj = h == 0 ? t : s;
             // test panel i: forward when start(i) is the shared vertex (swap), else reversed; trial j opposite
             const double bi[2] = {swap ? si : 1.0 - si, swap ? 1.0 - si : si};
             const double bj[2] = {swap ? 1.0 - sj : sj, swap ? sj : 1.0 - sj};
-            const double* o = geo + (size_t)nd * 6;
             const unsigned fh = f >> (3 * h);
-            const double cdv = (fh & 1) ? 0.0 : o[2];
-            const double ctv = (fh & 2) ? 0.0 : o[3];
-            const double bt = (fh & 4) ? 0.0 : 2.0 * o[4];
-            node_accum<D>(A, kap, o[0], o[1], w, bi, bj, cdv, ctv, bt, o[5], false, 0.0);
+            const double cdv = (fh & 1) ? 0.0 : cur.o[2];
+            const double ctv = (fh & 2) ? 0.0 : cur.o[3];
+            const double bt = (fh & 4) ? 0.0 : 2.0 * cur.o[4];
+            node_accum<D>(A, kap, cur.o[0], cur.o[1], w, bi, bj, cdv, ctv, bt, cur.o[5], false, 0.0);
+            cur = nxt;
         }
     } else {
         const int nc = P.nc;
         const double f0 = (f & 1) ? 0.0 : 1.0, f1 = (f & 8) ? 0.0 : 1.0, fb = (f & 4) ? 0.0 : 2.0;
-        for (int nd = lo + g; nd < hi; nd += G) {
-            const int a = nd / nc, b = nd - a * nc;
-            const double t = P.xc[b], s = t * (1.0 - P.xc[a]);
-            const double w = t * P.wc[a] * P.wc[b];
+        NodeIn cur, nxt;
+        int nd = lo + g;
+        // xa = xc[b], xb = xc[a] (t = xc[b], s = t (1 - xc[a]))
+        if (nd < hi) fetch_node(cur, geo, nd, P.xc, P.wc, nd % nc, nd / nc);
+        for (; nd < hi; nd += G) {
+            if (nd + G < hi) fetch_node(nxt, geo, nd + G, P.xc, P.wc, (nd + G) % nc, (nd + G) / nc);
+            const double t = cur.xa, s = t * (1.0 - cur.xb);
+            const double w = t * cur.wa * cur.wb;
             const double bi[2] = {s, 1.0 - s}, bj[2] = {t, 1.0 - t};
-            const double* o = geo + (size_t)nd * 6;
-            node_accum<D>(A, kap, o[0], o[1], w, bi, bj, f0 * o[2], 0.0, fb * o[4], o[5], true, f1 * o[3]);
+            node_accum<D>(A, kap, cur.o[0], cur.o[1], w, bi, bj, f0 * cur.o[2], 0.0, fb * cur.o[4], cur.o[5], true,
+                          f1 * cur.o[3]);
+            cur = nxt;
         }
     }
 
@@ -500,7 +544,8 @@ template <int CLS, int D>
 static void launch_asm_g(const AsmParams& P, int G, hipStream_t st) {
     const long long nitems = (long long)P.nent * P.nchunk * P.nk;
     if (nitems <= 0) return;
-    const long long waves = (nitems * G + 63) / 64;
+    // G = 1: one wave per (pair, chunk, block of 64 slots); else 64/G items per wave
+    const long long waves = G == 1 ? (long long)P.nent * P.nchunk * ((P.nk + 63) / 64) : (nitems * G + 63) / 64;
     const int blocks = (int)((waves + 3) / 4);
     switch (G) {
     case 1: hipLaunchKernelGGL((k_assemble<CLS, D, 1>), dim3(blocks), dim3(256), 0, st, P); break;

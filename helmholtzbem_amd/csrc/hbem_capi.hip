@@ -157,7 +157,6 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
         P.ngeo = l.ngeo;
         P.flags = l.flags;
         P.pidx = l.pidx;
-        P.wave_uniform = (l.G == 1 && nk % 64 == 0) ? 1 : 0;
         // general pairs: one chunk; singular pairs: chunks of ~512 nodes per lane-group
         const int chunk_nodes = l.cls == 2 ? l.nodes : (l.G == 1 ? 512 : 64 * 48);
         P.chunk = chunk_nodes;

@@ -42,24 +42,41 @@ constexpr double kDblEps = 2.220446049250313080847e-16;
 // loads stay inside the loop.
 static __constant__ double kBesselTable[HBEM_TAB_SIZE] = HBEM_BESSEL_TABLE_INIT;
 
-__device__ __forceinline__ const double* bessel_table() {
-    const double* t = kBesselTable;
-    asm volatile("" : "+s"(t));
-    return t;
+// pointer into the constant address space: uniform reads through it compile
+// to scalar loads (s_load), which keep the coefficients in SGPRs
+typedef const __attribute__((address_space(4))) double* ctab_ptr;
+
+__device__ __forceinline__ ctab_ptr bessel_table() {
+    unsigned long long a = (unsigned long long)(const void*)kBesselTable;
+    asm volatile("" : "+s"(a));
+    return (ctab_ptr)a;
 }
 
-__device__ __forceinline__ double horner_pq(const double* c, double z) {
-    double v = c[HBEM_PQ_DEG];
-#pragma unroll
-    for (int i = HBEM_PQ_DEG - 1; i >= 0; --i) v = fma(v, z, c[i]);
-    return v;
+// a * b + c with the wave-uniform c taken straight from its SGPR pair.  Left
+// to itself the compiler selects v_fmac_f64, whose addend must sit in the
+// destination VGPR, and copies every coefficient from SGPRs to VGPRs first.
+__device__ __forceinline__ double fma_su(double a, double b, double c) {
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+    return r;
 }
 
-__device__ __forceinline__ double horner_small(const double* c, double y) {
-    double v = c[HBEM_SMALL_DEG];
+// Four polynomials of degree DEG in one variable, evaluated in lockstep
+// (independent Horner chains interleaved); c[4 i + f] is coefficient i of
+// polynomial f.
+template <int DEG>
+__device__ __forceinline__ void horner4(ctab_ptr c, double z, double& v0, double& v1, double& v2, double& v3) {
+    v0 = fma_su(z, c[4 * DEG + 0], c[4 * DEG - 4]);
+    v1 = fma_su(z, c[4 * DEG + 1], c[4 * DEG - 3]);
+    v2 = fma_su(z, c[4 * DEG + 2], c[4 * DEG - 2]);
+    v3 = fma_su(z, c[4 * DEG + 3], c[4 * DEG - 1]);
 #pragma unroll
-    for (int i = HBEM_SMALL_DEG - 1; i >= 0; --i) v = fma(v, y, c[i]);
-    return v;
+    for (int i = DEG - 2; i >= 0; --i) {
+        v0 = fma_su(v0, z, c[4 * i + 0]);
+        v1 = fma_su(v1, z, c[4 * i + 1]);
+        v2 = fma_su(v2, z, c[4 * i + 2]);
+        v3 = fma_su(v3, z, c[4 * i + 3]);
+    }
 }
 
 // J0, Y0, J1, Y1 of x.  The branch points are the reference's
@@ -78,9 +95,8 @@ __device__ __forceinline__ void bessel_jy01(double x, double& j0, double& y0, do
         const double xs = x > 0.0 ? x : 1.0;
         const double h = 0.125 * xs;
         const double y = h * h;
-        const double* tab = bessel_table();
-        const double A = horner_small(tab + HBEM_TAB_A, y), B = horner_small(tab + HBEM_TAB_B, y);
-        const double C = horner_small(tab + HBEM_TAB_C, y), D = horner_small(tab + HBEM_TAB_D, y);
+        double A, B, C, D;
+        horner4<HBEM_SMALL_DEG>(bessel_table() + HBEM_TAB_SMALL, y, A, B, C, D);
         const double L2 = kTwoOverPi * log(0.5 * xs);
         const double xinv = 1.0 / xs;
         const double J1 = xs * B;
@@ -97,9 +113,10 @@ __device__ __forceinline__ void bessel_jy01(double x, double& j0, double& y0, do
         if (x <= 1e6) {
             const double xinv = 1.0 / x;
             const double z = 64.0 * xinv * xinv;
-            const double* tab = bessel_table();
-            const double P0 = horner_pq(tab + HBEM_TAB_P0, z), Q0 = horner_pq(tab + HBEM_TAB_Q0, z) * xinv;
-            const double P1 = horner_pq(tab + HBEM_TAB_P1, z), Q1 = horner_pq(tab + HBEM_TAB_Q1, z) * xinv;
+            double P0, Q0, P1, Q1;
+            horner4<HBEM_PQ_DEG>(bessel_table() + HBEM_TAB_PQ, z, P0, Q0, P1, Q1);
+            Q0 *= xinv;
+            Q1 *= xinv;
             j0 = b * fma(P0, cs, -Q0 * sc);
             y0 = b * fma(P0, sc, Q0 * cs);
             j1 = b * fma(P1, sc, Q1 * cs);

@@ -22,7 +22,6 @@ struct AsmParams {
     int nent;                    // entries of this class
     int nk;                      // wavenumber slots in the batch
     int nchunk, chunk;           // node chunks per entry, nodes per chunk
-    int wave_uniform;            // 1: nk % 64 == 0, a G = 1 wavefront shares (entry, chunk)
     const double* kap;           // kappa = k sqrt(c) per slot
     const double* kr;            // k per slot
     const double* cv;            // c per slot
