@@ -695,11 +695,13 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
     const int D = want_der ? (want_der >= 2 ? 2 : 1) : 0;
     const size_t nn = (size_t)n * n;
     // Batch of B wavenumbers -> 2B slots ordered [c_i for every k | c_o for every k]:
-    // with B = 64 each wavefront of the assembly sees a single refraction index
-    // and 64 consecutive k, so the arguments kappa r of its lanes stay close
-    // (fewer divergent Bessel branches).  B is bounded by the working set:
-    // local matrices, chunk partials and T, T', T'' per wavenumber.
-    int B = 64;
+    // each wavefront of the assembly sees a single refraction index and 64
+    // consecutive k, so the arguments kappa r of its lanes stay close (fewer
+    // divergent Bessel branches).  B = 256 gives the dense stage (one
+    // workgroup per matrix in the panel factorisation and the thin solves)
+    // one matrix per CU.  B is bounded by the working set: local matrices,
+    // chunk partials and T, T', T'' per wavenumber.
+    int B = 256;
     const size_t nslot = 16 * (size_t)(D + 1);
     const size_t maxchunks = (size_t)g->nadj * ((2 * (size_t)g->nc * g->nc + 511) / 512);
     const size_t per = nn * sizeof(cdbl) * (D + 1) +

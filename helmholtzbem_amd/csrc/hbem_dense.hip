@@ -239,80 +239,128 @@ __device__ __forceinline__ cdbl tri_el(const cdbl* lu, int n, int kind, int i, i
 }
 
 constexpr int kMaxNc = 8;
+constexpr int kTS = 65;   // padded row stride of the 64 x 64 LDS tile
+
+// tile[i * kTS + j] = T(i0 + i, j0 + j) for i < rows, j < cols, 0 elsewhere.
+// The loop order follows the memory layout of the operand (column-major LU
+// for kinds 0/1, its rows for kinds 2/3), so the global reads coalesce.
+__device__ __forceinline__ void load_tile(const cdbl* lu, int n, int kind, int i0, int rows, int j0, int cols,
+                                          cdbl* tile) {
+    for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
+        int i, j;
+        if (kind <= 1) { i = e & 63; j = e >> 6; } else { j = e & 63; i = e >> 6; }
+        tile[i * kTS + j] = (i < rows && j < cols) ? tri_el(lu, n, kind, i0 + i, j0 + j) : mk(0, 0);
+    }
+}
+
+// Diagonal block solve by wavefront 0 on w[r0 .. r0 + bs), the block's
+// operand in `tile`; lower (forward) or upper (backward).  The solution is
+// also left in xb[j * kMaxNc + c] (zero for j >= bs).
+template <int NC>
+__device__ void diag_solve(const cdbl* tile, int r0, int bs, bool lower, bool unit, cdbl* w, int n, cdbl* xb) {
+    const int lane = threadIdx.x & 63;
+    cdbl x[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) x[c] = lane < bs ? w[r0 + lane + (size_t)c * n] : mk(0, 0);
+    for (int step = 0; step < bs; ++step) {
+        const int j = lower ? step : bs - 1 - step;
+        if (!unit && lane == j) {
+            const cdbl inv = crecip(tile[j * kTS + j]);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) x[c] = x[c] * inv;
+        }
+        const bool below = lower ? (lane > j && lane < bs) : (lane < j);
+        const cdbl l = below ? tile[lane * kTS + j] : mk(0, 0);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const cdbl xj = shfl_c(x[c], j);
+            if (below) cfms(x[c], l, xj);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        if (lane < bs) w[r0 + lane + (size_t)c * n] = x[c];
+        xb[lane * kMaxNc + c] = lane < bs ? x[c] : mk(0, 0);
+    }
+}
+
+// Rows [ia, ib) of w -= T(rows, r0 : r0 + bs) xb, in 64-row tiles through LDS;
+// four partial sums over 16 columns each per row, added in a fixed order.
+template <int NC>
+__device__ void trail_update(const cdbl* lu, int n, int kind, int r0, int bs, int ia, int ib, cdbl* w,
+                             const cdbl* xb, cdbl* tile, cdbl* part) {
+    const int tid = threadIdx.x;
+    for (int i0 = ia; i0 < ib; i0 += 64) {
+        const int rows = ib - i0 < 64 ? ib - i0 : 64;
+        load_tile(lu, n, kind, i0, rows, r0, bs, tile);
+        __syncthreads();
+        const int i = tid & 63, pt = tid >> 6;
+        cdbl s[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) s[c] = mk(0, 0);
+#pragma unroll 4
+        for (int jj = 0; jj < 16; ++jj) {
+            const int j = pt * 16 + jj;
+            const cdbl t = tile[i * kTS + j];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) cfma(s[c], t, xb[j * kMaxNc + c]);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) part[(pt * 64 + i) * kMaxNc + c] = s[c];
+        __syncthreads();
+        if (tid < rows) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const cdbl acc = ((part[tid * kMaxNc + c] + part[(64 + tid) * kMaxNc + c]) +
+                                  part[(128 + tid) * kMaxNc + c]) + part[(192 + tid) * kMaxNc + c];
+                cdbl& y = w[i0 + tid + (size_t)c * n];
+                y = y - acc;
+            }
+        }
+    }
+}
+
+struct SolveLds {
+    cdbl tile[64 * kTS];
+    cdbl xb[64 * kMaxNc];
+    cdbl part[256 * kMaxNc];
+};
 
 // forward (lower) substitution, unit or not, on w (n x nc, ld n)
-__device__ void fwd_subst(const cdbl* lu, int n, int kind, bool unit, cdbl* w, int nc) {
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+template <int NC>
+__device__ void fwd_subst(const cdbl* lu, int n, int kind, bool unit, cdbl* w, SolveLds& S) {
     for (int r0 = 0; r0 < n; r0 += 64) {
         const int bs = n - r0 < 64 ? n - r0 : 64;
-        if (wv == 0) {
-            cdbl x[kMaxNc];
-            for (int c = 0; c < nc; ++c) x[c] = lane < bs ? w[r0 + lane + (size_t)c * n] : mk(0, 0);
-            for (int j = 0; j < bs; ++j) {
-                if (!unit && lane == j) {
-                    const cdbl inv = crecip(tri_el(lu, n, kind, r0 + j, r0 + j));
-                    for (int c = 0; c < nc; ++c) x[c] = x[c] * inv;
-                }
-                const cdbl l = (lane > j && lane < bs) ? tri_el(lu, n, kind, r0 + lane, r0 + j) : mk(0, 0);
-                for (int c = 0; c < nc; ++c) {
-                    const cdbl xj = shfl_c(x[c], j);
-                    if (lane > j) cfms(x[c], l, xj);
-                }
-            }
-            if (lane < bs) for (int c = 0; c < nc; ++c) w[r0 + lane + (size_t)c * n] = x[c];
-        }
+        load_tile(lu, n, kind, r0, bs, r0, bs, S.tile);
         __syncthreads();
-        for (int i = r0 + bs + tid; i < n; i += blockDim.x) {
-            cdbl s[kMaxNc];
-            for (int c = 0; c < nc; ++c) s[c] = w[i + (size_t)c * n];
-            for (int j = 0; j < bs; ++j) {
-                const cdbl l = tri_el(lu, n, kind, i, r0 + j);
-                for (int c = 0; c < nc; ++c) cfms(s[c], l, w[r0 + j + (size_t)c * n]);
-            }
-            for (int c = 0; c < nc; ++c) w[i + (size_t)c * n] = s[c];
-        }
+        if (threadIdx.x < 64) diag_solve<NC>(S.tile, r0, bs, true, unit, w, n, S.xb);
+        __syncthreads();
+        trail_update<NC>(lu, n, kind, r0, bs, r0 + bs, n, w, S.xb, S.tile, S.part);
         __syncthreads();
     }
 }
 
 // backward (upper) substitution
-__device__ void bwd_subst(const cdbl* lu, int n, int kind, bool unit, cdbl* w, int nc) {
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+template <int NC>
+__device__ void bwd_subst(const cdbl* lu, int n, int kind, bool unit, cdbl* w, SolveLds& S) {
     for (int r1 = n; r1 > 0; r1 -= 64) {
         const int r0 = r1 - 64 > 0 ? r1 - 64 : 0;
         const int bs = r1 - r0;
-        if (wv == 0) {
-            cdbl x[kMaxNc];
-            for (int c = 0; c < nc; ++c) x[c] = lane < bs ? w[r0 + lane + (size_t)c * n] : mk(0, 0);
-            for (int j = bs - 1; j >= 0; --j) {
-                if (!unit && lane == j) {
-                    const cdbl inv = crecip(tri_el(lu, n, kind, r0 + j, r0 + j));
-                    for (int c = 0; c < nc; ++c) x[c] = x[c] * inv;
-                }
-                const cdbl u = (lane < j) ? tri_el(lu, n, kind, r0 + lane, r0 + j) : mk(0, 0);
-                for (int c = 0; c < nc; ++c) {
-                    const cdbl xj = shfl_c(x[c], j);
-                    if (lane < j) cfms(x[c], u, xj);
-                }
-            }
-            if (lane < bs) for (int c = 0; c < nc; ++c) w[r0 + lane + (size_t)c * n] = x[c];
-        }
+        load_tile(lu, n, kind, r0, bs, r0, bs, S.tile);
         __syncthreads();
-        for (int i = tid; i < r0; i += blockDim.x) {
-            cdbl s[kMaxNc];
-            for (int c = 0; c < nc; ++c) s[c] = w[i + (size_t)c * n];
-            for (int j = 0; j < bs; ++j) {
-                const cdbl u = tri_el(lu, n, kind, i, r0 + j);
-                for (int c = 0; c < nc; ++c) cfms(s[c], u, w[r0 + j + (size_t)c * n]);
-            }
-            for (int c = 0; c < nc; ++c) w[i + (size_t)c * n] = s[c];
-        }
+        if (threadIdx.x < 64) diag_solve<NC>(S.tile, r0, bs, false, unit, w, n, S.xb);
+        __syncthreads();
+        trail_update<NC>(lu, n, kind, r0, bs, 0, r0, w, S.xb, S.tile, S.part);
         __syncthreads();
     }
 }
 
+// one workgroup of 256 threads per matrix (trail_update assumes 256)
+template <int NC>
 __global__ __launch_bounds__(256) void k_solve_thin(const cdbl* __restrict__ LU, const int* __restrict__ perm, int n,
-                                                    cdbl* __restrict__ X, int nc, int adj, cdbl* __restrict__ work) {
+                                                    cdbl* __restrict__ X, int adj, cdbl* __restrict__ work) {
+    const int nc = NC;
+    __shared__ SolveLds S;
     const int b = blockIdx.x;
     const cdbl* lu = LU + (size_t)b * n * n;
     const int* p = perm + (size_t)b * n;
@@ -326,15 +374,15 @@ __global__ __launch_bounds__(256) void k_solve_thin(const cdbl* __restrict__ LU,
             w[e] = x[p[i] + (size_t)c * n];
         }
         __syncthreads();
-        fwd_subst(lu, n, 0, true, w, nc);
-        bwd_subst(lu, n, 1, false, w, nc);
+        fwd_subst<NC>(lu, n, 0, true, w, S);
+        bwd_subst<NC>(lu, n, 1, false, w, S);
         for (int e = tid; e < n * nc; e += blockDim.x) x[e] = w[e];
     } else {
         // A^H = U^H L^H P:  U^H y = x, L^H z = y, x = P^T z
         for (int e = tid; e < n * nc; e += blockDim.x) w[e] = x[e];
         __syncthreads();
-        fwd_subst(lu, n, 2, false, w, nc);
-        bwd_subst(lu, n, 3, true, w, nc);
+        fwd_subst<NC>(lu, n, 2, false, w, S);
+        bwd_subst<NC>(lu, n, 3, true, w, S);
         for (int e = tid; e < n * nc; e += blockDim.x) {
             const int i = e % n, c = e / n;
             x[p[i] + (size_t)c * n] = w[e];
@@ -344,7 +392,17 @@ __global__ __launch_bounds__(256) void k_solve_thin(const cdbl* __restrict__ LU,
 
 void lu_solve_thin_batched(const cdbl* LU, const int* perm, int n, int batch, cdbl* X, int nc, bool adj, cdbl* work,
                            hipStream_t st) {
-    hipLaunchKernelGGL(k_solve_thin, dim3(batch), dim3(256), 0, st, LU, perm, n, X, nc, adj ? 1 : 0, work);
+    const int a = adj ? 1 : 0;
+    switch (nc) {
+    case 1: hipLaunchKernelGGL(k_solve_thin<1>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
+    case 2: hipLaunchKernelGGL(k_solve_thin<2>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
+    case 3: hipLaunchKernelGGL(k_solve_thin<3>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
+    case 4: hipLaunchKernelGGL(k_solve_thin<4>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
+    case 5: hipLaunchKernelGGL(k_solve_thin<5>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
+    case 6: hipLaunchKernelGGL(k_solve_thin<6>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
+    case 7: hipLaunchKernelGGL(k_solve_thin<7>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
+    default: hipLaunchKernelGGL(k_solve_thin<8>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
+    }
 }
 
 // ---------------------------------------------------------------------------
