@@ -429,6 +429,299 @@ __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Paired-lane assembly for batches of wavenumbers.
+//
+// A wavefront takes one (pair, node chunk) and 32 wavenumber slots; lane
+// j + 32 h (h = 0, 1) serves slot j.  The workgroup's four wavefronts share
+// the (pair, chunk) and stage its node records in LDS, 256 nodes at a time,
+// with the kappa-free factors of the Hankel arguments (1/r, ln r, 1/sqrt r)
+// and the quadrature weight products computed once per node there.  Nodes are
+// taken two at a time: lane half h evaluates J0, Y0, J1, Y1 at node p + h,
+// and v_permlane32_swap hands both halves both nodes' values.  Half h keeps
+// the sums of the shape products with trial function n = h (entries q = m + 2h,
+// m = 0, 1), so a lane holds 9 (D + 1) complex sums instead of 17 (D + 1).
+// ---------------------------------------------------------------------------
+constexpr int kRec = 14;     // doubles per staged node record
+constexpr int kStage = 256;  // nodes per staging round
+
+// record: 0 r, 1 r^2, 2 1/r, 3 (2/pi) ln r, 4 1/sqrt r, 5 w cd, 6 w ct (coinciding: w cd2),
+//         7 w bt, 8 w ttp, 9 weight of the curl sum, 10 bi0, 11 bi1, 12 bj0, 13 bj1
+template <int CLS>
+__device__ __forceinline__ void stage_node(const AsmParams& P, const double* __restrict__ geo, unsigned f, bool swap,
+                                           int nd, double* __restrict__ rec) {
+    const double* o = geo + (size_t)nd * 6;
+    double w, bi0, bi1, bj0, bj1, cd, ct, bt;
+    if (CLS == 2) {
+        const int ns = P.ns, a = nd / ns, b = nd - a * ns;
+        const double s = P.xg[a], t = P.xg[b];
+        w = P.wg[a] * P.wg[b];
+        bi0 = s; bi1 = 1.0 - s; bj0 = t; bj1 = 1.0 - t;
+        cd = (f & 1) ? 0.0 : o[2];
+        ct = (f & 2) ? 0.0 : o[3];
+        bt = (f & 4) ? 0.0 : 2.0 * o[4];
+    } else if (CLS == 1) {
+        const int nc = P.nc, nh = nc * nc;
+        const int h = nd >= nh ? 1 : 0, gg = nd - h * nh, a = gg / nc, b = gg - a * nc;
+        const double t = P.xc[a], s = t * P.xc[b];
+        w = t * P.wc[a] * P.wc[b];
+        const double si = h == 0 ? s : t, sj = h == 0 ? t : s;
+        bi0 = swap ? si : 1.0 - si; bi1 = swap ? 1.0 - si : si;
+        bj0 = swap ? 1.0 - sj : sj; bj1 = swap ? sj : 1.0 - sj;
+        const unsigned fh = f >> (3 * h);
+        cd = (fh & 1) ? 0.0 : o[2];
+        ct = (fh & 2) ? 0.0 : o[3];
+        bt = (fh & 4) ? 0.0 : 2.0 * o[4];
+    } else {
+        const int nc = P.nc, a = nd / nc, b = nd - a * nc;
+        const double t = P.xc[b], s = t * (1.0 - P.xc[a]);
+        w = t * P.wc[a] * P.wc[b];
+        bi0 = s; bi1 = 1.0 - s; bj0 = t; bj1 = 1.0 - t;
+        cd = (f & 1) ? 0.0 : o[2];
+        ct = (f & 8) ? 0.0 : o[3];     // second triangle's K factor
+        bt = (f & 4) ? 0.0 : 2.0 * o[4];
+    }
+    const double r = o[0];
+    rec[0] = r;
+    rec[1] = o[1];
+    rec[2] = r > 0.0 ? 1.0 / r : 0.0;
+    rec[3] = r > 0.0 ? kTwoOverPi * log(r) : 0.0;
+    rec[4] = r > 0.0 ? 1.0 / sqrt(r) : 0.0;
+    rec[5] = w * cd;
+    rec[6] = w * ct;
+    rec[7] = w * bt;
+    rec[8] = w * o[5];
+    rec[9] = CLS == 0 ? 2.0 * w : w;
+    rec[10] = bi0; rec[11] = bi1; rec[12] = bj0; rec[13] = bj1;
+}
+
+// padding record for an odd node count: finite argument, zero weights
+__device__ __forceinline__ void stage_pad(double* rec) {
+    rec[0] = 1.0; rec[1] = 1.0; rec[2] = 1.0; rec[3] = 0.0; rec[4] = 1.0;
+#pragma unroll
+    for (int i = 5; i < kRec; ++i) rec[i] = 0.0;
+}
+
+// both lane halves get the value of the lower half (v0) and of the upper half (v1)
+__device__ __forceinline__ void halves(double v, double& v0, double& v1) {
+    const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    v0 = __hiloint2double((int)b[0], (int)a[0]);
+    v1 = __hiloint2double((int)b[1], (int)a[1]);
+}
+
+template <int D>
+struct AccX {
+    cdbl kd[D + 1][2], kt[D + 1][2], wb[D + 1][2], v[D + 1][2], wa[D + 1];
+};
+
+// J0, Y0, J1, Y1 of x = kappa r from the node's kappa-free factors.
+//   y = x^2 / 64 = k64 r^2, L2 = (2/pi) ln(x/2) = lk + lr, 1/x = kinv / r,
+//   1/sqrt(pi x) = ksq / sqrt(r)
+__device__ __forceinline__ void bessel_jy01_f(double x, double y, double L2, double xinv, double b, double& j0,
+                                              double& y0, double& j1, double& y1) {
+    if (x < 8.0) {
+        double A, B, C, Dd;
+        horner4<HBEM_SMALL_DEG>(bessel_table() + HBEM_TAB_SMALL, y, A, B, C, Dd);
+        const bool pos = x > 0.0;
+        const double J1 = x * B;
+        j0 = pos ? A : 1.0;
+        j1 = pos ? J1 : 0.0;
+        y0 = pos ? fma(L2, A, C) : 0.0;
+        y1 = pos ? fma(L2, J1, fma(x, Dd, -kTwoOverPi * xinv)) : 0.0;
+    } else {
+        double s, c;
+        sincos(x, &s, &c);
+        const double cs = c + s, sc = s - c;
+        if (x <= 1e6) {
+            const double z = 64.0 * xinv * xinv;
+            double P0, Q0, P1, Q1;
+            horner4<HBEM_PQ_DEG>(bessel_table() + HBEM_TAB_PQ, z, P0, Q0, P1, Q1);
+            Q0 *= xinv;
+            Q1 *= xinv;
+            j0 = b * fma(P0, cs, -Q0 * sc);
+            y0 = b * fma(P0, sc, Q0 * cs);
+            j1 = b * fma(P1, sc, Q1 * cs);
+            y1 = b * fma(Q1, sc, -P1 * cs);
+        } else {
+            j0 = b * cs; y0 = b * sc; j1 = b * sc; y1 = -b * cs;
+        }
+    }
+}
+
+// one node's contribution to this lane's sums (entries q = m + 2h)
+template <int CLS, int D>
+__device__ __forceinline__ void node_accum_x(AccX<D>& A, const double* __restrict__ R, int h, double j0, double y0,
+                                             double j1, double y1) {
+    const double r = R[0], r2 = R[1], rinv = R[2];
+    const cdbl h0 = mk(j0, y0);
+    const cdbl h1r = mk(j1 * rinv, y1 * rinv);
+    const cdbl hr = mk(j1 * r, y1 * r);
+    const cdbl h0r2 = mk(j0 * r2, y0 * r2);
+    const double wcd = R[5], wct = R[6], wbt = R[7], wtt = R[8], wa = R[9];
+    const double bih = R[10 + h], bjh = R[12 + h];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const double pbb = R[10 + m] * bjh;   // test m at the panel-i point, trial h at the panel-j point
+        const double pbt = R[12 + m] * bih;   // exchanged roles
+        double ckd, ckt, cb, cg;
+        if (CLS == 0) {
+            ckd = wcd * pbb + wct * pbt;
+            ckt = 0.0;
+            cb = wbt * (pbb + pbt);
+            cg = wtt * (pbb + pbt);
+        } else {
+            ckd = wcd * pbb;
+            ckt = wct * pbt;
+            cb = wbt * pbb;
+            cg = wtt * pbb;
+        }
+        axpy(A.kd[0][m], ckd, h1r);
+        if (CLS != 0) axpy(A.kt[0][m], ckt, h1r);
+        axpy(A.wb[0][m], cb, h0);
+        axpy(A.v[0][m], cg, h0);
+        if (D >= 1) {
+            axpy(A.kd[1][m], ckd, h0);
+            if (CLS != 0) axpy(A.kt[1][m], ckt, h0);
+            axpy(A.wb[1][m], cb, hr);
+            axpy(A.v[1][m], cg, hr);
+        }
+        if (D >= 2) {
+            axpy(A.kd[2][m], ckd, hr);
+            if (CLS != 0) axpy(A.kt[2][m], ckt, hr);
+            axpy(A.wb[2][m], cb, h0r2);
+            axpy(A.v[2][m], cg, h0r2);
+        }
+    }
+    axpy(A.wa[0], wa, h0);
+    if (D >= 1) axpy(A.wa[1], wa, hr);
+    if (D >= 2) axpy(A.wa[2], wa, h0r2);
+}
+
+// cls: 0 coinciding, 1 adjacent, 2 general; blockDim 256
+template <int CLS, int D>
+__global__ __launch_bounds__(256) void k_assemble_x2(AsmParams P) {
+    __shared__ double lds[(kStage + 1) * kRec];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int h = lane >> 5, j = lane & 31;
+    const int nqb = (P.nk + 31) >> 5;        // blocks of 32 slots
+    const int ngrp = (nqb + 3) >> 2;         // groups of 4 blocks = one workgroup
+    const int grp = blockIdx.x % ngrp;
+    const int rest = blockIdx.x / ngrp;
+    const int chunk = rest % P.nchunk;
+    const int e = rest / P.nchunk;           // < nent by the grid size
+    const int qb = grp * 4 + wv;
+    int q = qb * 32 + j;
+    const bool active = qb < nqb && q < P.nk;
+    if (q >= P.nk) q = P.nk - 1;
+    const double kap = P.kap[q];
+    const double k64 = kap * kap * (1.0 / 64.0);
+    const double lk = kTwoOverPi * log(0.5 * kap);
+    const double kinv = 1.0 / kap;
+    const double ksq = 1.0 / sqrt(kPi * kap);
+    const unsigned f = P.flags[e];
+    const bool swap = CLS == 1 ? P.ent[3 * e + 2] != 0 : false;
+    const int nn = CLS == 2 ? P.ns * P.ns : (CLS == 1 ? 2 * P.nc * P.nc : P.nc * P.nc);
+    const int lo = chunk * P.chunk;
+    const int hi = min(nn, lo + P.chunk);
+    const double* __restrict__ geo = P.ngeo + (size_t)e * nn * 6;
+
+    AccX<D> A;
+#pragma unroll
+    for (int t = 0; t <= D; ++t) {
+        A.wa[t] = mk(0, 0);
+#pragma unroll
+        for (int m = 0; m < 2; ++m) { A.kd[t][m] = mk(0, 0); A.kt[t][m] = mk(0, 0); A.wb[t][m] = mk(0, 0); A.v[t][m] = mk(0, 0); }
+    }
+
+    for (int base = lo; base < hi; base += kStage) {
+        const int cnt = min(kStage, hi - base);
+        __syncthreads();   // the previous round's records are no longer read
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) stage_node<CLS>(P, geo, f, swap, base + i, lds + i * kRec);
+        if ((cnt & 1) && threadIdx.x == 0) stage_pad(lds + cnt * kRec);
+        __syncthreads();
+        if (qb < nqb) {
+            for (int p = 0; p < cnt; p += 2) {
+                const double* R0 = lds + p * kRec;
+                const double* Rown = R0 + h * kRec;
+                const double r = Rown[0];
+                const double x = kap * r;
+                double j0, y0, j1, y1;
+                bessel_jy01_f(x, k64 * Rown[1], lk + Rown[3], kinv * Rown[2], ksq * Rown[4], j0, y0, j1, y1);
+                double a0, a1, b0, b1, c0, c1, d0, d1;
+                halves(j0, a0, a1);
+                halves(y0, b0, b1);
+                halves(j1, c0, c1);
+                halves(y1, d0, d1);
+                node_accum_x<CLS, D>(A, R0, h, a0, b0, c0, d0);
+                node_accum_x<CLS, D>(A, R0 + kRec, h, a1, b1, c1, d1);
+            }
+        }
+    }
+
+    // Sums -> this lane's entries q = m + 2h of the reference's local matrices
+    // (same scalings as acc_store)
+    const double kr = P.kr[q], c = P.cv[q];
+    const double sqc = sqrt(c), k2h = 0.5 * kap * kap;
+    cdbl Ko[D + 1][2], Kto[D + 1][2], Wo[D + 1][2], Vo[D + 1][2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const int qq = m + 2 * h;
+        const double dd = (qq == 0 || qq == 3) ? 1.0 : -1.0;   // b'_m b'_n for P1 (+1, -1)
+        Ko[0][m] = kap * A.kd[0][m]; Kto[0][m] = kap * A.kt[0][m];
+        Wo[0][m] = dd * A.wa[0] - k2h * A.wb[0][m];
+        Vo[0][m] = A.v[0][m];
+        if (D >= 1) {
+            Ko[1][m] = (kr * c) * A.kd[1][m]; Kto[1][m] = (kr * c) * A.kt[1][m];
+            Wo[1][m] = (-sqc) * (dd * A.wa[1] - k2h * A.wb[1][m] + kap * A.wb[0][m]);
+            Vo[1][m] = (-sqc) * A.v[1][m];
+        }
+        if (D >= 2) {
+            Ko[2][m] = c * (A.kd[1][m] - kap * A.kd[2][m]); Kto[2][m] = c * (A.kt[1][m] - kap * A.kt[2][m]);
+            Wo[2][m] = (-c) * (dd * (A.wa[2] - kinv * A.wa[1]) - k2h * (A.wb[2][m] - kinv * A.wb[1][m]) -
+                              (2.0 * kap) * A.wb[1][m] + A.wb[0][m]);
+            Vo[2][m] = c * (kinv * A.v[1][m] - A.v[2][m]);
+        }
+    }
+    if (CLS == 0) {
+        // entry (0,1) of W and V = entry (1,0) (galerkin_builder.cpp:571-573, 744-746):
+        // q = 2 is (m = 0, h = 1), q = 1 is (m = 1, h = 0)
+        cdbl W10[D + 1], V10[D + 1];
+#pragma unroll
+        for (int t = 0; t <= D; ++t) {
+            double u0, u1;
+            halves(Wo[t][1].x, u0, u1); W10[t].x = u0;
+            halves(Wo[t][1].y, u0, u1); W10[t].y = u0;
+            halves(Vo[t][1].x, u0, u1); V10[t].x = u0;
+            halves(Vo[t][1].y, u0, u1); V10[t].y = u0;
+        }
+        if (h == 1) {
+#pragma unroll
+            for (int t = 0; t <= D; ++t) { Wo[t][0] = W10[t]; Vo[t][0] = V10[t]; }
+            if (P.qflags & 1) {
+                // reference quirk: the d = 0 sum of entry (1,0), scaled for order t
+                if (D >= 1) Vo[1][0] = (-sqc) * V10[0];
+                if (D >= 2) Vo[2][0] = c * V10[0];
+            }
+        }
+    }
+    if (active) {
+        cdbl* out = P.nchunk == 1 ? P.out + ((size_t)q * P.npairs_total + P.pidx[e]) * (16 * (D + 1))
+                                  : P.out + (((size_t)q * P.nent + e) * P.nchunk + chunk) * (16 * (D + 1));
+#pragma unroll
+        for (int t = 0; t <= D; ++t)
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                const int qq = m + 2 * h;
+                out[16 * t + qq] = Ko[t][m];
+                out[16 * t + 4 + qq] = CLS == 0 ? mk(0, 0) : Kto[t][m];
+                out[16 * t + 8 + qq] = Wo[t][m];
+                out[16 * t + 12 + qq] = Vo[t][m];
+            }
+    }
+}
+
 // Sums the node-chunk partials of one class in chunk order into the pair buffer.
 __global__ void k_chunk_sum(const cdbl* __restrict__ part, int nent, int nchunk, int nk, int nslot,
                             const long long* __restrict__ pidx, long long npairs, cdbl* __restrict__ L) {
@@ -544,6 +837,13 @@ template <int CLS, int D>
 static void launch_asm_g(const AsmParams& P, int G, hipStream_t st) {
     const long long nitems = (long long)P.nent * P.nchunk * P.nk;
     if (nitems <= 0) return;
+    if (G == 0) {
+        // paired-lane kernel: one workgroup per (pair, chunk, 4 blocks of 32 slots)
+        const long long groups = ((P.nk + 31) / 32 + 3) / 4;
+        const long long blocks = (long long)P.nent * P.nchunk * groups;
+        hipLaunchKernelGGL((k_assemble_x2<CLS, D>), dim3((unsigned)blocks), dim3(256), 0, st, P);
+        return;
+    }
     // G = 1: one wave per (pair, chunk, block of 64 slots); else 64/G items per wave
     const long long waves = G == 1 ? (long long)P.nent * P.nchunk * ((P.nk + 63) / 64) : (nitems * G + 63) / 64;
     const int blocks = (int)((waves + 3) / 4);

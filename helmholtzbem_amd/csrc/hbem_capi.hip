@@ -132,9 +132,10 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
     P.ns = g->ns;
     P.nc = g->nc;
     P.qflags = g->flags;
-    // lanes per item: one lane per wavenumber for wide batches, node groups otherwise
-    const int Gg = nk >= 64 ? 1 : (nk >= 16 ? 4 : 16);
-    const int Gs = nk >= 64 ? 1 : 64;
+    // lanes per item: G = 0 selects the paired-lane kernel for wide batches
+    // (32 wavenumbers x 2 nodes per wavefront); node groups of G lanes otherwise
+    const int Gg = nk >= 64 ? 0 : (nk >= 16 ? 4 : 16);
+    const int Gs = nk >= 64 ? 0 : 64;
     struct Launch {
         int cls, which, G, nent, nodes;
         const int* ent;
@@ -157,8 +158,9 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
         P.ngeo = l.ngeo;
         P.flags = l.flags;
         P.pidx = l.pidx;
-        // general pairs: one chunk; singular pairs: chunks of ~512 nodes per lane-group
-        const int chunk_nodes = l.cls == 2 ? l.nodes : (l.G == 1 ? 512 : 64 * 48);
+        // general pairs: one chunk; singular pairs: chunks of 2048 nodes (paired-lane
+        // kernel, 8 staging rounds) or 64 x 48 nodes (one wavefront per item)
+        const int chunk_nodes = l.cls == 2 ? l.nodes : (l.G == 0 ? 2048 : (l.G == 1 ? 512 : 64 * 48));
         P.chunk = chunk_nodes;
         P.nchunk = (l.nodes + chunk_nodes - 1) / chunk_nodes;
         cdbl* part = nullptr;
