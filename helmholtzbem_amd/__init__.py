@@ -296,6 +296,28 @@ class randomized_svd:
         return out
 
 
+def assemble_dense_batch(builder_data: BuilderData, ks, c, layers=LayerType.ALL, d=0):
+    """GalerkinBuilder::assembleDense(k, c, layers, d) for every real k of `ks`
+    in one batched pass.  Returns {LayerType: array (nk, d+1, N, N)} with the
+    matrices GalerkinBuilder.getDoubleLayer / getHypersingular / getSingleLayer
+    would return for each k."""
+    g = builder_data._ctx(True)
+    ks = np.ascontiguousarray(ks, dtype=np.float64)
+    N = builder_data.getNumPanels()
+    layers = LayerType(int(layers))
+    outs, args = {}, []
+    for lt in (LayerType.DOUBLE, LayerType.HYPERSINGULAR, LayerType.SINGLE):
+        if layers & lt:
+            outs[lt] = np.zeros((ks.size, d + 1, N, N), dtype=np.complex128)
+            args.append(outs[lt])
+        else:
+            args.append(None)
+    check(lib().hbem_assemble_dense_batch(g.h, ks.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ks.size,
+                                          float(c), int(layers), int(d), *[_vptr(a) for a in args]))
+    # slices are column-major N x N
+    return {lt: np.swapaxes(a, 2, 3).copy() for lt, a in outs.items()}
+
+
 def sigma_sweep(builder_data: BuilderData, ks, c_o, c_i, R, q=2, der=0, proj=True):
     """sigma_min(k) for every k (and dsigma/dk when der >= 1; der = 2 assembles
     the operators with their first and second k-derivatives)."""

@@ -18,7 +18,7 @@ EXPORTED = [
     "hbem_last_error", "hbem_init", "hbem_geometry_create", "hbem_geometry_destroy", "hbem_geometry_info",
     "hbem_assemble_dense", "hbem_mass_matrix", "hbem_projection_factor", "hbem_gen_sol_op", "hbem_projection",
     "hbem_rsvd_sv", "hbem_rsvd_sv_der", "hbem_rsvd_sv_der2", "hbem_sigma_sweep", "hbem_profile_enable",
-    "hbem_profile_query",
+    "hbem_profile_query", "hbem_assemble_dense_batch",
 ]
 
 _lib = None
@@ -44,6 +44,7 @@ def lib():
         L.hbem_geometry_destroy.argtypes = [vp]
         L.hbem_geometry_info.argtypes = [vp, ip, ip, ip, ip]
         L.hbem_assemble_dense.argtypes = [vp, d, d, d, i, i, vp, vp, vp]
+        L.hbem_assemble_dense_batch.argtypes = [vp, dp, i, d, i, i, vp, vp, vp]
         L.hbem_mass_matrix.argtypes = [vp, dp]
         L.hbem_projection_factor.argtypes = [vp, dp, ip]
         L.hbem_gen_sol_op.argtypes = [vp, d, d, d, d, i, vp, vp, vp]

@@ -29,6 +29,10 @@
 //   [0..3] K direct (test on panel i, trial on j), [4..7] K transposed block,
 //   [8..11] W, [12..15] V; entry (m, n) at m + 2n, m/n = P1 local index
 //   (b0(t) = t, b1(t) = 1 - t; continuous_space.hpp:115-120).
+// Buffer layout: [pair][slot][wavenumber] -- the wavenumber index runs
+// fastest, so the assembly stores (lanes over wavenumbers) and the gathers
+// (threads over wavenumbers) are coalesced.  Chunk partials of the singular
+// classes: [entry][chunk][slot][wavenumber].
 #include "hbem_device.hpp"
 #include "hbem_internal.hpp"
 
@@ -264,7 +268,8 @@ __device__ __forceinline__ void node_accum(Acc<D>& A, double kap, double r, doub
 // afterwards.
 template <int D>
 __device__ __forceinline__ void acc_store(const Acc<D>& A, double kap, double kr, double c, bool coinc, int qflags,
-                                          cdbl* __restrict__ out) {
+                                          cdbl* __restrict__ dst, long long stride) {
+    cdbl out[16 * (D + 1)];
     const double sqc = sqrt(c), k2h = 0.5 * kap * kap, kinv = 1.0 / kap;
     const double dd[4] = {1.0, -1.0, -1.0, 1.0};   // b'_m b'_n for P1 (+1, -1)
 #pragma unroll
@@ -302,6 +307,8 @@ __device__ __forceinline__ void acc_store(const Acc<D>& A, double kap, double kr
             if (D >= 2) out[32 + 12 + 2] = c * out[12 + 1];
         }
     }
+#pragma unroll
+    for (int i = 0; i < 16 * (D + 1); ++i) dst[(size_t)i * stride] = out[i];
 }
 
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -423,9 +430,9 @@ __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
 
     acc_reduce<D, G>(A);
     if (active && g == 0) {
-        cdbl* out = P.nchunk == 1 ? P.out + ((size_t)q * P.npairs_total + P.pidx[e]) * (16 * (D + 1))
-                                  : P.out + (((size_t)q * P.nent + e) * P.nchunk + chunk) * (16 * (D + 1));
-        acc_store<D>(A, kap, P.kr[q], P.cv[q], CLS == 0, P.qflags, out);
+        const size_t rec = P.nchunk == 1 ? (size_t)P.pidx[e] : (size_t)e * P.nchunk + chunk;
+        cdbl* out = P.out + rec * (16 * (D + 1)) * P.nk + q;
+        acc_store<D>(A, kap, P.kr[q], P.cv[q], CLS == 0, P.qflags, out, P.nk);
     }
 }
 
@@ -707,17 +714,18 @@ __global__ __launch_bounds__(256) void k_assemble_x2(AsmParams P) {
         }
     }
     if (active) {
-        cdbl* out = P.nchunk == 1 ? P.out + ((size_t)q * P.npairs_total + P.pidx[e]) * (16 * (D + 1))
-                                  : P.out + (((size_t)q * P.nent + e) * P.nchunk + chunk) * (16 * (D + 1));
+        const size_t rec = P.nchunk == 1 ? (size_t)P.pidx[e] : (size_t)e * P.nchunk + chunk;
+        const size_t nk = P.nk;
+        cdbl* out = P.out + rec * (16 * (D + 1)) * nk + q;
 #pragma unroll
         for (int t = 0; t <= D; ++t)
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
                 const int qq = m + 2 * h;
-                out[16 * t + qq] = Ko[t][m];
-                out[16 * t + 4 + qq] = CLS == 0 ? mk(0, 0) : Kto[t][m];
-                out[16 * t + 8 + qq] = Wo[t][m];
-                out[16 * t + 12 + qq] = Vo[t][m];
+                out[(16 * t + qq) * nk] = Ko[t][m];
+                out[(16 * t + 4 + qq) * nk] = CLS == 0 ? mk(0, 0) : Kto[t][m];
+                out[(16 * t + 8 + qq) * nk] = Wo[t][m];
+                out[(16 * t + 12 + qq) * nk] = Vo[t][m];
             }
     }
 }
@@ -728,13 +736,13 @@ __global__ void k_chunk_sum(const cdbl* __restrict__ part, int nent, int nchunk,
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)nk * nent * nslot;
     if (id >= total) return;
-    const int s = (int)(id % nslot);
-    const long long qe = id / nslot;
-    const int e = (int)(qe % nent), q = (int)(qe / nent);
-    const cdbl* p = part + ((size_t)qe * nchunk) * nslot + s;
+    const int q = (int)(id % nk);
+    const long long es = id / nk;
+    const int s = (int)(es % nslot), e = (int)(es / nslot);
+    const cdbl* p = part + ((size_t)e * nchunk * nslot + s) * nk + q;
     cdbl acc = mk(0, 0);
-    for (int c = 0; c < nchunk; ++c) acc = acc + p[(size_t)c * nslot];
-    L[((size_t)q * npairs + pidx[e]) * nslot + s] = acc;
+    for (int c = 0; c < nchunk; ++c) acc = acc + p[(size_t)c * nslot * nk];
+    L[((size_t)pidx[e] * nslot + s) * nk + q] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -746,10 +754,9 @@ __global__ void k_chunk_sum(const cdbl* __restrict__ part, int nent, int nchunk,
 __device__ __forceinline__ long long tri(int i, int j) { return (long long)i * (i + 1) / 2 + j; }
 
 // layer: 0 K, 1 W, 2 V
-__device__ __forceinline__ cdbl op_entry(const cdbl* __restrict__ L, long long npt, int nslot, int q, int layer,
-                                         int t, int m, int n, int N) {
-    cdbl acc = mk(0, 0);
-    const cdbl* base = L + (size_t)q * npt * nslot + 16 * t;
+__device__ __forceinline__ cdbl op_entry(const cdbl* __restrict__ L, int nk, int nslot, int q, int layer, int t,
+                                         int m, int n, int N) {
+    cdbl e[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
         const int P = a == 0 ? (m == 0 ? N - 1 : m - 1) : m;
@@ -765,14 +772,17 @@ __device__ __forceinline__ cdbl op_entry(const cdbl* __restrict__ L, long long n
                 p = tri(Q, P);
                 slot = layer == 0 ? 4 + a + 2 * b : (layer == 1 ? 8 : 12) + b + 2 * a;
             }
-            acc = acc + base[p * nslot + slot];
+            e[a][b] = L[((size_t)p * nslot + 16 * t + slot) * nk + q];
         }
     }
+    // (e00 + e11) + (e01 + e10): under (m, n) -> (n, m) the symmetric layers
+    // exchange e01 and e10 only, so V and W come out exactly symmetric
+    const cdbl acc = (e[0][0] + e[1][1]) + (e[0][1] + e[1][0]);
     return mk(-0.25 * acc.y, 0.25 * acc.x);   // * i/4
 }
 
 // Builder matrices for wavenumber slot q: out[layer][t] (N x N column-major)
-__global__ void k_gather_layers(const cdbl* __restrict__ L, long long npt, int nslot, int q, int N, int D, int layers,
+__global__ void k_gather_layers(const cdbl* __restrict__ L, int nk, int nslot, int q, int N, int D, int layers,
                                 cdbl* __restrict__ Kout, cdbl* __restrict__ Wout, cdbl* __restrict__ Vout) {
     const long long NN = (long long)N * N;
     long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -780,36 +790,44 @@ __global__ void k_gather_layers(const cdbl* __restrict__ L, long long npt, int n
     const int t = (int)(id / NN);
     const long long e = id - t * NN;
     const int m = (int)(e % N), n = (int)(e / N);
-    if ((layers & 1) && Kout) Kout[id] = op_entry(L, npt, nslot, q, 0, t, m, n, N);
-    if ((layers & 4) && Wout) Wout[id] = op_entry(L, npt, nslot, q, 1, t, m, n, N);
-    if ((layers & 2) && Vout) Vout[id] = op_entry(L, npt, nslot, q, 2, t, m, n, N);
+    if ((layers & 1) && Kout) Kout[id] = op_entry(L, nk, nslot, q, 0, t, m, n, N);
+    if ((layers & 4) && Wout) Wout[id] = op_entry(L, nk, nslot, q, 1, t, m, n, N);
+    if ((layers & 2) && Vout) Vout[id] = op_entry(L, nk, nslot, q, 2, t, m, n, N);
 }
 
 // Transmission matrix (assemble_operator, gen_sol_op.cpp:59-74, unprojected):
 // T = [[K_i - K_o + M, V_o - V_i], [W_o - W_i, K_o - K_i + M^T]], M only for t = 0.
 // Output b uses wavenumber slots qi[b] (inner c_i), qo[b] (outer c_o); T_t of
 // output b is written at Tt + b (2N)^2.
-__global__ void k_form_T(const cdbl* __restrict__ L, long long npt, int nslot, const int* __restrict__ qi,
-                         const int* __restrict__ qo, int nb, int N, int D, const double* __restrict__ M,
-                         cdbl* __restrict__ T0, cdbl* __restrict__ T1, cdbl* __restrict__ T2) {
+// Workgroup = 8 consecutive rows m of one column n for 32 outputs b; thread
+// (i, jb) = (threadIdx & 7, threadIdx >> 3) forms the four block entries
+// (m, n), (m + N, n + N), (m, n + N), (m + N, n) of every order t.  Reads run
+// over 8 wavenumbers per record slot, writes over 8 rows per column.
+__global__ __launch_bounds__(256) void k_form_T(const cdbl* __restrict__ L, int nk, int nslot,
+                                                const int* __restrict__ qi, const int* __restrict__ qo, int nb, int N,
+                                                int D, const double* __restrict__ M, cdbl* __restrict__ T0,
+                                                cdbl* __restrict__ T1, cdbl* __restrict__ T2) {
+    const int i = threadIdx.x & 7, jb = threadIdx.x >> 3;
+    const int mt = (N + 7) >> 3;
+    const int m = (blockIdx.x % mt) * 8 + i;
+    const int n = blockIdx.x / mt;
+    const int b = blockIdx.y * 32 + jb;
+    if (m >= N || b >= nb) return;
     const int n2 = 2 * N;
-    const long long NN = (long long)n2 * n2;
-    long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long total = NN * (D + 1) * nb;
-    if (id >= total) return;
-    const long long mat = id / NN;
-    const long long e = id - mat * NN;
-    const int b = (int)(mat / (D + 1)), t = (int)(mat % (D + 1));
-    const int r = (int)(e % n2), s = (int)(e / n2);
-    const int br = r >= N, bs = s >= N;
-    const int m = r - br * N, n = s - bs * N;
-    const int layer = (br == bs) ? 0 : (br ? 1 : 2);
-    const cdbl xi = op_entry(L, npt, nslot, qi[b], layer, t, m, n, N);
-    const cdbl xo = op_entry(L, npt, nslot, qo[b], layer, t, m, n, N);
-    cdbl v = (br == 0 && bs == 0) ? (xi - xo) : (xo - xi);
-    if (t == 0 && br == bs) v.x += br == 0 ? M[m + (size_t)n * N] : M[n + (size_t)m * N];
-    cdbl* Tt = t == 0 ? T0 : (t == 1 ? T1 : T2);
-    Tt[(size_t)b * NN + e] = v;
+    const size_t NN = (size_t)n2 * n2;
+    const int a = qi[b], o = qo[b];
+    for (int t = 0; t <= D; ++t) {
+        const cdbl Ki = op_entry(L, nk, nslot, a, 0, t, m, n, N), Ko = op_entry(L, nk, nslot, o, 0, t, m, n, N);
+        const cdbl Wi = op_entry(L, nk, nslot, a, 1, t, m, n, N), Wo = op_entry(L, nk, nslot, o, 1, t, m, n, N);
+        const cdbl Vi = op_entry(L, nk, nslot, a, 2, t, m, n, N), Vo = op_entry(L, nk, nslot, o, 2, t, m, n, N);
+        cdbl k00 = Ki - Ko, k11 = Ko - Ki;
+        if (t == 0) { k00.x += M[m + (size_t)n * N]; k11.x += M[n + (size_t)m * N]; }
+        cdbl* Tt = (t == 0 ? T0 : (t == 1 ? T1 : T2)) + (size_t)b * NN;
+        Tt[m + (size_t)n * n2] = k00;
+        Tt[(m + N) + (size_t)(n + N) * n2] = k11;
+        Tt[m + (size_t)(n + N) * n2] = Vo - Vi;
+        Tt[(m + N) + (size_t)n * n2] = Wo - Wi;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -873,18 +891,18 @@ void launch_chunk_sum(const cdbl* part, int nent, int nchunk, int nk, int nslot,
                        nslot, pidx, npairs, L);
 }
 
-void launch_gather_layers(const cdbl* L, long long npt, int nslot, int q, int N, int D, int layers, cdbl* K, cdbl* W,
+void launch_gather_layers(const cdbl* L, int nk, int nslot, int q, int N, int D, int layers, cdbl* K, cdbl* W,
                           cdbl* V, hipStream_t st) {
     long long n = (long long)N * N * (D + 1);
-    hipLaunchKernelGGL(k_gather_layers, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, npt, nslot, q, N, D,
+    hipLaunchKernelGGL(k_gather_layers, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, nk, nslot, q, N, D,
                        layers, K, W, V);
 }
 
-void launch_form_T(const cdbl* L, long long npt, int nslot, const int* qi, const int* qo, int nb, int N, int D,
+void launch_form_T(const cdbl* L, int nk, int nslot, const int* qi, const int* qo, int nb, int N, int D,
                    const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st) {
-    long long n = 4LL * N * N * (D + 1) * nb;
-    hipLaunchKernelGGL(k_form_T, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, npt, nslot, qi, qo, nb, N, D, M,
-                       T0, T1, T2);
+    const unsigned mt = (unsigned)((N + 7) / 8);
+    hipLaunchKernelGGL(k_form_T, dim3(mt * (unsigned)N, (unsigned)((nb + 31) / 32)), dim3(256), 0, st, L, nk, nslot, qi,
+                       qo, nb, N, D, M, T0, T1, T2);
 }
 
 }  // namespace hbem

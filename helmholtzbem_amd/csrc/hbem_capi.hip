@@ -434,11 +434,43 @@ int hbem_assemble_dense(hbem_geometry_t g, double k_re, double k_im, double c, i
     const int N = g->N;
     const size_t NN = (size_t)N * N * (d + 1);
     cdbl* out = g->w_out.as<cdbl>(3 * NN);
-    hbem::launch_gather_layers(L, g->npairs, 16 * (d + 1), 0, N, d, layers, out, out + NN, out + 2 * NN, g->st);
+    hbem::launch_gather_layers(L, 1, 16 * (d + 1), 0, N, d, layers, out, out + NN, out + 2 * NN, g->st);
     HIPC(hipGetLastError());
     if (layers & HBEM_LAYER_DOUBLE) copy_out_square(out, K, NN, g->st);
     if (layers & HBEM_LAYER_HYPERSINGULAR) copy_out_square(out + NN, W, NN, g->st);
     if (layers & HBEM_LAYER_SINGLE) copy_out_square(out + 2 * NN, V, NN, g->st);
+    HIPC(hipStreamSynchronize(g->st));
+    API_END
+}
+
+int hbem_assemble_dense_batch(hbem_geometry_t g, const double* ks, int nk, double c, int layers, int d, void* K,
+                              void* W, void* V) {
+    API_BEGIN
+    if (!g || (nk > 0 && !ks)) throw Error(HBEM_EINVAL, "null argument");
+    if (nk < 0) throw Error(HBEM_EINVAL, "negative number of wavenumbers");
+    if (d < 0 || d > 2) throw Error(HBEM_EINVAL, "Invalid order of derivative");
+    for (int i = 0; i < nk; ++i) check_k(ks[i], 0.0, c);
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPC(hipSetDevice(g->dev));
+    const double sc = std::sqrt(c);
+    const int N = g->N;
+    const size_t NN = (size_t)N * N * (d + 1);
+    cdbl* out = g->w_out.as<cdbl>(3 * NN);
+    const int B = 256;
+    for (int k0 = 0; k0 < nk; k0 += B) {
+        const int nb = std::min(B, nk - k0);
+        std::vector<double> kap(nb), kr(nb), cv(nb, c);
+        for (int b = 0; b < nb; ++b) { kap[b] = ks[k0 + b] * sc; kr[b] = ks[k0 + b]; }
+        cdbl* L = assemble_locals(g, kap, kr, cv, d);
+        for (int b = 0; b < nb; ++b) {
+            hbem::launch_gather_layers(L, nb, 16 * (d + 1), b, N, d, layers, out, out + NN, out + 2 * NN, g->st);
+            HIPC(hipGetLastError());
+            const size_t off = (size_t)(k0 + b) * NN * sizeof(cdbl);
+            if ((layers & HBEM_LAYER_DOUBLE) && K) copy_out_square(out, (char*)K + off, NN, g->st);
+            if ((layers & HBEM_LAYER_HYPERSINGULAR) && W) copy_out_square(out + NN, (char*)W + off, NN, g->st);
+            if ((layers & HBEM_LAYER_SINGLE) && V) copy_out_square(out + 2 * NN, (char*)V + off, NN, g->st);
+        }
+    }
     HIPC(hipStreamSynchronize(g->st));
     API_END
 }
@@ -477,7 +509,7 @@ int hbem_gen_sol_op(hbem_geometry_t g, double k_re, double k_im, double c_o, dou
     std::vector<int> qi = {0}, qo = {1};
     const int* dqi = upload(g->w_qi, qi, g->st);
     const int* dqo = upload(g->w_qo, qo, g->st);
-    hbem::launch_form_T(L, g->npairs, 16 * (d + 1), dqi, dqo, 1, g->N, d, (const double*)g->d_M.p, T0, T1, T2, g->st);
+    hbem::launch_form_T(L, 2, 16 * (d + 1), dqi, dqo, 1, g->N, d, (const double*)g->d_M.p, T0, T1, T2, g->st);
     HIPC(hipGetLastError());
     cdbl* Ts[3] = {T0, T1, T2};
     void* hs[3] = {T, Td, Tdd};
@@ -728,7 +760,7 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
         cdbl* T2 = D >= 2 ? g->w_T2.as<cdbl>(nn * nb) : nullptr;
         const int* dqi = upload(g->w_qi, qi, g->st);
         const int* dqo = upload(g->w_qo, qo, g->st);
-        hbem::launch_form_T(L, g->npairs, 16 * (D + 1), dqi, dqo, nb, g->N, D, (const double*)g->d_M.p, T0, T1, T2, g->st);
+        hbem::launch_form_T(L, 2 * nb, 16 * (D + 1), dqi, dqo, nb, g->N, D, (const double*)g->d_M.p, T0, T1, T2, g->st);
         int* piv = g->w_piv.as<int>((size_t)n * nb);
         int* perm = g->w_perm.as<int>((size_t)n * nb);
         hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -26,8 +26,8 @@ struct AsmParams {
     const double* kr;            // k per slot
     const double* cv;            // c per slot
     long long npairs_total;      // N (N + 1) / 2
-    cdbl* out;                   // nchunk == 1: local matrices [nk][npairs_total][16 (D+1)];
-                                 // else partials [nk][nent][nchunk][16 (D+1)]
+    cdbl* out;                   // nchunk == 1: local matrices [npairs_total][16 (D+1)][nk];
+                                 // else partials [nent][nchunk][16 (D+1)][nk]
     int ns, nc;                  // Gauss / composite rule sizes
     int qflags;                  // HBEM_FLAG_* quirk bits
 };
@@ -42,10 +42,11 @@ void launch_setup_singular(int cls, const int* ent, int nent, const double* pane
 void launch_assemble(int cls, int D, const AsmParams& P, int G, hipStream_t st);
 void launch_chunk_sum(const cdbl* part, int nent, int nchunk, int nk, int nslot, const long long* pidx, long long npairs,
                       cdbl* L, hipStream_t st);
-void launch_gather_layers(const cdbl* L, long long npt, int nslot, int q, int N, int D, int layers, cdbl* K, cdbl* W,
+// L: local matrices [pair][slot][wavenumber] of nk wavenumber slots
+void launch_gather_layers(const cdbl* L, int nk, int nslot, int q, int N, int D, int layers, cdbl* K, cdbl* W,
                           cdbl* V, hipStream_t st);
 // T_t for output b is written at T[t] + b * (2N)^2 (t = 0..D)
-void launch_form_T(const cdbl* L, long long npt, int nslot, const int* qi, const int* qo, int nb, int N, int D,
+void launch_form_T(const cdbl* L, int nk, int nslot, const int* qi, const int* qo, int nb, int N, int D,
                    const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st);
 
 // dense (hbem_dense.hip); all batched over `batch` matrices of order n with

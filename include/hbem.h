@@ -87,6 +87,17 @@ int hbem_geometry_info(hbem_geometry_t g, int* npanels, int* order, int* corder,
 int hbem_assemble_dense(hbem_geometry_t g, double k_re, double k_im, double c, int layers, int d,
                         void* K, void* W, void* V);
 
+/*
+ * assembleDense over a grid of real wavenumbers in one pass: the same
+ * matrices as nk calls of hbem_assemble_dense(g, ks[i], 0, c, layers, d, ...)
+ * (galerkin_builder.cpp:918-1018), assembled batch-wise (all panel pairs x all
+ * k per launch).  K, W, V: host arrays of nk (d+1) N x N matrices, slice t of
+ * wavenumber i at offset (i (d+1) + t) N N, or NULL.  The reference drivers
+ * make these calls one k at a time (examples/roots_brent_square_rsvd.cpp:167-171).
+ */
+int hbem_assemble_dense_batch(hbem_geometry_t g, const double* ks, int nk, double c, int layers, int d, void* K,
+                              void* W, void* V);
+
 /* mass_matrix::GalerkinMatrix as built by SolutionsOperator (mass_matrix.cpp:6-47,
  * gen_sol_op.cpp:30-33).  M: N x N real, column-major. */
 int hbem_mass_matrix(hbem_geometry_t g, double* M);

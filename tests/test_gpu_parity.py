@@ -48,6 +48,47 @@ def test_assemble_dense_parity(case, H, O):
             assert rel(gb.getSingleLayer(t), V[t]) <= MAT_TOL, (name, k, c, "V", t)
 
 
+def test_batched_assembly_parity(case, H, O):
+    """assembleDense over 70 wavenumbers in one pass -- the batched kernel of the
+    sweeps (32 k x 2 nodes per wavefront; 70 k leave a partial block and an idle
+    wavefront) -- against the oracle at every 9th k, for d = 0, 1, 2, plus the
+    quirk-free exact symmetry of V and W and agreement with the per-k path."""
+    name, mesh, order, bd = case
+    ks = np.linspace(0.5, 6.0, 70)
+    out = H.assemble_dense_batch(bd, ks, 20.0, H.LayerType.ALL, 2)
+    gb = H.GalerkinBuilder(bd)
+    for i in range(0, 70, 9):
+        K, W, V = O.assemble(mesh, order, ks[i], 20.0, d=2)
+        for t in range(3):
+            assert rel(out[H.LayerType.DOUBLE][i, t], K[t]) <= MAT_TOL, (name, i, "K", t)
+            assert rel(out[H.LayerType.HYPERSINGULAR][i, t], W[t]) <= MAT_TOL, (name, i, "W", t)
+            assert rel(out[H.LayerType.SINGLE][i, t], V[t]) <= MAT_TOL, (name, i, "V", t)
+            for lt in (H.LayerType.HYPERSINGULAR, H.LayerType.SINGLE):
+                assert np.array_equal(out[lt][i, t], out[lt][i, t].T), (name, i, lt, t)
+    gb.assembleDense(ks[33], 20.0, H.LayerType.ALL, 2)
+    for t in range(3):
+        assert rel(out[H.LayerType.SINGLE][33, t], gb.getSingleLayer(t)) <= 1e-13
+    one = H.assemble_dense_batch(bd, ks, 20.0, H.LayerType.SINGLE, 0)
+    assert np.array_equal(one[H.LayerType.SINGLE][:, 0], out[H.LayerType.SINGLE][:, 0])
+    assert set(one) == {H.LayerType.SINGLE}
+
+
+def test_wide_sweep_parity(case, H, O):
+    """A sweep of 40 k (80 wavenumber slots: the batched kernel) against the
+    oracle's sigma and against narrow sweeps (per-item kernel) for dsigma/dk."""
+    name, mesh, order, bd = case
+    n = 2 * mesh.getNumPanels()
+    R = H.gaussian_test_matrix(n, 2, 3)
+    ks = np.linspace(1.0, 4.0, 40)
+    sig, dsig = H.sigma_sweep(bd, ks, 1.0, 20.0, R, 2, der=2)
+    so = O.SolOp(mesh, order).sigma_sweep(ks, 1.0, 20.0, R, 2)
+    assert np.max(np.abs(sig - so) / so) <= SIG_TOL, (name, sig, so)
+    sub = ks[::8]
+    sig_n, dsig_n = H.sigma_sweep(bd, sub, 1.0, 20.0, R, 2, der=1)
+    assert np.max(np.abs(sig_n - sig[::8]) / sig_n) <= SIG_TOL
+    assert np.max(np.abs(dsig_n - dsig[::8]) / np.maximum(1.0, np.abs(dsig_n))) <= 1e-9, (name, dsig_n, dsig[::8])
+
+
 def test_layer_subsets_and_orders(case, H, O):
     name, mesh, order, bd = case
     gb = H.GalerkinBuilder(bd)
