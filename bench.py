@@ -28,6 +28,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
+# HBM bytes per launch of the general-pair assembly kernel at this workload:
+# rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (KiB per dispatch, scaled by
+# the counters' reading of a plain 1 GiB copy), from
+# profiles/k_assemble_x2_general_star400_hbm_traffic.txt
+GENERAL_KERNEL_TRAFFIC_BYTES = 3.5131e10
 N_PANELS, ORDER, C_I, C_O, NK, K_MIN, K_MAX, NC, Q = 400, 16, 16.0, 1.0, 256, 1.0, 10.0, 2, 2
 
 
@@ -126,9 +131,9 @@ def main():
                                    "assembly_adjacent": ms_adj * 1e-3 / args.steps,
                                    "assembly_coinciding": ms_coi * 1e-3 / args.steps,
                                    "lu": ms_lu * 1e-3 / args.steps},
-        "roofline": {"kernel": "k_assemble<general, d=2> (general panel pairs)",
+        "roofline": {"kernel": "k_assemble_x2<general, d=2> (general panel pairs, 256 k x 2 c per launch)",
                      "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": GENERAL_KERNEL_TRAFFIC_BYTES,
                      "avg_launch_ms": avg_launch_s * 1e3, "launches": n_gen,
                      "flops_per_launch": fl_gen / max(n_gen, 1)},
     }

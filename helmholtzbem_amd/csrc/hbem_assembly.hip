@@ -537,23 +537,24 @@ __device__ __forceinline__ void bessel_jy01_f(double x, double y, double L2, dou
         j1 = pos ? J1 : 0.0;
         y0 = pos ? fma(L2, A, C) : 0.0;
         y1 = pos ? fma(L2, J1, fma(x, Dd, -kTwoOverPi * xinv)) : 0.0;
+    } else if (x <= 1e6) {
+        double s, c;
+        sincos_med(x, s, c);
+        const double cs = c + s, sc = s - c;
+        const double z = 64.0 * xinv * xinv;
+        double P0, Q0, P1, Q1;
+        horner4<HBEM_PQ_DEG>(bessel_table() + HBEM_TAB_PQ, z, P0, Q0, P1, Q1);
+        Q0 *= xinv;
+        Q1 *= xinv;
+        j0 = b * fma(P0, cs, -Q0 * sc);
+        y0 = b * fma(P0, sc, Q0 * cs);
+        j1 = b * fma(P1, sc, Q1 * cs);
+        y1 = b * fma(Q1, sc, -P1 * cs);
     } else {
         double s, c;
         sincos(x, &s, &c);
         const double cs = c + s, sc = s - c;
-        if (x <= 1e6) {
-            const double z = 64.0 * xinv * xinv;
-            double P0, Q0, P1, Q1;
-            horner4<HBEM_PQ_DEG>(bessel_table() + HBEM_TAB_PQ, z, P0, Q0, P1, Q1);
-            Q0 *= xinv;
-            Q1 *= xinv;
-            j0 = b * fma(P0, cs, -Q0 * sc);
-            y0 = b * fma(P0, sc, Q0 * cs);
-            j1 = b * fma(P1, sc, Q1 * cs);
-            y1 = b * fma(Q1, sc, -P1 * cs);
-        } else {
-            j0 = b * cs; y0 = b * sc; j1 = b * sc; y1 = -b * cs;
-        }
+        j0 = b * cs; y0 = b * sc; j1 = b * sc; y1 = -b * cs;
     }
 }
 

@@ -79,6 +79,33 @@ __device__ __forceinline__ void horner4(ctab_ptr c, double z, double& v0, double
     }
 }
 
+// sin x and cos x for 8 <= x <= 1e6: x = n pi/2 + r with n = rint(2x/pi) and r
+// from an FMA Cody-Waite reduction against pi/2 split into three doubles;
+// sin r = r + r u S(u), cos r = 1 - u/2 + u^2 C(u), u = r^2, with S and C from
+// hankel_tables.inc; the quadrant n mod 4 then swaps and negates.
+__device__ __forceinline__ void sincos_med(double x, double& s, double& c) {
+    const ctab_ptr T = bessel_table() + HBEM_TAB_SC;
+    const double n = rint(x * T[0]);
+    double r = fma(-n, T[1], x);
+    r = fma(-n, T[2], r);
+    r = fma(-n, T[3], r);
+    const double u = r * r;
+    const ctab_ptr k = T + 4;
+    double ps = fma_su(u, k[2 * HBEM_SC_DEG], k[2 * HBEM_SC_DEG - 2]);
+    double pc = fma_su(u, k[2 * HBEM_SC_DEG + 1], k[2 * HBEM_SC_DEG - 1]);
+#pragma unroll
+    for (int i = HBEM_SC_DEG - 2; i >= 0; --i) {
+        ps = fma_su(ps, u, k[2 * i]);
+        pc = fma_su(pc, u, k[2 * i + 1]);
+    }
+    const double sr = fma(r * u, ps, r);
+    const double cr = fma(u * u, pc, fma(-0.5, u, 1.0));
+    const int q = (int)n & 3;
+    const double s0 = (q & 1) ? cr : sr, c0 = (q & 1) ? sr : cr;
+    s = (q & 2) ? -s0 : s0;
+    c = ((q + 1) & 2) ? -c0 : c0;
+}
+
 // J0, Y0, J1, Y1 of x.  The branch points are the reference's
 // (Hankel1Real01::jy_01, cbessel.cpp:1896-1980):
 //   x <= 0       J0 = 1, J1 = 0, Y0 = Y1 = 0 (the reference zeroes Y there, cbessel.cpp:1976-1980)
@@ -107,7 +134,8 @@ __device__ __forceinline__ void bessel_jy01(double x, double& j0, double& y0, do
         y1 = pos ? fma(L2, J1, fma(xs, D, -kTwoOverPi * xinv)) : 0.0;
     } else {
         double s, c;
-        sincos(x, &s, &c);
+        if (x <= 1e6) sincos_med(x, s, c);
+        else sincos(x, &s, &c);
         const double b = 1.0 / sqrt(kPi * x);      // sqrt(2/(pi x)) / sqrt(2)
         const double cs = c + s, sc = s - c;       // sqrt2 cos(x - pi/4), sqrt2 sin(x - pi/4)
         if (x <= 1e6) {
