@@ -449,21 +449,23 @@ __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
 // the sums of the shape products with trial function n = h (entries q = m + 2h,
 // m = 0, 1), so a lane holds 9 (D + 1) complex sums instead of 17 (D + 1).
 // ---------------------------------------------------------------------------
-constexpr int kRec = 14;     // doubles per staged node record
+constexpr int kRec = 22;     // doubles per staged node record
 constexpr int kStage = 256;  // nodes per staging round
 
-// record: 0 r, 1 r^2, 2 1/r, 3 (2/pi) ln r, 4 1/sqrt r, 5 w cd, 6 w ct (coinciding: w cd2),
-//         7 w bt, 8 w ttp, 9 weight of the curl sum, 10 bi0, 11 bi1, 12 bj0, 13 bj1
+// record: 0 r, 1 r^2, 2 1/r, 3 (2/pi) ln r, 4 1/sqrt r, 5 weight of the curl sum,
+//         6 + 8h + {0,1} K direct, {2,3} K transposed, {4,5} W, {6,7} V coefficient
+//         of the entry q = m + 2h (m = 0, 1): quadrature weight x geometry factor x
+//         shape product, the per-node factors of node_accum for lane half h
 template <int CLS>
 __device__ __forceinline__ void stage_node(const AsmParams& P, const double* __restrict__ geo, unsigned f, bool swap,
                                            int nd, double* __restrict__ rec) {
     const double* o = geo + (size_t)nd * 6;
-    double w, bi0, bi1, bj0, bj1, cd, ct, bt;
+    double w, bi[2], bj[2], cd, ct, bt;
     if (CLS == 2) {
         const int ns = P.ns, a = nd / ns, b = nd - a * ns;
         const double s = P.xg[a], t = P.xg[b];
         w = P.wg[a] * P.wg[b];
-        bi0 = s; bi1 = 1.0 - s; bj0 = t; bj1 = 1.0 - t;
+        bi[0] = s; bi[1] = 1.0 - s; bj[0] = t; bj[1] = 1.0 - t;
         cd = (f & 1) ? 0.0 : o[2];
         ct = (f & 2) ? 0.0 : o[3];
         bt = (f & 4) ? 0.0 : 2.0 * o[4];
@@ -473,8 +475,8 @@ __device__ __forceinline__ void stage_node(const AsmParams& P, const double* __r
         const double t = P.xc[a], s = t * P.xc[b];
         w = t * P.wc[a] * P.wc[b];
         const double si = h == 0 ? s : t, sj = h == 0 ? t : s;
-        bi0 = swap ? si : 1.0 - si; bi1 = swap ? 1.0 - si : si;
-        bj0 = swap ? 1.0 - sj : sj; bj1 = swap ? sj : 1.0 - sj;
+        bi[0] = swap ? si : 1.0 - si; bi[1] = swap ? 1.0 - si : si;
+        bj[0] = swap ? 1.0 - sj : sj; bj[1] = swap ? sj : 1.0 - sj;
         const unsigned fh = f >> (3 * h);
         cd = (fh & 1) ? 0.0 : o[2];
         ct = (fh & 2) ? 0.0 : o[3];
@@ -483,7 +485,7 @@ __device__ __forceinline__ void stage_node(const AsmParams& P, const double* __r
         const int nc = P.nc, a = nd / nc, b = nd - a * nc;
         const double t = P.xc[b], s = t * (1.0 - P.xc[a]);
         w = t * P.wc[a] * P.wc[b];
-        bi0 = s; bi1 = 1.0 - s; bj0 = t; bj1 = 1.0 - t;
+        bi[0] = s; bi[1] = 1.0 - s; bj[0] = t; bj[1] = 1.0 - t;
         cd = (f & 1) ? 0.0 : o[2];
         ct = (f & 8) ? 0.0 : o[3];     // second triangle's K factor
         bt = (f & 4) ? 0.0 : 2.0 * o[4];
@@ -494,12 +496,27 @@ __device__ __forceinline__ void stage_node(const AsmParams& P, const double* __r
     rec[2] = r > 0.0 ? 1.0 / r : 0.0;
     rec[3] = r > 0.0 ? kTwoOverPi * log(r) : 0.0;
     rec[4] = r > 0.0 ? 1.0 / sqrt(r) : 0.0;
-    rec[5] = w * cd;
-    rec[6] = w * ct;
-    rec[7] = w * bt;
-    rec[8] = w * o[5];
-    rec[9] = CLS == 0 ? 2.0 * w : w;
-    rec[10] = bi0; rec[11] = bi1; rec[12] = bj0; rec[13] = bj1;
+    rec[5] = CLS == 0 ? 2.0 * w : w;
+    const double wcd = w * cd, wct = w * ct, wbt = w * bt, wtt = w * o[5];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const double pbb = bi[m] * bj[h];   // test m at the panel-i point, trial h at the panel-j point
+            const double pbt = bj[m] * bi[h];   // exchanged roles
+            double* c = rec + 6 + 8 * h;
+            if (CLS == 0) {
+                c[m] = wcd * pbb + wct * pbt;
+                c[2 + m] = 0.0;
+                c[4 + m] = wbt * (pbb + pbt);
+                c[6 + m] = wtt * (pbb + pbt);
+            } else {
+                c[m] = wcd * pbb;
+                c[2 + m] = wct * pbt;
+                c[4 + m] = wbt * pbb;
+                c[6 + m] = wtt * pbb;
+            }
+        }
 }
 
 // padding record for an odd node count: finite argument, zero weights
@@ -562,29 +579,15 @@ __device__ __forceinline__ void bessel_jy01_f(double x, double y, double L2, dou
 template <int CLS, int D>
 __device__ __forceinline__ void node_accum_x(AccX<D>& A, const double* __restrict__ R, int h, double j0, double y0,
                                              double j1, double y1) {
-    const double r = R[0], r2 = R[1], rinv = R[2];
+    const double r = R[0], r2 = R[1], rinv = R[2], wa = R[5];
     const cdbl h0 = mk(j0, y0);
     const cdbl h1r = mk(j1 * rinv, y1 * rinv);
     const cdbl hr = mk(j1 * r, y1 * r);
     const cdbl h0r2 = mk(j0 * r2, y0 * r2);
-    const double wcd = R[5], wct = R[6], wbt = R[7], wtt = R[8], wa = R[9];
-    const double bih = R[10 + h], bjh = R[12 + h];
+    const double* __restrict__ c = R + 6 + 8 * h;
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-        const double pbb = R[10 + m] * bjh;   // test m at the panel-i point, trial h at the panel-j point
-        const double pbt = R[12 + m] * bih;   // exchanged roles
-        double ckd, ckt, cb, cg;
-        if (CLS == 0) {
-            ckd = wcd * pbb + wct * pbt;
-            ckt = 0.0;
-            cb = wbt * (pbb + pbt);
-            cg = wtt * (pbb + pbt);
-        } else {
-            ckd = wcd * pbb;
-            ckt = wct * pbt;
-            cb = wbt * pbb;
-            cg = wtt * pbb;
-        }
+        const double ckd = c[m], ckt = c[2 + m], cb = c[4 + m], cg = c[6 + m];
         axpy(A.kd[0][m], ckd, h1r);
         if (CLS != 0) axpy(A.kt[0][m], ckt, h1r);
         axpy(A.wb[0][m], cb, h0);

@@ -63,19 +63,36 @@ __device__ __forceinline__ double fma_su(double a, double b, double c) {
 
 // Four polynomials of degree DEG in one variable, evaluated in lockstep
 // (independent Horner chains interleaved); c[4 i + f] is coefficient i of
-// polynomial f.
+// polynomial f.  The coefficients of two consecutive steps are read together
+// (one 64-byte scalar load, one wait per two steps).
 template <int DEG>
 __device__ __forceinline__ void horner4(ctab_ptr c, double z, double& v0, double& v1, double& v2, double& v3) {
-    v0 = fma_su(z, c[4 * DEG + 0], c[4 * DEG - 4]);
-    v1 = fma_su(z, c[4 * DEG + 1], c[4 * DEG - 3]);
-    v2 = fma_su(z, c[4 * DEG + 2], c[4 * DEG - 2]);
-    v3 = fma_su(z, c[4 * DEG + 3], c[4 * DEG - 1]);
+    double k[8];
 #pragma unroll
-    for (int i = DEG - 2; i >= 0; --i) {
-        v0 = fma_su(v0, z, c[4 * i + 0]);
-        v1 = fma_su(v1, z, c[4 * i + 1]);
-        v2 = fma_su(v2, z, c[4 * i + 2]);
-        v3 = fma_su(v3, z, c[4 * i + 3]);
+    for (int f = 0; f < 8; ++f) k[f] = c[4 * DEG - 4 + f];
+    v0 = fma_su(z, k[4], k[0]);
+    v1 = fma_su(z, k[5], k[1]);
+    v2 = fma_su(z, k[6], k[2]);
+    v3 = fma_su(z, k[7], k[3]);
+    int i = DEG - 2;
+#pragma unroll
+    for (; i >= 1; i -= 2) {
+#pragma unroll
+        for (int f = 0; f < 8; ++f) k[f] = c[4 * (i - 1) + f];
+        v0 = fma_su(v0, z, k[4]);
+        v1 = fma_su(v1, z, k[5]);
+        v2 = fma_su(v2, z, k[6]);
+        v3 = fma_su(v3, z, k[7]);
+        v0 = fma_su(v0, z, k[0]);
+        v1 = fma_su(v1, z, k[1]);
+        v2 = fma_su(v2, z, k[2]);
+        v3 = fma_su(v3, z, k[3]);
+    }
+    if (i == 0) {
+        v0 = fma_su(v0, z, c[0]);
+        v1 = fma_su(v1, z, c[1]);
+        v2 = fma_su(v2, z, c[2]);
+        v3 = fma_su(v3, z, c[3]);
     }
 }
 
