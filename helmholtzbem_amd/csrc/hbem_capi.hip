@@ -78,7 +78,7 @@ struct hbem_geometry {
     int ngen = 0, nadj = 0, ncoi = 0;
     DBuf d_M, d_Frm, d_Fcm, d_Finvcm, d_Finvc;
     // work space
-    DBuf w_L, w_T0, w_T1, w_T2, w_piv, w_perm, w_X, w_Y, w_work, w_R, w_u, w_v, w_a, w_b, w_sig, w_dsig, w_kap,
+    DBuf w_LUT, w_L, w_T0, w_T1, w_T2, w_piv, w_perm, w_X, w_Y, w_work, w_R, w_u, w_v, w_a, w_b, w_sig, w_dsig, w_kap,
         w_kr, w_cv, w_qi, w_qo, w_tmp, w_out, w_part;
     // profiling
     bool prof = false;
@@ -201,6 +201,7 @@ struct DenseCtx {
     hipStream_t st;
     int n, batch, nc, q;
     const cdbl* LU;
+    const cdbl* LUT;     // transposed factors (adjoint solves)
     const int* perm;
     const double* Frm;   // F row-major (apply F), null = identity
     const double* Fcm;   // F column-major (apply F^T)
@@ -213,12 +214,12 @@ struct DenseCtx {
 void tp_solve(DenseCtx& c, cdbl* X, cdbl* Y, bool adj) {
     if (c.Frm) {
         hbem::real_apply_thin_batched(c.Frm, c.n, c.batch, X, Y, c.nc, c.st);
-        hbem::lu_solve_thin_batched(c.LU, c.perm, c.n, c.batch, Y, c.nc, adj, c.work, c.st);
+        hbem::lu_solve_thin_batched(c.LU, c.LUT, c.perm, c.n, c.batch, Y, c.nc, adj, c.work, c.st);
         hbem::real_apply_thin_batched(c.Fcm, c.n, c.batch, Y, X, c.nc, c.st);
         HIPC(hipMemcpyAsync(Y, X, sizeof(cdbl) * (size_t)c.n * c.nc * c.batch, hipMemcpyDeviceToDevice, c.st));
     } else {
         if (Y != X) HIPC(hipMemcpyAsync(Y, X, sizeof(cdbl) * (size_t)c.n * c.nc * c.batch, hipMemcpyDeviceToDevice, c.st));
-        hbem::lu_solve_thin_batched(c.LU, c.perm, c.n, c.batch, Y, c.nc, adj, c.work, c.st);
+        hbem::lu_solve_thin_batched(c.LU, c.LUT, c.perm, c.n, c.batch, Y, c.nc, adj, c.work, c.st);
     }
 }
 
@@ -546,7 +547,7 @@ namespace {
 struct Standalone {
     std::mutex mu;
     hipStream_t st = nullptr;
-    DBuf T, T1, T2, piv, perm, X, Y, work, u, v, sig, out, W, Wpiv, Wperm, p, p2, ud, tmp;
+    DBuf T, T1, T2, piv, perm, X, Y, work, u, v, sig, out, W, Wpiv, Wperm, p, p2, ud, tmp, LUT;
     void init() {
         if (!st) {
             HIPC(hipSetDevice(g_device));
@@ -573,8 +574,11 @@ double standalone_sv(const void* T, int n, const void* R, int nc, int q, bool wa
     int* piv = s.piv.as<int>(n);
     int* perm = s.perm.as<int>(n);
     hbem::lu_factor_batched(dT, n, 1, piv, perm, s.st);
+    cdbl* dLUT = s.LUT.as<cdbl>(nn);
+    hbem::transpose_batched(dT, dLUT, n, 1, s.st);
     DenseCtx c;
-    c.st = s.st; c.n = n; c.batch = 1; c.nc = nc; c.q = q; c.LU = dT; c.perm = perm; c.Frm = nullptr; c.Fcm = nullptr;
+    c.st = s.st; c.n = n; c.batch = 1; c.nc = nc; c.q = q; c.LU = dT; c.LUT = dLUT; c.perm = perm;
+    c.Frm = nullptr; c.Fcm = nullptr;
     c.X = s.X.as<cdbl>((size_t)n * nc);
     c.Y = s.Y.as<cdbl>((size_t)n * nc);
     c.work = s.work.as<cdbl>(2 * (size_t)n * nc);
@@ -679,7 +683,7 @@ int hbem_rsvd_sv_der2(const void* T, const void* Td, const void* Tdd, int n, con
     hbem::gemv_batched(dT1, 0, du + n, p, n, 1, false, s.st);
     hbem::gemv_batched(dT1, 0, du, p + n, n, 1, true, s.st);
     cdbl* wk = s.work.as<cdbl>(2 * (size_t)m2);
-    hbem::lu_solve_thin_batched(W, wperm, m2, 1, p, 1, false, wk, s.st);   // p <- W^{-1} p
+    hbem::lu_solve_thin_batched(W, nullptr, wperm, m2, 1, p, 1, false, wk, s.st);   // p <- W^{-1} p
     std::vector<cdbl> ud(m2);
     HIPC(hipMemcpyAsync(ud.data(), p, m2 * sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
     HIPC(hipStreamSynchronize(s.st));
@@ -705,7 +709,7 @@ int hbem_rsvd_sv_der2(const void* T, const void* Td, const void* Tdd, int n, con
         rhs[l] = hbem::mk(v.real(), v.imag());
     }
     HIPC(hipMemcpyAsync(p, rhs.data(), m2 * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
-    hbem::lu_solve_thin_batched(W, wperm, m2, 1, p, 1, false, wk, s.st);
+    hbem::lu_solve_thin_batched(W, nullptr, wperm, m2, 1, p, 1, false, wk, s.st);
     cdbl sol;
     HIPC(hipMemcpyAsync(&sol, p + m, sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
     HIPC(hipStreamSynchronize(s.st));
@@ -738,7 +742,7 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
     int B = 256;
     const size_t nslot = 16 * (size_t)(D + 1);
     const size_t maxchunks = (size_t)g->nadj * ((2 * (size_t)g->nc * g->nc + 511) / 512);
-    const size_t per = nn * sizeof(cdbl) * (D + 1) +
+    const size_t per = nn * sizeof(cdbl) * (D + 2) +   // T, T', T'' and the transposed factors
                        (size_t)2 * ((size_t)g->npairs + maxchunks) * nslot * sizeof(cdbl);
     while (B > 1 && (size_t)B * per > ((size_t)64 << 30)) B /= 2;
     cdbl* dR = g->w_R.as<cdbl>((size_t)n * nc);
@@ -777,8 +781,10 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
         }
+        cdbl* LUT = g->w_LUT.as<cdbl>(nn * nb);
+        hbem::transpose_batched(T0, LUT, n, nb, g->st);
         DenseCtx c;
-        c.st = g->st; c.n = n; c.batch = nb; c.nc = nc; c.q = q; c.LU = T0; c.perm = perm;
+        c.st = g->st; c.n = n; c.batch = nb; c.nc = nc; c.q = q; c.LU = T0; c.LUT = LUT; c.perm = perm;
         c.Frm = g->proj ? (const double*)g->d_Frm.p : nullptr;
         c.Fcm = g->proj ? (const double*)g->d_Fcm.p : nullptr;
         c.X = g->w_X.as<cdbl>((size_t)n * nc * nb);

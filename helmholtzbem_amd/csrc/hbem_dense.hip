@@ -228,181 +228,162 @@ void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStrea
 // ---------------------------------------------------------------------------
 // Thin triangular solves
 // ---------------------------------------------------------------------------
-// Triangular operand of the current pass.
-//   kind 0: L  (unit lower)      L(i,j) = LU(i,j)
-//   kind 1: U  (upper)           U(i,j) = LU(i,j)
-//   kind 2: U^H (lower)          (U^H)(i,j) = conj(LU(j,i))
-//   kind 3: L^H (unit upper)     (L^H)(i,j) = conj(LU(j,i))
-__device__ __forceinline__ cdbl tri_el(const cdbl* lu, int n, int kind, int i, int j) {
-    if (kind <= 1) return lu[i + (size_t)j * n];
-    return cconj(lu[j + (size_t)i * n]);
+// Triangular operand of a pass, always read column-major and coalesced:
+//   kind 0: L  (unit lower)    L(i,j) = LU(i,j)
+//   kind 1: U  (upper)         U(i,j) = LU(i,j)
+//   kind 2: U^H (lower)        U^H(i,j) = conj(LU(j,i)) = conj(LUT(i,j))
+//   kind 3: L^H (unit upper)   L^H(i,j) = conj(LUT(i,j))
+// with LUT the transposed copy of the factors made after the factorisation.
+__device__ __forceinline__ cdbl op_el(const cdbl* M, int n, int kind, int i, int j) {
+    const cdbl v = M[i + (size_t)j * n];
+    return kind <= 1 ? v : cconj(v);
 }
 
 constexpr int kMaxNc = 8;
-constexpr int kTS = 65;   // padded row stride of the 64 x 64 LDS tile
-
-// tile[i * kTS + j] = T(i0 + i, j0 + j) for i < rows, j < cols, 0 elsewhere.
-// The loop order follows the memory layout of the operand (column-major LU
-// for kinds 0/1, its rows for kinds 2/3), so the global reads coalesce.
-__device__ __forceinline__ void load_tile(const cdbl* lu, int n, int kind, int i0, int rows, int j0, int cols,
-                                          cdbl* tile) {
-    for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
-        int i, j;
-        if (kind <= 1) { i = e & 63; j = e >> 6; } else { j = e & 63; i = e >> 6; }
-        tile[i * kTS + j] = (i < rows && j < cols) ? tri_el(lu, n, kind, i0 + i, j0 + j) : mk(0, 0);
-    }
-}
-
-// Diagonal block solve by wavefront 0 on w[r0 .. r0 + bs), the block's
-// operand in `tile`; lower (forward) or upper (backward).  The solution is
-// also left in xb[j * kMaxNc + c] (zero for j >= bs).
-template <int NC>
-__device__ void diag_solve(const cdbl* tile, int r0, int bs, bool lower, bool unit, cdbl* w, int n, cdbl* xb) {
-    const int lane = threadIdx.x & 63;
-    cdbl x[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) x[c] = lane < bs ? w[r0 + lane + (size_t)c * n] : mk(0, 0);
-    for (int step = 0; step < bs; ++step) {
-        const int j = lower ? step : bs - 1 - step;
-        if (!unit && lane == j) {
-            const cdbl inv = crecip(tile[j * kTS + j]);
-#pragma unroll
-            for (int c = 0; c < NC; ++c) x[c] = x[c] * inv;
-        }
-        const bool below = lower ? (lane > j && lane < bs) : (lane < j);
-        const cdbl l = below ? tile[lane * kTS + j] : mk(0, 0);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const cdbl xj = shfl_c(x[c], j);
-            if (below) cfms(x[c], l, xj);
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-        if (lane < bs) w[r0 + lane + (size_t)c * n] = x[c];
-        xb[lane * kMaxNc + c] = lane < bs ? x[c] : mk(0, 0);
-    }
-}
-
-// Rows [ia, ib) of w -= T(rows, r0 : r0 + bs) xb, in 64-row tiles through LDS;
-// four partial sums over 16 columns each per row, added in a fixed order.
-template <int NC>
-__device__ void trail_update(const cdbl* lu, int n, int kind, int r0, int bs, int ia, int ib, cdbl* w,
-                             const cdbl* xb, cdbl* tile, cdbl* part) {
-    const int tid = threadIdx.x;
-    for (int i0 = ia; i0 < ib; i0 += 64) {
-        const int rows = ib - i0 < 64 ? ib - i0 : 64;
-        load_tile(lu, n, kind, i0, rows, r0, bs, tile);
-        __syncthreads();
-        const int i = tid & 63, pt = tid >> 6;
-        cdbl s[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) s[c] = mk(0, 0);
-#pragma unroll 4
-        for (int jj = 0; jj < 16; ++jj) {
-            const int j = pt * 16 + jj;
-            const cdbl t = tile[i * kTS + j];
-#pragma unroll
-            for (int c = 0; c < NC; ++c) cfma(s[c], t, xb[j * kMaxNc + c]);
-        }
-#pragma unroll
-        for (int c = 0; c < NC; ++c) part[(pt * 64 + i) * kMaxNc + c] = s[c];
-        __syncthreads();
-        if (tid < rows) {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                const cdbl acc = ((part[tid * kMaxNc + c] + part[(64 + tid) * kMaxNc + c]) +
-                                  part[(128 + tid) * kMaxNc + c]) + part[(192 + tid) * kMaxNc + c];
-                cdbl& y = w[i0 + tid + (size_t)c * n];
-                y = y - acc;
-            }
-        }
-    }
-}
+constexpr int kSolveThreads = 1024;
+constexpr int kTS = 65;   // padded row stride of the 64 x 64 LDS diagonal block
 
 struct SolveLds {
     cdbl tile[64 * kTS];
     cdbl xb[64 * kMaxNc];
-    cdbl part[256 * kMaxNc];
 };
 
-// forward (lower) substitution, unit or not, on w (n x nc, ld n)
+// Blocked substitution in 64-row blocks [64 b, 64 b + bs): the diagonal block
+// is staged in LDS and solved by wavefront 0 with cross-lane broadcasts; the
+// rows outside the block are then updated with one thread per row (loads of
+// the operand's column j coalesce across threads).
 template <int NC>
-__device__ void fwd_subst(const cdbl* lu, int n, int kind, bool unit, cdbl* w, SolveLds& S) {
-    for (int r0 = 0; r0 < n; r0 += 64) {
-        const int bs = n - r0 < 64 ? n - r0 : 64;
-        load_tile(lu, n, kind, r0, bs, r0, bs, S.tile);
+__device__ void subst(const cdbl* M, int n, int kind, bool lower, bool unit, cdbl* w, SolveLds& S) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int nblk = (n + 63) / 64;
+    for (int s = 0; s < nblk; ++s) {
+        const int blk = lower ? s : nblk - 1 - s;
+        const int r0 = 64 * blk, bs = min(64, n - r0);
+        for (int e = tid; e < 64 * 64; e += blockDim.x) {
+            const int i = e & 63, j = e >> 6;
+            S.tile[i * kTS + j] = (i < bs && j < bs) ? op_el(M, n, kind, r0 + i, r0 + j) : mk(0, 0);
+        }
         __syncthreads();
-        if (threadIdx.x < 64) diag_solve<NC>(S.tile, r0, bs, true, unit, w, n, S.xb);
+        if (tid < 64) {
+            cdbl x[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) x[c] = lane < bs ? w[r0 + lane + (size_t)c * n] : mk(0, 0);
+            for (int step = 0; step < bs; ++step) {
+                const int j = lower ? step : bs - 1 - step;
+                if (!unit && lane == j) {
+                    const cdbl inv = crecip(S.tile[j * kTS + j]);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) x[c] = x[c] * inv;
+                }
+                const bool below = lower ? (lane > j && lane < bs) : (lane < j);
+                const cdbl l = below ? S.tile[lane * kTS + j] : mk(0, 0);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    const cdbl xj = shfl_c(x[c], j);
+                    if (below) cfms(x[c], l, xj);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                if (lane < bs) w[r0 + lane + (size_t)c * n] = x[c];
+                S.xb[lane * kMaxNc + c] = lane < bs ? x[c] : mk(0, 0);
+            }
+        }
         __syncthreads();
-        trail_update<NC>(lu, n, kind, r0, bs, r0 + bs, n, w, S.xb, S.tile, S.part);
-        __syncthreads();
+        const int ia = lower ? r0 + bs : 0, ib = lower ? n : r0;
+        for (int i = ia + tid; i < ib; i += blockDim.x) {
+            cdbl acc[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) acc[c] = mk(0, 0);
+#pragma unroll 8
+            for (int j = 0; j < bs; ++j) {
+                const cdbl t = op_el(M, n, kind, i, r0 + j);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) cfma(acc[c], t, S.xb[j * kMaxNc + c]);
+            }
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                cdbl& y = w[i + (size_t)c * n];
+                y = y - acc[c];
+            }
+        }
+        // the next round's diagonal solve reads w after a barrier (tile staging)
     }
+    __syncthreads();
 }
 
-// backward (upper) substitution
+// one workgroup of kSolveThreads threads per matrix
 template <int NC>
-__device__ void bwd_subst(const cdbl* lu, int n, int kind, bool unit, cdbl* w, SolveLds& S) {
-    for (int r1 = n; r1 > 0; r1 -= 64) {
-        const int r0 = r1 - 64 > 0 ? r1 - 64 : 0;
-        const int bs = r1 - r0;
-        load_tile(lu, n, kind, r0, bs, r0, bs, S.tile);
-        __syncthreads();
-        if (threadIdx.x < 64) diag_solve<NC>(S.tile, r0, bs, false, unit, w, n, S.xb);
-        __syncthreads();
-        trail_update<NC>(lu, n, kind, r0, bs, 0, r0, w, S.xb, S.tile, S.part);
-        __syncthreads();
-    }
-}
-
-// one workgroup of 256 threads per matrix (trail_update assumes 256)
-template <int NC>
-__global__ __launch_bounds__(256) void k_solve_thin(const cdbl* __restrict__ LU, const int* __restrict__ perm, int n,
-                                                    cdbl* __restrict__ X, int adj, cdbl* __restrict__ work) {
-    const int nc = NC;
+__global__ __launch_bounds__(kSolveThreads) void k_solve_thin(const cdbl* __restrict__ LU, const cdbl* __restrict__ LUT,
+                                                              const int* __restrict__ perm, int n, cdbl* __restrict__ X,
+                                                              int adj, cdbl* __restrict__ work) {
     __shared__ SolveLds S;
     const int b = blockIdx.x;
-    const cdbl* lu = LU + (size_t)b * n * n;
     const int* p = perm + (size_t)b * n;
-    cdbl* x = X + (size_t)b * n * nc;
-    cdbl* w = work + (size_t)b * n * nc;
+    cdbl* x = X + (size_t)b * n * NC;
+    cdbl* w = work + (size_t)b * n * NC;
     const int tid = threadIdx.x;
     if (!adj) {
         // A = P^T L U:  w = P x, L y = w, U z = y
-        for (int e = tid; e < n * nc; e += blockDim.x) {
+        const cdbl* lu = LU + (size_t)b * n * n;
+        for (int e = tid; e < n * NC; e += blockDim.x) {
             const int i = e % n, c = e / n;
             w[e] = x[p[i] + (size_t)c * n];
         }
         __syncthreads();
-        fwd_subst<NC>(lu, n, 0, true, w, S);
-        bwd_subst<NC>(lu, n, 1, false, w, S);
-        for (int e = tid; e < n * nc; e += blockDim.x) x[e] = w[e];
+        subst<NC>(lu, n, 0, true, true, w, S);
+        subst<NC>(lu, n, 1, false, false, w, S);
+        for (int e = tid; e < n * NC; e += blockDim.x) x[e] = w[e];
     } else {
         // A^H = U^H L^H P:  U^H y = x, L^H z = y, x = P^T z
-        for (int e = tid; e < n * nc; e += blockDim.x) w[e] = x[e];
+        const cdbl* lut = LUT + (size_t)b * n * n;
+        for (int e = tid; e < n * NC; e += blockDim.x) w[e] = x[e];
         __syncthreads();
-        fwd_subst<NC>(lu, n, 2, false, w, S);
-        bwd_subst<NC>(lu, n, 3, true, w, S);
-        for (int e = tid; e < n * nc; e += blockDim.x) {
+        subst<NC>(lut, n, 2, true, false, w, S);
+        subst<NC>(lut, n, 3, false, true, w, S);
+        for (int e = tid; e < n * NC; e += blockDim.x) {
             const int i = e % n, c = e / n;
             x[p[i] + (size_t)c * n] = w[e];
         }
     }
 }
 
-void lu_solve_thin_batched(const cdbl* LU, const int* perm, int n, int batch, cdbl* X, int nc, bool adj, cdbl* work,
-                           hipStream_t st) {
+void lu_solve_thin_batched(const cdbl* LU, const cdbl* LUT, const int* perm, int n, int batch, cdbl* X, int nc,
+                           bool adj, cdbl* work, hipStream_t st) {
     const int a = adj ? 1 : 0;
+    const dim3 g(batch), t(kSolveThreads);
     switch (nc) {
-    case 1: hipLaunchKernelGGL(k_solve_thin<1>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
-    case 2: hipLaunchKernelGGL(k_solve_thin<2>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
-    case 3: hipLaunchKernelGGL(k_solve_thin<3>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
-    case 4: hipLaunchKernelGGL(k_solve_thin<4>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
-    case 5: hipLaunchKernelGGL(k_solve_thin<5>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
-    case 6: hipLaunchKernelGGL(k_solve_thin<6>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
-    case 7: hipLaunchKernelGGL(k_solve_thin<7>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
-    default: hipLaunchKernelGGL(k_solve_thin<8>, dim3(batch), dim3(256), 0, st, LU, perm, n, X, a, work); break;
+    case 1: hipLaunchKernelGGL(k_solve_thin<1>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
+    case 2: hipLaunchKernelGGL(k_solve_thin<2>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
+    case 3: hipLaunchKernelGGL(k_solve_thin<3>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
+    case 4: hipLaunchKernelGGL(k_solve_thin<4>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
+    case 5: hipLaunchKernelGGL(k_solve_thin<5>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
+    case 6: hipLaunchKernelGGL(k_solve_thin<6>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
+    case 7: hipLaunchKernelGGL(k_solve_thin<7>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
+    default: hipLaunchKernelGGL(k_solve_thin<8>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
     }
+}
+
+// AT_b = A_b^T for batch square matrices (32 x 32 tiles through LDS)
+__global__ __launch_bounds__(256) void k_transpose(const cdbl* __restrict__ A, cdbl* __restrict__ AT, int n) {
+    __shared__ cdbl t[32][33];
+    const size_t off = (size_t)blockIdx.z * n * n;
+    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + tx, j = j0 + r;
+        if (i < n && j < n) t[r][tx] = A[off + i + (size_t)j * n];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int i = j0 + tx, j = i0 + r;
+        if (i < n && j < n) AT[off + i + (size_t)j * n] = t[tx][r];
+    }
+}
+
+void transpose_batched(const cdbl* A, cdbl* AT, int n, int batch, hipStream_t st) {
+    const dim3 g((n + 31) / 32, (n + 31) / 32, batch);
+    hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, A, AT, n);
 }
 
 // ---------------------------------------------------------------------------

@@ -53,9 +53,11 @@ void launch_form_T(const cdbl* L, int nk, int nslot, const int* qi, const int* q
 // stride n*n (column-major, lda = n) unless noted.
 void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStream_t st);
 // X (n x nc, stride n*nc per matrix) <- op(A)^{-1} X with A = P^T L U from lu_factor_batched;
-// adj = false solves A X = B, adj = true solves A^H X = B.
-void lu_solve_thin_batched(const cdbl* LU, const int* perm, int n, int batch, cdbl* X, int nc, bool adj,
-                           cdbl* work, hipStream_t st);
+// adj = false solves A X = B, adj = true solves A^H X = B and reads LUT, the
+// transposed factors (transpose_batched), instead of LU.
+void lu_solve_thin_batched(const cdbl* LU, const cdbl* LUT, const int* perm, int n, int batch, cdbl* X, int nc,
+                           bool adj, cdbl* work, hipStream_t st);
+void transpose_batched(const cdbl* A, cdbl* AT, int n, int batch, hipStream_t st);
 // Y = op(F) X for a real dense n x n F given row-major (so op = F) -- pass the
 // column-major array to apply F^T.
 void real_apply_thin_batched(const double* Frm, int n, int batch, const cdbl* X, cdbl* Y, int nc, hipStream_t st);
