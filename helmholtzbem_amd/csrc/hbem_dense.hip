@@ -36,92 +36,169 @@ __device__ __forceinline__ cdbl shfl_c(cdbl v, int src) {
 // ---------------------------------------------------------------------------
 // LU with partial pivoting
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_lu_panel(cdbl* __restrict__ A, int n, int j0, int jb, int* __restrict__ ipiv) {
+// Panel factorisation of columns [j0, j0 + jb) (jb <= 32), one workgroup of
+// 1024 threads per matrix: per column the pivot (largest |a|^2, first index on
+// ties) by wavefront shuffles and one LDS exchange, the row interchange inside
+// the panel, then the rank-1 update of the panel with one thread per row and
+// the pivot row staged in LDS.
+__global__ __launch_bounds__(1024) void k_lu_panel(cdbl* __restrict__ A, int n, int j0, int jb, int* __restrict__ ipiv) {
     cdbl* a = A + (size_t)blockIdx.x * n * n;
     int* ip = ipiv + (size_t)blockIdx.x * n;
-    __shared__ double sv[256];
-    __shared__ int si[256];
-    const int tid = threadIdx.x;
+    __shared__ double sv[16];
+    __shared__ int si[16];
+    __shared__ int spiv;
+    __shared__ cdbl prow[32];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int c = 0; c < jb; ++c) {
         const int col = j0 + c;
         double best = -1.0;
         int bi = n;
-        for (int i = col + tid; i < n; i += 256) {
-            double v = cabs2(a[i + (size_t)col * n]);
+        for (int i = col + tid; i < n; i += blockDim.x) {
+            const double v = cabs2(a[i + (size_t)col * n]);
             if (v > best) { best = v; bi = i; }
         }
-        sv[tid] = best;
-        si[tid] = bi;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (tid < s) {
-                double v2 = sv[tid + s];
-                int i2 = si[tid + s];
-                if (v2 > sv[tid] || (v2 == sv[tid] && i2 < si[tid])) { sv[tid] = v2; si[tid] = i2; }
-            }
-            __syncthreads();
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double v2 = __shfl_xor(best, o, 64);
+            const int i2 = __shfl_xor(bi, o, 64);
+            if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
         }
-        const int piv = si[0];
+        if (lane == 0) { sv[wv] = best; si[wv] = bi; }
         __syncthreads();
-        if (tid == 0) ip[col] = piv;
-        if (piv != col)
-            for (int cc = tid; cc < jb; cc += 256) {
-                cdbl t = a[col + (size_t)(j0 + cc) * n];
-                a[col + (size_t)(j0 + cc) * n] = a[piv + (size_t)(j0 + cc) * n];
-                a[piv + (size_t)(j0 + cc) * n] = t;
+        if (tid < 64) {
+            const int nw = blockDim.x >> 6;
+            best = lane < nw ? sv[lane] : -1.0;
+            bi = lane < nw ? si[lane] : n;
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) {
+                const double v2 = __shfl_xor(best, o, 64);
+                const int i2 = __shfl_xor(bi, o, 64);
+                if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
             }
+            // no candidate (a column of NaNs): keep the row, never index past n
+            if (tid == 0) { const int pv = bi < n ? bi : col; spiv = pv; ip[col] = pv; }
+        }
         __syncthreads();
-        const cdbl d = a[col + (size_t)col * n];
+        const int piv = spiv;
+        if (tid < jb) {
+            cdbl* pc = a + (size_t)(j0 + tid) * n;
+            const cdbl u = pc[piv];
+            if (piv != col) { pc[piv] = pc[col]; pc[col] = u; }
+            prow[tid] = u;
+        }
+        __syncthreads();
+        const cdbl d = prow[c];
         const cdbl inv = (d.x == 0.0 && d.y == 0.0) ? mk(0, 0) : crecip(d);
-        for (int i = col + 1 + tid; i < n; i += 256) {
+        for (int i = col + 1 + tid; i < n; i += blockDim.x) {
             const cdbl l = a[i + (size_t)col * n] * inv;
             a[i + (size_t)col * n] = l;
-            for (int cc = c + 1; cc < jb; ++cc) cfms(a[i + (size_t)(j0 + cc) * n], l, a[col + (size_t)(j0 + cc) * n]);
+            // eight columns per round, loads issued together
+            for (int cc0 = c + 1; cc0 < jb; cc0 += 8) {
+                cdbl v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (cc0 + u < jb) v[u] = a[i + (size_t)(j0 + cc0 + u) * n];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (cc0 + u < jb) {
+                        cfms(v[u], l, prow[cc0 + u]);
+                        a[i + (size_t)(j0 + cc0 + u) * n] = v[u];
+                    }
+            }
         }
         __syncthreads();
     }
 }
 
-__global__ void k_lu_swap(cdbl* __restrict__ A, int n, int j0, int jb, const int* __restrict__ ipiv) {
-    const int b = blockIdx.y;
-    cdbl* a = A + (size_t)b * n * n;
-    const int* ip = ipiv + (size_t)b * n;
-    int co = blockIdx.x * blockDim.x + threadIdx.x;
-    if (co >= n - jb) return;
-    const int c = co < j0 ? co : co + jb;
-    for (int k = j0; k < j0 + jb; ++k) {
-        const int p = ip[k];
-        if (p != k) {
-            cdbl t = a[k + (size_t)c * n];
-            a[k + (size_t)c * n] = a[p + (size_t)c * n];
-            a[p + (size_t)c * n] = t;
-        }
-    }
-}
-
-// A12 <- L11^{-1} A12 (L11 unit lower jb x jb)
-__global__ __launch_bounds__(256) void k_lu_trsm(cdbl* __restrict__ A, int n, int j0, int jb) {
-    const int b = blockIdx.y;
-    cdbl* a = A + (size_t)b * n * n;
+// Row interchanges of the panel [j0, j0 + jb) applied to every other column,
+// and for the columns right of the panel U12 <- L11^{-1} A12 (L11 unit lower).
+// A workgroup takes 64 columns of one matrix: the panel rows and the distinct
+// pivot rows below the panel are staged in LDS (panel rows read as 512-byte
+// column segments), the interchanges run in LDS in their sequential order, the
+// triangular solve per column, and the rows are written back.
+__global__ __launch_bounds__(256) void k_lu_rows(cdbl* __restrict__ A, int n, int j0, int jb,
+                                                 const int* __restrict__ ipiv) {
+    cdbl* a = A + (size_t)blockIdx.y * n * n;
+    const int* ip = ipiv + (size_t)blockIdx.y * n;
+    __shared__ cdbl top[32][65];
+    __shared__ cdbl ext[32][65];
     __shared__ cdbl L[32][33];
-    for (int e = threadIdx.x; e < jb * jb; e += 256) {
+    __shared__ int erow[32];    // distinct pivot rows below the panel
+    __shared__ int slot[32];    // for interchange k: 0..31 panel row, 32 + e external row e
+    __shared__ int next;
+    const int tid = threadIdx.x;
+    const int nout = n - jb;
+    const int c0 = blockIdx.x * 64;
+    const int ncol = min(64, nout - c0);
+    // column index of local column cc (skipping the panel)
+    auto colof = [&](int cc) { const int q = c0 + cc; return q < j0 ? q : q + jb; };
+    __shared__ int pv[32];
+    if (tid < jb) pv[tid] = ip[j0 + tid];
+    __syncthreads();
+    if (tid < 64) {
+        // interchange k: panel row (slot p - j0) or external row; external rows
+        // are numbered in order of first appearance (wavefront ballot)
+        const int k = tid;
+        const int p = k < jb ? pv[k] : 0;
+        const bool ex = k < jb && p >= j0 + jb;
+        int first = k;
+        if (ex)
+            for (int k2 = 0; k2 < k; ++k2)
+                if (pv[k2] == p) { first = k2; break; }
+        const unsigned long long firsts = __ballot(ex && first == k);
+        const int e = __popcll(firsts & ((1ull << first) - 1ull));
+        if (k < jb) slot[k] = ex ? 32 + e : p - j0;
+        if (ex && first == k) erow[e] = p;
+        if (k == 0) next = __popcll(firsts);
+    }
+    for (int e = tid; e < jb * jb; e += blockDim.x) {
         const int r = e % jb, c = e / jb;
         L[r][c] = a[(j0 + r) + (size_t)(j0 + c) * n];
     }
-    __syncthreads();
-    const int col = j0 + jb + blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= n) return;
-    cdbl x[32];
-#pragma unroll
-    for (int k = 0; k < 32; ++k) if (k < jb) x[k] = a[(j0 + k) + (size_t)col * n];
-#pragma unroll
-    for (int k = 0; k < 32; ++k) {
-        if (k >= jb) break;
-#pragma unroll
-        for (int i = k + 1; i < 32; ++i) if (i < jb) cfms(x[i], L[i][k], x[k]);
+    for (int e = tid; e < 32 * 64; e += blockDim.x) {
+        const int r = e & 31, cc = e >> 5;
+        if (r < jb && cc < ncol) top[r][cc] = a[(j0 + r) + (size_t)colof(cc) * n];
     }
+    __syncthreads();
+    const int ne = next;
+    for (int e = tid; e < ne * 64; e += blockDim.x) {
+        const int r = e >> 6, cc = e & 63;
+        if (cc < ncol) ext[r][cc] = a[erow[r] + (size_t)colof(cc) * n];
+    }
+    __syncthreads();
+    if (tid < ncol) {
+        const int cc = tid;
+        for (int k = 0; k < jb; ++k) {
+            const int sk = slot[k];
+            if (sk == k) continue;
+            cdbl& other = sk < 32 ? top[sk][cc] : ext[sk - 32][cc];
+            const cdbl t = top[k][cc];
+            top[k][cc] = other;
+            other = t;
+        }
+        if (colof(cc) >= j0 + jb) {
+            // column in registers (jb <= 32), L11 read from LDS
+            cdbl x[32];
 #pragma unroll
-    for (int k = 0; k < 32; ++k) if (k < jb) a[(j0 + k) + (size_t)col * n] = x[k];
+            for (int k = 0; k < 32; ++k) x[k] = k < jb ? top[k][cc] : mk(0, 0);
+#pragma unroll
+            for (int k = 0; k < 32; ++k)
+#pragma unroll
+                for (int i = k + 1; i < 32; ++i)
+                    if (i < jb) cfms(x[i], L[i][k], x[k]);
+#pragma unroll
+            for (int k = 0; k < 32; ++k) if (k < jb) top[k][cc] = x[k];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < 32 * 64; e += blockDim.x) {
+        const int r = e & 31, cc = e >> 5;
+        if (r < jb && cc < ncol) a[(j0 + r) + (size_t)colof(cc) * n] = top[r][cc];
+    }
+    for (int e = tid; e < ne * 64; e += blockDim.x) {
+        const int r = e >> 6, cc = e & 63;
+        if (cc < ncol) a[erow[r] + (size_t)colof(cc) * n] = ext[r][cc];
+    }
 }
 
 // final row permutation: (P A)(i, :) = A(perm[i], :)
@@ -212,12 +289,11 @@ void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStrea
     const long long s = (long long)n * n;
     for (int j0 = 0; j0 < n; j0 += nb) {
         const int jb = n - j0 < nb ? n - j0 : nb;
-        hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(256), 0, st, A, n, j0, jb, ipiv);
+        hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(1024), 0, st, A, n, j0, jb, ipiv);
         const int nout = n - jb;
-        if (nout > 0) hipLaunchKernelGGL(k_lu_swap, dim3((nout + 255) / 256, batch), dim3(256), 0, st, A, n, j0, jb, ipiv);
+        if (nout > 0) hipLaunchKernelGGL(k_lu_rows, dim3((nout + 63) / 64, batch), dim3(256), 0, st, A, n, j0, jb, ipiv);
         const int nr = n - j0 - jb;
         if (nr > 0) {
-            hipLaunchKernelGGL(k_lu_trsm, dim3((nr + 255) / 256, batch), dim3(256), 0, st, A, n, j0, jb);
             zgemm_batched(nr, nr, jb, mk(-1, 0), A + (j0 + jb) + (size_t)j0 * n, n, s, A + j0 + (size_t)(j0 + jb) * n, n,
                           s, false, mk(1, 0), A + (j0 + jb) + (size_t)(j0 + jb) * n, n, s, batch, st);
         }
