@@ -13,6 +13,9 @@
 //   * for 8 <= x <= 1e6 J0/Y0/J1/Y1 come from glibc's j0/y0/j1/y1, which
 //     implement the same published fdlibm P/Q rational forms (pR8/pS8/...) that
 //     cbessel.cpp:1807-1845 restates.
+//   * for k not real positive, H0/H1 of complex argument come from restated
+//     published methods (series / Steed-Temme continued fraction for K0, K1)
+//     instead of the reference's Miller recurrence; see hankel01_cplx.
 //   * the coinciding single-layer symmetry copy (galerkin_builder.cpp:744-746)
 //     is reproduced literally only when HBEM_QUIRK_SLP_COINC_DER is set; by
 //     default the mathematically symmetric copy is used (see flags below).
@@ -141,6 +144,81 @@ static void jy01(double x, double& j0, double& y0, double& j1, double& y1) {
 }
 
 // ------------------------------------------------------------------------
+// H0(z), H1(z) of the first kind for complex z with Im z >= 0 -- the path of
+// GalerkinBuilder::H1_01_cplx (galerkin_builder.cpp:224-235) for k not real
+// positive: complex_bessel::HankelH1 -> H_v0 / H_v1 (cbessel.cpp:1210-1287),
+// H0(z) = -(2i/pi) K0(-iz), H1(z) = -(2/pi) K1(-iz) (A&S 9.6.4), with
+// Re(-iz) = Im z >= 0.  Non-finite values (z = 0) become 0 as in
+// replace_non_finite_elements_with_zeros (galerkin_builder.cpp:216-222).
+// K0, K1 (restated methods, not the reference's code):
+//   |w| <= 2  ascending series A&S 9.6.10-9.6.13 (the reference also switches
+//             at |w| = 2, cbessel.cpp:979);
+//   |w| > 2   Temme's continued fraction for the U function evaluated by
+//             Steed's algorithm (Thompson & Barnett, J. Comput. Phys. 64 (1986)),
+//             where the reference runs the same U recurrence backwards
+//             (Miller) from Campbell's starting index (cbessel.cpp:999-1007).
+// Both are accurate to a few ulp (tests pin them against mpmath).
+// ------------------------------------------------------------------------
+static void bessel_k01(cd w, cd& k0, cd& k1) {
+    if (std::abs(w) <= 2.0) {
+        const cd t = 0.25 * w * w, lw = std::log(0.5 * w);
+        cd term = 1.0, u = 1.0, i0 = 1.0, s0 = 0.0, i1s = 1.0;
+        double H = 0.0;
+        cd s1 = (1.0 - 2.0 * kEuler) * u;                  // k = 0: psi(1) + psi(2)
+        for (int k = 1; k < 200; ++k) {
+            term *= t / double(k * k);                      // t^k / (k!)^2
+            u *= t / double(k * (k + 1));                   // t^k / (k! (k+1)!)
+            const double Hk = H + 1.0 / k, Hk1 = Hk + 1.0 / (k + 1);
+            i0 += term;
+            s0 += Hk * term;
+            i1s += u;
+            s1 += (Hk + Hk1 - 2.0 * kEuler) * u;
+            H = Hk;
+            if (std::abs(term) < 1e-17 * std::abs(i0) && std::abs(u) < 1e-17 * std::abs(i1s)) break;
+        }
+        k0 = -(lw + kEuler) * i0 + s0;                      // A&S 9.6.13
+        const cd i1 = 0.5 * w * i1s;
+        k1 = 1.0 / w + lw * i1 - 0.25 * w * s1;              // A&S 9.6.11, n = 1
+        return;
+    }
+    // order 0: b_i = 2 (w + i), a_i = -(1/4 + i (i - 1) ...) via the running A
+    cd b = 2.0 * (1.0 + w), d = 1.0 / b, h = d, dh = d;
+    cd q0 = 0.0, q1 = 1.0;
+    const double a1 = 0.25;
+    cd Q = a1, C = a1, S = 1.0 + Q * dh;
+    double A = -a1;
+    for (int i = 1; i < 100000; ++i) {
+        A -= 2.0 * i;
+        C = -A * C / (i + 1.0);
+        const cd qn = (q0 - b * q1) / A;
+        q0 = q1;
+        q1 = qn;
+        Q += C * qn;
+        b += 2.0;
+        d = 1.0 / (b + A * d);
+        dh = (b * d - 1.0) * dh;
+        h += dh;
+        const cd dS = Q * dh;
+        S += dS;
+        if (std::abs(dS) < 1e-17 * std::abs(S)) break;
+    }
+    h *= a1;
+    k0 = std::sqrt(kPi / (2.0 * w)) * std::exp(-w) / S;
+    k1 = k0 * (w + 0.5 - h) / w;
+}
+
+static void hankel01_cplx(cd z, cd& h0, cd& h1) {
+    if (z == cd(0.0)) { h0 = 0.0; h1 = 0.0; return; }
+    cd k0, k1;
+    bessel_k01(cd(z.imag(), -z.real()), k0, k1);           // w = -i z
+    h0 = cd(0.0, -2.0 / kPi) * k0;
+    h1 = (-2.0 / kPi) * k1;
+    auto fin = [](cd v) { return std::isfinite(v.real()) && std::isfinite(v.imag()); };
+    if (!fin(h0)) h0 = 0.0;
+    if (!fin(h1)) h1 = 0.0;
+}
+
+// ------------------------------------------------------------------------
 // Panels.  ParametrizedLine (parametrized_line.cpp:43-107),
 // ParametrizedCircularArc (parametrized_circular_arc.cpp:54-121) and an
 // analytic Fourier-sum curve segment z(t) = sum c_m e^{imt}, t in [t0,t1].
@@ -238,9 +316,27 @@ struct Local {
     void zero() { std::memset((void*)this, 0, sizeof(*this)); }
 };
 
+// GalerkinBuilder::initialize_constants (galerkin_builder.cpp:899-916)
 struct Consts {
-    double kr, c, sqrtc, kap, kap_inv, kap2h;   // k (real), c, sqrt(c), kappa=k sqrt c, 1/kappa, kappa^2/2
+    cd kr, kap, kap_inv, kap2h;   // k, kappa = k sqrt(c), 1/kappa, kappa^2/2
+    double c, sqrtc;
+    bool k_rp;                    // Im k == 0 && Re k > 0: the real Hankel path
 };
+
+static Consts make_consts(cd k, double c) {
+    if (c < 1.0) throw std::runtime_error("Refraction index must not be smaller than 1");
+    if (k.imag() < 0.0)
+        throw std::runtime_error("Wavenumber does not meet conditions for uniqueness of the solution (Im(k) >= 0)");
+    Consts C;
+    C.kr = k;
+    C.c = c;
+    C.sqrtc = std::sqrt(c);
+    C.kap = k * C.sqrtc;
+    C.kap_inv = 1.0 / C.kap;
+    C.kap2h = C.kap * C.kap * 0.5;
+    C.k_rp = k.imag() == 0.0 && k.real() > 0.0;
+    return C;
+}
 
 struct Quad {
     int ns, nc;
@@ -257,8 +353,12 @@ struct Node {
 };
 
 static void hankel_node(Node& nd, const Consts& C) {
+    if (!C.k_rp) {
+        hankel01_cplx(C.kap * nd.r, nd.h0, nd.h1);
+        return;
+    }
     double j0, y0, j1, y1;
-    jy01(C.kap * nd.r, j0, y0, j1, y1);
+    jy01(C.kap.real() * nd.r, j0, y0, j1, y1);
     nd.h0 = cd(j0, y0);
     nd.h1 = cd(j1, y1);
 }
@@ -301,7 +401,7 @@ static void hyp_accum(const std::vector<Node>& nodes, const std::vector<double>&
                 if (lower_only && m < n) continue;
                 double A = dshp(m) * dshp(n) * nd.w;
                 double B = tz ? 0.0 : (nd.bi[m] * nd.bj[n] * nd.w) * tdt2[q];
-                double cf = A - C.kap2h * B;
+                cd cf = A - C.kap2h * B;
                 out[0][m][n] += cf * nd.h0;
                 if (d > 0) out[1][m][n] += hr * cf + C.kap * nd.h0 * B;
                 if (d > 1) out[2][m][n] += a2 * cf - B * a3;
@@ -523,13 +623,9 @@ static Quad make_quad(int order) {
 // GalerkinBuilder::assembleDense (galerkin_builder.cpp:918-1018) for equal
 // test/trial P1 spaces: all pairs i >= j, local matrices, local-to-global
 // scatter with transposed writes, final factor i/4.
-static void assemble(const Geo& g, const Quad& Q, double k, double c, int layers, int d, int flags,
+static void assemble(const Geo& g, const Quad& Q, cd k, double c, int layers, int d, int flags,
                      int nthreads, cd* K, cd* W, cd* V) {
-    if (c < 1.0) throw std::runtime_error("Refraction index must not be smaller than 1");
-    if (!(k > 0.0)) throw std::runtime_error("oracle supports real positive k only");
-    Consts C;
-    C.kr = k; C.c = c; C.sqrtc = std::sqrt(c); C.kap = k * C.sqrtc; C.kap_inv = 1.0 / C.kap;
-    C.kap2h = C.kap * C.kap * 0.5;
+    const Consts C = make_consts(k, c);
     const int N = g.N;
     const size_t NN = (size_t)N * N;
     int npairs = N * (N + 1) / 2;
@@ -991,7 +1087,7 @@ static void assemble_operator(const SolOp& S, cd* T, bool mass, bool proj, const
     if (proj) projection(S, T);
 }
 
-static void gen_sol_op(const SolOp& S, double k, double c_o, double c_i, int d, bool proj, int nthreads,
+static void gen_sol_op(const SolOp& S, cd k, double c_o, double c_i, int d, bool proj, int nthreads,
                        cd* T, cd* Td, cd* Tdd) {
     const int N = S.N;
     const size_t NN = (size_t)N * N;
@@ -1028,6 +1124,17 @@ int orc_composite_rule(int n, double* x, double* w) {
     ORC_TRY composite_rule(n, kEps, x, w); ORC_CATCH
 }
 
+// H0, H1 of complex z (Im z >= 0); z: n complex, out: 2n complex {H0, H1} per point
+int orc_hankel01_cplx(const double* z, int n, double* out) {
+    ORC_TRY
+    for (int i = 0; i < n; ++i) {
+        cd h0, h1;
+        hankel01_cplx(cd(z[2 * i], z[2 * i + 1]), h0, h1);
+        out[4 * i] = h0.real(); out[4 * i + 1] = h0.imag(); out[4 * i + 2] = h1.real(); out[4 * i + 3] = h1.imag();
+    }
+    ORC_CATCH
+}
+
 int orc_jy01(const double* x, int n, double* out) {
     ORC_TRY
     for (int i = 0; i < n; ++i) jy01(x[i], out[4 * i], out[4 * i + 1], out[4 * i + 2], out[4 * i + 3]);
@@ -1062,12 +1169,11 @@ int orc_classify(const double* panels, int npanels, const double* coefs, int nco
 
 // local matrices of pair (i, j): out = 48 complex (K[3][2][2][2], W[3][2][2], V[3][2][2]) as in struct Local
 int orc_pair_local(const double* panels, int npanels, const double* coefs, int ncoef, int order,
-                   double k, double c, int i, int j, int d, int flags, double* out) {
+                   double k_re, double k_im, double c, int i, int j, int d, int flags, double* out) {
     ORC_TRY
     Geo g = make_geo(panels, npanels, coefs, ncoef);
     Quad Q = make_quad(order);
-    Consts C;
-    C.kr = k; C.c = c; C.sqrtc = std::sqrt(c); C.kap = k * C.sqrtc; C.kap_inv = 1.0 / C.kap; C.kap2h = C.kap * C.kap * 0.5;
+    const Consts C = make_consts(cd(k_re, k_im), c);
     Local L;
     pair_local(g, Q, C, i, j, 7, d, flags, L);
     std::memcpy(out, &L, sizeof(L));
@@ -1075,13 +1181,13 @@ int orc_pair_local(const double* panels, int npanels, const double* coefs, int n
 }
 
 int orc_assemble(const double* panels, int npanels, const double* coefs, int ncoef, int order,
-                 double k, double c, int layers, int d, int flags, int nthreads,
+                 double k_re, double k_im, double c, int layers, int d, int flags, int nthreads,
                  double* K, double* W, double* V) {
     ORC_TRY
     if (d > 2) throw std::runtime_error("Invalid order of derivative");
     Geo g = make_geo(panels, npanels, coefs, ncoef);
     Quad Q = make_quad(order);
-    assemble(g, Q, k, c, layers, d, flags, nthreads, (cd*)K, (cd*)W, (cd*)V);
+    assemble(g, Q, cd(k_re, k_im), c, layers, d, flags, nthreads, (cd*)K, (cd*)W, (cd*)V);
     ORC_CATCH
 }
 
@@ -1130,11 +1236,12 @@ int orc_solop_mass(void* h, double* M) {
     ORC_CATCH
 }
 
-int orc_gen_sol_op(void* h, double k, double c_o, double c_i, int d, int nthreads, double* T, double* Td, double* Tdd) {
+int orc_gen_sol_op(void* h, double k_re, double k_im, double c_o, double c_i, int d, int nthreads, double* T,
+                   double* Td, double* Tdd) {
     ORC_TRY
     OrcSolOp* s = (OrcSolOp*)h;
     if (c_o < 1.0 || c_i < 1.0) throw std::runtime_error("Refraction index must not be smaller than 1");
-    gen_sol_op(s->S, k, c_o, c_i, d, s->proj, nthreads, (cd*)T, (cd*)Td, (cd*)Tdd);
+    gen_sol_op(s->S, cd(k_re, k_im), c_o, c_i, d, s->proj, nthreads, (cd*)T, (cd*)Td, (cd*)Tdd);
     ORC_CATCH
 }
 

@@ -89,6 +89,15 @@ def jy01(x):
     return out[:, 0], out[:, 1], out[:, 2], out[:, 3]
 
 
+def hankel01_cplx(z):
+    """H0(z), H1(z) of the first kind for complex z (Im z >= 0), as the
+    builder's complex path computes them; returns two complex arrays."""
+    z = np.ascontiguousarray(np.atleast_1d(np.asarray(z, dtype=np.complex128)))
+    out = np.empty(2 * z.size, dtype=np.complex128)
+    _check(lib().orc_hankel01_cplx(_p(z), z.size, _p(out)))
+    return out[0::2], out[1::2]
+
+
 def panel_eval(mesh, i, what, s):
     pan, n, coefs, nc = _geom(mesh)
     s = np.ascontiguousarray(np.atleast_1d(np.asarray(s, dtype=np.float64)))
@@ -108,8 +117,9 @@ def pair_local(mesh, order, k, c, i, j, d=2, flags=0):
     """Local matrices of pair (i >= j): dict K[d,o,m,n], W[d,m,n], V[d,m,n]."""
     pan, n, coefs, nc = _geom(mesh)
     out = np.zeros(96)
-    _check(lib().orc_pair_local(_p(pan), n, _p(coefs), nc, order, ctypes.c_double(k), ctypes.c_double(c),
-                                i, j, d, flags, _p(out)))
+    k = complex(k)
+    _check(lib().orc_pair_local(_p(pan), n, _p(coefs), nc, order, ctypes.c_double(k.real), ctypes.c_double(k.imag),
+                                ctypes.c_double(c), i, j, d, flags, _p(out)))
     z = out[0::2] + 1j * out[1::2]
     return {"K": z[:24].reshape(3, 2, 2, 2), "W": z[24:36].reshape(3, 2, 2), "V": z[36:48].reshape(3, 2, 2)}
 
@@ -121,8 +131,9 @@ def assemble(mesh, order, k, c, layers=LAYER_ALL, d=0, flags=0, nthreads=8):
     K = np.zeros(shp, dtype=np.complex128, order="F")
     W = np.zeros(shp, dtype=np.complex128, order="F")
     V = np.zeros(shp, dtype=np.complex128, order="F")
-    _check(lib().orc_assemble(_p(pan), n, _p(coefs), nc, order, ctypes.c_double(k), ctypes.c_double(c),
-                              layers, d, flags, nthreads, _p(K), _p(W), _p(V)))
+    k = complex(k)
+    _check(lib().orc_assemble(_p(pan), n, _p(coefs), nc, order, ctypes.c_double(k.real), ctypes.c_double(k.imag),
+                              ctypes.c_double(c), layers, d, flags, nthreads, _p(K), _p(W), _p(V)))
     return (np.moveaxis(K, 2, 0), np.moveaxis(W, 2, 0), np.moveaxis(V, 2, 0))
 
 
@@ -175,8 +186,10 @@ class SolOp:
         n = 2 * self.N
         Ts = [np.zeros((n, n), dtype=np.complex128, order="F") for _ in range(d + 1)]
         ptrs = [_p(T) for T in Ts] + [None] * (2 - d)
-        _check(lib().orc_gen_sol_op(ctypes.c_void_p(self.h), ctypes.c_double(k), ctypes.c_double(c_o),
-                                    ctypes.c_double(c_i), ctypes.c_int(d), ctypes.c_int(nthreads), *ptrs))
+        k = complex(k)
+        _check(lib().orc_gen_sol_op(ctypes.c_void_p(self.h), ctypes.c_double(k.real), ctypes.c_double(k.imag),
+                                    ctypes.c_double(c_o), ctypes.c_double(c_i), ctypes.c_int(d),
+                                    ctypes.c_int(nthreads), *ptrs))
         return Ts
 
     def projection(self, T):

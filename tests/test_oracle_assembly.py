@@ -100,8 +100,8 @@ def panel_fn(O, mesh, i):
     return z, dz
 
 
-def oracle_local(O, mesh, order, i, j, d):
-    L = O.pair_local(mesh, order, K0, C0, i, j, d=d)
+def oracle_local(O, mesh, order, i, j, d, k=K0):
+    L = O.pair_local(mesh, order, k, C0, i, j, d=d)
     return {"V": L["V"][d], "W": L["W"][d], "Kd": L["K"][d, 0], "Kt": L["K"][d, 1]}
 
 
@@ -109,10 +109,7 @@ def _meshes(H):
     return {"square": H.square_mesh(1.0, 2), "circle": H.circle_mesh(8, 0.6, (0.1, -0.2)), "star": H.star_mesh(12)}
 
 
-@pytest.mark.parametrize("name,i,j", [("square", 5, 0), ("square", 4, 1), ("circle", 4, 0), ("circle", 6, 1),
-                                      ("star", 6, 0), ("star", 9, 2)])
-@pytest.mark.parametrize("d", [0, 1, 2])
-def test_general_pair_vs_textbook_integral(O, H, name, i, j, d):
+def _general_vs_textbook(O, H, name, i, j, d, k):
     mesh = _meshes(H)[name]
     assert O.classify(mesh)[i * (i + 1) // 2 + j] == 3
     xg, wg = np.polynomial.legendre.leggauss(48)
@@ -125,12 +122,29 @@ def test_general_pair_vs_textbook_integral(O, H, name, i, j, d):
     y = pj_(T.ravel()).reshape(T.shape)
     ti = ti_(S.ravel()).reshape(S.shape)
     tj = tj_(T.ravel()).reshape(T.shape)
-    ker = kernels(x, y, ti, tj, S, T, K0, C0, d)
-    got = oracle_local(O, mesh, 20, i, j, d)
+    ker = kernels(x, y, ti, tj, S, T, k, C0, d)
+    got = oracle_local(O, mesh, 20, i, j, d, k)
     for key in ("V", "Kd", "Kt", "W"):
         ref = np.array([[np.sum(W2 * ker[key][m, n]) for n in range(2)] for m in range(2)])
         scale = np.abs(ref).max()
         assert np.abs(got[key] - ref).max() <= 1e-11 * scale, (key, got[key], ref)
+
+
+@pytest.mark.parametrize("name,i,j", [("square", 5, 0), ("square", 4, 1), ("circle", 4, 0), ("circle", 6, 1),
+                                      ("star", 6, 0), ("star", 9, 2)])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_general_pair_vs_textbook_integral(O, H, name, i, j, d):
+    _general_vs_textbook(O, H, name, i, j, d, K0)
+
+
+# Complex wavenumbers (galerkin_builder.cpp:224-235 path): Im k > 0, a negative
+# real k (H of the negative real axis, principal branch) and a large imaginary
+# part; the integrands use scipy's complex Hankel functions.
+@pytest.mark.parametrize("name,i,j", [("circle", 4, 0), ("star", 6, 0), ("square", 5, 0)])
+@pytest.mark.parametrize("k", [2.0 + 0.7j, -1.5 + 0.0j, 0.3 + 2.0j])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_general_pair_complex_k_vs_textbook_integral(O, H, name, i, j, k, d):
+    _general_vs_textbook(O, H, name, i, j, d, k)
 
 
 def _coinciding_exact(L, kap, c, d, layer, m, n):
@@ -177,16 +191,16 @@ def _coinciding_exact(L, kap, c, d, layer, m, n):
     return complex(val)
 
 
-@pytest.mark.parametrize("d", [0, 1, 2])
-def test_coinciding_straight_panel_exact(O, H, d):
+@pytest.mark.parametrize("d,k", [(0, K0), (1, K0), (2, K0), (0, 1.5 + 0.5j), (2, 1.5 + 0.5j)])
+def test_coinciding_straight_panel_exact(O, H, d, k):
     mesh = H.square_mesh(1.3, 3)
     i = 4
     p = mesh.packed[i]
     Lp = math.hypot(p[3] - p[1], p[4] - p[2])
-    kap = K0 * math.sqrt(C0)
+    kap = k * math.sqrt(C0)
     # The composite rule omits [0, sigma^(n-1)] (gauleg.hpp:94-119): at order 16
     # that truncation is sigma^15 ~ 3e-12 of the integral, so 1e-9 pins the formulas.
-    got = oracle_local(O, mesh, 16, i, i, d)
+    got = oracle_local(O, mesh, 16, i, i, d, k)
     assert np.all(got["Kd"] == 0) and np.all(got["Kt"] == 0)     # straight panel: v parallel to tangent
     for layer in ("V", "W"):
         ref = np.array([[_coinciding_exact(Lp, kap, C0, d, layer, m, n) for n in range(2)] for m in range(2)])
@@ -205,7 +219,7 @@ def _graded_rule():
     return np.concatenate(xs), np.concatenate(ws)
 
 
-def _adjacent_exact(O, mesh, i, j, d):
+def _adjacent_exact(O, mesh, i, j, d, k=K0):
     """All local entries of an adjacent pair by Duffy splitting at the shared
     vertex: (sigma_i, sigma_j) = (rho u, rho) and (rho, rho u), Gauss-Legendre
     in u, the graded rule in rho.  sigma = parameter distance from the vertex."""
@@ -223,20 +237,21 @@ def _adjacent_exact(O, mesh, i, j, d):
         si_, sj_ = (RHO * U, RHO) if tri == 0 else (RHO, RHO * U)
         s = si_ if swap else 1.0 - si_
         t = 1.0 - sj_ if swap else sj_
-        ker = kernels(pi_(s), pj_(t), ti_(s), tj_(t), s, t, K0, C0, d)
+        ker = kernels(pi_(s), pj_(t), ti_(s), tj_(t), s, t, k, C0, d)
         for key in ("V", "W", "Kd", "Kt"):
             val = np.array([[np.sum(WW * ker[key][m, n]) for n in range(2)] for m in range(2)])
             out[key] = out.get(key, 0) + val
     return out
 
 
-@pytest.mark.parametrize("name,i,j", [("square", 2, 1), ("square", 7, 0), ("square", 1, 0), ("circle", 3, 2),
-                                      ("circle", 7, 0)])
-def test_adjacent_pair_vs_duffy_quadpack(O, H, name, i, j):
+@pytest.mark.parametrize("name,i,j,k", [("square", 2, 1, K0), ("square", 7, 0, K0), ("square", 1, 0, K0),
+                                        ("circle", 3, 2, K0), ("circle", 7, 0, K0), ("circle", 3, 2, 2.0 + 0.7j),
+                                        ("square", 2, 1, -1.2 + 0.4j)])
+def test_adjacent_pair_vs_duffy_quadpack(O, H, name, i, j, k):
     mesh = _meshes(H)[name]
     d = 1
-    got = oracle_local(O, mesh, 12, i, j, d)
-    exact = _adjacent_exact(O, mesh, i, j, d)
+    got = oracle_local(O, mesh, 12, i, j, d, k)
+    exact = _adjacent_exact(O, mesh, i, j, d, k)
     for key in ("V", "W", "Kd", "Kt"):
         if np.all(got[key] == 0):
             # skipped by the reference's all-|v.n| < eps test: collinear panels

@@ -50,3 +50,31 @@ def test_wronskian(O):
     j0, y0, j1, y1 = O.jy01(x)
     w = j1 * y0 - j0 * y1          # = 2/(pi x)
     assert np.max(np.abs(w * np.pi * x / 2 - 1.0)) < 5e-13
+
+
+def test_complex_hankel_vs_mpmath(O):
+    """H0, H1 of complex argument (the builder's path for k not real positive,
+    galerkin_builder.cpp:224-235) against 50-digit mpmath values."""
+    with open(os.path.join(GOLD, "hankel_cplx_mpmath.json")) as f:
+        rows = json.load(f)["rows"]
+    z = np.array([r["re"] + 1j * r["im"] for r in rows])
+    h0, h1 = O.hankel01_cplx(z)
+    for k, r in enumerate(rows):
+        for got, ref in ((h0[k], r["h0"]), (h1[k], r["h1"])):
+            ref = float(ref[0]) + 1j * float(ref[1])
+            # the phase of e^{iz} is conditioned by |z| eps; values below 1e-300
+            # are subnormal in double precision
+            assert abs(got - ref) <= (4e-14 + 2e-16 * abs(z[k])) * abs(ref) + 1e-300, (z[k], got, ref)
+
+
+def test_complex_hankel_zero_and_continuity(O):
+    """z = 0 gives 0 (non-finite values are replaced by zeros,
+    galerkin_builder.cpp:216-222); approaching the positive real axis the
+    complex path meets the real one."""
+    h0, h1 = O.hankel01_cplx([0.0])
+    assert h0[0] == 0 and h1[0] == 0
+    x = np.array([0.3, 1.9, 2.1, 5.0, 11.0, 40.0])
+    j0, y0, j1, y1 = O.jy01(x)
+    c0, c1 = O.hankel01_cplx(x + 1e-300j)
+    assert np.allclose(c0, j0 + 1j * y0, rtol=1e-14, atol=0)
+    assert np.allclose(c1, j1 + 1j * y1, rtol=1e-14, atol=0)
