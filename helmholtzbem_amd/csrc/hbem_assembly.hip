@@ -207,17 +207,26 @@ __device__ __forceinline__ void acc_reduce(Acc<D>& A) {
 // Per lane: kappa.  `sym` selects the coinciding form, where the two
 // triangles of [0,1]^2 share the node (roles of s and t exchanged; cd2 is the
 // second triangle's K factor) and only the entries m >= n of W and V are kept.
-template <int D>
-__device__ __forceinline__ void node_accum(Acc<D>& A, double kap, double r, double r2, double w, const double bi[2],
-                                           const double bj[2], double cd, double ct, double bt, double ttp, bool sym,
-                                           double cd2) {
+// H0, H1 at kappa r: real positive kappa (the reference's Hankel1Real01
+// path) or any other kappa (its complex path)
+__device__ __forceinline__ void hankel_at(double kap, double r, cdbl& h0, cdbl& h1) {
     double j0, y0, j1, y1;
     bessel_jy01(kap * r, j0, y0, j1, y1);
-    const cdbl h0 = mk(j0, y0);
-    const cdbl hr = mk(j1 * r, y1 * r);
+    h0 = mk(j0, y0);
+    h1 = mk(j1, y1);
+}
+__device__ __forceinline__ void hankel_at(cdbl kap, double r, cdbl& h0, cdbl& h1) { hankel01_cplx(r * kap, h0, h1); }
+
+template <int D, typename KT>
+__device__ __forceinline__ void node_accum(Acc<D>& A, KT kap, double r, double r2, double w, const double bi[2],
+                                           const double bj[2], double cd, double ct, double bt, double ttp, bool sym,
+                                           double cd2) {
+    cdbl h0, h1;
+    hankel_at(kap, r, h0, h1);
+    const cdbl hr = r * h1;
     const double rinv = r > 0.0 ? 1.0 / r : 0.0;
-    const cdbl h1r = mk(j1 * rinv, y1 * rinv);
-    const cdbl h0r2 = mk(j0 * r2, y0 * r2);
+    const cdbl h1r = rinv * h1;
+    const cdbl h0r2 = r2 * h0;
     double pbb[4], pbt[4];
 #pragma unroll
     for (int n = 0; n < 2; ++n)
@@ -266,11 +275,12 @@ __device__ __forceinline__ void node_accum(Acc<D>& A, double kap, double r, doub
 // double_layer_*, hypersingular_*, single_layer_*; before the factor i/4).
 // Linear in the accumulators, so partial sums over node chunks may be summed
 // afterwards.
-template <int D>
-__device__ __forceinline__ void acc_store(const Acc<D>& A, double kap, double kr, double c, bool coinc, int qflags,
+template <int D, typename KT>
+__device__ __forceinline__ void acc_store(const Acc<D>& A, KT kap, KT kr, double c, bool coinc, int qflags,
                                           cdbl* __restrict__ dst, long long stride) {
     cdbl out[16 * (D + 1)];
-    const double sqc = sqrt(c), k2h = 0.5 * kap * kap, kinv = 1.0 / kap;
+    const double sqc = sqrt(c);
+    const KT k2h = 0.5 * kap * kap, kinv = recip(kap);
     const double dd[4] = {1.0, -1.0, -1.0, 1.0};   // b'_m b'_n for P1 (+1, -1)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -328,8 +338,18 @@ __device__ __forceinline__ void fetch_node(NodeIn& n, const double* __restrict__
     n.xa = x[a]; n.xb = x[b]; n.wa = w[a]; n.wb = w[b];
 }
 
-// cls: 0 coinciding, 1 adjacent, 2 general
-template <int CLS, int D, int G>
+__device__ __forceinline__ void load_k(const AsmParams& P, int q, double& kap, double& kr) {
+    kap = P.kap[q];
+    kr = P.kr[q];
+}
+__device__ __forceinline__ void load_k(const AsmParams& P, int q, cdbl& kap, cdbl& kr) {
+    kap = mk(P.kap[q], P.kapi[q]);
+    kr = mk(P.kr[q], P.kri[q]);
+}
+
+// cls: 0 coinciding, 1 adjacent, 2 general; KT: double (real positive
+// wavenumbers) or cdbl (all others)
+template <int CLS, int D, int G, typename KT>
 __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
     const int lane = threadIdx.x & 63;
     const int g = lane % G;
@@ -360,7 +380,8 @@ __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
         chunk = (int)(rest % P.nchunk);
         e = (int)(rest / P.nchunk);
     }
-    const double kap = P.kap[q];
+    KT kap, kr;
+    load_k(P, q, kap, kr);
     const unsigned f = P.flags[e];
     const int nn = CLS == 2 ? P.ns * P.ns : (CLS == 1 ? 2 * P.nc * P.nc : P.nc * P.nc);
     const int lo = chunk * P.chunk;
@@ -384,7 +405,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
             const double s = cur.xa, t = cur.xb;
             const double w = cur.wa * cur.wb;
             const double bi[2] = {s, 1.0 - s}, bj[2] = {t, 1.0 - t};
-            node_accum<D>(A, kap, cur.o[0], cur.o[1], w, bi, bj, fd * cur.o[2], ft * cur.o[3], fb * cur.o[4],
+            node_accum<D, KT>(A, kap, cur.o[0], cur.o[1], w, bi, bj, fd * cur.o[2], ft * cur.o[3], fb * cur.o[4],
                           cur.o[5], false, 0.0);
             cur = nxt;
         }
@@ -407,7 +428,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
             const double cdv = (fh & 1) ? 0.0 : cur.o[2];
             const double ctv = (fh & 2) ? 0.0 : cur.o[3];
             const double bt = (fh & 4) ? 0.0 : 2.0 * cur.o[4];
-            node_accum<D>(A, kap, cur.o[0], cur.o[1], w, bi, bj, cdv, ctv, bt, cur.o[5], false, 0.0);
+            node_accum<D, KT>(A, kap, cur.o[0], cur.o[1], w, bi, bj, cdv, ctv, bt, cur.o[5], false, 0.0);
             cur = nxt;
         }
     } else {
@@ -422,7 +443,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
             const double t = cur.xa, s = t * (1.0 - cur.xb);
             const double w = t * cur.wa * cur.wb;
             const double bi[2] = {s, 1.0 - s}, bj[2] = {t, 1.0 - t};
-            node_accum<D>(A, kap, cur.o[0], cur.o[1], w, bi, bj, f0 * cur.o[2], 0.0, fb * cur.o[4], cur.o[5], true,
+            node_accum<D, KT>(A, kap, cur.o[0], cur.o[1], w, bi, bj, f0 * cur.o[2], 0.0, fb * cur.o[4], cur.o[5], true,
                           f1 * cur.o[3]);
             cur = nxt;
         }
@@ -432,7 +453,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
     if (active && g == 0) {
         const size_t rec = P.nchunk == 1 ? (size_t)P.pidx[e] : (size_t)e * P.nchunk + chunk;
         cdbl* out = P.out + rec * (16 * (D + 1)) * P.nk + q;
-        acc_store<D>(A, kap, P.kr[q], P.cv[q], CLS == 0, P.qflags, out, P.nk);
+        acc_store<D, KT>(A, kap, kr, P.cv[q], CLS == 0, P.qflags, out, P.nk);
     }
 }
 
@@ -869,11 +890,19 @@ static void launch_asm_g(const AsmParams& P, int G, hipStream_t st) {
     // G = 1: one wave per (pair, chunk, block of 64 slots); else 64/G items per wave
     const long long waves = G == 1 ? (long long)P.nent * P.nchunk * ((P.nk + 63) / 64) : (nitems * G + 63) / 64;
     const int blocks = (int)((waves + 3) / 4);
+    if (P.kapi) {
+        switch (G) {
+        case 4: hipLaunchKernelGGL((k_assemble<CLS, D, 4, cdbl>), dim3(blocks), dim3(256), 0, st, P); break;
+        case 16: hipLaunchKernelGGL((k_assemble<CLS, D, 16, cdbl>), dim3(blocks), dim3(256), 0, st, P); break;
+        default: hipLaunchKernelGGL((k_assemble<CLS, D, 64, cdbl>), dim3(blocks), dim3(256), 0, st, P); break;
+        }
+        return;
+    }
     switch (G) {
-    case 1: hipLaunchKernelGGL((k_assemble<CLS, D, 1>), dim3(blocks), dim3(256), 0, st, P); break;
-    case 4: hipLaunchKernelGGL((k_assemble<CLS, D, 4>), dim3(blocks), dim3(256), 0, st, P); break;
-    case 16: hipLaunchKernelGGL((k_assemble<CLS, D, 16>), dim3(blocks), dim3(256), 0, st, P); break;
-    default: hipLaunchKernelGGL((k_assemble<CLS, D, 64>), dim3(blocks), dim3(256), 0, st, P); break;
+    case 1: hipLaunchKernelGGL((k_assemble<CLS, D, 1, double>), dim3(blocks), dim3(256), 0, st, P); break;
+    case 4: hipLaunchKernelGGL((k_assemble<CLS, D, 4, double>), dim3(blocks), dim3(256), 0, st, P); break;
+    case 16: hipLaunchKernelGGL((k_assemble<CLS, D, 16, double>), dim3(blocks), dim3(256), 0, st, P); break;
+    default: hipLaunchKernelGGL((k_assemble<CLS, D, 64, double>), dim3(blocks), dim3(256), 0, st, P); break;
     }
 }
 

@@ -78,7 +78,7 @@ struct hbem_geometry {
     int ngen = 0, nadj = 0, ncoi = 0;
     DBuf d_M, d_Frm, d_Fcm, d_Finvcm, d_Finvc;
     // work space
-    DBuf w_LUT, w_L, w_T0, w_T1, w_T2, w_piv, w_perm, w_X, w_Y, w_work, w_R, w_u, w_v, w_a, w_b, w_sig, w_dsig, w_kap,
+    DBuf w_kapi, w_kri, w_LUT, w_L, w_T0, w_T1, w_T2, w_piv, w_perm, w_X, w_Y, w_work, w_R, w_u, w_v, w_a, w_b, w_sig, w_dsig, w_kap,
         w_kr, w_cv, w_qi, w_qo, w_tmp, w_out, w_part;
     // profiling
     bool prof = false;
@@ -102,19 +102,24 @@ int fail(const std::exception& e) {
     catch (const std::exception& e) { return fail(e); } \
     return HBEM_OK;
 
+// GalerkinBuilder::initialize_constants' validation (galerkin_builder.cpp:899-916)
 void check_k(double k_re, double k_im, double c) {
     if (c < 1.0) throw Error(HBEM_EINVAL, "Refraction index must not be smaller than 1");
     if (k_im < 0.0)
         throw Error(HBEM_EINVAL, "Wavenumber does not meet conditions for uniqueness of the solution (Im(k) >= 0)");
-    if (k_im != 0.0 || !(k_re > 0.0))
-        throw Error(HBEM_ENOTSUP, "only real positive wavenumbers are supported by the device path");
+    if (!std::isfinite(k_re) || !std::isfinite(k_im)) throw Error(HBEM_EINVAL, "wavenumber must be finite");
 }
 
-// Assemble the local matrices of all pairs for wavenumber slots (kap, kr, cv);
-// returns the local buffer.  Uses G = 1 (lanes over wavenumbers) when the
-// batch is wide, node-parallel groups otherwise.
+// k_rp of initialize_constants: the real Hankel path
+bool real_positive(double k_re, double k_im) { return k_im == 0.0 && k_re > 0.0; }
+
+// Assemble the local matrices of all pairs for wavenumber slots (kap, kr, cv)
+// with imaginary parts (kapi, kri) when any slot is not real positive; returns
+// the local buffer.  Wide real batches run on the paired-lane kernel, the rest
+// on node-parallel groups of lanes per item.
 cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const std::vector<double>& kr,
-                      const std::vector<double>& cv, int D) {
+                      const std::vector<double>& cv, int D, const std::vector<double>& kapi = {},
+                      const std::vector<double>& kri = {}) {
     const int nk = (int)kap.size();
     const int nslot = 16 * (D + 1);
     cdbl* L = g->w_L.as<cdbl>((size_t)nk * g->npairs * nslot);
@@ -128,14 +133,19 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
     P.kap = upload(g->w_kap, kap, g->st);
     P.kr = upload(g->w_kr, kr, g->st);
     P.cv = upload(g->w_cv, cv, g->st);
+    const bool cplx = !kapi.empty();
+    if (cplx) {
+        P.kapi = upload(g->w_kapi, kapi, g->st);
+        P.kri = upload(g->w_kri, kri, g->st);
+    }
     P.npairs_total = g->npairs;
     P.ns = g->ns;
     P.nc = g->nc;
     P.qflags = g->flags;
     // lanes per item: G = 0 selects the paired-lane kernel for wide batches
     // (32 wavenumbers x 2 nodes per wavefront); node groups of G lanes otherwise
-    const int Gg = nk >= 64 ? 0 : (nk >= 16 ? 4 : 16);
-    const int Gs = nk >= 64 ? 0 : 64;
+    const int Gg = cplx ? 16 : (nk >= 64 ? 0 : (nk >= 16 ? 4 : 16));
+    const int Gs = (nk >= 64 && !cplx) ? 0 : 64;
     struct Launch {
         int cls, which, G, nent, nodes;
         const int* ent;
@@ -431,7 +441,8 @@ int hbem_assemble_dense(hbem_geometry_t g, double k_re, double k_im, double c, i
     std::lock_guard<std::mutex> lk(g->mu);
     HIPC(hipSetDevice(g->dev));
     const double sc = std::sqrt(c);
-    cdbl* L = assemble_locals(g, {k_re * sc}, {k_re}, {c}, d);
+    cdbl* L = real_positive(k_re, k_im) ? assemble_locals(g, {k_re * sc}, {k_re}, {c}, d)
+                                        : assemble_locals(g, {k_re * sc}, {k_re}, {c}, d, {k_im * sc}, {k_im});
     const int N = g->N;
     const size_t NN = (size_t)N * N * (d + 1);
     cdbl* out = g->w_out.as<cdbl>(3 * NN);
@@ -461,8 +472,14 @@ int hbem_assemble_dense_batch(hbem_geometry_t g, const double* ks, int nk, doubl
     for (int k0 = 0; k0 < nk; k0 += B) {
         const int nb = std::min(B, nk - k0);
         std::vector<double> kap(nb), kr(nb), cv(nb, c);
-        for (int b = 0; b < nb; ++b) { kap[b] = ks[k0 + b] * sc; kr[b] = ks[k0 + b]; }
-        cdbl* L = assemble_locals(g, kap, kr, cv, d);
+        bool allpos = true;
+        for (int b = 0; b < nb; ++b) {
+            kap[b] = ks[k0 + b] * sc;
+            kr[b] = ks[k0 + b];
+            allpos = allpos && ks[k0 + b] > 0.0;
+        }
+        const std::vector<double> zero(allpos ? 0 : nb, 0.0);
+        cdbl* L = assemble_locals(g, kap, kr, cv, d, zero, zero);
         for (int b = 0; b < nb; ++b) {
             hbem::launch_gather_layers(L, nb, 16 * (d + 1), b, N, d, layers, out, out + NN, out + 2 * NN, g->st);
             HIPC(hipGetLastError());
@@ -501,7 +518,11 @@ int hbem_gen_sol_op(hbem_geometry_t g, double k_re, double k_im, double c_o, dou
     check_k(k_re, k_im, c_o);
     std::lock_guard<std::mutex> lk(g->mu);
     HIPC(hipSetDevice(g->dev));
-    cdbl* L = assemble_locals(g, {k_re * std::sqrt(c_i), k_re * std::sqrt(c_o)}, {k_re, k_re}, {c_i, c_o}, d);
+    const double si = std::sqrt(c_i), so = std::sqrt(c_o);
+    cdbl* L = real_positive(k_re, k_im)
+                  ? assemble_locals(g, {k_re * si, k_re * so}, {k_re, k_re}, {c_i, c_o}, d)
+                  : assemble_locals(g, {k_re * si, k_re * so}, {k_re, k_re}, {c_i, c_o}, d, {k_im * si, k_im * so},
+                                    {k_im, k_im});
     const int n = 2 * g->N;
     const size_t nn = (size_t)n * n;
     cdbl* T0 = g->w_T0.as<cdbl>(nn);
@@ -758,7 +779,10 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
             qi[b] = b;
             qo[b] = nb + b;
         }
-        cdbl* L = assemble_locals(g, kap, kr, cv, D);
+        bool allpos = true;
+        for (int b = 0; b < nb; ++b) allpos = allpos && ks[k0 + b] > 0.0;
+        const std::vector<double> zero(allpos ? 0 : 2 * nb, 0.0);
+        cdbl* L = assemble_locals(g, kap, kr, cv, D, zero, zero);
         cdbl* T0 = g->w_T0.as<cdbl>(nn * nb);
         cdbl* T1 = D >= 1 ? g->w_T1.as<cdbl>(nn * nb) : nullptr;
         cdbl* T2 = D >= 2 ? g->w_T2.as<cdbl>(nn * nb) : nullptr;

@@ -23,6 +23,7 @@ __host__ __device__ __forceinline__ cdbl operator+(cdbl a, cdbl b) { return cdbl
 __host__ __device__ __forceinline__ cdbl operator-(cdbl a, cdbl b) { return cdbl{a.x - b.x, a.y - b.y}; }
 __host__ __device__ __forceinline__ cdbl operator*(double s, cdbl a) { return cdbl{s * a.x, s * a.y}; }
 __host__ __device__ __forceinline__ cdbl operator*(cdbl a, cdbl b) { return cdbl{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__host__ __device__ __forceinline__ cdbl operator*(cdbl a, double s) { return cdbl{s * a.x, s * a.y}; }
 __host__ __device__ __forceinline__ cdbl cconj(cdbl a) { return cdbl{a.x, -a.y}; }
 __host__ __device__ __forceinline__ double cabs2(cdbl a) { return a.x * a.x + a.y * a.y; }
 // acc += s * a  (s real)
@@ -170,6 +171,95 @@ __device__ __forceinline__ void bessel_jy01(double x, double& j0, double& y0, do
             j0 = b * cs; y0 = b * sc; j1 = b * sc; y1 = -b * cs;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Complex helpers and H0, H1 of complex argument (wavenumbers that are not
+// real positive: GalerkinBuilder::H1_01_cplx, galerkin_builder.cpp:224-235)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double recip(double a) { return 1.0 / a; }
+__device__ __forceinline__ cdbl recip(cdbl a) {
+    const double s = 1.0 / (a.x * a.x + a.y * a.y);
+    return mk(a.x * s, -a.y * s);
+}
+__device__ __forceinline__ cdbl cdiv(cdbl a, cdbl b) { return a * recip(b); }
+__device__ __forceinline__ double cabsd(cdbl a) { return hypot(a.x, a.y); }
+__device__ __forceinline__ cdbl clogd(cdbl a) { return mk(log(cabsd(a)), atan2(a.y, a.x)); }
+__device__ __forceinline__ cdbl cexpd(cdbl a) {
+    double s, c;
+    sincos(a.y, &s, &c);
+    const double e = exp(a.x);
+    return mk(e * c, e * s);
+}
+__device__ __forceinline__ cdbl csqrtd(cdbl a) {   // principal branch
+    const double r = cabsd(a);
+    if (r == 0.0) return mk(0.0, 0.0);
+    const double t = sqrt(0.5 * (r + fabs(a.x)));
+    if (a.x >= 0.0) return mk(t, 0.5 * a.y / t);
+    return mk(0.5 * fabs(a.y) / t, a.y < 0.0 ? -t : t);
+}
+
+// K0(w), K1(w) for Re w >= 0, w != 0: the ascending series (A&S 9.6.10-9.6.13)
+// for |w| <= 2, Temme's continued fraction evaluated by Steed's algorithm
+// (Thompson & Barnett 1986) above; the loops end on convergence and are
+// bounded (a non-finite input runs to the bound and yields non-finite values).
+__device__ inline void bessel_k01_cplx(cdbl w, cdbl& k0, cdbl& k1) {
+    if (cabsd(w) <= 2.0) {
+        const cdbl t = 0.25 * (w * w), lw = clogd(0.5 * w);
+        cdbl term = mk(1.0, 0.0), u = mk(1.0, 0.0), i0 = mk(1.0, 0.0), s0 = mk(0.0, 0.0), i1s = mk(1.0, 0.0);
+        double H = 0.0;
+        cdbl s1 = mk(1.0 - 2.0 * kEuler, 0.0);
+        for (int k = 1; k < 200; ++k) {
+            term = term * t * (1.0 / double(k * k));
+            u = u * t * (1.0 / double(k * (k + 1)));
+            const double Hk = H + 1.0 / k, Hk1 = Hk + 1.0 / (k + 1);
+            i0 = i0 + term;
+            s0 = s0 + Hk * term;
+            i1s = i1s + u;
+            s1 = s1 + (Hk + Hk1 - 2.0 * kEuler) * u;
+            H = Hk;
+            if (cabsd(term) < 1e-17 * cabsd(i0) && cabsd(u) < 1e-17 * cabsd(i1s)) break;
+        }
+        const cdbl l = lw + mk(kEuler, 0.0);
+        k0 = s0 - l * i0;
+        const cdbl i1 = (0.5 * w) * i1s;
+        k1 = recip(w) + lw * i1 - (0.25 * w) * s1;
+        return;
+    }
+    cdbl b = 2.0 * (mk(1.0, 0.0) + w), d = recip(b), h = d, dh = d;
+    cdbl q0 = mk(0.0, 0.0), q1 = mk(1.0, 0.0);
+    cdbl Q = mk(0.25, 0.0), C = mk(0.25, 0.0), S = mk(1.0, 0.0) + Q * dh;
+    double A = -0.25;
+    for (int i = 1; i < 1000; ++i) {
+        A -= 2.0 * i;
+        C = (-A / (i + 1.0)) * C;
+        const cdbl qn = (q0 - b * q1) * (1.0 / A);
+        q0 = q1;
+        q1 = qn;
+        Q = Q + C * qn;
+        b = b + mk(2.0, 0.0);
+        d = recip(b + A * d);
+        dh = (b * d - mk(1.0, 0.0)) * dh;
+        h = h + dh;
+        const cdbl dS = Q * dh;
+        S = S + dS;
+        if (cabsd(dS) < 1e-17 * cabsd(S)) break;
+    }
+    h = 0.25 * h;
+    k0 = csqrtd(cdiv(mk(kPi, 0.0), 2.0 * w)) * cdiv(cexpd(mk(-w.x, -w.y)), S);
+    k1 = cdiv(k0 * (w + mk(0.5, 0.0) - h), w);
+}
+
+// H0(z), H1(z) for Im z >= 0: -(2i/pi) K0(-iz), -(2/pi) K1(-iz) (A&S 9.6.4);
+// z = 0 and other non-finite results give 0 (galerkin_builder.cpp:216-222)
+__device__ inline void hankel01_cplx(cdbl z, cdbl& h0, cdbl& h1) {
+    if (z.x == 0.0 && z.y == 0.0) { h0 = mk(0.0, 0.0); h1 = mk(0.0, 0.0); return; }
+    cdbl k0, k1;
+    bessel_k01_cplx(mk(z.y, -z.x), k0, k1);
+    h0 = mk(kTwoOverPi * k0.y, -kTwoOverPi * k0.x);
+    h1 = (-kTwoOverPi) * k1;
+    if (!(isfinite(h0.x) && isfinite(h0.y))) h0 = mk(0.0, 0.0);
+    if (!(isfinite(h1.x) && isfinite(h1.y))) h1 = mk(0.0, 0.0);
 }
 
 // ---------------------------------------------------------------------------

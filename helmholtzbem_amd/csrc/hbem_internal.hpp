@@ -22,8 +22,10 @@ struct AsmParams {
     int nent;                    // entries of this class
     int nk;                      // wavenumber slots in the batch
     int nchunk, chunk;           // node chunks per entry, nodes per chunk
-    const double* kap;           // kappa = k sqrt(c) per slot
-    const double* kr;            // k per slot
+    const double* kap;           // kappa = k sqrt(c) per slot (real part)
+    const double* kr;            // k per slot (real part)
+    const double* kapi;          // imaginary parts, or null: every slot real positive
+    const double* kri;
     const double* cv;            // c per slot
     long long npairs_total;      // N (N + 1) / 2
     cdbl* out;                   // nchunk == 1: local matrices [npairs_total][16 (D+1)][nk];

@@ -35,7 +35,7 @@ extern "C" {
 /* error codes */
 #define HBEM_OK 0
 #define HBEM_EINVAL (-1)     /* invalid argument (reference: std::runtime_error / out_of_range) */
-#define HBEM_ENOTSUP (-2)    /* not supported by this build (e.g. complex wavenumber) */
+#define HBEM_ENOTSUP (-2)    /* not supported by this build (e.g. submatrix assembly) */
 #define HBEM_EDEVICE (-3)    /* HIP runtime error */
 #define HBEM_ENOMEM (-4)     /* allocation failed */
 
@@ -81,8 +81,9 @@ int hbem_geometry_info(hbem_geometry_t g, int* npanels, int* order, int* corder,
  * getDoubleLayer(t) / getHypersingular(t) / getSingleLayer(t), t = 0..d
  * (galerkin_builder.hpp:221-276, galerkin_builder.cpp:918-1018, 1050-1068).
  * K, W, V: host arrays of (d+1) N x N matrices each (slice t at offset t*N*N),
- * or NULL to skip copying that layer.  Only real k > 0 is supported
- * (HBEM_ENOTSUP otherwise).
+ * or NULL to skip copying that layer.  Any k with Im k >= 0: real positive k
+ * takes the real Hankel path, every other k the complex one
+ * (galerkin_builder.cpp:224-235, 899-916).
  */
 int hbem_assemble_dense(hbem_geometry_t g, double k_re, double k_im, double c, int layers, int d,
                         void* K, void* W, void* V);
