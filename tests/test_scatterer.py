@@ -1,0 +1,137 @@
+"""Scatterer input files and polygonal meshing (scatterer.cpp:29-262): the
+reference's own data files (copied as fixtures), the panel-count rules, the
+tiling of the polygon, the grading toward corners, the C++ facade header
+against the Python host code bit for bit, and (GPU) a sigma_min sweep on a
+mesh read from a file against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helmholtzbem_amd import scatterer as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DATA = os.path.join(ROOT, "tests", "golden", "scatterers")
+
+
+def path(name):
+    return os.path.join(DATA, "scatterer_%s.dat" % name)
+
+
+def ends(panels):
+    return np.array([[p.start[0], p.start[1], p.end[0], p.end[1]] for p in panels])
+
+
+def test_read_reference_files():
+    for name, nv in (("square", 4), ("poly", 10), ("cavity", 12), ("flower", 399), ("astroid", 99), ("rect", 163)):
+        x, y = S.read(path(name))
+        assert x.size == nv and y.size == nv
+    x, y = S.read(path("square"), scale=2.5)
+    assert np.array_equal(x, [0, 2.5, 2.5, 0]) and np.array_equal(y, [0, 0, 2.5, 2.5])
+    assert S.length(*S.read(path("square"))) == 4.0
+
+
+def test_read_format_rules(tmp_path):
+    f = tmp_path / "a.dat"
+    f.write_text(" 1 , 2\n\n3,4,\n\t5,6 \n")
+    assert np.array_equal(S.read_array(str(f)), [[1, 2], [3, 4], [5, 6]])
+    f.write_text("1,2\n3,x\n")
+    assert S.read_array(str(f)) is None and S.read(str(f)) is None
+    f.write_text("1,2,3\n4,5,6\n")
+    assert S.read(str(f)) is None                      # not two columns
+    f.write_text("1,2\n3\n")
+    assert np.array_equal(S.read_array(str(f)), [[1, 2], [3, 0]])
+    assert S.read_array(str(tmp_path / "missing.dat")) is None
+
+
+def _check_tiling(x, y, panels):
+    e = ends(panels)
+    # consecutive panels share endpoints and the chain is closed (to a few ulp:
+    # ParametrizedLine::split evaluates the [-1, 1] form, parametrized_line.cpp:23-29)
+    tol = 4e-16 * max(np.abs(x).max(), np.abs(y).max())
+    assert np.abs(e[1:, :2] - e[:-1, 2:]).max() <= tol and np.abs(e[0, :2] - e[-1, 2:]).max() <= tol
+    # every vertex is a panel start
+    for xi, yi in zip(x, y):
+        assert np.min(np.hypot(e[:, 0] - xi, e[:, 1] - yi)) <= tol
+    lens = np.hypot(e[:, 2] - e[:, 0], e[:, 3] - e[:, 1])
+    assert abs(lens.sum() - S.length(x, y)) <= 1e-12 * S.length(x, y)
+    return lens
+
+
+@pytest.mark.parametrize("name,N", [("square", 100), ("square", 37), ("poly", 64), ("cavity", 200)])
+def test_make_N_polygonal_panels(name, N):
+    x, y = S.read(path(name))
+    p = S.make_N_polygonal_panels(x, y, N)
+    assert len(p) == N
+    lens = _check_tiling(x, y, p)
+    if name == "square" and N == 100:
+        assert np.allclose(lens, 0.04, rtol=1e-14)
+    # per edge: equal panels (ParametrizedLine::split)
+    e = ends(p)
+    assert np.all(lens > 0)
+
+
+@pytest.mark.parametrize("name,N,rf", [("square", 100, 0.5), ("poly", 80, 0.3), ("cavity", 120, 0.5),
+                                       ("square", 60, 1.0)])
+def test_make_polygonal_panels_graded(name, N, rf):
+    x, y = S.read(path(name))
+    p = S.make_polygonal_panels(x, y, N, rf)
+    lens = _check_tiling(x, y, p)
+    assert abs(len(p) - N) <= len(x)          # counts are rounded per edge
+    if rf >= 1.0:
+        # uniform per edge
+        e = ends(p)
+        assert np.allclose(lens, lens[0], rtol=1e-12) or name != "square"
+    elif name == "square":
+        # right angles: f = rf + (1 - rf)/2 at both ends, panels shrink toward the corners
+        per = len(p) // 4
+        side = lens[:per]
+        assert side[0] < side[per // 2] and side[-1] < side[per // 2]
+        assert abs(side[0] - side[-1]) <= 1e-12
+
+
+def test_auto_num_panels():
+    x, y = S.read(path("square"))
+    assert S.auto_num_panels(x, y, 0.25) == 16
+    with pytest.raises(ValueError):
+        S.auto_num_panels(*S.read(path("rect")), 1.0)   # the file repeats a vertex
+
+
+def _cpp(tmp_path):
+    exe = str(tmp_path / "scatterer_dump")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "scatterer_dump.cpp"), "-o", exe],
+                   check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    return exe
+
+
+def test_cpp_header_matches_python(tmp_path):
+    exe = _cpp(tmp_path)
+    cases = [("square", 100, 0, None), ("poly", 77, 0, None), ("cavity", 50, 0, None), ("square", 100, 1, 0.5),
+             ("poly", 80, 1, 0.3), ("cavity", 120, 1, 0.5), ("astroid", 150, 1, 0.7), ("poly", 40, 1, 1.0)]
+    for name, N, mode, rf in cases:
+        args = [exe, path(name), str(N), str(mode)] + ([repr(rf)] if rf is not None else [])
+        out = subprocess.run(args, check=True, stdout=subprocess.PIPE, text=True).stdout.split("\n")
+        head = out[0].split()
+        x, y = S.read(path(name))
+        p = S.make_N_polygonal_panels(x, y, N) if mode == 0 else S.make_polygonal_panels(x, y, N, rf)
+        assert int(head[0]) == len(p), (name, N, mode)
+        assert float(head[1]) == S.length(x, y)
+        got = np.array([[float(v) for v in ln.split()] for ln in out[1:1 + len(p)]])
+        assert np.array_equal(got, ends(p)), (name, N, mode, rf)
+
+
+@pytest.mark.gpu
+def test_sigma_on_file_mesh_matches_oracle(H, O):
+    """The path on a mesh built from a reference data file: graded panels of
+    scatterer_poly.dat, sigma_min sweep vs the oracle with the same R."""
+    mesh = S.mesh_from_file(path("poly"), 60, refinement_factor=0.5)
+    sp = H.ContinuousSpace1()
+    bd = H.BuilderData(mesh, sp, sp, 8)
+    n = 2 * mesh.getNumPanels()
+    R = H.gaussian_test_matrix(n, 2, 4)
+    ks = np.linspace(2.0, 9.0, 5)
+    sig = H.sigma_sweep(bd, ks, 1.0, 9.0, R, 2)
+    so = O.SolOp(mesh, 8).sigma_sweep(ks, 1.0, 9.0, R, 2)
+    assert np.max(np.abs(sig - so) / so) <= 1e-10
