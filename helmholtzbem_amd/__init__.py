@@ -72,6 +72,30 @@ class ContinuousSpace1:
             raise IndexError("Panel/RSF index out of range!")
         return n if q == 2 else (1 if n % N == 0 else n + 1)
 
+    def Interpolate_helmholtz(self, func, mesh):
+        """Coefficients of the P1 interpolant: func(x1, x2) at the start point
+        of every panel (continuous_space.hpp:67-82)."""
+        panels = mesh.getPanels()
+        out = np.zeros(self.getSpaceDim(len(panels)), dtype=np.complex128)
+        for i, p in enumerate(panels):
+            x = p[0.0]
+            out[i] = func(x[0], x[1])
+        return out
+
+    def Interpolate_helmholtz_neu(self, func, mesh):
+        """Normal derivatives n . func(x1, x2) at the panel start points, with
+        func the gradient and n = (t_y, -t_x) / |t| from the panel tangent t
+        there (continuous_space.hpp:84-101)."""
+        panels = mesh.getPanels()
+        out = np.zeros(self.getSpaceDim(len(panels)), dtype=np.complex128)
+        for i, p in enumerate(panels):
+            x = p[0.0]
+            t = p.Derivative_01(0.0)
+            nrm = math.sqrt(t[1] * t[1] + t[0] * t[0])
+            g = func(x[0], x[1])
+            out[i] = (t[1] / nrm) * g[0] + (-t[0] / nrm) * g[1]
+        return out
+
 
 class _Geometry:
     """Owns one device context (hbem_geometry_t)."""
@@ -213,6 +237,8 @@ class SolutionsOperator:
         self.builder_data = builder_data
         self.do_projection = bool(proj)
         self._g = builder_data._ctx(self.do_projection)
+        self._last = None
+        self._ops = {}
 
     def _gen(self, k, c_o, c_i, d):
         k = complex(k)
@@ -220,7 +246,43 @@ class SolutionsOperator:
         outs = [np.zeros((n, n), dtype=np.complex128, order="F") for _ in range(d + 1)]
         ptrs = [_vptr(o) for o in outs] + [None] * (2 - d)
         check(lib().hbem_gen_sol_op(self._g.h, k.real, k.imag, float(c_o), float(c_i), int(d), *ptrs))
+        self._remember(k, c_o, c_i)
         return outs
+
+    def _remember(self, k, c_o, c_i):
+        self._last = (complex(k), float(c_o), float(c_i))
+        self._ops = {}
+
+    def _layer(self, layer, inner):
+        """Operator of the last gen_sol_op* call (gen_sol_op.hpp:120-125),
+        assembled on request on the device."""
+        if self._last is None:
+            raise RuntimeError("no operator has been generated yet")
+        key = (layer, inner)
+        if key not in self._ops:
+            k, c_o, c_i = self._last
+            b = GalerkinBuilder(self.builder_data)
+            b.assembleDense(k, c_i if inner else c_o, layer)
+            self._ops[key] = b._get(layer, 0)
+        return self._ops[key]
+
+    def K_int(self):
+        return self._layer(LayerType.DOUBLE, True)
+
+    def K_ext(self):
+        return self._layer(LayerType.DOUBLE, False)
+
+    def W_int(self):
+        return self._layer(LayerType.HYPERSINGULAR, True)
+
+    def W_ext(self):
+        return self._layer(LayerType.HYPERSINGULAR, False)
+
+    def V_int(self):
+        return self._layer(LayerType.SINGLE, True)
+
+    def V_ext(self):
+        return self._layer(LayerType.SINGLE, False)
 
     def gen_sol_op(self, k, c_o, c_i, builder=None):
         return self._gen(k, c_o, c_i, 0)[0]

@@ -12,13 +12,13 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhbem.so")
 
-HBEM_EINVAL, HBEM_ENOTSUP, HBEM_EDEVICE, HBEM_ENOMEM = -1, -2, -3, -4
+HBEM_EINVAL, HBEM_ENOTSUP, HBEM_EDEVICE, HBEM_ENOMEM, HBEM_ENUMERIC = -1, -2, -3, -4, -5
 
 EXPORTED = [
     "hbem_last_error", "hbem_init", "hbem_geometry_create", "hbem_geometry_destroy", "hbem_geometry_info",
     "hbem_assemble_dense", "hbem_mass_matrix", "hbem_projection_factor", "hbem_gen_sol_op", "hbem_projection",
     "hbem_rsvd_sv", "hbem_rsvd_sv_der", "hbem_rsvd_sv_der2", "hbem_sigma_sweep", "hbem_profile_enable",
-    "hbem_profile_query", "hbem_assemble_dense_batch",
+    "hbem_profile_query", "hbem_assemble_dense_batch", "hbem_tp_A_rhs", "hbem_tp_solve",
 ]
 
 _lib = None
@@ -55,6 +55,8 @@ def lib():
         L.hbem_sigma_sweep.argtypes = [vp, dp, i, d, d, vp, i, i, i, dp, dp]
         L.hbem_profile_enable.argtypes = [vp, i]
         L.hbem_profile_query.argtypes = [vp, i, dp, ctypes.POINTER(ctypes.c_longlong), dp]
+        L.hbem_tp_A_rhs.argtypes = [vp, d, d, d, d, vp, i, vp, vp]
+        L.hbem_tp_solve.argtypes = [vp, d, d, d, d, vp, i, vp]
         _lib = L
     return _lib
 

@@ -855,6 +855,25 @@ __global__ __launch_bounds__(256) void k_form_T(const cdbl* __restrict__ L, int 
     }
 }
 
+// Right-hand-side operator of the transmission solve (tp::direct_second_kind::
+// A_rhs, solvers.cpp:116-122) from the outer operators of wavenumber slot o:
+// A_o = [[-K_o + M/2, V_o], [W_o, K_o^T + M/2]] (2N x 2N, column-major).
+// One thread per (m, n) writes the four block entries; K_o^T(m, n) = K_o(n, m)
+// is gathered from the local matrices directly.
+__global__ __launch_bounds__(256) void k_form_Ao(const cdbl* __restrict__ L, int nk, int nslot, int o, int N,
+                                                 const double* __restrict__ M, cdbl* __restrict__ A) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)N * N) return;
+    const int m = (int)(id % N), n = (int)(id / N);
+    const size_t n2 = 2 * (size_t)N;
+    const double hm = 0.5 * M[m + (size_t)n * N];
+    const cdbl Ko = op_entry(L, nk, nslot, o, 0, 0, m, n, N), KoT = op_entry(L, nk, nslot, o, 0, 0, n, m, N);
+    A[m + (size_t)n * n2] = mk(-Ko.x + hm, -Ko.y);
+    A[m + (size_t)(n + N) * n2] = op_entry(L, nk, nslot, o, 2, 0, m, n, N);
+    A[(m + N) + (size_t)n * n2] = op_entry(L, nk, nslot, o, 1, 0, m, n, N);
+    A[(m + N) + (size_t)(n + N) * n2] = mk(KoT.x + hm, KoT.y);
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -936,6 +955,12 @@ void launch_form_T(const cdbl* L, int nk, int nslot, const int* qi, const int* q
     const unsigned mt = (unsigned)((N + 7) / 8);
     hipLaunchKernelGGL(k_form_T, dim3(mt * (unsigned)N, (unsigned)((nb + 31) / 32)), dim3(256), 0, st, L, nk, nslot, qi,
                        qo, nb, N, D, M, T0, T1, T2);
+}
+
+void launch_form_Ao(const cdbl* L, int nk, int nslot, int o, int N, const double* M, cdbl* A, hipStream_t st) {
+    const long long n = (long long)N * N;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_form_Ao, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, nk, nslot, o, N, M, A);
 }
 
 }  // namespace hbem

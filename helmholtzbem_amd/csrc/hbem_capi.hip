@@ -78,6 +78,7 @@ struct hbem_geometry {
     int ngen = 0, nadj = 0, ncoi = 0;
     DBuf d_M, d_Frm, d_Fcm, d_Finvcm, d_Finvc;
     // work space
+    DBuf w_Ao, w_U, w_rhs, w_flag;
     DBuf w_kapi, w_kri, w_LUT, w_L, w_T0, w_T1, w_T2, w_piv, w_perm, w_X, w_Y, w_work, w_R, w_u, w_v, w_a, w_b, w_sig, w_dsig, w_kap,
         w_kr, w_cv, w_qi, w_qo, w_tmp, w_out, w_part;
     // profiling
@@ -265,6 +266,41 @@ void project_inplace(hbem_geometry* g, cdbl* T, int count) {
     const cdbl* Fi = (const cdbl*)g->d_Finvc.p;
     hbem::zgemm_batched(n, n, n, hbem::mk(1, 0), Fi, n, 0, T, n, s, false, hbem::mk(0, 0), tmp, n, s, count, g->st);
     hbem::zgemm_batched(n, n, n, hbem::mk(1, 0), tmp, n, s, Fi, n, 0, true, hbem::mk(0, 0), T, n, s, count, g->st);
+}
+
+// The linear system of tp::direct_second_kind::A_rhs (solvers.cpp:106-131) on
+// the device: both wavenumber slots (c_i, c_o) are assembled in one pass, T is
+// gen_sol_op's unprojected operator, and rhs = A_o U for the nrhs host columns
+// U (2N x nrhs) with A_o = [[-K_o + M/2, V_o], [W_o, K_o^T + M/2]].
+void tp_system(hbem_geometry* g, double k_re, double k_im, double c_o, double c_i, const void* U, int nrhs,
+               cdbl*& T, cdbl*& rhs) {
+    const double si = std::sqrt(c_i), so = std::sqrt(c_o);
+    cdbl* L = real_positive(k_re, k_im)
+                  ? assemble_locals(g, {k_re * si, k_re * so}, {k_re, k_re}, {c_i, c_o}, 0)
+                  : assemble_locals(g, {k_re * si, k_re * so}, {k_re, k_re}, {c_i, c_o}, 0, {k_im * si, k_im * so},
+                                    {k_im, k_im});
+    const int N = g->N, n = 2 * N;
+    const size_t nn = (size_t)n * n;
+    const double* dM = (const double*)g->d_M.p;
+    T = g->w_T0.as<cdbl>(nn);
+    std::vector<int> qi = {0}, qo = {1};
+    const int* dqi = upload(g->w_qi, qi, g->st);
+    const int* dqo = upload(g->w_qo, qo, g->st);
+    hbem::launch_form_T(L, 2, 16, dqi, dqo, 1, N, 0, dM, T, nullptr, nullptr, g->st);
+    HIPC(hipGetLastError());
+    cdbl* Ao = g->w_Ao.as<cdbl>(nn);
+    hbem::launch_form_Ao(L, 2, 16, 1, N, dM, Ao, g->st);
+    HIPC(hipGetLastError());
+    cdbl* dU = g->w_U.as<cdbl>((size_t)n * nrhs);
+    HIPC(hipMemcpyAsync(dU, U, sizeof(cdbl) * (size_t)n * nrhs, hipMemcpyHostToDevice, g->st));
+    rhs = g->w_rhs.as<cdbl>((size_t)n * nrhs);
+    hbem::zgemm_batched(n, nrhs, n, hbem::mk(1, 0), Ao, n, 0, dU, n, 0, false, hbem::mk(0, 0), rhs, n, 0, 1, g->st);
+    HIPC(hipGetLastError());
+}
+
+void check_tp_args(hbem_geometry* g, const void* U, int nrhs) {
+    if (!g || !U) throw Error(HBEM_EINVAL, "null argument");
+    if (nrhs < 1) throw Error(HBEM_EINVAL, "number of right-hand sides must be positive");
 }
 
 }  // namespace
@@ -556,6 +592,68 @@ int hbem_projection(hbem_geometry_t g, const void* T, void* Tp) {
     HIPC(hipMemcpyAsync(d, T, nn * sizeof(cdbl), hipMemcpyHostToDevice, g->st));
     project_inplace(g, d, 1);
     copy_out_square(d, Tp, nn, g->st);
+    HIPC(hipStreamSynchronize(g->st));
+    API_END
+}
+
+int hbem_tp_A_rhs(hbem_geometry_t g, double k_re, double k_im, double c_o, double c_i, const void* u_inc_N, int nrhs,
+                  void* A, void* rhs) {
+    API_BEGIN
+    check_tp_args(g, u_inc_N, nrhs);
+    check_k(k_re, k_im, c_i);
+    check_k(k_re, k_im, c_o);
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPC(hipSetDevice(g->dev));
+    cdbl *T = nullptr, *r = nullptr;
+    tp_system(g, k_re, k_im, c_o, c_i, u_inc_N, nrhs, T, r);
+    const int n = 2 * g->N;
+    if (A) {
+        if (g->proj) project_inplace(g, T, 1);
+        copy_out_square(T, A, (size_t)n * n, g->st);
+    }
+    if (rhs) copy_out_square(r, rhs, (size_t)n * nrhs, g->st);
+    HIPC(hipStreamSynchronize(g->st));
+    API_END
+}
+
+int hbem_tp_solve(hbem_geometry_t g, double k_re, double k_im, double c_o, double c_i, const void* u_inc_N, int nrhs,
+                  void* sol) {
+    API_BEGIN
+    check_tp_args(g, u_inc_N, nrhs);
+    if (!sol) throw Error(HBEM_EINVAL, "null argument");
+    check_k(k_re, k_im, c_i);
+    check_k(k_re, k_im, c_o);
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPC(hipSetDevice(g->dev));
+    cdbl *T = nullptr, *r = nullptr;
+    tp_system(g, k_re, k_im, c_o, c_i, u_inc_N, nrhs, T, r);
+    const int n = 2 * g->N;
+    const size_t nn = (size_t)n * n;
+    // solve (solvers.cpp:133-152): finiteness of rhs and T, then zgesv
+    int* flag = g->w_flag.as<int>(3);
+    HIPC(hipMemsetAsync(flag, 0, 3 * sizeof(int), g->st));
+    hbem::flag_nonfinite((const double*)r, 2LL * n * nrhs, flag, g->st);
+    hbem::flag_nonfinite((const double*)T, 2LL * (long long)nn, flag + 1, g->st);
+    HIPC(hipGetLastError());
+    int hf[3] = {0, 0, 0};
+    HIPC(hipMemcpyAsync(hf, flag, 2 * sizeof(int), hipMemcpyDeviceToHost, g->st));
+    HIPC(hipStreamSynchronize(g->st));
+    if (hf[0]) throw Error(HBEM_ENUMERIC, "Failed to solve for traces, right-hand side is undefined");
+    if (hf[1]) throw Error(HBEM_ENUMERIC, "Failed to solve for traces, solutions operator is undefined");
+    int* piv = g->w_piv.as<int>(n);
+    int* perm = g->w_perm.as<int>(n);
+    hbem::lu_factor_batched(T, n, 1, piv, perm, g->st);
+    hbem::flag_zero_diag(T, n, flag + 2, g->st);
+    cdbl* work = g->w_work.as<cdbl>((size_t)n * 8);
+    for (int j = 0; j < nrhs; j += 8) {
+        const int nc = std::min(8, nrhs - j);
+        hbem::lu_solve_thin_batched(T, nullptr, perm, n, 1, r + (size_t)j * n, nc, false, work, g->st);
+    }
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(hf + 2, flag + 2, sizeof(int), hipMemcpyDeviceToHost, g->st));
+    HIPC(hipStreamSynchronize(g->st));
+    if (hf[2]) throw Error(HBEM_ENUMERIC, "zgesv: LU is singular");
+    copy_out_square(r, sol, (size_t)n * nrhs, g->st);
     HIPC(hipStreamSynchronize(g->st));
     API_END
 }

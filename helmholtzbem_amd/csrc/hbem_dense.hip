@@ -463,6 +463,37 @@ void transpose_batched(const cdbl* A, cdbl* AT, int n, int batch, hipStream_t st
 }
 
 // ---------------------------------------------------------------------------
+// Checks of the dense solve: *flag = 1 when any of the cnt doubles at x is not
+// finite / when a diagonal entry of the n x n complex matrix A is exactly zero
+// (LAPACK zgesv's info > 0 after the LU).  Threads only ever store 1, so the
+// concurrent stores need no atomics; the host clears the flag first.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_flag_nonfinite(const double* __restrict__ x, long long cnt, int* flag) {
+    bool bad = false;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (long long)gridDim.x * blockDim.x)
+        bad = bad || !isfinite(x[i]);
+    if (bad) *flag = 1;
+}
+
+__global__ __launch_bounds__(256) void k_flag_zero_diag(const cdbl* __restrict__ A, int n, int* flag) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const cdbl d = A[i + (size_t)i * n];
+        if (d.x == 0.0 && d.y == 0.0) *flag = 1;
+    }
+}
+
+void flag_nonfinite(const double* x, long long cnt, int* flag, hipStream_t st) {
+    if (cnt <= 0) return;
+    const long long blocks = (cnt + 255) / 256;
+    hipLaunchKernelGGL(k_flag_nonfinite, dim3((unsigned)(blocks < 2048 ? blocks : 2048)), dim3(256), 0, st, x, cnt, flag);
+}
+
+void flag_zero_diag(const cdbl* A, int n, int* flag, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_flag_zero_diag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A, n, flag);
+}
+
+// ---------------------------------------------------------------------------
 // Y = F X for a real dense F given row-major; one wavefront per row.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_real_apply(const double* __restrict__ Frm, int n, const cdbl* __restrict__ X,

@@ -50,6 +50,8 @@ void launch_gather_layers(const cdbl* L, int nk, int nslot, int q, int N, int D,
 // T_t for output b is written at T[t] + b * (2N)^2 (t = 0..D)
 void launch_form_T(const cdbl* L, int nk, int nslot, const int* qi, const int* qo, int nb, int N, int D,
                    const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st);
+// A_o = [[-K_o + M/2, V_o], [W_o, K_o^T + M/2]] of wavenumber slot o (2N x 2N)
+void launch_form_Ao(const cdbl* L, int nk, int nslot, int o, int N, const double* M, cdbl* A, hipStream_t st);
 
 // dense (hbem_dense.hip); all batched over `batch` matrices of order n with
 // stride n*n (column-major, lda = n) unless noted.
@@ -60,6 +62,9 @@ void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStrea
 void lu_solve_thin_batched(const cdbl* LU, const cdbl* LUT, const int* perm, int n, int batch, cdbl* X, int nc,
                            bool adj, cdbl* work, hipStream_t st);
 void transpose_batched(const cdbl* A, cdbl* AT, int n, int batch, hipStream_t st);
+// *flag = 1 if any of cnt doubles is not finite / a diagonal entry of A is 0
+void flag_nonfinite(const double* x, long long cnt, int* flag, hipStream_t st);
+void flag_zero_diag(const cdbl* A, int n, int* flag, hipStream_t st);
 // Y = op(F) X for a real dense n x n F given row-major (so op = F) -- pass the
 // column-major array to apply F^T.
 void real_apply_thin_batched(const double* Frm, int n, int batch, const cdbl* X, cdbl* Y, int nc, hipStream_t st);

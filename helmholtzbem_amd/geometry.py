@@ -42,6 +42,16 @@ class ParametrizedLine:
         x2, y2 = self.end
         return (0.5 * (t * (x2 - x1) + (x2 + x1)), 0.5 * (t * (y2 - y1) + (y2 + y1)))
 
+    def __getitem__(self, s: float):
+        """Point at s in [0, 1] (operator[], parametrized_line.cpp:36-42)."""
+        x1, y1 = self.start
+        x2, y2 = self.end
+        return (s * (x2 - x1) + x1, s * (y2 - y1) + y1)
+
+    def Derivative_01(self, s: float):
+        """d/ds of operator[] (parametrized_line.cpp:78-84)."""
+        return (self.end[0] - self.start[0], self.end[1] - self.start[1])
+
     def split(self, n: int) -> List["ParametrizedLine"]:
         """parametrized_line.cpp:116-137."""
         parts = []
@@ -70,6 +80,17 @@ class ParametrizedCircularArc:
     radius: float = 0.25
     phi_start: float = 0.0
     phi_end: float = 2 * math.pi
+
+    def __getitem__(self, s: float):
+        """Point at s in [0, 1] (operator[], parametrized_circular_arc.cpp:46-53)."""
+        a = s * (self.phi_end - self.phi_start) + self.phi_start
+        return (self.center[0] + self.radius * math.cos(a), self.center[1] + self.radius * math.sin(a))
+
+    def Derivative_01(self, s: float):
+        """d/ds of operator[] (parametrized_circular_arc.cpp:92-99)."""
+        d = self.phi_end - self.phi_start
+        a = s * d + self.phi_start
+        return (-self.radius * d * math.sin(a), self.radius * d * math.cos(a))
 
     def split(self, n: int) -> List["ParametrizedCircularArc"]:
         """parametrized_circular_arc.cpp:134-148."""
@@ -114,6 +135,17 @@ class ParametrizedTrigCurve:
 
     def z(self, t):
         return sum(complex(c) * np.exp(1j * m * t) for m, c in self.coeffs)
+
+    def __getitem__(self, s: float):
+        """Point at s in [0, 1]: z(t0 + s (t1 - t0))."""
+        v = complex(self.z(self.t0 + s * (self.t1 - self.t0)))
+        return (v.real, v.imag)
+
+    def Derivative_01(self, s: float):
+        """d/ds of the point at s in [0, 1]."""
+        t = self.t0 + s * (self.t1 - self.t0)
+        v = (self.t1 - self.t0) * sum(1j * m * complex(c) * np.exp(1j * m * t) for m, c in self.coeffs)
+        return (float(np.real(v)), float(np.imag(v)))
 
     def length(self, n=64) -> float:
         t = np.linspace(self.t0, self.t1, n)

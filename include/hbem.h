@@ -38,6 +38,7 @@ extern "C" {
 #define HBEM_ENOTSUP (-2)    /* not supported by this build (e.g. submatrix assembly) */
 #define HBEM_EDEVICE (-3)    /* HIP runtime error */
 #define HBEM_ENOMEM (-4)     /* allocation failed */
+#define HBEM_ENUMERIC (-5)   /* numerical failure of a solve (reference: std::runtime_error) */
 
 /* LayerType bits (galerkin_builder.hpp:100-106) */
 #define HBEM_LAYER_DOUBLE 1
@@ -119,6 +120,38 @@ int hbem_gen_sol_op(hbem_geometry_t g, double k_re, double k_im, double c_o, dou
 
 /* SolutionsOperator::projection(T) = F^{-1} T F^{-T} (gen_sol_op.cpp:45-48). */
 int hbem_projection(hbem_geometry_t g, const void* T, void* Tp);
+
+/*
+ * tp::direct_second_kind::A_rhs (solvers.hpp:82-90, solvers.cpp:106-131):
+ * A = gen_sol_op(k, c_o, c_i) (2N x 2N, projected when the context was
+ * created with proj != 0, as the reference's sol_op argument decides) and
+ * rhs = A_o u_inc_N with A_o = [[-K_o + M/2, V_o], [W_o, K_o^T + M/2]], the
+ * operators K_o, W_o, V_o assembled at (k, c_o).  u_inc_N: 2N x nrhs, each
+ * column the P1 interpolation coefficients of one incoming wave -- its values
+ * at the panel start points (ContinuousSpace<1>::Interpolate_helmholtz,
+ * continuous_space.hpp:67-82), then the normal derivatives there with the
+ * normal (t_y, -t_x) / |t| of the panel tangent t (Interpolate_helmholtz_neu,
+ * continuous_space.hpp:84-101).  The reference takes one column; nrhs
+ * columns share one assembly.  A or rhs may be NULL.
+ */
+int hbem_tp_A_rhs(hbem_geometry_t g, double k_re, double k_im, double c_o, double c_i, const void* u_inc_N, int nrhs,
+                  void* A, void* rhs);
+
+/*
+ * tp::direct_second_kind::solve (solvers.hpp:107-113, solvers.cpp:133-152):
+ * the traces sol (2N x nrhs: Dirichlet, then Neumann coefficients of the
+ * total field) solving A sol = rhs with A the unprojected gen_sol_op (the
+ * reference builds its own SolutionsOperator(builder_data, false)),
+ * whatever proj the context was created with.  LU with partial pivoting on
+ * the device (the reference calls LAPACK zgesv, solvers.cpp:25-51); the
+ * factorisation is shared by the nrhs columns.  Errors (HBEM_ENUMERIC) carry
+ * the reference's messages: "Failed to solve for traces, right-hand side is
+ * undefined" / "... solutions operator is undefined" (non-finite entries)
+ * and "zgesv: LU is singular" (an exactly zero pivot); sol is then left
+ * unwritten.
+ */
+int hbem_tp_solve(hbem_geometry_t g, double k_re, double k_im, double c_o, double c_i, const void* u_inc_N, int nrhs,
+                  void* sol);
 
 /*
  * randomized_svd::sv / sv_der / sv_der2 (randsvd.hpp:23-30,

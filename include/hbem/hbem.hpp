@@ -13,6 +13,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -35,6 +36,7 @@ using MatrixXd = Eigen::MatrixXd;
 using VectorXcd = Eigen::VectorXcd;
 using Vector2d = Eigen::Vector2d;
 using Vector3d = Eigen::Vector3d;
+using Vector2cd = Eigen::Vector2cd;
 #else
 template <class T>
 class Matrix {
@@ -62,6 +64,13 @@ using MatrixXd = Matrix<double>;
 using VectorXcd = Matrix<std::complex<double>>;
 struct Vector2d { double v[2] = {0, 0}; double& operator()(int i) { return v[i]; } double operator()(int i) const { return v[i]; } };
 struct Vector3d { double v[3] = {0, 0, 0}; double& operator()(int i) { return v[i]; } double operator()(int i) const { return v[i]; } };
+struct Vector2cd {
+    std::complex<double> v[2] = {0.0, 0.0};
+    Vector2cd() = default;
+    Vector2cd(std::complex<double> a, std::complex<double> b) : v{a, b} {}
+    std::complex<double>& operator()(int i) { return v[i]; }
+    std::complex<double> operator()(int i) const { return v[i]; }
+};
 #endif
 
 inline void check(int rc) {
@@ -81,6 +90,9 @@ struct Point2 { double x, y; };
 class AbstractParametrizedCurve {
   public:
     virtual ~AbstractParametrizedCurve() = default;
+    // point at s in [0, 1] and its derivative in s (abstract_parametrized_curve.hpp:72, 92)
+    virtual Point2 operator[](double s) const = 0;
+    virtual Point2 Derivative_01(double s) const = 0;
     virtual void pack(double* out8, int coef_offset) const = 0;
     virtual bool isLineSegment() const = 0;
     virtual const std::vector<double>* coefficients() const { return nullptr; }
@@ -94,6 +106,9 @@ class ParametrizedLine : public AbstractParametrizedCurve {
     Point2 operator()(double t) const {
         return {0.5 * (t * (b_.x - a_.x) + (b_.x + a_.x)), 0.5 * (t * (b_.y - a_.y) + (b_.y + a_.y))};
     }
+    // parametrized_line.cpp:36-42, 78-84
+    Point2 operator[](double s) const override { return {s * (b_.x - a_.x) + a_.x, s * (b_.y - a_.y) + a_.y}; }
+    Point2 Derivative_01(double) const override { return {b_.x - a_.x, b_.y - a_.y}; }
     // parametrized_line.cpp:116-137
     PanelVector split(unsigned N) const {
         PanelVector out;
@@ -118,6 +133,15 @@ class ParametrizedCircularArc : public AbstractParametrizedCurve {
   public:
     ParametrizedCircularArc(Point2 center, double r, double phi_start, double phi_end)
         : c_(center), r_(r), p0_(phi_start), p1_(phi_end) {}
+    // parametrized_circular_arc.cpp:46-53, 92-99
+    Point2 operator[](double s) const override {
+        const double a = s * (p1_ - p0_) + p0_;
+        return {c_.x + r_ * std::cos(a), c_.y + r_ * std::sin(a)};
+    }
+    Point2 Derivative_01(double s) const override {
+        const double d = p1_ - p0_, a = s * d + p0_;
+        return {-r_ * d * std::sin(a), r_ * d * std::cos(a)};
+    }
     // parametrized_circular_arc.cpp:134-148
     PanelVector split(unsigned N) const {
         PanelVector out;
@@ -144,6 +168,23 @@ class ParametrizedTrigCurve : public AbstractParametrizedCurve {
   public:
     ParametrizedTrigCurve(std::shared_ptr<const std::vector<double>> coefs, double t0, double t1)
         : co_(std::move(coefs)), t0_(t0), t1_(t1) {}
+    // z(t0 + s (t1 - t0)) and its derivative in s
+    Point2 operator[](double s) const override {
+        const double t = t0_ + s * (t1_ - t0_);
+        std::complex<double> z = 0.0;
+        for (size_t i = 0; i + 2 < co_->size(); i += 3)
+            z += std::complex<double>((*co_)[i + 1], (*co_)[i + 2]) * std::polar(1.0, (*co_)[i] * t);
+        return {z.real(), z.imag()};
+    }
+    Point2 Derivative_01(double s) const override {
+        const double t = t0_ + s * (t1_ - t0_);
+        std::complex<double> z = 0.0;
+        for (size_t i = 0; i + 2 < co_->size(); i += 3)
+            z += std::complex<double>(0.0, (*co_)[i]) * std::complex<double>((*co_)[i + 1], (*co_)[i + 2]) *
+                 std::polar(1.0, (*co_)[i] * t);
+        z *= (t1_ - t0_);
+        return {z.real(), z.imag()};
+    }
     PanelVector split(unsigned N) const {
         PanelVector out;
         for (unsigned i = 0; i < N; ++i) {
@@ -229,6 +270,31 @@ template <> class ContinuousSpace<1> : public AbstractBEMSpace {
         return q == 2 ? n : ((n % N == 0) ? 1 : (n + 1));
     }
     unsigned getSpaceDim(unsigned numpanels) const override { return numpanels * (q_ - 1); }
+    // values of func at the panel start points (continuous_space.hpp:67-82)
+    VectorXcd Interpolate_helmholtz(const std::function<std::complex<double>(double, double)>& func,
+                                    const ParametrizedMesh& mesh) const {
+        const unsigned N = mesh.getNumPanels();
+        VectorXcd c(getSpaceDim(N), 1);
+        for (unsigned i = 0; i < N; ++i) {
+            const Point2 x = (*mesh.getPanels()[i])[0.0];
+            c(i) = func(x.x, x.y);
+        }
+        return c;
+    }
+    // n . func at the panel start points, n = (t_y, -t_x) / |t| (continuous_space.hpp:84-101)
+    VectorXcd Interpolate_helmholtz_neu(const std::function<Vector2cd(double, double)>& func,
+                                        const ParametrizedMesh& mesh) const {
+        const unsigned N = mesh.getNumPanels();
+        VectorXcd c(getSpaceDim(N), 1);
+        for (unsigned i = 0; i < N; ++i) {
+            const AbstractParametrizedCurve& p = *mesh.getPanels()[i];
+            const Point2 x = p[0.0], t = p.Derivative_01(0.0);
+            const double nrm = std::sqrt(t.y * t.y + t.x * t.x);
+            const Vector2cd g = func(x.x, x.y);
+            c(i) = (t.y / nrm) * g(0) + (-t.x / nrm) * g(1);
+        }
+        return c;
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -377,6 +443,19 @@ class SolutionsOperator {
         return out;
     }
     const BuilderData& getBuilderData() const { return builder_data; }
+    bool projects() const { return do_projection; }
+    // device calls behind tp::direct_second_kind (hbem/solvers.hpp): U holds
+    // nrhs columns of 2N incoming-wave coefficients
+    void transmission_system(const std::complex<double>& k, double c_o, double c_i, const std::complex<double>* U,
+                             int nrhs, std::complex<double>* A, std::complex<double>* rhs) {
+        check(hbem_tp_A_rhs(g_, k.real(), k.imag(), c_o, c_i, U, nrhs, A, rhs));
+        k_ = k; c_o_ = c_o; c_i_ = c_i; have_k_ = true;
+        for (auto& f : fresh_) f = false;
+    }
+    void transmission_solve(const std::complex<double>& k, double c_o, double c_i, const std::complex<double>* U,
+                            int nrhs, std::complex<double>* sol) {
+        check(hbem_tp_solve(g_, k.real(), k.imag(), c_o, c_i, U, nrhs, sol));
+    }
 
   private:
     void gen(const std::complex<double>& k, double c_o, double c_i, int d, MatrixXcd* T, MatrixXcd* Td, MatrixXcd* Tdd) {

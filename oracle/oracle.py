@@ -239,3 +239,30 @@ def svd_values(B):
     out = np.zeros(B.shape[1])
     _check(lib().orc_svd_values(_p(B), B.shape[0], B.shape[1], _p(out)))
     return out
+
+
+def tp_A_rhs(mesh, order, k, c_o, c_i, U, proj=False, nthreads=8):
+    """tp::direct_second_kind::A_rhs (solvers.cpp:106-131) composed from the
+    oracle's operators: A = gen_sol_op(k, c_o, c_i) and rhs = A_o U with
+    A_o = [[-K_o + M/2, V_o], [W_o, K_o^T + M/2]], K_o, W_o, V_o assembled at
+    (k, c_o).  U: 2N x nrhs interpolation coefficients of the incoming waves."""
+    S = SolOp(mesh, order, proj=proj)
+    A = S.gen_sol_op(k, c_o, c_i, d=0, nthreads=nthreads)[0]
+    M = S.mass_matrix()
+    K, W, V = assemble(mesh, order, k, c_o, nthreads=nthreads)
+    Ko = K[0]
+    Ao = np.block([[-Ko + 0.5 * M, V[0]], [W[0], Ko.T + 0.5 * M]])
+    return A, Ao @ np.asarray(U, dtype=np.complex128)
+
+
+def tp_solve(mesh, order, k, c_o, c_i, U, nthreads=8):
+    """tp::direct_second_kind::solve (solvers.cpp:133-152): the unprojected
+    system of tp_A_rhs solved by LAPACK's zgesv (numpy.linalg.solve), the
+    routine the reference calls (solvers.cpp:25-51), after its finiteness
+    checks."""
+    A, rhs = tp_A_rhs(mesh, order, k, c_o, c_i, U, proj=False, nthreads=nthreads)
+    if not np.all(np.isfinite(rhs)):
+        raise RuntimeError("Failed to solve for traces, right-hand side is undefined")
+    if not np.all(np.isfinite(A)):
+        raise RuntimeError("Failed to solve for traces, solutions operator is undefined")
+    return np.linalg.solve(A, rhs)
