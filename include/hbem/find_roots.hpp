@@ -5,13 +5,17 @@
 // zbrent: Brent's method (R. P. Brent, Algorithms for Minimization without
 // Derivatives, 1973, ch. 4) on [x1, x2]; rtsafe: Newton steps on fct with
 // fct_both(x) = [f(x), f'(x)] (a 1 x 2 matrix), bisecting whenever a step
-// leaves the bracket or does not halve the previous step.  Step for step the
-// same as helmholtzbem_amd/resonances.py, so both give the same iterates.
+// leaves the bracket or does not halve the previous step; brent_gsl /
+// brent_gsl_with_values (find_roots.hpp:262-265, find_roots.cpp:1426-1512):
+// Brent's minimiser (ch. 5 of the same book) in the form of the GSL
+// fminimizer the reference wraps.  Step for step the same as
+// helmholtzbem_amd/resonances.py, so both give the same iterates.
 #pragma once
 
 #include <cmath>
 #include <functional>
 #include <limits>
+#include <vector>
 
 #include "hbem.hpp"
 
@@ -119,4 +123,89 @@ inline double rtsafe(std::function<double(double)> fct, std::function<hbem::Matr
     }
     num_iter = (unsigned)kMaxIt;
     return rts;
+}
+
+namespace hbem_roots_detail {
+constexpr unsigned kMaxItMin = 100;
+
+// |b - a| < epsabs + epsrel min(|a|, |b|) for a bracket not holding 0
+inline bool min_test_interval(double a, double b, double epsabs, double epsrel) {
+    const double min_abs = ((a > 0 && b > 0) || (a < 0 && b < 0)) ? std::min(std::fabs(a), std::fabs(b)) : 0.0;
+    return std::fabs(b - a) < epsabs + epsrel * min_abs;
+}
+
+// Brent's minimiser on [lo, hi] from z with f(z) below f(lo), f(hi).
+inline double brent_min(double lo, double f_lo, double hi, double f_hi, double z, double f_z,
+                        const std::function<double(double)>& f, double tol, unsigned& ic) {
+    const double golden = 0.5 * (3.0 - std::sqrt(5.0)), sqrt_eps = std::sqrt(kEps);
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    ic = 0;
+    if (!(f_z < f_lo && f_z < f_hi)) return nan;
+    double v = lo + golden * (hi - lo), w = v;
+    double f_v = f(v), f_w = f_v, d = 0.0, e = 0.0;
+    for (unsigned it = 1; it <= kMaxItMin; ++it) {
+        ic = it;
+        if (!std::isfinite(f_z) || !std::isfinite(f_v) || !std::isfinite(f_w)) return nan;
+        const double tolerance = sqrt_eps * std::fabs(z), mid = 0.5 * (lo + hi);
+        double p = 0.0, q = 0.0, r = 0.0, dd = e, ee = d;
+        if (std::fabs(ee) > tolerance) {
+            r = (z - w) * (f_z - f_v);
+            q = (z - v) * (f_z - f_w);
+            p = (z - v) * q - (z - w) * r;
+            q = 2.0 * (q - r);
+            if (q > 0.0) p = -p;
+            else q = -q;
+            r = ee;
+            ee = dd;
+        }
+        if (std::fabs(p) < std::fabs(0.5 * q * r) && p < q * (z - lo) && p < q * (hi - z)) {
+            dd = p / q;
+            const double u = z + dd;
+            if ((u - lo) < 2.0 * tolerance || (hi - u) < 2.0 * tolerance) dd = z < mid ? tolerance : -tolerance;
+        } else {
+            ee = z < mid ? (hi - z) : -(z - lo);
+            dd = golden * ee;
+        }
+        const double u = std::fabs(dd) >= tolerance ? z + dd : z + (dd > 0.0 ? tolerance : -tolerance);
+        e = ee;
+        d = dd;
+        const double f_u = f(u);
+        if (!std::isfinite(f_u)) return nan;
+        if (f_u <= f_z) {
+            if (u < z) hi = z;
+            else lo = z;
+            v = w; f_v = f_w;
+            w = z; f_w = f_z;
+            z = u; f_z = f_u;
+        } else {
+            if (u < z) lo = u;
+            else hi = u;
+            if (f_u <= f_w || w == z) {
+                v = w; f_v = f_w;
+                w = u; f_w = f_u;
+            } else if (f_u <= f_v || v == z || v == w) {
+                v = u; f_v = f_u;
+            }
+        }
+        if (min_test_interval(lo, hi, 0.0, tol)) return z;
+    }
+    return z;
+}
+}  // namespace hbem_roots_detail
+
+// Local minimum of f in [a, b] from guess; NaN when [a, b] does not enclose a
+// minimum below f(guess) or f is not finite.  The progress and value
+// callbacks of the reference (rp, rv) are accepted and not called.
+inline double brent_gsl(double a, double b, double guess, std::function<double(double)>& f, double tol,
+                        unsigned int& ic, std::function<void(double)>* rp = nullptr,
+                        std::function<void(double, double)>* rv = nullptr) {
+    (void)rp;
+    (void)rv;
+    const double fa = f(a), fb = f(b), fg = f(guess);
+    return hbem_roots_detail::brent_min(a, fa, b, fb, guess, fg, f, tol, ic);
+}
+
+inline double brent_gsl_with_values(double a, double fa, double b, double fb, double guess, double fguess,
+                                    std::function<double(double)>& f, double tol, unsigned int& ic) {
+    return hbem_roots_detail::brent_min(a, fa, b, fb, guess, fguess, f, tol, ic);
 }

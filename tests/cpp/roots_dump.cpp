@@ -1,5 +1,6 @@
 // Runs zbrent / rtsafe of include/hbem/find_roots.hpp on fixed test functions
-// and prints root, root_found, num_iter per case (17 significant digits).
+// and prints root, root_found, num_iter per case (17 significant digits), then
+// brent_gsl on fixed minimisation cases: x_minimum and iterations per line.
 #include <cmath>
 #include <cstdio>
 
@@ -23,6 +24,19 @@ int main() {
         it = 0;
         r = rtsafe(c.f, both, c.a, c.b, c.tol, rf, it);
         std::printf("%.17g %d %u\n", r, (int)rf, it);
+    }
+    struct MinCase { double (*f)(double); double a, b, g; };
+    const MinCase mins[] = {
+        {[](double x) { return (x - 2.0) * (x - 2.0) + 1.0; }, 1.0, 3.5, 2.4},
+        {[](double x) { return std::cos(2.0 * x); }, 0.0, 2.0, 1.5},
+        {[](double x) { return std::fabs(x - 0.1234) + 1.0; }, -1.0, 1.0, 0.3},
+        {[](double x) { return std::exp(x) - 3.0 * x; }, 0.0, 3.0, 1.0},
+    };
+    for (const MinCase& c : mins) {
+        std::function<double(double)> f = c.f;
+        unsigned ic = 0;
+        const double m = brent_gsl(c.a, c.b, c.g, f, 1e-7, ic);
+        std::printf("%.17g %u\n", m, ic);
     }
     return 0;
 }
