@@ -334,10 +334,16 @@ int hbem_geometry_create(const double* panels, int npanels, const double* coefs,
     if (order < 1 || order > 40) throw Error(HBEM_EINVAL, "quadrature order out of range");
     for (int i = 0; i < npanels; ++i) {
         int t = (int)panels[8 * i];
-        if (t < 0 || t > 2 || panels[8 * i] != (double)t) throw Error(HBEM_EINVAL, "invalid panel type");
+        if (t < 0 || t > 3 || panels[8 * i] != (double)t) throw Error(HBEM_EINVAL, "invalid panel type");
         if (t == 2) {
             int c0 = (int)panels[8 * i + 3], nc = (int)panels[8 * i + 4];
             if (c0 < 0 || nc < 1 || c0 + nc > ncoefs) throw Error(HBEM_EINVAL, "trig-curve panel coefficients out of range");
+        }
+        if (t == 3) {
+            const long long c0 = (long long)panels[8 * i + 3], np = (long long)panels[8 * i + 4];
+            if (c0 < 0 || np < 1 || c0 + 3 * np + 1 > ncoefs)
+                throw Error(HBEM_EINVAL, "spline panel coefficients out of range");
+            if (!(panels[8 * i + 1] < panels[8 * i + 2])) throw Error(HBEM_EINVAL, "spline panel needs tmin < tmax");
         }
     }
     g = new hbem_geometry;

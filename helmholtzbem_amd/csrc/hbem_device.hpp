@@ -284,6 +284,22 @@ __device__ __forceinline__ void panel_eval(const double* __restrict__ p, const d
         z = mk(fma(p[3], cs, p[1]), fma(p[3], sn, p[2]));
         const double a = p[3] * d;
         tau = mk(-a * sn, a * cs);
+    } else if (type == 3) {
+        // spline piece: rows of 9 doubles per spline piece j from row p[3]:
+        // {t_j, x_j, y_j, bx, cx, dx, by, cy, dy}; the piece holding t by bisection
+        const double dt = p[2] - p[1];
+        const double t = swapped ? fma(s, -dt, p[2]) : fma(s, dt, p[1]);
+        const double* c = coefs + 3 * (int)p[3];
+        int lo = 0, hi = (int)p[4];
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (c[9 * mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const double* r = c + 9 * lo;
+        const double d = t - r[0];
+        z = mk(r[1] + d * (r[3] + d * (r[4] + d * r[5])), r[2] + d * (r[6] + d * (r[7] + d * r[8])));
+        tau = mk(dt * (r[3] + d * (2.0 * r[4] + 3.0 * d * r[5])), dt * (r[6] + d * (2.0 * r[7] + 3.0 * d * r[8])));
     } else {
         const double dt = p[2] - p[1];
         const double t = swapped ? fma(s, -dt, p[2]) : fma(s, dt, p[1]);

@@ -62,10 +62,32 @@ void composite_rule(int n, double* x, double* w) {
     }
 }
 
+// Spline piece at t (rows of 9 doubles per piece, see the device panel_eval):
+// point z and dz/dt.
+static void host_spline(const double* p, const double* coefs, double t, std::complex<double>& z,
+                        std::complex<double>& dz) {
+    const double* c = coefs + 3 * (int)p[3];
+    int lo = 0, hi = (int)p[4];
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (c[9 * mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    const double* r = c + 9 * lo;
+    const double d = t - r[0];
+    z = std::complex<double>(r[1] + d * (r[3] + d * (r[4] + d * r[5])), r[2] + d * (r[6] + d * (r[7] + d * r[8])));
+    dz = std::complex<double>(r[3] + d * (2.0 * r[4] + 3.0 * d * r[5]), r[6] + d * (2.0 * r[7] + 3.0 * d * r[8]));
+}
+
 // Host panel evaluation (same parametrizations as the device panel_eval).
 static std::complex<double> host_tangent(const double* p, const double* coefs, double s) {
     typedef std::complex<double> cd;
     switch ((int)p[0]) {
+    case 3: {
+        cd z, dz;
+        host_spline(p, coefs, p[1] + s * (p[2] - p[1]), z, dz);
+        return (p[2] - p[1]) * dz;
+    }
     case 0: return cd(p[3] - p[1], p[4] - p[2]);
     case 1: {
         double d = p[5] - p[4];
@@ -87,6 +109,11 @@ static std::complex<double> host_tangent(const double* p, const double* coefs, d
 static std::complex<double> host_point(const double* p, const double* coefs, double s) {
     typedef std::complex<double> cd;
     switch ((int)p[0]) {
+    case 3: {
+        cd z, dz;
+        host_spline(p, coefs, p[1] + s * (p[2] - p[1]), z, dz);
+        return z;
+    }
     case 0: return cd(p[3] - p[1], p[4] - p[2]) * s + cd(p[1], p[2]);
     case 1: {
         double d = p[5] - p[4];

@@ -12,10 +12,15 @@ bit-identical to the reference's:
   but has no evaluation methods, so smooth scatterers use this type.
 * ``ParametrizedMesh`` -- parametrized_mesh.cpp:13-81.
 
+* ``ParametrizedPolynomial`` -- a piece [tmin, tmax] of a complex Akima spline
+  (parametrized_polynomial.cpp:21-173, the panels of smooth scatterers).
+
 Panels are packed as 8 doubles ``{type, a0..a6}``:
 line ``{0, x1, y1, x2, y2}``, arc ``{1, cx, cy, r, phi0, phi1}``,
 trig-curve segment ``{2, t0, t1, first_coef, n_coefs}`` with a coefficient table of
-``{m, re, im}`` triples.  This is the layout the C-ABI (include/hbem.h) takes.
+``{m, re, im}`` triples, spline piece ``{3, tmin, tmax, first_row, n_pieces}`` with
+the spline's rows in the same table (helmholtzbem_amd/spline.py).  This is the
+layout the C-ABI (include/hbem.h) takes.
 """
 from __future__ import annotations
 
@@ -28,6 +33,7 @@ import numpy as np
 PANEL_LINE = 0
 PANEL_ARC = 1
 PANEL_TRIG = 2
+PANEL_SPLINE = 3
 
 
 @dataclass
@@ -158,6 +164,56 @@ class ParametrizedTrigCurve:
     is_line = False
 
 
+class ParametrizedPolynomial:
+    """Piece [tmin, tmax] of a complex spline z(t) (parametrized_polynomial.cpp:21-173).
+
+    The derivatives carry the chain-rule factor of the parameter map:
+    Derivative_01(s) = (tmax - tmin) z'(t).  The reference returns z'(t)
+    there (parametrized_polynomial.cpp:91-97) and positions from its
+    vectorised Derivative_01_swapped (132-144); both are not what the
+    Galerkin integrals need, so the intended derivative is used.
+    """
+
+    def __init__(self, spline, tmin: float, tmax: float, length: float = -1.0):
+        self.spline, self.tmin, self.tmax = spline, float(tmin), float(tmax)
+        self._length = float(length) if length >= 0.0 else spline.arc_length(self.tmin, self.tmax)
+
+    def _t01(self, s):
+        return s * (self.tmax - self.tmin) + self.tmin
+
+    def __call__(self, t: float):
+        """Point at t in [-1, 1] (parametrized_polynomial.cpp:25-32)."""
+        v = complex(self.spline.eval(t * (self.tmax - self.tmin) / 2 + (self.tmax + self.tmin) / 2))
+        return (v.real, v.imag)
+
+    def __getitem__(self, s: float):
+        v = complex(self.spline.eval(self._t01(s)))
+        return (v.real, v.imag)
+
+    def Derivative_01(self, s: float):
+        v = (self.tmax - self.tmin) * complex(self.spline.eval_der(self._t01(s)))
+        return (v.real, v.imag)
+
+    def split(self, n: int) -> List["ParametrizedPolynomial"]:
+        """parametrized_polynomial.cpp:154-169."""
+        parts = []
+        for i in range(n):
+            a = self.tmin + i * (self.tmax - self.tmin) / n
+            b = self.tmin + (i + 1) * (self.tmax - self.tmin) / n
+            if i == n - 1:
+                b = self.tmax
+            parts.append(ParametrizedPolynomial(self.spline, a, b))
+        return parts
+
+    def length(self) -> float:
+        return self._length
+
+    def pack(self, coef_offset: int = 0):
+        return [PANEL_SPLINE, self.tmin, self.tmax, float(coef_offset), float(self.spline.pieces), 0.0, 0.0, 0.0]
+
+    is_line = False
+
+
 class ParametrizedMesh:
     """Ordered panel list (parametrized_mesh.cpp:13-31) packed for the C-ABI."""
 
@@ -175,6 +231,12 @@ class ParametrizedMesh:
                     for m, c in p.coeffs:
                         c = complex(c)
                         coef_rows.append([float(m), c.real, c.imag])
+                off = offsets[key]
+            elif isinstance(p, ParametrizedPolynomial):
+                key = id(p.spline)
+                if key not in offsets:
+                    offsets[key] = len(coef_rows)
+                    coef_rows.extend(p.spline.rows())
                 off = offsets[key]
             rows.append(p.pack(off))
         self.packed = np.ascontiguousarray(np.array(rows, dtype=np.float64).reshape(-1, 8))

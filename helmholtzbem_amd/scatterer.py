@@ -18,7 +18,8 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
-from .geometry import ParametrizedLine, ParametrizedMesh
+from .geometry import ParametrizedLine, ParametrizedMesh, ParametrizedPolynomial
+from .spline import ComplexSpline
 
 _MAX_ITER_ALPHA = 5000
 
@@ -272,11 +273,81 @@ def _graded_panels(x, y, N, refinement_factor):
     return res
 
 
-def mesh_from_file(fname: str, N: int, refinement_factor: Optional[float] = None, scale: float = 1.0
-                   ) -> ParametrizedMesh:
+class SmoothScatterer:
+    """A smooth closed curve through the points of a scatterer file
+    (scatterer.hpp:140-250, scatterer.cpp:385-533): a periodic Akima spline
+    z(t), t in [0, 1], through the points closed by the first one, the
+    parameter of point i being the polygon length up to it over the
+    perimeter.  ``panelize(n)`` splits it into n ParametrizedPolynomial panels
+    of equal arc length."""
+
+    TOL = 1e-8                                   # scatterer.cpp:387
+    MAX_ITER = 1000                              # scatterer.cpp:500
+
+    def __init__(self, fname: str, scale: float = 1.0):
+        xy = read(fname, scale)
+        if xy is None:
+            raise RuntimeError("Failed to read data for smooth scatterer")
+        x = [float(v) for v in xy[0]] + [float(xy[0][0])]
+        y = [float(v) for v in xy[1]] + [float(xy[1][0])]
+        n = len(x) - 1
+        if n < 3:
+            raise RuntimeError("Failed to read data for smooth scatterer")
+        L = length(x, y)
+        t = [0.0]
+        for i in range(n):
+            dx, dy = x[i] - x[i + 1], y[i] - y[i + 1]
+            t.append(math.sqrt(dx * dx + dy * dy) / L + t[i])
+        self.t_vert = np.array(t)
+        self.spline = ComplexSpline(self.t_vert, np.array(x) + 1j * np.array(y), periodic=True)
+        self._total = -1.0
+
+    def curve_point(self, t: float):
+        z = complex(self.spline.eval(t))
+        return (z.real, z.imag)
+
+    def total_length(self) -> float:
+        """Arc length of the closed curve (scatterer.cpp:484-490)."""
+        if self._total < 0.0:
+            self._total = self.spline.arc_length(0.0, float(self.t_vert[-1]))
+        return self._total
+
+    def panelize(self, npanels: int) -> List[ParametrizedPolynomial]:
+        """npanels panels of equal arc length (scatterer.cpp:492-518): the end
+        t1 of each panel by Newton steps t1 += (L/n - length(t0, t1)) / |z'(t1)|
+        from t0 + (1 - t0) / (panels left), until the relative step is below
+        1e-8; the last panel ends at t = 1."""
+        total = self.total_length()
+        panel_length = total / npanels
+        t0 = 0.0
+        panels: List[ParametrizedPolynomial] = []
+        for i in range(1, npanels):
+            t1 = t0 + (1.0 - t0) / (npanels - i + 1)
+            L = 0.0
+            for _ in range(self.MAX_ITER):
+                L = self.spline.arc_length(t0, t1)
+                t2 = t1 + (panel_length - L) / abs(complex(self.spline.eval_der(t1)))
+                if abs(t2 - t1) / t1 < self.TOL:
+                    break
+                t1 = t2
+            panels.append(ParametrizedPolynomial(self.spline, t0, t1, L))
+            t0 = t1
+            total -= L
+        panels.append(ParametrizedPolynomial(self.spline, t0, 1.0, total))
+        return panels
+
+
+def mesh_from_file(fname: str, N: int, refinement_factor: Optional[float] = None, scale: float = 1.0,
+                   smooth: bool = False) -> ParametrizedMesh:
     """A ParametrizedMesh of a scatterer file: exactly N uniform-per-edge panels
-    (make_N_polygonal_panels) or, with a refinement factor, graded panels
-    (make_polygonal_panels)."""
+    (make_N_polygonal_panels), with a refinement factor graded panels
+    (make_polygonal_panels), or with smooth=True -- or a file name ending in
+    '@', the convention of examples/plot_solution_polygon.cpp:64-109 -- N
+    equal-length panels of the smooth curve through the points."""
+    if fname.endswith("@"):
+        fname, smooth = fname[:-1], True
+    if smooth:
+        return ParametrizedMesh(SmoothScatterer(fname, scale).panelize(N))
     xy = read(fname, scale)
     if xy is None:
         raise ValueError("cannot read scatterer file " + fname)

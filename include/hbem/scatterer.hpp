@@ -10,6 +10,8 @@
 //   auto_num_panels        round(perimeter / (shortest edge * f))
 //   make_N_polygonal_panels  exactly N panels, uniform per edge
 //   make_polygonal_panels    about N panels graded toward the corners
+//   SmoothScatterer          periodic Akima spline through the points, split
+//                            into panels of equal arc length (panelize)
 // The arithmetic is sequential and in the same order as helmholtzbem_amd/
 // scatterer.py, so both produce the same panels bit for bit (compile with
 // -ffp-contract=off).
@@ -26,6 +28,7 @@
 #include <vector>
 
 #include "hbem.hpp"
+#include "spline.hpp"
 
 namespace scatterer {
 
@@ -273,5 +276,63 @@ inline PanelVector make_polygonal_panels(const std::vector<double>& x, const std
     }
     return res;
 }
+
+// Smooth closed curve through the points of a scatterer file
+// (scatterer.hpp:140-250, scatterer.cpp:385-533): periodic Akima spline z(t),
+// t in [0, 1], through the points closed by the first one, point i at the
+// polygon length up to it over the perimeter.
+class SmoothScatterer {
+  public:
+    explicit SmoothScatterer(const std::string& fname, double scale = 1.0) {
+        std::vector<double> x, y;
+        if (!read(fname, x, y, scale) || x.size() < 3) throw std::runtime_error("Failed to read data for smooth scatterer");
+        const size_t n = x.size();
+        x.push_back(x[0]);
+        y.push_back(y[0]);
+        const double L = length(x, y);
+        t_.assign(n + 1, 0.0);
+        std::vector<std::complex<double>> z(n + 1);
+        for (size_t i = 0; i < n; ++i) {
+            const double dx = x[i] - x[i + 1], dy = y[i] - y[i + 1];
+            t_[i + 1] = std::sqrt(dx * dx + dy * dy) / L + t_[i];
+        }
+        for (size_t i = 0; i <= n; ++i) z[i] = {x[i], y[i]};
+        spline_ = std::make_shared<hbem::ComplexSpline>(t_, z, true);
+    }
+    hbem::Point2 curve_point(double t) const {
+        const std::complex<double> z = spline_->eval(t);
+        return {z.real(), z.imag()};
+    }
+    double total_length() {
+        if (total_ < 0.0) total_ = spline_->arc_length(0.0, t_.back());
+        return total_;
+    }
+    // npanels panels of equal arc length (scatterer.cpp:492-518)
+    PanelVector panelize(unsigned npanels) {
+        double len = total_length(), t0 = 0.0;
+        const double panel_length = len / npanels, tol = 1e-8;
+        PanelVector panels;
+        for (unsigned i = 1; i < npanels; ++i) {
+            double t1 = t0 + (1.0 - t0) / double(npanels - i + 1), L = 0.0;
+            for (unsigned ic = 0; ic < 1000; ++ic) {
+                L = spline_->arc_length(t0, t1);
+                const double t2 = t1 + (panel_length - L) / std::abs(spline_->eval_der(t1));
+                if (std::fabs(t2 - t1) / t1 < tol) break;
+                t1 = t2;
+            }
+            panels.push_back(std::make_shared<hbem::ParametrizedPolynomial>(spline_, t0, t1, L));
+            t0 = t1;
+            len -= L;
+        }
+        panels.push_back(std::make_shared<hbem::ParametrizedPolynomial>(spline_, t0, 1.0, len));
+        return panels;
+    }
+    std::shared_ptr<const hbem::ComplexSpline> spline() const { return spline_; }
+
+  private:
+    std::vector<double> t_;
+    std::shared_ptr<hbem::ComplexSpline> spline_;
+    double total_ = -1.0;
+};
 
 }  // namespace scatterer

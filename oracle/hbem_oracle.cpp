@@ -246,11 +246,29 @@ static void fourier_eval(const Geo& g, const double* p, double t, cd& z, cd& dz)
     }
 }
 
+// Spline piece (smooth-scatterer panel, parametrized_polynomial.cpp:21-173):
+// rows of 9 doubles per spline piece j from row p[3], {t_j, x_j, y_j, bx, cx,
+// dx, by, cy, dy}; the piece holding t by bisection over the p[4] pieces.
+static void spline_eval(const Geo& g, const double* p, double t, cd& z, cd& dz) {
+    const double* c = &g.coefs[3 * (int)p[3]];
+    int lo = 0, hi = (int)p[4];
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (c[9 * mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    const double* r = c + 9 * lo;
+    const double d = t - r[0];
+    z = cd(r[1] + d * (r[3] + d * (r[4] + d * r[5])), r[2] + d * (r[6] + d * (r[7] + d * r[8])));
+    dz = cd(r[3] + d * (2.0 * r[4] + 3.0 * d * r[5]), r[6] + d * (2.0 * r[7] + 3.0 * d * r[8]));
+}
+
 static cd pan_point(const Geo& g, int i, double s) {
     const double* p = g.P(i);
     switch ((int)p[0]) {
     case 0: return cd(p[3] - p[1], p[4] - p[2]) * s + cd(p[1], p[2]);
     case 1: { double d = p[5] - p[4]; return cd(p[1], p[2]) + p[3] * std::exp(cd(0.0, s * d + p[4])); }
+    case 3: { cd z, dz; spline_eval(g, p, p[1] + s * (p[2] - p[1]), z, dz); return z; }
     default: { cd z, dz; fourier_eval(g, p, p[1] + s * (p[2] - p[1]), z, dz); return z; }
     }
 }
@@ -259,6 +277,7 @@ static cd pan_swapped(const Geo& g, int i, double s) {
     switch ((int)p[0]) {
     case 0: return cd(p[1] - p[3], p[2] - p[4]) * s + cd(p[3], p[4]);
     case 1: { double d = p[4] - p[5]; return cd(p[1], p[2]) + p[3] * std::exp(cd(0.0, s * d + p[5])); }
+    case 3: { cd z, dz; spline_eval(g, p, p[2] + s * (p[1] - p[2]), z, dz); return z; }
     default: { cd z, dz; fourier_eval(g, p, p[2] + s * (p[1] - p[2]), z, dz); return z; }
     }
 }
@@ -267,6 +286,7 @@ static cd pan_tangent(const Geo& g, int i, double s) {
     switch ((int)p[0]) {
     case 0: return cd(p[3] - p[1], p[4] - p[2]);
     case 1: { double d = p[5] - p[4]; return cd(0.0, p[3] * d) * std::exp(cd(0.0, s * d + p[4])); }
+    case 3: { cd z, dz; spline_eval(g, p, p[1] + s * (p[2] - p[1]), z, dz); return (p[2] - p[1]) * dz; }
     default: { cd z, dz; fourier_eval(g, p, p[1] + s * (p[2] - p[1]), z, dz); return (p[2] - p[1]) * dz; }
     }
 }
@@ -275,6 +295,7 @@ static cd pan_tangent_sw(const Geo& g, int i, double s) {
     switch ((int)p[0]) {
     case 0: return -cd(p[1] - p[3], p[2] - p[4]);
     case 1: { double d = p[4] - p[5]; return -(cd(0.0, p[3] * d) * std::exp(cd(0.0, s * d + p[5]))); }
+    case 3: { cd z, dz; spline_eval(g, p, p[2] + s * (p[1] - p[2]), z, dz); return -((p[1] - p[2]) * dz); }
     default: { cd z, dz; fourier_eval(g, p, p[2] + s * (p[1] - p[2]), z, dz); return -((p[1] - p[2]) * dz); }
     }
 }

@@ -87,6 +87,27 @@ def panel_fn(O, mesh, i):
         c, r, a, b = p[1] + 1j * p[2], p[3], p[4], p[5]
         return (lambda s: c + r * np.exp(1j * (a + (b - a) * np.asarray(s))),
                 lambda s: 1j * r * (b - a) * np.exp(1j * (a + (b - a) * np.asarray(s))))
+    if typ == 3:
+        # spline piece: 9 doubles per cubic piece {t_j, x_j, y_j, bx, cx, dx, by, cy, dy}
+        t0, t1, c0, npc = p[1], p[2], int(p[3]), int(p[4])
+        rows = mesh.coefs[c0:c0 + 3 * npc + 1]
+        knots = rows[0::3, 0]
+
+        def piece(s):
+            t = t0 + (t1 - t0) * np.asarray(s, dtype=float)
+            j = np.clip(np.searchsorted(knots, t, side="right") - 1, 0, npc - 1)
+            return t - rows[3 * j, 0], rows[3 * j], rows[3 * j + 1], rows[3 * j + 2]
+
+        def zs(s):
+            d, r0, r1, r2 = piece(s)
+            return (r0[..., 1] + d * (r1[..., 0] + d * (r1[..., 1] + d * r1[..., 2]))
+                    + 1j * (r0[..., 2] + d * (r2[..., 0] + d * (r2[..., 1] + d * r2[..., 2]))))
+
+        def dzs(s):
+            d, r0, r1, r2 = piece(s)
+            return (t1 - t0) * ((r1[..., 0] + d * (2 * r1[..., 1] + 3 * d * r1[..., 2]))
+                                + 1j * (r2[..., 0] + d * (2 * r2[..., 1] + 3 * d * r2[..., 2])))
+        return zs, dzs
     t0, t1, c0, nc = p[1], p[2], int(p[3]), int(p[4])
     co = mesh.coefs[c0:c0 + nc]
 
