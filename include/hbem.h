@@ -18,7 +18,11 @@
  *       type 0  line        {x1, y1, x2, y2}            ParametrizedLine
  *       type 1  arc         {cx, cy, r, phi0, phi1}     ParametrizedCircularArc
  *       type 2  trig curve  {t0, t1, first, count}      z(t) = sum c_m e^{imt}
- *     with a coefficient table of {m, re, im} triples for type 2.
+ *       type 3  spline      {tmin, tmax, first, pieces} ParametrizedPolynomial
+ *     with a coefficient table of rows of 3 doubles: {m, re, im} for type 2;
+ *     for type 3 three rows per cubic piece j of the spline, {t_j, x_j, y_j},
+ *     {bx, cx, dx}, {by, cy, dy} (z = z_j + b d + c d^2 + e d^3, d = t - t_j),
+ *     and a closing row {t_K, x_K, y_K}.
  *   - Spaces: test = trial = continuous P1 (ContinuousSpace<1>), the space of
  *     every sigma_min driver in the reference.
  *   - One GPU; all work of a geometry context runs on the context's stream.
