@@ -29,7 +29,7 @@ from .geometry import (ParametrizedCircularArc, ParametrizedLine, ParametrizedMe
                        square_mesh, star_curve, star_mesh)
 
 __all__ = ["LayerType", "ContinuousSpace1", "BuilderData", "GalerkinBuilder", "SolutionsOperator",
-           "randomized_svd", "sigma_sweep", "HbemError", "ParametrizedLine", "ParametrizedCircularArc",
+           "randomized_svd", "sigma_sweep", "sigma_sweep_multi", "HbemError", "ParametrizedLine", "ParametrizedCircularArc",
            "ParametrizedTrigCurve", "ParametrizedMesh", "square_mesh", "star_mesh", "circle_mesh",
            "polygon_mesh", "random_polygon_mesh", "gaussian_test_matrix", "Xoshiro256ss", "star_curve"]
 
@@ -391,6 +391,26 @@ def sigma_sweep(builder_data: BuilderData, ks, c_o, c_i, R, q=2, der=0, proj=Tru
     check(lib().hbem_sigma_sweep(g.h, _dptr(ks), ks.size, float(c_o), float(c_i), _vptr(R), R.shape[1], int(q),
                                  int(der), _dptr(sig), _dptr(dsig) if der else None))
     return (sig, dsig) if der else sig
+
+
+def sigma_sweep_multi(instances, q=2, der=0, proj=True, max_workers=None):
+    """sigma_sweep for many independent scatterers at once -- the drivers'
+    loop over instances of uneven size.  instances: (builder_data, ks, c_o,
+    c_i, R) tuples.  Every geometry context owns its device buffers and
+    stream, so the sweeps are issued from a pool of host threads (ctypes
+    releases the GIL) and the GPU overlaps them: small instances, whose
+    batched LU and assembly launches do not fill the GPU alone, share it.
+    Returns the list of sigma_sweep results in the order of the instances."""
+    from concurrent.futures import ThreadPoolExecutor
+    items = list(instances)
+    if not items:
+        return []
+    for bd, *_ in items:
+        bd._ctx(proj)                          # contexts are created on this thread
+    workers = max_workers or min(len(items), 8)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        futs = [ex.submit(sigma_sweep, bd, ks, c_o, c_i, R, q, der, proj) for bd, ks, c_o, c_i, R in items]
+        return [f.result() for f in futs]
 
 
 def profile_enable(builder_data: BuilderData, on=True, proj=True):
