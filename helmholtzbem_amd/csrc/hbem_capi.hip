@@ -402,18 +402,18 @@ int hbem_geometry_create(const double* panels, int npanels, const double* coefs,
     upload(g->d_gent, gent, st);
     upload(g->d_gpidx, gpi, st);
     g->d_gflags.get(std::max(1, g->ngen));
-    double* gngeo = g->d_gngeo.as<double>(std::max<size_t>(1, (size_t)g->ngen * g->ns * g->ns * 6));
+    double* gngeo = g->d_gngeo.as<double>(std::max<size_t>(1, (size_t)g->ngen * g->ns * g->ns * hbem::kNodeStride));
     hbem::launch_setup_general((const int*)g->d_gent.p, g->ngen, gtab, g->ns, gngeo, (unsigned char*)g->d_gflags.p, st);
     upload(g->d_aent, aent, st);
     upload(g->d_apidx, api, st);
     g->d_aflags.get(std::max(1, g->nadj));
-    double* angeo = g->d_angeo.as<double>(std::max<size_t>(1, (size_t)g->nadj * 2 * g->nc * g->nc * 6));
+    double* angeo = g->d_angeo.as<double>(std::max<size_t>(1, (size_t)g->nadj * 2 * g->nc * g->nc * hbem::kNodeStride));
     hbem::launch_setup_singular(1, (const int*)g->d_aent.p, g->nadj, dpan, dcoef, dxc, g->nc, angeo,
                                 (unsigned char*)g->d_aflags.p, st);
     upload(g->d_cent, cent, st);
     upload(g->d_cpidx, cpi, st);
     g->d_cflags.get(std::max(1, g->ncoi));
-    double* cngeo = g->d_cngeo.as<double>(std::max<size_t>(1, (size_t)g->ncoi * g->nc * g->nc * 6));
+    double* cngeo = g->d_cngeo.as<double>(std::max<size_t>(1, (size_t)g->ncoi * g->nc * g->nc * hbem::kNodeStride));
     hbem::launch_setup_singular(0, (const int*)g->d_cent.p, g->ncoi, dpan, dcoef, dxc, g->nc, cngeo,
                                 (unsigned char*)g->d_cflags.p, st);
     HIPC(hipGetLastError());

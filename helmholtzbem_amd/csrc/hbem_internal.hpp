@@ -10,13 +10,19 @@
 
 namespace hbem {
 
+// Node table record: r, r^2, the double-layer factors of the direct and the
+// transposed block, tau_i.tau_j, |tau_i||tau_j|, then 1/r, (2/pi) ln r, 1/sqrt(r)
+// (all three 0 at r = 0): the r-dependent factors every wavenumber slot needs,
+// computed once per geometry instead of once per staging workgroup.
+constexpr int kNodeStride = 9;
+
 struct AsmParams {
     const double* xg;            // Gauss rule on [0,1]
     const double* wg;
     const double* xc;            // composite rule
     const double* wc;
     const int* ent;              // class entries (i, j, swap)
-    const double* ngeo;          // node geometry [entry][nodes][6]
+    const double* ngeo;          // node geometry [entry][nodes][kNodeStride]
     const unsigned char* flags;  // per entry skip flags
     const long long* pidx;       // triangular pair index per entry
     int nent;                    // entries of this class
