@@ -110,94 +110,114 @@ __global__ __launch_bounds__(1024) void k_lu_panel(cdbl* __restrict__ A, int n, 
     }
 }
 
-// Row interchanges of the panel [j0, j0 + jb) applied to every other column,
-// and for the columns right of the panel U12 <- L11^{-1} A12 (L11 unit lower).
-// A workgroup takes 64 columns of one matrix: the panel rows and the distinct
-// pivot rows below the panel are staged in LDS (panel rows read as 512-byte
-// column segments), the interchanges run in LDS in their sequential order, the
-// triangular solve per column, and the rows are written back.
+// Row interchanges of the panel [j0, j0 + jb) (jb <= 32) and the U12 solve.
+// MODE bit 0: apply the panel's interchanges to every column outside the panel;
+// MODE bit 1: for the columns right of the panel, U12 <- L11^{-1} A12 (L11 unit lower).
+// A workgroup takes 32 columns of one matrix (50 KB of LDS: three per CU).
+//   * Interchanges: wavefront 0 numbers the distinct pivot rows below the panel
+//     in order of first appearance (ballot) and replays the jb swaps on the
+//     slot -> source index, one slot per lane, with wave-uniform readlanes; the
+//     rows involved (panel rows, then those pivot rows) are staged in LDS in
+//     their permuted order and written back.
+//   * Solve: a wavefront takes 8 columns, 8 lanes per column holding 4 rows
+//     each in registers; step k broadcasts row k within its column by a lane
+//     permute and updates the rows below it with L11 read from LDS.
+template <int MODE>
 __global__ __launch_bounds__(256) void k_lu_rows(cdbl* __restrict__ A, int n, int j0, int jb,
                                                  const int* __restrict__ ipiv) {
     cdbl* a = A + (size_t)blockIdx.y * n * n;
     const int* ip = ipiv + (size_t)blockIdx.y * n;
-    __shared__ cdbl top[32][65];
-    __shared__ cdbl ext[32][65];
+    __shared__ cdbl rows[64][33];   // slot t < jb: panel row j0 + t; slot jb + e: pivot row erow[e]
     __shared__ cdbl L[32][33];
-    __shared__ int erow[32];    // distinct pivot rows below the panel
-    __shared__ int slot[32];    // for interchange k: 0..31 panel row, 32 + e external row e
-    __shared__ int next;
-    const int tid = threadIdx.x;
+    __shared__ int erow[32];
+    __shared__ int src[64];
+    __shared__ int nes;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int nout = n - jb;
-    const int c0 = blockIdx.x * 64;
-    const int ncol = min(64, nout - c0);
+    const int c0 = blockIdx.x * 32;
+    const int ncol = min(32, nout - c0);
+    if (MODE == 2 && c0 + ncol <= j0) return;             // only columns left of the panel: nothing to solve
     // column index of local column cc (skipping the panel)
     auto colof = [&](int cc) { const int q = c0 + cc; return q < j0 ? q : q + jb; };
-    __shared__ int pv[32];
-    if (tid < jb) pv[tid] = ip[j0 + tid];
-    __syncthreads();
-    if (tid < 64) {
-        // interchange k: panel row (slot p - j0) or external row; external rows
-        // are numbered in order of first appearance (wavefront ballot)
-        const int k = tid;
-        const int p = k < jb ? pv[k] : 0;
-        const bool ex = k < jb && p >= j0 + jb;
-        int first = k;
-        if (ex)
-            for (int k2 = 0; k2 < k; ++k2)
-                if (pv[k2] == p) { first = k2; break; }
-        const unsigned long long firsts = __ballot(ex && first == k);
-        const int e = __popcll(firsts & ((1ull << first) - 1ull));
-        if (k < jb) slot[k] = ex ? 32 + e : p - j0;
-        if (ex && first == k) erow[e] = p;
-        if (k == 0) next = __popcll(firsts);
-    }
-    for (int e = tid; e < jb * jb; e += blockDim.x) {
-        const int r = e % jb, c = e / jb;
-        L[r][c] = a[(j0 + r) + (size_t)(j0 + c) * n];
-    }
-    for (int e = tid; e < 32 * 64; e += blockDim.x) {
-        const int r = e & 31, cc = e >> 5;
-        if (r < jb && cc < ncol) top[r][cc] = a[(j0 + r) + (size_t)colof(cc) * n];
-    }
-    __syncthreads();
-    const int ne = next;
-    for (int e = tid; e < ne * 64; e += blockDim.x) {
-        const int r = e >> 6, cc = e & 63;
-        if (cc < ncol) ext[r][cc] = a[erow[r] + (size_t)colof(cc) * n];
-    }
-    __syncthreads();
-    if (tid < ncol) {
-        const int cc = tid;
-        for (int k = 0; k < jb; ++k) {
-            const int sk = slot[k];
-            if (sk == k) continue;
-            cdbl& other = sk < 32 ? top[sk][cc] : ext[sk - 32][cc];
-            const cdbl t = top[k][cc];
-            top[k][cc] = other;
-            other = t;
+    auto touched = [&](int cc) { return cc < ncol && (MODE != 2 || colof(cc) >= j0 + jb); };
+    int ne = 0;
+    if (MODE & 1) {
+        if (wv == 0) {
+            const int k = lane;
+            const int p = k < jb ? ip[j0 + k] : 0;
+            const bool ex = k < jb && p >= j0 + jb;
+            int first = k;
+            if (ex)
+                for (int k2 = 0; k2 < k; ++k2)
+                    if (ip[j0 + k2] == p) { first = k2; break; }
+            const unsigned long long firsts = __ballot(ex && first == k);
+            const int e = __popcll(firsts & ((1ull << first) - 1ull));
+            if (ex && first == k) erow[e] = p;
+            const int partner = k < jb ? (ex ? jb + e : p - j0) : k;   // slot swapped with slot k
+            int v = k;
+            for (int sw = 0; sw < jb; ++sw) {
+                const int b = __builtin_amdgcn_readlane(partner, sw);
+                if (b != sw) {
+                    const int va = __builtin_amdgcn_readlane(v, sw);
+                    const int vb = __builtin_amdgcn_readlane(v, b);
+                    v = k == sw ? vb : (k == b ? va : v);
+                }
+            }
+            src[k] = v;
+            if (k == 0) nes = __popcll(firsts);
         }
-        if (colof(cc) >= j0 + jb) {
-            // column in registers (jb <= 32), L11 read from LDS
-            cdbl x[32];
-#pragma unroll
-            for (int k = 0; k < 32; ++k) x[k] = k < jb ? top[k][cc] : mk(0, 0);
-#pragma unroll
-            for (int k = 0; k < 32; ++k)
-#pragma unroll
-                for (int i = k + 1; i < 32; ++i)
-                    if (i < jb) cfms(x[i], L[i][k], x[k]);
-#pragma unroll
-            for (int k = 0; k < 32; ++k) if (k < jb) top[k][cc] = x[k];
+        __syncthreads();
+        ne = nes;
+    }
+    const int nrow = jb + ne;
+    if (MODE & 2)
+        for (int e2 = tid; e2 < jb * jb; e2 += blockDim.x) {
+            const int r = e2 % jb, c = e2 / jb;
+            L[r][c] = a[(j0 + r) + (size_t)(j0 + c) * n];
         }
+    // staged already permuted: slot t receives the row that the swaps move into it
+    auto row_of = [&](int t) { return t < jb ? j0 + t : erow[t - jb]; };
+    for (int e2 = tid; e2 < nrow * 32; e2 += blockDim.x) {
+        const int t = e2 % nrow, cc = e2 / nrow;
+        if (touched(cc)) rows[t][cc] = a[row_of((MODE & 1) ? src[t] : t) + (size_t)colof(cc) * n];
     }
     __syncthreads();
-    for (int e = tid; e < 32 * 64; e += blockDim.x) {
-        const int r = e & 31, cc = e >> 5;
-        if (r < jb && cc < ncol) a[(j0 + r) + (size_t)colof(cc) * n] = top[r][cc];
+    if (MODE & 2) {
+        const int c8 = lane & 7, g = lane >> 3;
+        const int cc = 8 * wv + c8;
+        const bool act = cc < ncol && colof(cc) >= j0 + jb;
+        cdbl x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = g + 8 * u;
+            x[u] = (act && i < jb) ? rows[i][cc] : mk(0, 0);
+        }
+        // row k = 8 uk + kk sits in register uk of lane kk * 8 + c8
+#pragma unroll
+        for (int uk = 0; uk < 4; ++uk) {
+            for (int kk = 0; kk < 8; ++kk) {
+                const int k = 8 * uk + kk;
+                if (k + 1 >= jb) break;
+                const cdbl xk = shfl_c(x[uk], (kk << 3) | c8);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int i = g + 8 * u;
+                    if (i > k && i < jb) cfms(x[u], L[i][k], xk);
+                }
+            }
+        }
+        if (act) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = g + 8 * u;
+                if (i < jb) rows[i][cc] = x[u];
+            }
+        }
+        __syncthreads();
     }
-    for (int e = tid; e < ne * 64; e += blockDim.x) {
-        const int r = e >> 6, cc = e & 63;
-        if (cc < ncol) a[erow[r] + (size_t)colof(cc) * n] = ext[r][cc];
+    for (int e2 = tid; e2 < nrow * 32; e2 += blockDim.x) {
+        const int t = e2 % nrow, cc = e2 / nrow;
+        if (touched(cc)) a[row_of(t) + (size_t)colof(cc) * n] = rows[t][cc];
     }
 }
 
@@ -284,19 +304,50 @@ void zgemm_batched(int m, int n, int k, cdbl alpha, const cdbl* A, int lda, long
         hipLaunchKernelGGL(k_zgemm<false>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc);
 }
 
+// Blocked right-looking LU in steps of 64 columns, each factorised as two
+// 32-column sub-panels: the second sub-panel, and its future U rows right of
+// it, are brought up to date with the first one's k = 32 product, and the
+// trailing matrix receives one k = 64 update per step.  The trailing update
+// reads and writes the trailing matrix once per step -- the traffic that bounds
+// it at these sizes -- so 64-column steps halve it.
+//   sub-panel 1: panel, interchanges everywhere + U1 for every column right of it
+//   sub-panel 2: A22 -= L1 U1 on its columns; panel; its interchanges everywhere;
+//                A[j1:j2, j2:] -= L1[j1:j2] U1[:, j2:] with the interchanged L1 rows;
+//                U2 = L22^{-1} A[j1:j2, j2:]
+//   trailing:    A[j2:, j2:] -= [L1 L2][j2:] [U1; U2]
 void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStream_t st) {
-    const int nb = 32;
+    const int sb = 32;
     const long long s = (long long)n * n;
-    for (int j0 = 0; j0 < n; j0 += nb) {
-        const int jb = n - j0 < nb ? n - j0 : nb;
-        hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(1024), 0, st, A, n, j0, jb, ipiv);
+    auto at = [&](int r, int c) { return A + r + (size_t)c * n; };
+    auto rows = [&](int mode, int j0, int jb) {
         const int nout = n - jb;
-        if (nout > 0) hipLaunchKernelGGL(k_lu_rows, dim3((nout + 63) / 64, batch), dim3(256), 0, st, A, n, j0, jb, ipiv);
-        const int nr = n - j0 - jb;
-        if (nr > 0) {
-            zgemm_batched(nr, nr, jb, mk(-1, 0), A + (j0 + jb) + (size_t)j0 * n, n, s, A + j0 + (size_t)(j0 + jb) * n, n,
-                          s, false, mk(1, 0), A + (j0 + jb) + (size_t)(j0 + jb) * n, n, s, batch, st);
+        if (nout <= 0) return;
+        const dim3 g((nout + 31) / 32, batch), t(256);
+        if (mode == 3) hipLaunchKernelGGL(k_lu_rows<3>, g, t, 0, st, A, n, j0, jb, ipiv);
+        else if (mode == 1) hipLaunchKernelGGL(k_lu_rows<1>, g, t, 0, st, A, n, j0, jb, ipiv);
+        else hipLaunchKernelGGL(k_lu_rows<2>, g, t, 0, st, A, n, j0, jb, ipiv);
+    };
+    for (int j0 = 0; j0 < n; j0 += 2 * sb) {
+        const int jb1 = n - j0 < sb ? n - j0 : sb;
+        hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(1024), 0, st, A, n, j0, jb1, ipiv);
+        rows(3, j0, jb1);
+        const int j1 = j0 + jb1;
+        const int jb2 = n - j1 < sb ? n - j1 : sb;
+        const int j2 = j1 + jb2;
+        if (jb2 > 0) {
+            zgemm_batched(n - j1, jb2, jb1, mk(-1, 0), at(j1, j0), n, s, at(j0, j1), n, s, false, mk(1, 0), at(j1, j1),
+                          n, s, batch, st);
+            hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(1024), 0, st, A, n, j1, jb2, ipiv);
+            rows(1, j1, jb2);
+            if (n - j2 > 0) {
+                zgemm_batched(jb2, n - j2, jb1, mk(-1, 0), at(j1, j0), n, s, at(j0, j2), n, s, false, mk(1, 0),
+                              at(j1, j2), n, s, batch, st);
+                rows(2, j1, jb2);
+            }
         }
+        if (n - j2 > 0)
+            zgemm_batched(n - j2, n - j2, j2 - j0, mk(-1, 0), at(j2, j0), n, s, at(j0, j2), n, s, false, mk(1, 0),
+                          at(j2, j2), n, s, batch, st);
     }
     hipLaunchKernelGGL(k_lu_perm, dim3(batch), dim3(64), 0, st, n, ipiv, perm);
 }
