@@ -843,34 +843,63 @@ __global__ void k_gather_layers(const cdbl* __restrict__ L, int nk, int nslot, i
 // T = [[K_i - K_o + M, V_o - V_i], [W_o - W_i, K_o - K_i + M^T]], M only for t = 0.
 // Output b uses wavenumber slots qi[b] (inner c_i), qo[b] (outer c_o); T_t of
 // output b is written at Tt + b (2N)^2.
-// Workgroup = 8 consecutive rows m of one column n for 32 outputs b; thread
-// (i, jb) = (threadIdx & 7, threadIdx >> 3) forms the four block entries
-// (m, n), (m + N, n + N), (m, n + N), (m + N, n) of every order t.  Reads run
-// over 8 wavenumbers per record slot, writes over 8 rows per column.
+// Workgroup = one pair of 8 x 8 tiles (I, J), I >= J, of the N x N blocks for
+// 32 outputs b; thread (i, jb) = (threadIdx & 7, threadIdx >> 3) forms the
+// four block entries (m, n), (m + N, n + N), (m, n + N), (m + N, n) of row
+// m = 8 I + i over the 8 columns of tile (I, J), then (for I > J) of row
+// 8 J + i over the columns of tile (J, I).  op_entry sums the symmetric layers
+// W and V bitwise symmetric, so tile (J, I) takes them from tile (I, J)
+// through LDS: every local-matrix record is read once.  Reads run over 32
+// wavenumbers per record slot, writes over 8 rows per column.
 __global__ __launch_bounds__(256) void k_form_T(const cdbl* __restrict__ L, int nk, int nslot,
                                                 const int* __restrict__ qi, const int* __restrict__ qo, int nb, int N,
                                                 int D, const double* __restrict__ M, cdbl* __restrict__ T0,
                                                 cdbl* __restrict__ T1, cdbl* __restrict__ T2) {
-    const int i = threadIdx.x & 7, jb = threadIdx.x >> 3;
+    __shared__ cdbl sW[8][8][32];   // [column in tile (I, J)][row in tile (I, J)][output]
+    __shared__ cdbl sV[8][8][32];
     const int mt = (N + 7) >> 3;
-    const int m = (blockIdx.x % mt) * 8 + i;
-    const int n = blockIdx.x / mt;
+    const int I = blockIdx.x % mt, J = blockIdx.x / mt;
+    if (J > I) return;                       // whole workgroup: no barrier is skipped
+    const int i = threadIdx.x & 7, jb = threadIdx.x >> 3;
     const int b = blockIdx.y * 32 + jb;
-    if (m >= N || b >= nb) return;
+    const bool act = b < nb;                 // idle threads still reach every barrier
     const int n2 = 2 * N;
     const size_t NN = (size_t)n2 * n2;
-    const int a = qi[b], o = qo[b];
+    const int a = act ? qi[b] : 0, o = act ? qo[b] : 0;
+    const bool mirror = I != J;
     for (int t = 0; t <= D; ++t) {
-        const cdbl Ki = op_entry(L, nk, nslot, a, 0, t, m, n, N), Ko = op_entry(L, nk, nslot, o, 0, t, m, n, N);
-        const cdbl Wi = op_entry(L, nk, nslot, a, 1, t, m, n, N), Wo = op_entry(L, nk, nslot, o, 1, t, m, n, N);
-        const cdbl Vi = op_entry(L, nk, nslot, a, 2, t, m, n, N), Vo = op_entry(L, nk, nslot, o, 2, t, m, n, N);
-        cdbl k00 = Ki - Ko, k11 = Ko - Ki;
-        if (t == 0) { k00.x += M[m + (size_t)n * N]; k11.x += M[n + (size_t)m * N]; }
-        cdbl* Tt = (t == 0 ? T0 : (t == 1 ? T1 : T2)) + (size_t)b * NN;
-        Tt[m + (size_t)n * n2] = k00;
-        Tt[(m + N) + (size_t)(n + N) * n2] = k11;
-        Tt[m + (size_t)(n + N) * n2] = Vo - Vi;
-        Tt[(m + N) + (size_t)n * n2] = Wo - Wi;
+        cdbl* Tt = (t == 0 ? T0 : (t == 1 ? T1 : T2)) + (size_t)(act ? b : 0) * NN;
+        const int m = 8 * I + i;
+        for (int c = 0; c < 8; ++c) {
+            const int n = 8 * J + c;
+            if (!act || m >= N || n >= N) continue;
+            const cdbl Ki = op_entry(L, nk, nslot, a, 0, t, m, n, N), Ko = op_entry(L, nk, nslot, o, 0, t, m, n, N);
+            const cdbl Wi = op_entry(L, nk, nslot, a, 1, t, m, n, N), Wo = op_entry(L, nk, nslot, o, 1, t, m, n, N);
+            const cdbl Vi = op_entry(L, nk, nslot, a, 2, t, m, n, N), Vo = op_entry(L, nk, nslot, o, 2, t, m, n, N);
+            cdbl k00 = Ki - Ko, k11 = Ko - Ki;
+            if (t == 0) { k00.x += M[m + (size_t)n * N]; k11.x += M[n + (size_t)m * N]; }
+            const cdbl w = Wo - Wi, v = Vo - Vi;
+            Tt[m + (size_t)n * n2] = k00;
+            Tt[(m + N) + (size_t)(n + N) * n2] = k11;
+            Tt[m + (size_t)(n + N) * n2] = v;
+            Tt[(m + N) + (size_t)n * n2] = w;
+            if (mirror) { sW[c][i][jb] = w; sV[c][i][jb] = v; }
+        }
+        if (!mirror) continue;
+        __syncthreads();
+        const int r = 8 * J + i;
+        for (int c = 0; c < 8; ++c) {
+            const int col = 8 * I + c;
+            if (!act || r >= N || col >= N) continue;
+            const cdbl Ki = op_entry(L, nk, nslot, a, 0, t, r, col, N), Ko = op_entry(L, nk, nslot, o, 0, t, r, col, N);
+            cdbl k00 = Ki - Ko, k11 = Ko - Ki;
+            if (t == 0) { k00.x += M[r + (size_t)col * N]; k11.x += M[col + (size_t)r * N]; }
+            Tt[r + (size_t)col * n2] = k00;
+            Tt[(r + N) + (size_t)(col + N) * n2] = k11;
+            Tt[r + (size_t)(col + N) * n2] = sV[i][c][jb];   // V(r, col) = V(col, r)
+            Tt[(r + N) + (size_t)col * n2] = sW[i][c][jb];
+        }
+        __syncthreads();                     // the next order overwrites sW / sV
     }
 }
 
@@ -972,8 +1001,8 @@ void launch_gather_layers(const cdbl* L, int nk, int nslot, int q, int N, int D,
 void launch_form_T(const cdbl* L, int nk, int nslot, const int* qi, const int* qo, int nb, int N, int D,
                    const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st) {
     const unsigned mt = (unsigned)((N + 7) / 8);
-    hipLaunchKernelGGL(k_form_T, dim3(mt * (unsigned)N, (unsigned)((nb + 31) / 32)), dim3(256), 0, st, L, nk, nslot, qi,
-                       qo, nb, N, D, M, T0, T1, T2);
+    hipLaunchKernelGGL(k_form_T, dim3(mt * mt, (unsigned)((nb + 31) / 32)), dim3(256), 0, st, L, nk, nslot, qi, qo, nb,
+                       N, D, M, T0, T1, T2);
 }
 
 void launch_form_Ao(const cdbl* L, int nk, int nslot, int o, int N, const double* M, cdbl* A, hipStream_t st) {

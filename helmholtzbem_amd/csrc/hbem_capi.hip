@@ -55,6 +55,26 @@ T* upload(DBuf& b, const std::vector<T>& v, hipStream_t st) {
     return d;
 }
 
+// Real sparse n x n matrix in compressed-row form on the device.
+struct DCsr {
+    DBuf rp, ci, v;
+    // rows of op(A) for a dense column-major A (op = A or A^T); exact zeros are dropped
+    void build(const std::vector<double>& A, int n, bool trans, hipStream_t st) {
+        std::vector<int> r(n + 1, 0), c;
+        std::vector<double> x;
+        for (int i = 0; i < n; ++i) {
+            for (int j = 0; j < n; ++j) {
+                const double a = trans ? A[j + (size_t)i * n] : A[i + (size_t)j * n];
+                if (a != 0.0) { c.push_back(j); x.push_back(a); }
+            }
+            r[i + 1] = (int)c.size();
+        }
+        upload(rp, r, st);
+        upload(ci, c, st);
+        upload(v, x, st);
+    }
+};
+
 // Algorithmic FP64 flops per node (frozen model): Hankel node (13 geometry + 171
 // Bessel) and accumulation for P1 x P1, three layers, derivative order d.
 double flops_per_node(int d) { return 184.0 + (d == 0 ? 108.0 : d == 1 ? 192.0 : 314.0); }
@@ -76,7 +96,8 @@ struct hbem_geometry {
     DBuf d_aent, d_apidx, d_aflags, d_angeo;
     DBuf d_cent, d_cpidx, d_cflags, d_cngeo;
     int ngen = 0, nadj = 0, ncoi = 0;
-    DBuf d_M, d_Frm, d_Fcm, d_Finvcm, d_Finvc;
+    DBuf d_M, d_Finvc;
+    DCsr s_F, s_FT, s_FinvT;   // F, F^T and Finv^T of the projection
     // work space
     DBuf w_Ao, w_U, w_rhs, w_flag;
     DBuf w_kapi, w_kri, w_LUT, w_L, w_T0, w_T1, w_T2, w_piv, w_perm, w_X, w_Y, w_work, w_R, w_u, w_v, w_a, w_b, w_sig, w_dsig, w_kap,
@@ -205,53 +226,52 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
 }
 
 // Dense stage shared by sv, sv_der and the sweep: given LU factors of the
-// unprojected T (batch matrices), the projection factor (F row-major /
-// column-major, or null for no projection) and R, returns sigma and the
-// singular pair (u, v) of T_p = F^{-1} T F^{-T} in d_u / d_v.
+// unprojected T (batch matrices), the projection factor (sparse F and F^T, or
+// none) and R, returns sigma and the singular pair (u, v) of
+// T_p = F^{-1} T F^{-T} in d_u / d_v.
 struct DenseCtx {
     hipStream_t st;
     int n, batch, nc, q;
     const cdbl* LU;
     const cdbl* LUT;     // transposed factors (adjoint solves)
     const int* perm;
-    const double* Frm;   // F row-major (apply F), null = identity
-    const double* Fcm;   // F column-major (apply F^T)
+    const DCsr* F;       // projection factor, null = no projection
+    const DCsr* FT;      // its transpose
     cdbl* X;             // n x nc x batch
     cdbl* Y;
-    cdbl* work;
+    cdbl* work;          // 2 n nc batch: thin-solve scratch, then the projected right-hand sides
 };
 
-// Y <- T_p^{-1} X  or  T_p^{-H} X  (T_p^{-1} = F^T T^{-1} F, T_p^{-H} = F^T T^{-H} F)
-void tp_solve(DenseCtx& c, cdbl* X, cdbl* Y, bool adj) {
-    if (c.Frm) {
-        hbem::real_apply_thin_batched(c.Frm, c.n, c.batch, X, Y, c.nc, c.st);
-        hbem::lu_solve_thin_batched(c.LU, c.LUT, c.perm, c.n, c.batch, Y, c.nc, adj, c.work, c.st);
-        hbem::real_apply_thin_batched(c.Fcm, c.n, c.batch, Y, X, c.nc, c.st);
-        HIPC(hipMemcpyAsync(Y, X, sizeof(cdbl) * (size_t)c.n * c.nc * c.batch, hipMemcpyDeviceToDevice, c.st));
+// Y <- T_p^{-1} X  or  T_p^{-H} X  (T_p^{-1} = F^T T^{-1} F, T_p^{-H} = F^T T^{-H} F); X is kept.
+void tp_solve(DenseCtx& c, const cdbl* X, cdbl* Y, bool adj) {
+    const long long cols = (long long)c.nc * c.batch;
+    if (c.F) {
+        cdbl* Z = c.work + (size_t)c.n * cols;
+        hbem::csr_apply_batched((const int*)c.F->rp.p, (const int*)c.F->ci.p, (const double*)c.F->v.p, c.n, X, Z, cols,
+                                c.st);
+        hbem::lu_solve_thin_batched(c.LU, c.LUT, c.perm, c.n, c.batch, Z, c.nc, adj, c.work, c.st);
+        hbem::csr_apply_batched((const int*)c.FT->rp.p, (const int*)c.FT->ci.p, (const double*)c.FT->v.p, c.n, Z, Y,
+                                cols, c.st);
     } else {
-        if (Y != X) HIPC(hipMemcpyAsync(Y, X, sizeof(cdbl) * (size_t)c.n * c.nc * c.batch, hipMemcpyDeviceToDevice, c.st));
+        HIPC(hipMemcpyAsync(Y, X, sizeof(cdbl) * (size_t)c.n * cols, hipMemcpyDeviceToDevice, c.st));
         hbem::lu_solve_thin_batched(c.LU, c.LUT, c.perm, c.n, c.batch, Y, c.nc, adj, c.work, c.st);
     }
 }
 
-// randomized_svd::sv (randsvd.cpp:99-146) on the batch; X holds R replicated per matrix.
+// randomized_svd::sv (randsvd.cpp:99-146) on the batch; X holds R replicated
+// per matrix.  The power iteration alternates between X and Y.
 void rsvd_core(DenseCtx& c, double* d_sigma, cdbl* d_u, cdbl* d_v) {
     tp_solve(c, c.X, c.Y, false);                              // Y = T_p^{-1} R
     hbem::orthonormalize_thin_batched(c.Y, c.n, c.batch, c.nc, c.st);   // Q
     for (int it = 0; it < c.q; ++it) {
-        HIPC(hipMemcpyAsync(c.X, c.Y, sizeof(cdbl) * (size_t)c.n * c.nc * c.batch, hipMemcpyDeviceToDevice, c.st));
-        tp_solve(c, c.X, c.Y, true);
-        hbem::orthonormalize_thin_batched(c.Y, c.n, c.batch, c.nc, c.st);
-        HIPC(hipMemcpyAsync(c.X, c.Y, sizeof(cdbl) * (size_t)c.n * c.nc * c.batch, hipMemcpyDeviceToDevice, c.st));
+        tp_solve(c, c.Y, c.X, true);
+        hbem::orthonormalize_thin_batched(c.X, c.n, c.batch, c.nc, c.st);
         tp_solve(c, c.X, c.Y, false);
         hbem::orthonormalize_thin_batched(c.Y, c.n, c.batch, c.nc, c.st);
     }
-    // X = Q, Y = T_p^{-H} Q
-    HIPC(hipMemcpyAsync(c.X, c.Y, sizeof(cdbl) * (size_t)c.n * c.nc * c.batch, hipMemcpyDeviceToDevice, c.st));
-    cdbl* Q = c.work + (size_t)c.n * c.nc * c.batch;   // second half of work holds Q
-    HIPC(hipMemcpyAsync(Q, c.X, sizeof(cdbl) * (size_t)c.n * c.nc * c.batch, hipMemcpyDeviceToDevice, c.st));
-    tp_solve(c, c.X, c.Y, true);
-    hbem::sigma_from_thin_batched(c.Y, Q, c.n, c.batch, c.nc, d_sigma, d_u, d_v, c.st);
+    // Q = Y, X = T_p^{-H} Q
+    tp_solve(c, c.Y, c.X, true);
+    hbem::sigma_from_thin_batched(c.X, c.Y, c.n, c.batch, c.nc, d_sigma, d_u, d_v, c.st);
 }
 
 void copy_out_square(const cdbl* d, void* h, size_t count, hipStream_t st) {
@@ -434,11 +454,9 @@ int hbem_geometry_create(const double* panels, int npanels, const double* coefs,
         hbem::ldlt_projection_factor(A.data(), n, g->F.data(), g->tr.data());
         g->Finv.resize((size_t)n * n);
         hbem::invert_projection_factor(g->F.data(), g->tr.data(), n, g->Finv.data());
-        std::vector<double> Frm((size_t)n * n);
-        for (int j = 0; j < n; ++j) for (int i = 0; i < n; ++i) Frm[(size_t)i * n + j] = g->F[i + (size_t)j * n];
-        upload(g->d_Frm, Frm, st);
-        upload(g->d_Fcm, g->F, st);
-        upload(g->d_Finvcm, g->Finv, st);
+        g->s_F.build(g->F, n, false, st);
+        g->s_FT.build(g->F, n, true, st);
+        g->s_FinvT.build(g->Finv, n, true, st);
         std::vector<cdbl> Fic((size_t)n * n);
         for (size_t e = 0; e < Fic.size(); ++e) Fic[e] = hbem::mk(g->Finv[e], 0.0);
         upload(g->d_Finvc, Fic, st);
@@ -703,7 +721,7 @@ double standalone_sv(const void* T, int n, const void* R, int nc, int q, bool wa
     hbem::transpose_batched(dT, dLUT, n, 1, s.st);
     DenseCtx c;
     c.st = s.st; c.n = n; c.batch = 1; c.nc = nc; c.q = q; c.LU = dT; c.LUT = dLUT; c.perm = perm;
-    c.Frm = nullptr; c.Fcm = nullptr;
+    c.F = nullptr; c.FT = nullptr;
     c.X = s.X.as<cdbl>((size_t)n * nc);
     c.Y = s.Y.as<cdbl>((size_t)n * nc);
     c.work = s.work.as<cdbl>(2 * (size_t)n * nc);
@@ -913,13 +931,12 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
         hbem::transpose_batched(T0, LUT, n, nb, g->st);
         DenseCtx c;
         c.st = g->st; c.n = n; c.batch = nb; c.nc = nc; c.q = q; c.LU = T0; c.LUT = LUT; c.perm = perm;
-        c.Frm = g->proj ? (const double*)g->d_Frm.p : nullptr;
-        c.Fcm = g->proj ? (const double*)g->d_Fcm.p : nullptr;
+        c.F = g->proj ? &g->s_F : nullptr;
+        c.FT = g->proj ? &g->s_FT : nullptr;
         c.X = g->w_X.as<cdbl>((size_t)n * nc * nb);
         c.Y = g->w_Y.as<cdbl>((size_t)n * nc * nb);
         c.work = g->w_work.as<cdbl>(2 * (size_t)n * nc * nb);
-        for (int b = 0; b < nb; ++b)
-            HIPC(hipMemcpyAsync(c.X + (size_t)b * n * nc, dR, (size_t)n * nc * sizeof(cdbl), hipMemcpyDeviceToDevice, g->st));
+        hbem::replicate_batched(dR, (long long)n * nc, nb, c.X, g->st);
         double* dsig = g->w_sig.as<double>(nb);
         cdbl* du = D ? g->w_u.as<cdbl>((size_t)n * nb) : nullptr;
         cdbl* dv = D ? g->w_v.as<cdbl>((size_t)n * nb) : nullptr;
@@ -929,8 +946,9 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
             cdbl* a = g->w_a.as<cdbl>((size_t)n * nb);
             cdbl* bb = g->w_b.as<cdbl>((size_t)n * nb);
             if (g->proj) {
-                hbem::real_apply_thin_batched((const double*)g->d_Finvcm.p, n, nb, du, a, 1, g->st);
-                hbem::real_apply_thin_batched((const double*)g->d_Finvcm.p, n, nb, dv, bb, 1, g->st);
+                const DCsr& S = g->s_FinvT;
+                hbem::csr_apply_batched((const int*)S.rp.p, (const int*)S.ci.p, (const double*)S.v.p, n, du, a, nb, g->st);
+                hbem::csr_apply_batched((const int*)S.rp.p, (const int*)S.ci.p, (const double*)S.v.p, n, dv, bb, nb, g->st);
             } else {
                 HIPC(hipMemcpyAsync(a, du, (size_t)n * nb * sizeof(cdbl), hipMemcpyDeviceToDevice, g->st));
                 HIPC(hipMemcpyAsync(bb, dv, (size_t)n * nb * sizeof(cdbl), hipMemcpyDeviceToDevice, g->st));

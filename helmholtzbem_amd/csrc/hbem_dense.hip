@@ -545,32 +545,45 @@ void flag_zero_diag(const cdbl* A, int n, int* flag, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// Y = F X for a real dense F given row-major; one wavefront per row.
+// Y = S X for a real sparse n x n S in compressed-row form; X and Y are
+// n x ncols column-major (leading dimension n) -- every thin block of a batch
+// at once, since the blocks of a batch are stored back to back.  One thread
+// per output entry; consecutive threads take consecutive rows of one column,
+// so the writes are coalesced and the gathered reads of a banded S stay in a
+// few cache lines.  The row sum runs in stored (ascending column) order.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_real_apply(const double* __restrict__ Frm, int n, const cdbl* __restrict__ X,
-                                                    cdbl* __restrict__ Y, int nc) {
-    const int b = blockIdx.y;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= n) return;
-    const cdbl* x = X + (size_t)b * n * nc;
-    const double* f = Frm + (size_t)row * n;
-    cdbl s[kMaxNc];
-    for (int c = 0; c < nc; ++c) s[c] = mk(0, 0);
-    for (int j = lane; j < n; j += 64) {
-        const double fv = f[j];
-        if (fv != 0.0)
-            for (int c = 0; c < nc; ++c) axpy(s[c], fv, x[j + (size_t)c * n]);
-    }
-    for (int c = 0; c < nc; ++c) {
-        double re = s[c].x, im = s[c].y;
-        for (int o = 32; o > 0; o >>= 1) { re += __shfl_xor(re, o, 64); im += __shfl_xor(im, o, 64); }
-        if (lane == 0) Y[(size_t)b * n * nc + row + (size_t)c * n] = mk(re, im);
-    }
+__global__ __launch_bounds__(256) void k_csr_apply(const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                                                   const double* __restrict__ val, int n,
+                                                   const cdbl* __restrict__ X, cdbl* __restrict__ Y, long long ncols) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)n * ncols) return;
+    const int i = (int)(id % n);
+    const cdbl* x = X + (id - i);
+    cdbl s = mk(0, 0);
+    const int p1 = rowptr[i + 1];
+    for (int p = rowptr[i]; p < p1; ++p) axpy(s, val[p], x[colidx[p]]);
+    Y[id] = s;
 }
 
-void real_apply_thin_batched(const double* Frm, int n, int batch, const cdbl* X, cdbl* Y, int nc, hipStream_t st) {
-    hipLaunchKernelGGL(k_real_apply, dim3((n + 3) / 4, batch), dim3(256), 0, st, Frm, n, X, Y, nc);
+void csr_apply_batched(const int* rowptr, const int* colidx, const double* val, int n, const cdbl* X, cdbl* Y,
+                       long long ncols, hipStream_t st) {
+    const long long total = (long long)n * ncols;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_csr_apply, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, rowptr, colidx, val, n, X,
+                       Y, ncols);
+}
+
+// dst[b * count + i] = src[i] for every matrix b of the batch
+__global__ __launch_bounds__(256) void k_replicate(const cdbl* __restrict__ src, long long count, long long total,
+                                                   cdbl* __restrict__ dst) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < total) dst[id] = src[id % count];
+}
+
+void replicate_batched(const cdbl* src, long long count, int batch, cdbl* dst, hipStream_t st) {
+    const long long total = count * batch;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_replicate, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, src, count, total, dst);
 }
 
 // ---------------------------------------------------------------------------

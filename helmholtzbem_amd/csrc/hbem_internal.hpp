@@ -71,9 +71,12 @@ void transpose_batched(const cdbl* A, cdbl* AT, int n, int batch, hipStream_t st
 // *flag = 1 if any of cnt doubles is not finite / a diagonal entry of A is 0
 void flag_nonfinite(const double* x, long long cnt, int* flag, hipStream_t st);
 void flag_zero_diag(const cdbl* A, int n, int* flag, hipStream_t st);
-// Y = op(F) X for a real dense n x n F given row-major (so op = F) -- pass the
-// column-major array to apply F^T.
-void real_apply_thin_batched(const double* Frm, int n, int batch, const cdbl* X, cdbl* Y, int nc, hipStream_t st);
+// Y = S X for a real sparse n x n S in compressed-row form (rowptr[n + 1],
+// column indices ascending within a row); X, Y: n x ncols, column-major, ld n.
+void csr_apply_batched(const int* rowptr, const int* colidx, const double* val, int n, const cdbl* X, cdbl* Y,
+                       long long ncols, hipStream_t st);
+// dst = batch copies of the count entries at src, back to back
+void replicate_batched(const cdbl* src, long long count, int batch, cdbl* dst, hipStream_t st);
 void orthonormalize_thin_batched(cdbl* X, int n, int batch, int nc, hipStream_t st);
 // sigma = 1 / sigma_max(B); u = left singular vector of B for sigma_max,
 // y = right singular vector (nc) -- v = Q y is formed by the caller.
