@@ -488,7 +488,13 @@ constexpr int kStage = 256;  // nodes per staging round
 // record: 0 r, 1 r^2, 2 1/r, 3 (2/pi) ln r, 4 1/sqrt r, 5 weight of the curl sum,
 //         6 + 8h + {0,1} K direct, {2,3} K transposed, {4,5} W, {6,7} V coefficient
 //         of the entry q = m + 2h (m = 0, 1): quadrature weight x geometry factor x
-//         shape product, the per-node factors of node_accum for lane half h
+//         shape product, the per-node factors of node_accum for lane half h.
+// General pairs (tensor Gauss rule, node a * ns + b at s = x_a on panel i,
+// t = x_b on panel j): the test shape b_m(s) of K direct, W and V is constant
+// along a row a, so those records hold the coefficient without it --
+//         6 + 8h + 0 K direct, 1 W, 2 V (weight x geometry x b_h(t)),
+//         3 + {0,1} K transposed of the entry q = m + 2h as above --
+// and the kernel applies b_m(s) once per row (row_fold).
 template <int CLS>
 __device__ __forceinline__ void stage_node(const AsmParams& P, const double* __restrict__ geo, unsigned f, bool swap,
                                            int nd, double* __restrict__ rec) {
@@ -531,6 +537,18 @@ __device__ __forceinline__ void stage_node(const AsmParams& P, const double* __r
     rec[4] = o[8];
     rec[5] = CLS == 0 ? 2.0 * w : w;
     const double wcd = w * cd, wct = w * ct, wbt = w * bt, wtt = w * o[5];
+    if (CLS == 2) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            double* c = rec + 6 + 8 * h;
+            c[0] = wcd * bj[h];
+            c[1] = wbt * bj[h];
+            c[2] = wtt * bj[h];
+            c[3] = wct * (bj[0] * bi[h]);
+            c[4] = wct * (bj[1] * bi[h]);
+        }
+        return;
+    }
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -613,7 +631,9 @@ __device__ __forceinline__ void bessel_jy01_f(double x, double y, double L2, dou
     }
 }
 
-// one node's contribution to this lane's sums (entries q = m + 2h)
+// one node's contribution to this lane's sums (entries q = m + 2h).  General
+// pairs: K direct, W and V go to the running row-free sums U in slot m = 1
+// (see row_fold); K transposed to both entries as for the other classes.
 template <int CLS, int D>
 __device__ __forceinline__ void node_accum_x(AccX<D>& A, const double* __restrict__ R, int h, double j0, double y0,
                                              double j1, double y1) {
@@ -623,6 +643,25 @@ __device__ __forceinline__ void node_accum_x(AccX<D>& A, const double* __restric
     const cdbl hr = mk(j1 * r, y1 * r);
     const cdbl h0r2 = mk(j0 * r2, y0 * r2);
     const double* __restrict__ c = R + 6 + 8 * h;
+    if (CLS == 2) {
+        const double ukd = c[0], ub = c[1], ug = c[2];
+        axpy(A.kd[0][1], ukd, h1r);
+        axpy(A.wb[0][1], ub, h0);
+        axpy(A.v[0][1], ug, h0);
+        if (D >= 1) { axpy(A.kd[1][1], ukd, h0); axpy(A.wb[1][1], ub, hr); axpy(A.v[1][1], ug, hr); }
+        if (D >= 2) { axpy(A.kd[2][1], ukd, hr); axpy(A.wb[2][1], ub, h0r2); axpy(A.v[2][1], ug, h0r2); }
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const double ckt = c[3 + m];
+            axpy(A.kt[0][m], ckt, h1r);
+            if (D >= 1) axpy(A.kt[1][m], ckt, h0);
+            if (D >= 2) axpy(A.kt[2][m], ckt, hr);
+        }
+        axpy(A.wa[0], wa, h0);
+        if (D >= 1) axpy(A.wa[1], wa, hr);
+        if (D >= 2) axpy(A.wa[2], wa, h0r2);
+        return;
+    }
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
         const double ckd = c[m], ckt = c[2 + m], cb = c[4 + m], cg = c[6 + m];
@@ -646,6 +685,30 @@ __device__ __forceinline__ void node_accum_x(AccX<D>& A, const double* __restric
     axpy(A.wa[0], wa, h0);
     if (D >= 1) axpy(A.wa[1], wa, hr);
     if (D >= 2) axpy(A.wa[2], wa, h0r2);
+}
+
+// General pairs, end of row a of the tensor rule.  With U_a the running sum of
+// a term after row a and T_a = U_a - U_{a-1} the row's own sum, the test shape
+// b_0(s) = s gives sum_a s_a T_a = sum_a (s_a - s_{a+1}) U_a (s_{ns} = 0): one
+// update per row in slot m = 0, where the other classes pay one per node.
+// b_1 = 1 - s is then U - V (row_finish).
+template <int D>
+__device__ __forceinline__ void row_fold(AccX<D>& A, double delta) {
+#pragma unroll
+    for (int t = 0; t <= D; ++t) {
+        axpy(A.kd[t][0], delta, A.kd[t][1]);
+        axpy(A.wb[t][0], delta, A.wb[t][1]);
+        axpy(A.v[t][0], delta, A.v[t][1]);
+    }
+}
+template <int D>
+__device__ __forceinline__ void row_finish(AccX<D>& A) {
+#pragma unroll
+    for (int t = 0; t <= D; ++t) {
+        A.kd[t][1] = A.kd[t][1] - A.kd[t][0];
+        A.wb[t][1] = A.wb[t][1] - A.wb[t][0];
+        A.v[t][1] = A.v[t][1] - A.v[t][0];
+    }
 }
 
 // cls: 0 coinciding, 1 adjacent, 2 general; blockDim 256.  Three waves per
@@ -706,10 +769,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 halves(j1, c0, c1);
                 halves(y1, d0, d1);
                 node_accum_x<CLS, D>(A, R0, h, a0, b0, c0, d0);
+                if (CLS == 2) {
+                    const int nd = base + p, a = nd / P.ns;
+                    if (nd - a * P.ns == P.ns - 1) row_fold(A, P.xg[a] - (a + 1 < P.ns ? P.xg[a + 1] : 0.0));
+                }
                 node_accum_x<CLS, D>(A, R0 + kRec, h, a1, b1, c1, d1);
+                if (CLS == 2) {
+                    const int nd = base + p + 1, a = nd / P.ns;
+                    if (nd < hi && nd - a * P.ns == P.ns - 1)
+                        row_fold(A, P.xg[a] - (a + 1 < P.ns ? P.xg[a + 1] : 0.0));
+                }
             }
         }
     }
+    if (CLS == 2) row_finish(A);
 
     // Sums -> this lane's entries q = m + 2h of the reference's local matrices
     // (same scalings as acc_store)

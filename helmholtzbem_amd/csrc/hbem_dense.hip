@@ -234,10 +234,25 @@ __global__ void k_lu_perm(int n, const int* __restrict__ ipiv, int* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Tiled complex GEMM: C = alpha A op(B) + beta C
+// Complex GEMM on the matrix cores: C = alpha A op(B) + beta C
 // ---------------------------------------------------------------------------
+// v_mfma_f64_16x16x4f64 multiplies a 16 x 4 by a 4 x 16 FP64 block: lane l
+// supplies row (l & 15), column (l >> 4) of the first operand and row (l >> 4),
+// column (l & 15) of the second, and holds D(4 v + (l >> 4), l & 15) in element
+// v of the accumulator.  The complex product takes three real ones (the 3M
+// form): P1 = Re A Re B, P2 = Im A Im B, P3 = (Re A + Im A)(Re B + Im B),
+// accumulated over k, then Re C = P1 - P2, Im C = P3 - P1 - P2 -- normwise as
+// accurate as the four-product form, with three independent accumulators per
+// block.  The product is formed transposed -- first operand B^T, second A^T --
+// so element v of lane l is C(r0 + (l & 15), c0 + 4 v + (l >> 4)): sixteen
+// consecutive rows of one column per lane group, for coalesced stores.
+// Workgroup = 64 x 64 tile of C as four wavefronts of 32 x 32 (2 x 2 blocks of
+// 16 x 16); k advances in steps of 16 through LDS while the next step's tile
+// is fetched into registers.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
 template <bool TB>
-__global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cdbl alpha, const cdbl* __restrict__ A, int lda,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_zgemm(int m, int n, int k, cdbl alpha, const cdbl* __restrict__ A, int lda,
                                                long long sa, const cdbl* __restrict__ B, int ldb, long long sb,
                                                cdbl beta, cdbl* __restrict__ C, int ldc, long long sc) {
     A += blockIdx.z * sa;
@@ -245,53 +260,83 @@ __global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cdbl alpha, 
     C += blockIdx.z * sc;
     __shared__ cdbl As[16][65];
     __shared__ cdbl Bs[16][65];
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int l16 = lane & 15, lk = lane >> 4;
+    const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
     const int row0 = blockIdx.x * 64, col0 = blockIdx.y * 64;
-    cdbl acc[4][4];
+    dbl4 p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = mk(0, 0);
-    for (int kb = 0; kb < k; kb += 16) {
-        for (int e = tid; e < 1024; e += 256) {
-            const int r = e & 63, kk = e >> 6;
-            As[kk][r] = (row0 + r < m && kb + kk < k) ? A[(row0 + r) + (size_t)(kb + kk) * lda] : mk(0, 0);
+        for (int j = 0; j < 2; ++j) {
+            p1[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+            p2[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+            p3[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
         }
-        for (int e = tid; e < 1024; e += 256) {
-            int kk, cc;
-            if (TB) { cc = e & 63; kk = e >> 6; } else { kk = e & 15; cc = e >> 4; }
+    cdbl pa[4], pb[4];
+    auto fetch = [&](int kb) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u, r = e & 63, kk = e >> 6;
+            pa[u] = (row0 + r < m && kb + kk < k) ? A[(row0 + r) + (size_t)(kb + kk) * lda] : mk(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            const int kk = TB ? (e >> 6) : (e & 15), cc = TB ? (e & 63) : (e >> 4);
             cdbl v = mk(0, 0);
             if (col0 + cc < n && kb + kk < k)
                 v = TB ? B[(col0 + cc) + (size_t)(kb + kk) * ldb] : B[(kb + kk) + (size_t)(col0 + cc) * ldb];
-            Bs[kk][cc] = v;
+            pb[u] = v;
+        }
+    };
+    fetch(0);
+    for (int kb = 0; kb < k; kb += 16) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            As[e >> 6][e & 63] = pa[u];
+            const int kk = TB ? (e >> 6) : (e & 15), cc = TB ? (e & 63) : (e >> 4);
+            Bs[kk][cc] = pb[u];
         }
         __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < 16; ++kk) {
-            cdbl av[4], bv[4];
+        if (kb + 16 < k) fetch(kb + 16);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) av[i] = As[kk][tx + 16 * i];
+        for (int k4 = 0; k4 < 16; k4 += 4) {
+            cdbl a[2], b[2];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][ty + 16 * j];
+            for (int i = 0; i < 2; ++i) a[i] = As[k4 + lk][wr + 16 * i + l16];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 2; ++j) b[j] = Bs[k4 + lk][wc + 16 * j + l16];
+            const double as[2] = {a[0].x + a[0].y, a[1].x + a[1].y};
+            const double bs[2] = {b[0].x + b[0].y, b[1].x + b[1].y};
 #pragma unroll
-                for (int j = 0; j < 4; ++j) cfma(acc[i][j], av[i], bv[j]);
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j].x, a[i].x, p1[i][j], 0, 0, 0);
+                    p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j].y, a[i].y, p2[i][j], 0, 0, 0);
+                    p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bs[j], as[i], p3[i][j], 0, 0, 0);
+                }
         }
         __syncthreads();
     }
+    const bool rd = beta.x != 0.0 || beta.y != 0.0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int r = row0 + tx + 16 * i, c = col0 + ty + 16 * j;
-            if (r < m && c < n) {
-                cdbl* p = C + r + (size_t)c * ldc;
-                cdbl v = alpha * acc[i][j];
-                if (beta.x != 0.0 || beta.y != 0.0) v = v + beta * (*p);
-                *p = v;
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = row0 + wr + 16 * i + l16, c = col0 + wc + 16 * j + 4 * v + lk;
+                if (r < m && c < n) {
+                    cdbl* p = C + r + (size_t)c * ldc;
+                    const double q1 = p1[i][j][v], q2 = p2[i][j][v];
+                    cdbl x = alpha * mk(q1 - q2, p3[i][j][v] - q1 - q2);
+                    if (rd) x = x + beta * (*p);
+                    *p = x;
+                }
             }
-        }
 }
 
 void zgemm_batched(int m, int n, int k, cdbl alpha, const cdbl* A, int lda, long long sa, const cdbl* B, int ldb,
