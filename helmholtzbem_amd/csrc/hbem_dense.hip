@@ -110,6 +110,147 @@ __global__ __launch_bounds__(1024) void k_lu_panel(cdbl* __restrict__ A, int n, 
     }
 }
 
+// The same panel factorisation with the rows held in registers, eight panel
+// columns (a chunk) at a time, for panels of at most 1024 rows: thread t owns
+// row j0 + t.  Inside a chunk every column step -- pivot search, exchange of
+// the two rows' chunk values through LDS, scaling, rank-1 update -- works on
+// registers; the interchange of the other panel columns goes to memory, one
+// thread per column.  After the chunk, its rows are stored, U12 = L11^{-1} A12
+// is solved for the panel columns right of it (one thread per column) and
+// those columns receive A22 -= L21 U12 in one pass.  The panel is read and
+// written about three times instead of once per column step.
+constexpr int kPanelChunk = 8;
+
+__global__ __launch_bounds__(1024) void k_lu_panel_rg(cdbl* __restrict__ A, int n, int j0, int jb,
+                                                      int* __restrict__ ipiv) {
+    cdbl* a = A + (size_t)blockIdx.x * n * n;
+    int* ip = ipiv + (size_t)blockIdx.x * n;
+    __shared__ double sv[16];
+    __shared__ int si[16];
+    __shared__ int spiv;
+    __shared__ cdbl prow[kPanelChunk];          // pivot row (chunk columns)
+    __shared__ cdbl xrow[kPanelChunk];          // the row it displaces
+    __shared__ cdbl L11[kPanelChunk][kPanelChunk];
+    __shared__ cdbl U12[kPanelChunk][32];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int row = j0 + tid;
+    for (int c0 = 0; c0 < jb; c0 += kPanelChunk) {
+        const int w = min(kPanelChunk, jb - c0);
+        const int cb = j0 + c0;                  // first column of the chunk = its first diagonal row
+        const bool own = row < n && row >= cb;
+        cdbl v[kPanelChunk];
+#pragma unroll
+        for (int u = 0; u < kPanelChunk; ++u) v[u] = (own && u < w) ? a[row + (size_t)(cb + u) * n] : mk(0, 0);
+#pragma unroll
+        for (int c = 0; c < kPanelChunk; ++c) {
+            if (c >= w) break;
+            const int col = cb + c;
+            double best = -1.0;
+            int bi = n;
+            if (own && row >= col) {
+                const double x = cabs2(v[c]);
+                if (x > best) { best = x; bi = row; }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const double v2 = __shfl_xor(best, o, 64);
+                const int i2 = __shfl_xor(bi, o, 64);
+                if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
+            }
+            if (lane == 0) { sv[wv] = best; si[wv] = bi; }
+            __syncthreads();
+            if (tid < 64) {
+                best = lane < 16 ? sv[lane] : -1.0;
+                bi = lane < 16 ? si[lane] : n;
+#pragma unroll
+                for (int o = 8; o > 0; o >>= 1) {
+                    const double v2 = __shfl_xor(best, o, 64);
+                    const int i2 = __shfl_xor(bi, o, 64);
+                    if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
+                }
+                // no candidate (a column of NaNs): keep the row, never index past n
+                if (tid == 0) { const int pv = bi < n ? bi : col; spiv = pv; ip[col] = pv; }
+            }
+            __syncthreads();
+            const int piv = spiv;
+            if (row == piv) {
+#pragma unroll
+                for (int u = 0; u < kPanelChunk; ++u) prow[u] = v[u];
+            }
+            if (row == col) {
+#pragma unroll
+                for (int u = 0; u < kPanelChunk; ++u) xrow[u] = v[u];
+            }
+            if (piv != col && tid < jb && (tid < c0 || tid >= c0 + w)) {
+                cdbl* pc = a + (size_t)(j0 + tid) * n;
+                const cdbl t = pc[piv];
+                pc[piv] = pc[col];
+                pc[col] = t;
+            }
+            __syncthreads();
+            if (piv != col) {
+                if (row == piv) {
+#pragma unroll
+                    for (int u = 0; u < kPanelChunk; ++u) v[u] = xrow[u];
+                } else if (row == col) {
+#pragma unroll
+                    for (int u = 0; u < kPanelChunk; ++u) v[u] = prow[u];
+                }
+            }
+            const cdbl d = prow[c];
+            const cdbl inv = (d.x == 0.0 && d.y == 0.0) ? mk(0, 0) : crecip(d);
+            if (own && row > col) {
+                const cdbl l = v[c] * inv;
+                v[c] = l;
+#pragma unroll
+                for (int u = c + 1; u < kPanelChunk; ++u)
+                    if (u < w) cfms(v[u], l, prow[u]);
+            }
+        }
+        // chunk rows back to memory; L11 of the chunk for the U12 solve
+        if (own) {
+#pragma unroll
+            for (int u = 0; u < kPanelChunk; ++u)
+                if (u < w) a[row + (size_t)(cb + u) * n] = v[u];
+            const int r = row - cb;
+            if (r < w) {
+#pragma unroll
+                for (int u = 0; u < kPanelChunk; ++u) L11[r][u] = v[u];
+            }
+        }
+        const int rem = jb - c0 - w;             // panel columns right of the chunk
+        if (rem <= 0) break;
+        __syncthreads();
+        if (tid < rem) {
+            cdbl* pc = a + (size_t)(cb + w + tid) * n + cb;
+            cdbl x[kPanelChunk];
+#pragma unroll
+            for (int r = 0; r < kPanelChunk; ++r) {
+                if (r < w) {
+                    x[r] = pc[r];
+#pragma unroll
+                    for (int u = 0; u < kPanelChunk; ++u)
+                        if (u < r) cfms(x[r], L11[r][u], x[u]);
+                    pc[r] = x[r];
+                    U12[r][tid] = x[r];
+                }
+            }
+        }
+        __syncthreads();
+        if (row < n && row >= cb + w) {
+            for (int cc = 0; cc < rem; ++cc) {
+                cdbl* p = a + row + (size_t)(cb + w + cc) * n;
+                cdbl y = *p;
+#pragma unroll
+                for (int u = 0; u < kPanelChunk; ++u)
+                    if (u < w) cfms(y, v[u], U12[u][cc]);
+                *p = y;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Row interchanges of the panel [j0, j0 + jb) (jb <= 32) and the U12 solve.
 // MODE bit 0: apply the panel's interchanges to every column outside the panel;
 // MODE bit 1: for the columns right of the panel, U12 <- L11^{-1} A12 (L11 unit lower).
@@ -372,9 +513,13 @@ void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStrea
         else if (mode == 1) hipLaunchKernelGGL(k_lu_rows<1>, g, t, 0, st, A, n, j0, jb, ipiv);
         else hipLaunchKernelGGL(k_lu_rows<2>, g, t, 0, st, A, n, j0, jb, ipiv);
     };
+    auto panel = [&](int j, int jb) {
+        if (n - j <= 1024) hipLaunchKernelGGL(k_lu_panel_rg, dim3(batch), dim3(1024), 0, st, A, n, j, jb, ipiv);
+        else hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(1024), 0, st, A, n, j, jb, ipiv);
+    };
     for (int j0 = 0; j0 < n; j0 += 2 * sb) {
         const int jb1 = n - j0 < sb ? n - j0 : sb;
-        hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(1024), 0, st, A, n, j0, jb1, ipiv);
+        panel(j0, jb1);
         rows(3, j0, jb1);
         const int j1 = j0 + jb1;
         const int jb2 = n - j1 < sb ? n - j1 : sb;
@@ -382,7 +527,7 @@ void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStrea
         if (jb2 > 0) {
             zgemm_batched(n - j1, jb2, jb1, mk(-1, 0), at(j1, j0), n, s, at(j0, j1), n, s, false, mk(1, 0), at(j1, j1),
                           n, s, batch, st);
-            hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(1024), 0, st, A, n, j1, jb2, ipiv);
+            panel(j1, jb2);
             rows(1, j1, jb2);
             if (n - j2 > 0) {
                 zgemm_batched(jb2, n - j2, jb1, mk(-1, 0), at(j1, j0), n, s, at(j0, j2), n, s, false, mk(1, 0),
