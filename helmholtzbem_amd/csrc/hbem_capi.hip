@@ -100,7 +100,7 @@ struct hbem_geometry {
     DCsr s_F, s_FT, s_FinvT;   // F, F^T and Finv^T of the projection
     // work space
     DBuf w_Ao, w_U, w_rhs, w_flag;
-    DBuf w_kapi, w_kri, w_LUT, w_L, w_T0, w_T1, w_T2, w_piv, w_perm, w_X, w_Y, w_work, w_R, w_u, w_v, w_a, w_b, w_sig, w_dsig, w_kap,
+    DBuf w_kapi, w_kri, w_L, w_T0, w_T1, w_T2, w_piv, w_perm, w_X, w_Y, w_work, w_R, w_u, w_v, w_a, w_b, w_sig, w_dsig, w_kap,
         w_kr, w_cv, w_qi, w_qo, w_tmp, w_out, w_part;
     // profiling
     bool prof = false;
@@ -233,7 +233,7 @@ struct DenseCtx {
     hipStream_t st;
     int n, batch, nc, q;
     const cdbl* LU;
-    const cdbl* LUT;     // transposed factors (adjoint solves)
+    long long ms;        // matrix stride of LU
     const int* perm;
     const DCsr* F;       // projection factor, null = no projection
     const DCsr* FT;      // its transpose
@@ -249,12 +249,12 @@ void tp_solve(DenseCtx& c, const cdbl* X, cdbl* Y, bool adj) {
         cdbl* Z = c.work + (size_t)c.n * cols;
         hbem::csr_apply_batched((const int*)c.F->rp.p, (const int*)c.F->ci.p, (const double*)c.F->v.p, c.n, X, Z, cols,
                                 c.st);
-        hbem::lu_solve_thin_batched(c.LU, c.LUT, c.perm, c.n, c.batch, Z, c.nc, adj, c.work, c.st);
+        hbem::lu_solve_thin_batched(c.LU, c.ms, c.perm, c.n, c.batch, Z, c.nc, adj, c.work, c.st);
         hbem::csr_apply_batched((const int*)c.FT->rp.p, (const int*)c.FT->ci.p, (const double*)c.FT->v.p, c.n, Z, Y,
                                 cols, c.st);
     } else {
         HIPC(hipMemcpyAsync(Y, X, sizeof(cdbl) * (size_t)c.n * cols, hipMemcpyDeviceToDevice, c.st));
-        hbem::lu_solve_thin_batched(c.LU, c.LUT, c.perm, c.n, c.batch, Y, c.nc, adj, c.work, c.st);
+        hbem::lu_solve_thin_batched(c.LU, c.ms, c.perm, c.n, c.batch, Y, c.nc, adj, c.work, c.st);
     }
 }
 
@@ -671,7 +671,7 @@ int hbem_tp_solve(hbem_geometry_t g, double k_re, double k_im, double c_o, doubl
     cdbl* work = g->w_work.as<cdbl>((size_t)n * 8);
     for (int j = 0; j < nrhs; j += 8) {
         const int nc = std::min(8, nrhs - j);
-        hbem::lu_solve_thin_batched(T, nullptr, perm, n, 1, r + (size_t)j * n, nc, false, work, g->st);
+        hbem::lu_solve_thin_batched(T, (long long)n * n, perm, n, 1, r + (size_t)j * n, nc, false, work, g->st);
     }
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(hf + 2, flag + 2, sizeof(int), hipMemcpyDeviceToHost, g->st));
@@ -690,7 +690,7 @@ namespace {
 struct Standalone {
     std::mutex mu;
     hipStream_t st = nullptr;
-    DBuf T, T1, T2, piv, perm, X, Y, work, u, v, sig, out, W, Wpiv, Wperm, p, p2, ud, tmp, LUT;
+    DBuf T, T1, T2, piv, perm, X, Y, work, u, v, sig, out, W, Wpiv, Wperm, p, p2, ud, tmp;
     void init() {
         if (!st) {
             HIPC(hipSetDevice(g_device));
@@ -717,10 +717,8 @@ double standalone_sv(const void* T, int n, const void* R, int nc, int q, bool wa
     int* piv = s.piv.as<int>(n);
     int* perm = s.perm.as<int>(n);
     hbem::lu_factor_batched(dT, n, 1, piv, perm, s.st);
-    cdbl* dLUT = s.LUT.as<cdbl>(nn);
-    hbem::transpose_batched(dT, dLUT, n, 1, s.st);
     DenseCtx c;
-    c.st = s.st; c.n = n; c.batch = 1; c.nc = nc; c.q = q; c.LU = dT; c.LUT = dLUT; c.perm = perm;
+    c.st = s.st; c.n = n; c.batch = 1; c.nc = nc; c.q = q; c.LU = dT; c.ms = (long long)n * n; c.perm = perm;
     c.F = nullptr; c.FT = nullptr;
     c.X = s.X.as<cdbl>((size_t)n * nc);
     c.Y = s.Y.as<cdbl>((size_t)n * nc);
@@ -760,8 +758,8 @@ int hbem_rsvd_sv_der(const void* T, const void* Td, int n, const void* R, int nc
     const size_t nn = (size_t)n * n;
     cdbl* dT1 = s.T1.as<cdbl>(nn);
     HIPC(hipMemcpyAsync(dT1, Td, nn * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
-    double* dout = s.out.as<double>(1);
-    hbem::bilinear_re_batched(dT1, (long long)nn, (const cdbl*)s.u.p, (const cdbl*)s.v.p, n, 1, dout, s.st);
+    double* dout = s.out.as<double>(1 + hbem::kBilinearParts);
+    hbem::bilinear_re_batched(dT1, (long long)nn, (const cdbl*)s.u.p, (const cdbl*)s.v.p, n, 1, dout, dout + 1, s.st);
     HIPC(hipGetLastError());
     double der = 0.0;
     HIPC(hipMemcpyAsync(&der, dout, sizeof(double), hipMemcpyDeviceToHost, s.st));
@@ -788,8 +786,8 @@ int hbem_rsvd_sv_der2(const void* T, const void* Td, const void* Tdd, int n, con
     cdbl* dT2 = s.T2.as<cdbl>(nn);
     HIPC(hipMemcpyAsync(dT1, Td, nn * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
     HIPC(hipMemcpyAsync(dT2, Tdd, nn * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
-    double* dout = s.out.as<double>(1);
-    hbem::bilinear_re_batched(dT1, (long long)nn, (const cdbl*)s.u.p, (const cdbl*)s.v.p, n, 1, dout, s.st);
+    double* dout = s.out.as<double>(1 + hbem::kBilinearParts);
+    hbem::bilinear_re_batched(dT1, (long long)nn, (const cdbl*)s.u.p, (const cdbl*)s.v.p, n, 1, dout, dout + 1, s.st);
     double der = 0.0;
     HIPC(hipMemcpyAsync(&der, dout, sizeof(double), hipMemcpyDeviceToHost, s.st));
     // x = [u; v] / sqrt2, normalised by its largest-modulus entry
@@ -826,7 +824,7 @@ int hbem_rsvd_sv_der2(const void* T, const void* Td, const void* Tdd, int n, con
     hbem::gemv_batched(dT1, 0, du + n, p, n, 1, false, s.st);
     hbem::gemv_batched(dT1, 0, du, p + n, n, 1, true, s.st);
     cdbl* wk = s.work.as<cdbl>(2 * (size_t)m2);
-    hbem::lu_solve_thin_batched(W, nullptr, wperm, m2, 1, p, 1, false, wk, s.st);   // p <- W^{-1} p
+    hbem::lu_solve_thin_batched(W, (long long)m2 * m2, wperm, m2, 1, p, 1, false, wk, s.st);   // p <- W^{-1} p
     std::vector<cdbl> ud(m2);
     HIPC(hipMemcpyAsync(ud.data(), p, m2 * sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
     HIPC(hipStreamSynchronize(s.st));
@@ -852,7 +850,7 @@ int hbem_rsvd_sv_der2(const void* T, const void* Td, const void* Tdd, int n, con
         rhs[l] = hbem::mk(v.real(), v.imag());
     }
     HIPC(hipMemcpyAsync(p, rhs.data(), m2 * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
-    hbem::lu_solve_thin_batched(W, nullptr, wperm, m2, 1, p, 1, false, wk, s.st);
+    hbem::lu_solve_thin_batched(W, (long long)m2 * m2, wperm, m2, 1, p, 1, false, wk, s.st);
     cdbl sol;
     HIPC(hipMemcpyAsync(&sol, p + m, sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
     HIPC(hipStreamSynchronize(s.st));
@@ -885,7 +883,7 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
     int B = 256;
     const size_t nslot = 16 * (size_t)(D + 1);
     const size_t maxchunks = (size_t)g->nadj * ((2 * (size_t)g->nc * g->nc + 511) / 512);
-    const size_t per = nn * sizeof(cdbl) * (D + 2) +   // T, T', T'' and the transposed factors
+    const size_t per = nn * sizeof(cdbl) * (D + 1) +   // T (factorised in place), T', T''
                        (size_t)2 * ((size_t)g->npairs + maxchunks) * nslot * sizeof(cdbl);
     while (B > 1 && (size_t)B * per > ((size_t)64 << 30)) B /= 2;
     cdbl* dR = g->w_R.as<cdbl>((size_t)n * nc);
@@ -927,10 +925,8 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
         }
-        cdbl* LUT = g->w_LUT.as<cdbl>(nn * nb);
-        hbem::transpose_batched(T0, LUT, n, nb, g->st);
         DenseCtx c;
-        c.st = g->st; c.n = n; c.batch = nb; c.nc = nc; c.q = q; c.LU = T0; c.LUT = LUT; c.perm = perm;
+        c.st = g->st; c.n = n; c.batch = nb; c.nc = nc; c.q = q; c.LU = T0; c.ms = (long long)nn; c.perm = perm;
         c.F = g->proj ? &g->s_F : nullptr;
         c.FT = g->proj ? &g->s_FT : nullptr;
         c.X = g->w_X.as<cdbl>((size_t)n * nc * nb);
@@ -953,8 +949,8 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
                 HIPC(hipMemcpyAsync(a, du, (size_t)n * nb * sizeof(cdbl), hipMemcpyDeviceToDevice, g->st));
                 HIPC(hipMemcpyAsync(bb, dv, (size_t)n * nb * sizeof(cdbl), hipMemcpyDeviceToDevice, g->st));
             }
-            double* dd = g->w_dsig.as<double>(nb);
-            hbem::bilinear_re_batched(T1, (long long)nn, a, bb, n, nb, dd, g->st);
+            double* dd = g->w_dsig.as<double>((size_t)nb * (1 + hbem::kBilinearParts));
+            hbem::bilinear_re_batched(T1, (long long)nn, a, bb, n, nb, dd, dd + nb, g->st);
             HIPC(hipMemcpyAsync(dsigma + k0, dd, nb * sizeof(double), hipMemcpyDeviceToHost, g->st));
         }
         HIPC(hipGetLastError());

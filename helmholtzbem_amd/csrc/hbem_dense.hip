@@ -545,119 +545,239 @@ void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStrea
 // ---------------------------------------------------------------------------
 // Thin triangular solves
 // ---------------------------------------------------------------------------
-// Triangular operand of a pass, always read column-major and coalesced:
-//   kind 0: L  (unit lower)    L(i,j) = LU(i,j)
-//   kind 1: U  (upper)         U(i,j) = LU(i,j)
-//   kind 2: U^H (lower)        U^H(i,j) = conj(LU(j,i)) = conj(LUT(i,j))
-//   kind 3: L^H (unit upper)   L^H(i,j) = conj(LUT(i,j))
-// with LUT the transposed copy of the factors made after the factorisation.
-__device__ __forceinline__ cdbl op_el(const cdbl* M, int n, int kind, int i, int j) {
-    const cdbl v = M[i + (size_t)j * n];
-    return kind <= 1 ? v : cconj(v);
-}
-
+// A = P^T L U with L (unit lower) and U in one column-major array LU.  All four
+// substitutions read LU in place, always down its columns:
+//   A x = b:    L y = P b forward, U x = y backward -- right-looking: once the
+//               block's unknowns are known, columns of L (below) / U (above)
+//               update the other rows;
+//   A^H x = b:  U^H y = b forward, L^H z = y backward, x = P^T z -- left-
+//               looking: (U^H y)_i = sum_j conj(U(j, i)) y_j runs down column
+//               i of U above the diagonal (of L below it for L^H), so each
+//               block first gathers its rows' sums from the finished unknowns.
+// Blocks of 64 rows; the diagonal block is staged in LDS (as the operator of
+// the pass) and solved by wavefront 0 with cross-lane broadcasts, while the
+// other wavefronts stage the next block.
 constexpr int kMaxNc = 8;
 constexpr int kSolveThreads = 1024;
 constexpr int kTS = 65;   // padded row stride of the 64 x 64 LDS diagonal block
 
 struct SolveLds {
-    cdbl tile[64 * kTS];
-    cdbl xb[64 * kMaxNc];
+    cdbl tile[2][64 * kTS];   // the current diagonal block and the next one
+    cdbl xb[64 * kMaxNc];     // solution of the last block
 };
 
-// Blocked substitution in 64-row blocks [64 b, 64 b + bs): the diagonal block
-// is staged in LDS and solved by wavefront 0 with cross-lane broadcasts; the
-// rows outside the block are then updated with one thread per row (loads of
-// the operand's column j coalesce across threads).
+// Stage the diagonal block [r0, r0 + bs)^2 of the pass operator with threads
+// t0, t0 + stride, ...: conj = false: op(i, j) = LU(i, j); conj = true:
+// op(i, j) = conj(LU(j, i)).
+__device__ __forceinline__ void stage_diag(const cdbl* M, int n, int r0, int bs, bool conj, cdbl* tile, int t0,
+                                           int stride) {
+    for (int e = t0; e < 64 * 64; e += stride) {
+        const int p = e & 63, q = e >> 6;   // p runs down a column of LU
+        const bool in = p < bs && q < bs;
+        const cdbl v = in ? M[(r0 + p) + (size_t)(r0 + q) * n] : mk(0, 0);
+        if (conj) tile[q * kTS + p] = cconj(v);
+        else tile[p * kTS + q] = v;
+    }
+}
+
+// Wavefront 0: solve the staged block for w[r0, r0 + bs) in place and leave
+// the solution in xb (zero beyond bs).  The reciprocal pivots are formed by
+// all lanes at once and the tile entries are read eight steps ahead, so a
+// step's dependence chain is one broadcast and one complex multiply-add.
 template <int NC>
-__device__ void subst(const cdbl* M, int n, int kind, bool lower, bool unit, cdbl* w, SolveLds& S) {
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int nblk = (n + 63) / 64;
-    for (int s = 0; s < nblk; ++s) {
-        const int blk = lower ? s : nblk - 1 - s;
-        const int r0 = 64 * blk, bs = min(64, n - r0);
-        for (int e = tid; e < 64 * 64; e += blockDim.x) {
-            const int i = e & 63, j = e >> 6;
-            S.tile[i * kTS + j] = (i < bs && j < bs) ? op_el(M, n, kind, r0 + i, r0 + j) : mk(0, 0);
+__device__ __forceinline__ void diag_solve(int n, int r0, int bs, bool lower, bool unit, cdbl* w,
+                                           const cdbl* __restrict__ tile, cdbl* xb) {
+    const int lane = threadIdx.x & 63;
+    cdbl x[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) x[c] = lane < bs ? w[r0 + lane + (size_t)c * n] : mk(0, 0);
+    const cdbl dv = tile[lane * kTS + lane];
+    const cdbl dinv = (unit || lane >= bs) ? mk(1, 0) : crecip(dv);
+    for (int s0 = 0; s0 < bs; s0 += 8) {
+        cdbl lv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int step = min(s0 + u, bs - 1);
+            const int j = lower ? step : bs - 1 - step;
+            lv[u] = tile[lane * kTS + j];
         }
-        __syncthreads();
-        if (tid < 64) {
-            cdbl x[NC];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) x[c] = lane < bs ? w[r0 + lane + (size_t)c * n] : mk(0, 0);
-            for (int step = 0; step < bs; ++step) {
-                const int j = lower ? step : bs - 1 - step;
-                if (!unit && lane == j) {
-                    const cdbl inv = crecip(S.tile[j * kTS + j]);
+        for (int u = 0; u < 8; ++u) {
+            const int step = s0 + u;
+            if (step >= bs) break;
+            const int j = lower ? step : bs - 1 - step;
+            if (!unit && lane == j) {
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) x[c] = x[c] * inv;
-                }
-                const bool below = lower ? (lane > j && lane < bs) : (lane < j);
-                const cdbl l = below ? S.tile[lane * kTS + j] : mk(0, 0);
-#pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    const cdbl xj = shfl_c(x[c], j);
-                    if (below) cfms(x[c], l, xj);
-                }
+                for (int c = 0; c < NC; ++c) x[c] = x[c] * dinv;
             }
+            const bool other = lower ? (lane > j && lane < bs) : (lane < j);
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                if (lane < bs) w[r0 + lane + (size_t)c * n] = x[c];
-                S.xb[lane * kMaxNc + c] = lane < bs ? x[c] : mk(0, 0);
+                const cdbl xj = shfl_c(x[c], j);
+                if (other) cfms(x[c], lv[u], xj);
             }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        if (lane < bs) w[r0 + lane + (size_t)c * n] = x[c];
+        xb[lane * kMaxNc + c] = lane < bs ? x[c] : mk(0, 0);
+    }
+}
+
+// Right-looking pass with the column-major factor: L (lower, unit) forward or
+// U (upper) backward.  After block b, rows outside it (below / above) take
+// w_i -= sum_j M(i, r0 + j) x_j: a wavefront covers 16 rows x 4 groups of 16
+// columns (lane = row + 16 group), so every thread keeps 16 loads in flight
+// (8 for more than two right-hand sides, within the register budget).
+template <int NC>
+__device__ __forceinline__ void subst_right(const cdbl* M, int n, bool lower, cdbl* w, SolveLds& S) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
+    const int l16 = lane & 15, grp = lane >> 4;
+    const int nblk = (n + 63) / 64;
+    auto rb = [&](int s) { return 64 * (lower ? s : nblk - 1 - s); };   // first row of the s-th block solved
+    auto bsz = [&](int s) { return min(64, n - rb(s)); };
+    stage_diag(M, n, rb(0), bsz(0), false, S.tile[0], tid, blockDim.x);
+    for (int s = 0; s < nblk; ++s) {
+        const int r0 = rb(s), bs = bsz(s);
+        __syncthreads();                          // block s staged, block s - 1's update done
+        if (wv == 0) {
+            diag_solve<NC>(n, r0, bs, lower, lower, w, S.tile[s & 1], S.xb);
+        } else if (s + 1 < nblk) {
+            stage_diag(M, n, rb(s + 1), bsz(s + 1), false, S.tile[(s + 1) & 1], tid - 64, blockDim.x - 64);
         }
         __syncthreads();
         const int ia = lower ? r0 + bs : 0, ib = lower ? n : r0;
-        for (int i = ia + tid; i < ib; i += blockDim.x) {
+        for (int i0 = ia + 16 * wv; i0 < ib; i0 += 16 * nwv) {
+            const int i = i0 + l16;
             cdbl acc[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) acc[c] = mk(0, 0);
-#pragma unroll 8
-            for (int j = 0; j < bs; ++j) {
-                const cdbl t = op_el(M, n, kind, i, r0 + j);
+            if (i < ib) {
+#pragma unroll(NC <= 2 ? 16 : 8)
+                for (int u = 0; u < 16; ++u) {
+                    const int j = 16 * grp + u;
+                    if (j < bs) {
+                        const cdbl t = M[i + (size_t)(r0 + j) * n];
 #pragma unroll
-                for (int c = 0; c < NC; ++c) cfma(acc[c], t, S.xb[j * kMaxNc + c]);
+                        for (int c = 0; c < NC; ++c) cfma(acc[c], t, S.xb[j * kMaxNc + c]);
+                    }
+                }
             }
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                cdbl& y = w[i + (size_t)c * n];
-                y = y - acc[c];
+                acc[c].x += __shfl_xor(acc[c].x, 16, 64); acc[c].y += __shfl_xor(acc[c].y, 16, 64);
+                acc[c].x += __shfl_xor(acc[c].x, 32, 64); acc[c].y += __shfl_xor(acc[c].y, 32, 64);
+            }
+            if (grp == 0 && i < ib) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    cdbl& y = w[i + (size_t)c * n];
+                    y = y - acc[c];
+                }
             }
         }
-        // the next round's diagonal solve reads w after a barrier (tile staging)
     }
     __syncthreads();
 }
 
+// Left-looking pass with the conjugate transpose of the factor: U^H (lower)
+// forward or L^H (upper, unit) backward.  Block b first takes
+// w_i -= sum_j conj(M(j, i)) w_j over the finished unknowns j (above the block
+// for U^H, below it for L^H): wavefront v sums columns i = r0 + 4 v .. + 3
+// (CP of them per sweep down the column, CP = 4 / 2 / 1 for up to 2 / 4 / 8
+// right-hand sides) with its lanes down the column, then the diagonal block is
+// solved.
+template <int NC>
+__device__ __forceinline__ void subst_left(const cdbl* M, int n, bool lower, cdbl* w, SolveLds& S) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nblk = (n + 63) / 64;
+    auto rb = [&](int s) { return 64 * (lower ? s : nblk - 1 - s); };
+    auto bsz = [&](int s) { return min(64, n - rb(s)); };
+    stage_diag(M, n, rb(0), bsz(0), true, S.tile[0], tid, blockDim.x);
+    for (int s = 0; s < nblk; ++s) {
+        const int r0 = rb(s), bs = bsz(s);
+        const int ja = lower ? 0 : r0 + bs, jb = lower ? r0 : n;
+        constexpr int CP = NC <= 2 ? 4 : (NC <= 4 ? 2 : 1);
+        for (int q0 = 0; q0 < 4; q0 += CP) {
+            if (4 * wv + q0 >= bs) break;        // wave-uniform
+            const int i0 = r0 + 4 * wv + q0;
+            cdbl acc[CP][NC];
+#pragma unroll
+            for (int q = 0; q < CP; ++q)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) acc[q][c] = mk(0, 0);
+            for (int j = ja + lane; j < jb; j += 64) {
+                cdbl xj[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) xj[c] = w[j + (size_t)c * n];
+#pragma unroll
+                for (int q = 0; q < CP; ++q) {
+                    if (4 * wv + q0 + q < bs) {
+                        const cdbl t = cconj(M[j + (size_t)(i0 + q) * n]);
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) cfma(acc[q][c], t, xj[c]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < CP; ++q)
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) {
+                        acc[q][c].x += __shfl_xor(acc[q][c].x, o, 64);
+                        acc[q][c].y += __shfl_xor(acc[q][c].y, o, 64);
+                    }
+            // lane q stores column i0 + q (register arrays are indexed with constants)
+#pragma unroll
+            for (int q = 0; q < CP; ++q)
+                if (lane == q && 4 * wv + q0 + q < bs) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        cdbl& y = w[i0 + q + (size_t)c * n];
+                        y = y - acc[q][c];
+                    }
+                }
+        }
+        __syncthreads();                          // sums in, block s staged
+        if (wv == 0) {
+            diag_solve<NC>(n, r0, bs, lower, !lower, w, S.tile[s & 1], S.xb);
+        } else if (s + 1 < nblk) {
+            stage_diag(M, n, rb(s + 1), bsz(s + 1), true, S.tile[(s + 1) & 1], tid - 64, blockDim.x - 64);
+        }
+        __syncthreads();
+    }
+}
+
 // one workgroup of kSolveThreads threads per matrix
 template <int NC>
-__global__ __launch_bounds__(kSolveThreads) void k_solve_thin(const cdbl* __restrict__ LU, const cdbl* __restrict__ LUT,
-                                                              const int* __restrict__ perm, int n, cdbl* __restrict__ X,
-                                                              int adj, cdbl* __restrict__ work) {
+__global__ __launch_bounds__(kSolveThreads) void k_solve_thin(const cdbl* __restrict__ LU, long long ms,
+                                                              const int* __restrict__ perm, int n,
+                                                              cdbl* __restrict__ X, int adj, cdbl* __restrict__ work) {
     __shared__ SolveLds S;
     const int b = blockIdx.x;
     const int* p = perm + (size_t)b * n;
     cdbl* x = X + (size_t)b * n * NC;
     cdbl* w = work + (size_t)b * n * NC;
+    const cdbl* lu = LU + (size_t)b * ms;
     const int tid = threadIdx.x;
     if (!adj) {
         // A = P^T L U:  w = P x, L y = w, U z = y
-        const cdbl* lu = LU + (size_t)b * n * n;
         for (int e = tid; e < n * NC; e += blockDim.x) {
             const int i = e % n, c = e / n;
             w[e] = x[p[i] + (size_t)c * n];
         }
         __syncthreads();
-        subst<NC>(lu, n, 0, true, true, w, S);
-        subst<NC>(lu, n, 1, false, false, w, S);
+        subst_right<NC>(lu, n, true, w, S);
+        subst_right<NC>(lu, n, false, w, S);
         for (int e = tid; e < n * NC; e += blockDim.x) x[e] = w[e];
     } else {
         // A^H = U^H L^H P:  U^H y = x, L^H z = y, x = P^T z
-        const cdbl* lut = LUT + (size_t)b * n * n;
         for (int e = tid; e < n * NC; e += blockDim.x) w[e] = x[e];
         __syncthreads();
-        subst<NC>(lut, n, 2, true, false, w, S);
-        subst<NC>(lut, n, 3, false, true, w, S);
+        subst_left<NC>(lu, n, true, w, S);
+        subst_left<NC>(lu, n, false, w, S);
         for (int e = tid; e < n * NC; e += blockDim.x) {
             const int i = e % n, c = e / n;
             x[p[i] + (size_t)c * n] = w[e];
@@ -665,42 +785,20 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_thin(const cdbl* __rest
     }
 }
 
-void lu_solve_thin_batched(const cdbl* LU, const cdbl* LUT, const int* perm, int n, int batch, cdbl* X, int nc,
-                           bool adj, cdbl* work, hipStream_t st) {
+void lu_solve_thin_batched(const cdbl* LU, long long ms, const int* perm, int n, int batch, cdbl* X, int nc, bool adj,
+                           cdbl* work, hipStream_t st) {
     const int a = adj ? 1 : 0;
     const dim3 g(batch), t(kSolveThreads);
     switch (nc) {
-    case 1: hipLaunchKernelGGL(k_solve_thin<1>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
-    case 2: hipLaunchKernelGGL(k_solve_thin<2>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
-    case 3: hipLaunchKernelGGL(k_solve_thin<3>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
-    case 4: hipLaunchKernelGGL(k_solve_thin<4>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
-    case 5: hipLaunchKernelGGL(k_solve_thin<5>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
-    case 6: hipLaunchKernelGGL(k_solve_thin<6>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
-    case 7: hipLaunchKernelGGL(k_solve_thin<7>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
-    default: hipLaunchKernelGGL(k_solve_thin<8>, g, t, 0, st, LU, LUT, perm, n, X, a, work); break;
+    case 1: hipLaunchKernelGGL(k_solve_thin<1>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    case 2: hipLaunchKernelGGL(k_solve_thin<2>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    case 3: hipLaunchKernelGGL(k_solve_thin<3>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    case 4: hipLaunchKernelGGL(k_solve_thin<4>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    case 5: hipLaunchKernelGGL(k_solve_thin<5>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    case 6: hipLaunchKernelGGL(k_solve_thin<6>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    case 7: hipLaunchKernelGGL(k_solve_thin<7>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    default: hipLaunchKernelGGL(k_solve_thin<8>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
     }
-}
-
-// AT_b = A_b^T for batch square matrices (32 x 32 tiles through LDS)
-__global__ __launch_bounds__(256) void k_transpose(const cdbl* __restrict__ A, cdbl* __restrict__ AT, int n) {
-    __shared__ cdbl t[32][33];
-    const size_t off = (size_t)blockIdx.z * n * n;
-    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    for (int r = ty; r < 32; r += 8) {
-        const int i = i0 + tx, j = j0 + r;
-        if (i < n && j < n) t[r][tx] = A[off + i + (size_t)j * n];
-    }
-    __syncthreads();
-    for (int r = ty; r < 32; r += 8) {
-        const int i = j0 + tx, j = i0 + r;
-        if (i < n && j < n) AT[off + i + (size_t)j * n] = t[tx][r];
-    }
-}
-
-void transpose_batched(const cdbl* A, cdbl* AT, int n, int batch, hipStream_t st) {
-    const dim3 g((n + 31) / 32, (n + 31) / 32, batch);
-    hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, A, AT, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -911,16 +1009,22 @@ void sigma_from_thin_batched(const cdbl* B, const cdbl* Q, int n, int batch, int
     hipLaunchKernelGGL(k_sigma, dim3(batch), dim3(256), 0, st, B, Q, n, nc, sigma, u, v);
 }
 
-// out = Re(a^H T b): one wavefront per column j computes (a^H T)_j
+// out = Re(a^H T b).  The columns of each matrix are split over kBilinearParts
+// workgroups (enough loads in flight to stream T at the memory rate); one
+// wavefront per column j forms (a^H T)_j b_j, each workgroup stores its
+// partial sum and a second kernel adds the partials in a fixed order.
+
 __global__ __launch_bounds__(256) void k_bilinear(const cdbl* __restrict__ T, long long tstride, const cdbl* __restrict__ a,
-                                                  const cdbl* __restrict__ bv, int n, double* __restrict__ out) {
+                                                  const cdbl* __restrict__ bv, int n, double* __restrict__ part) {
     __shared__ cdbl sh[4];
-    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int cs = (n + kBilinearParts - 1) / kBilinearParts;
+    const int jlo = blockIdx.x * cs, jhi = min(n, jlo + cs);
     const cdbl* Tm = T + (size_t)b * tstride;
     const cdbl* av = a + (size_t)b * n;
     const cdbl* bb = bv + (size_t)b * n;
     double acc = 0.0;
-    for (int j = wv; j < n; j += 4) {
+    for (int j = jlo + wv; j < jhi; j += 4) {
         cdbl s = mk(0, 0);
         for (int i = lane; i < n; i += 64) cfma(s, cconj(av[i]), Tm[i + (size_t)j * n]);
         for (int o = 32; o > 0; o >>= 1) { s.x += __shfl_xor(s.x, o, 64); s.y += __shfl_xor(s.y, o, 64); }
@@ -928,12 +1032,22 @@ __global__ __launch_bounds__(256) void k_bilinear(const cdbl* __restrict__ T, lo
         acc += t.x;
     }
     const cdbl r = block_sum(mk(lane == 0 ? acc : 0.0, 0.0), sh);
-    if (tid == 0) out[b] = r.x;
+    if (tid == 0) part[(size_t)b * kBilinearParts + blockIdx.x] = r.x;
+}
+
+__global__ void k_bilinear_sum(const double* __restrict__ part, int batch, double* __restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    double s = 0.0;
+    for (int q = 0; q < kBilinearParts; ++q) s += part[(size_t)b * kBilinearParts + q];
+    out[b] = s;
 }
 
 void bilinear_re_batched(const cdbl* T, long long tstride, const cdbl* a, const cdbl* bvec, int n, int batch,
-                         double* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_bilinear, dim3(batch), dim3(256), 0, st, T, tstride, a, bvec, n, out);
+                         double* out, double* part, hipStream_t st) {
+    if (batch <= 0) return;
+    hipLaunchKernelGGL(k_bilinear, dim3(kBilinearParts, batch), dim3(256), 0, st, T, tstride, a, bvec, n, part);
+    hipLaunchKernelGGL(k_bilinear_sum, dim3((batch + 255) / 256), dim3(256), 0, st, part, batch, out);
 }
 
 // y = T x or T^H x; one wavefront per output row (T^H: row i of T^H = column i of T)

@@ -62,12 +62,11 @@ void launch_form_Ao(const cdbl* L, int nk, int nslot, int o, int N, const double
 // dense (hbem_dense.hip); all batched over `batch` matrices of order n with
 // stride n*n (column-major, lda = n) unless noted.
 void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStream_t st);
-// X (n x nc, stride n*nc per matrix) <- op(A)^{-1} X with A = P^T L U from lu_factor_batched;
-// adj = false solves A X = B, adj = true solves A^H X = B and reads LUT, the
-// transposed factors (transpose_batched), instead of LU.
-void lu_solve_thin_batched(const cdbl* LU, const cdbl* LUT, const int* perm, int n, int batch, cdbl* X, int nc,
-                           bool adj, cdbl* work, hipStream_t st);
-void transpose_batched(const cdbl* A, cdbl* AT, int n, int batch, hipStream_t st);
+// X (n x nc, stride n*nc per matrix) <- op(A)^{-1} X with A = P^T L U from
+// lu_factor_batched (factors of matrix b at LU + b * ms); adj = false solves
+// A X = B, adj = true solves A^H X = B.  work: n * nc * batch.
+void lu_solve_thin_batched(const cdbl* LU, long long ms, const int* perm, int n, int batch, cdbl* X, int nc, bool adj,
+                           cdbl* work, hipStream_t st);
 // *flag = 1 if any of cnt doubles is not finite / a diagonal entry of A is 0
 void flag_nonfinite(const double* x, long long cnt, int* flag, hipStream_t st);
 void flag_zero_diag(const cdbl* A, int n, int* flag, hipStream_t st);
@@ -82,9 +81,11 @@ void orthonormalize_thin_batched(cdbl* X, int n, int batch, int nc, hipStream_t 
 // y = right singular vector (nc) -- v = Q y is formed by the caller.
 void sigma_from_thin_batched(const cdbl* B, const cdbl* Q, int n, int batch, int nc, double* sigma, cdbl* u, cdbl* v,
                              hipStream_t st);
-// out[b] = Re(a_b^H T_b b_b) for T_b column-major n x n (stride tstride)
+// out[b] = Re(a_b^H T_b b_b) for T_b column-major n x n (stride tstride);
+// part: scratch of kBilinearParts * batch doubles
+constexpr int kBilinearParts = 8;
 void bilinear_re_batched(const cdbl* T, long long tstride, const cdbl* a, const cdbl* bvec, int n, int batch,
-                         double* out, hipStream_t st);
+                         double* out, double* part, hipStream_t st);
 // y_b = T_b x_b (adj = false) or T_b^H x_b (adj = true), T_b n x n with stride tstride
 void gemv_batched(const cdbl* T, long long tstride, const cdbl* x, cdbl* y, int n, int batch, bool adj, hipStream_t st);
 // W = [[0, T], [T^H, 0]] - s I  (2n x 2n) for a single matrix
