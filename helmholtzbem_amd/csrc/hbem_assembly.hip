@@ -484,6 +484,10 @@ __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
 // ---------------------------------------------------------------------------
 constexpr int kRec = 22;     // doubles per staged node record
 constexpr int kStage = 256;  // nodes per staging round
+// general pairs: 16-double records, 128 nodes per round (their LDS leaves room
+// for the row-folded sums, see row_fold)
+template <int CLS> constexpr int rec_len() { return CLS == 2 ? 16 : kRec; }
+template <int CLS> constexpr int stage_len() { return CLS == 2 ? 128 : kStage; }
 
 // record: 0 r, 1 r^2, 2 1/r, 3 (2/pi) ln r, 4 1/sqrt r, 5 weight of the curl sum,
 //         6 + 8h + {0,1} K direct, {2,3} K transposed, {4,5} W, {6,7} V coefficient
@@ -492,7 +496,7 @@ constexpr int kStage = 256;  // nodes per staging round
 // General pairs (tensor Gauss rule, node a * ns + b at s = x_a on panel i,
 // t = x_b on panel j): the test shape b_m(s) of K direct, W and V is constant
 // along a row a, so those records hold the coefficient without it --
-//         6 + 8h + 0 K direct, 1 W, 2 V (weight x geometry x b_h(t)),
+//         6 + 5h + 0 K direct, 1 W, 2 V (weight x geometry x b_h(t)),
 //         3 + {0,1} K transposed of the entry q = m + 2h as above --
 // and the kernel applies b_m(s) once per row (row_fold).
 template <int CLS>
@@ -540,7 +544,7 @@ __device__ __forceinline__ void stage_node(const AsmParams& P, const double* __r
     if (CLS == 2) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            double* c = rec + 6 + 8 * h;
+            double* c = rec + 6 + 5 * h;
             c[0] = wcd * bj[h];
             c[1] = wbt * bj[h];
             c[2] = wtt * bj[h];
@@ -571,10 +575,11 @@ __device__ __forceinline__ void stage_node(const AsmParams& P, const double* __r
 }
 
 // padding record for an odd node count: finite argument, zero weights
+template <int CLS>
 __device__ __forceinline__ void stage_pad(double* rec) {
     rec[0] = 1.0; rec[1] = 1.0; rec[2] = 1.0; rec[3] = 0.0; rec[4] = 1.0;
 #pragma unroll
-    for (int i = 5; i < kRec; ++i) rec[i] = 0.0;
+    for (int i = 5; i < rec_len<CLS>(); ++i) rec[i] = 0.0;
 }
 
 // both lane halves get the value of the lower half (v0) and of the upper half (v1)
@@ -642,7 +647,7 @@ __device__ __forceinline__ void node_accum_x(AccX<D>& A, const double* __restric
     const cdbl h1r = mk(j1 * rinv, y1 * rinv);
     const cdbl hr = mk(j1 * r, y1 * r);
     const cdbl h0r2 = mk(j0 * r2, y0 * r2);
-    const double* __restrict__ c = R + 6 + 8 * h;
+    const double* __restrict__ c = R + 6 + (CLS == 2 ? 5 : 8) * h;
     if (CLS == 2) {
         const double ukd = c[0], ub = c[1], ug = c[2];
         axpy(A.kd[0][1], ukd, h1r);
@@ -690,21 +695,32 @@ __device__ __forceinline__ void node_accum_x(AccX<D>& A, const double* __restric
 // General pairs, end of row a of the tensor rule.  With U_a the running sum of
 // a term after row a and T_a = U_a - U_{a-1} the row's own sum, the test shape
 // b_0(s) = s gives sum_a s_a T_a = sum_a (s_a - s_{a+1}) U_a (s_{ns} = 0): one
-// update per row in slot m = 0, where the other classes pay one per node.
+// update per row, where the other classes pay one per node.  These nine sums
+// V (K direct, W, V for every order) are touched once per row, so they live in
+// LDS (vacc, one column per thread) and leave the registers to the node loop.
 // b_1 = 1 - s is then U - V (row_finish).
 template <int D>
-__device__ __forceinline__ void row_fold(AccX<D>& A, double delta) {
+__device__ __forceinline__ void row_fold(const AccX<D>& A, double delta, cdbl* vacc) {
+    const int t0 = threadIdx.x;
 #pragma unroll
     for (int t = 0; t <= D; ++t) {
-        axpy(A.kd[t][0], delta, A.kd[t][1]);
-        axpy(A.wb[t][0], delta, A.wb[t][1]);
-        axpy(A.v[t][0], delta, A.v[t][1]);
+        cdbl* v = vacc + (3 * t) * 256 + t0;
+        cdbl a = v[0], b = v[256], c = v[512];
+        axpy(a, delta, A.kd[t][1]);
+        axpy(b, delta, A.wb[t][1]);
+        axpy(c, delta, A.v[t][1]);
+        v[0] = a; v[256] = b; v[512] = c;
     }
 }
 template <int D>
-__device__ __forceinline__ void row_finish(AccX<D>& A) {
+__device__ __forceinline__ void row_finish(AccX<D>& A, const cdbl* vacc) {
+    const int t0 = threadIdx.x;
 #pragma unroll
     for (int t = 0; t <= D; ++t) {
+        const cdbl* v = vacc + (3 * t) * 256 + t0;
+        A.kd[t][0] = v[0];
+        A.wb[t][0] = v[256];
+        A.v[t][0] = v[512];
         A.kd[t][1] = A.kd[t][1] - A.kd[t][0];
         A.wb[t][1] = A.wb[t][1] - A.wb[t][0];
         A.v[t][1] = A.v[t][1] - A.v[t][0];
@@ -717,7 +733,9 @@ __device__ __forceinline__ void row_finish(AccX<D>& A) {
 // spill reloads cost (about 3% faster than two waves without spills).
 template <int CLS, int D>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_assemble_x2(AsmParams P) {
-    __shared__ double lds[(kStage + 1) * kRec];
+    constexpr int REC = rec_len<CLS>(), STG = stage_len<CLS>();
+    __shared__ double lds[(STG + 1) * REC];
+    __shared__ cdbl vacc[CLS == 2 ? 3 * (D + 1) * 256 : 1];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int h = lane >> 5, j = lane & 31;
     const int nqb = (P.nk + 31) >> 5;        // blocks of 32 slots
@@ -749,16 +767,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         for (int m = 0; m < 2; ++m) { A.kd[t][m] = mk(0, 0); A.kt[t][m] = mk(0, 0); A.wb[t][m] = mk(0, 0); A.v[t][m] = mk(0, 0); }
     }
 
-    for (int base = lo; base < hi; base += kStage) {
-        const int cnt = min(kStage, hi - base);
+    if (CLS == 2) {
+#pragma unroll
+        for (int t = 0; t < 3 * (D + 1); ++t) vacc[t * 256 + threadIdx.x] = mk(0, 0);
+    }
+    for (int base = lo; base < hi; base += STG) {
+        const int cnt = min(STG, hi - base);
         __syncthreads();   // the previous round's records are no longer read
-        for (int i = threadIdx.x; i < cnt; i += blockDim.x) stage_node<CLS>(P, geo, f, swap, base + i, lds + i * kRec);
-        if ((cnt & 1) && threadIdx.x == 0) stage_pad(lds + cnt * kRec);
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) stage_node<CLS>(P, geo, f, swap, base + i, lds + i * REC);
+        if ((cnt & 1) && threadIdx.x == 0) stage_pad<CLS>(lds + cnt * REC);
         __syncthreads();
         if (qb < nqb) {
             for (int p = 0; p < cnt; p += 2) {
-                const double* R0 = lds + p * kRec;
-                const double* Rown = R0 + h * kRec;
+                const double* R0 = lds + p * REC;
+                const double* Rown = R0 + h * REC;
                 const double r = Rown[0];
                 const double x = kap * r;
                 double j0, y0, j1, y1;
@@ -771,18 +793,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 node_accum_x<CLS, D>(A, R0, h, a0, b0, c0, d0);
                 if (CLS == 2) {
                     const int nd = base + p, a = nd / P.ns;
-                    if (nd - a * P.ns == P.ns - 1) row_fold(A, P.xg[a] - (a + 1 < P.ns ? P.xg[a + 1] : 0.0));
+                    if (nd - a * P.ns == P.ns - 1) row_fold(A, P.xg[a] - (a + 1 < P.ns ? P.xg[a + 1] : 0.0), vacc);
                 }
-                node_accum_x<CLS, D>(A, R0 + kRec, h, a1, b1, c1, d1);
+                node_accum_x<CLS, D>(A, R0 + REC, h, a1, b1, c1, d1);
                 if (CLS == 2) {
                     const int nd = base + p + 1, a = nd / P.ns;
                     if (nd < hi && nd - a * P.ns == P.ns - 1)
-                        row_fold(A, P.xg[a] - (a + 1 < P.ns ? P.xg[a + 1] : 0.0));
+                        row_fold(A, P.xg[a] - (a + 1 < P.ns ? P.xg[a + 1] : 0.0), vacc);
                 }
             }
         }
     }
-    if (CLS == 2) row_finish(A);
+    if (CLS == 2) row_finish(A, vacc);
 
     // Sums -> this lane's entries q = m + 2h of the reference's local matrices
     // (same scalings as acc_store)
