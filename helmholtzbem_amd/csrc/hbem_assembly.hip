@@ -945,18 +945,21 @@ __global__ void k_gather_layers(const cdbl* __restrict__ L, int nk, int nslot, i
 // 8 J + i over the columns of tile (J, I).  op_entry sums the symmetric layers
 // W and V bitwise symmetric, so tile (J, I) takes them from tile (I, J)
 // through LDS: every local-matrix record is read once.  Reads run over 32
-// wavenumbers per record slot, writes over 8 rows per column.
+// wavenumbers per record slot, writes over 8 rows per column; the output
+// chunks of one tile pair are neighbouring workgroups, so the chunks of a
+// record slot are read at about the same time (long runs in each DRAM page).
 __global__ __launch_bounds__(256) void k_form_T(const cdbl* __restrict__ L, int nk, int nslot,
                                                 const int* __restrict__ qi, const int* __restrict__ qo, int nb, int N,
                                                 int D, const double* __restrict__ M, cdbl* __restrict__ T0,
                                                 cdbl* __restrict__ T1, cdbl* __restrict__ T2) {
     __shared__ cdbl sW[8][8][32];   // [column in tile (I, J)][row in tile (I, J)][output]
     __shared__ cdbl sV[8][8][32];
-    const int mt = (N + 7) >> 3;
-    const int I = blockIdx.x % mt, J = blockIdx.x / mt;
+    const int mt = (N + 7) >> 3, nbc = (nb + 31) >> 5;
+    const int bc = blockIdx.x % nbc, tile = blockIdx.x / nbc;
+    const int I = tile % mt, J = tile / mt;
     if (J > I) return;                       // whole workgroup: no barrier is skipped
     const int i = threadIdx.x & 7, jb = threadIdx.x >> 3;
-    const int b = blockIdx.y * 32 + jb;
+    const int b = bc * 32 + jb;
     const bool act = b < nb;                 // idle threads still reach every barrier
     const int n2 = 2 * N;
     const size_t NN = (size_t)n2 * n2;
@@ -1096,8 +1099,8 @@ void launch_gather_layers(const cdbl* L, int nk, int nslot, int q, int N, int D,
 void launch_form_T(const cdbl* L, int nk, int nslot, const int* qi, const int* qo, int nb, int N, int D,
                    const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st) {
     const unsigned mt = (unsigned)((N + 7) / 8);
-    hipLaunchKernelGGL(k_form_T, dim3(mt * mt, (unsigned)((nb + 31) / 32)), dim3(256), 0, st, L, nk, nslot, qi, qo, nb,
-                       N, D, M, T0, T1, T2);
+    hipLaunchKernelGGL(k_form_T, dim3(mt * mt * (unsigned)((nb + 31) / 32)), dim3(256), 0, st, L, nk, nslot, qi, qo,
+                       nb, N, D, M, T0, T1, T2);
 }
 
 void launch_form_Ao(const cdbl* L, int nk, int nslot, int o, int N, const double* M, cdbl* A, hipStream_t st) {

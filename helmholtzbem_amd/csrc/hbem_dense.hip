@@ -389,22 +389,32 @@ __global__ void k_lu_perm(int n, const int* __restrict__ ipiv, int* __restrict__
 // consecutive rows of one column per lane group, for coalesced stores.
 // Workgroup = 64 x 64 tile of C as four wavefronts of 32 x 32 (2 x 2 blocks of
 // 16 x 16); k advances in steps of 16 through LDS while the next step's tile
-// is fetched into registers.
+// is fetched into registers.  Workgroups are dealt to the eight XCDs in turn
+// (id % 8); the grid is renumbered so that each XCD gets a contiguous run of
+// (matrix, tile) indices, i.e. whole matrices, whose A and B panels then stay
+// in that XCD's L2 instead of being fetched by all eight.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 template <bool TB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_zgemm(int m, int n, int k, cdbl alpha, const cdbl* __restrict__ A, int lda,
                                                long long sa, const cdbl* __restrict__ B, int ldb, long long sb,
-                                               cdbl beta, cdbl* __restrict__ C, int ldc, long long sc) {
-    A += blockIdx.z * sa;
-    B += blockIdx.z * sb;
-    C += blockIdx.z * sc;
+                                               cdbl beta, cdbl* __restrict__ C, int ldc, long long sc, int batch) {
+    const int tm = (m + 63) >> 6, tn = (n + 63) >> 6;
+    const long long total = (long long)tm * tn * batch;
+    const long long per = (total + 7) >> 3;
+    const long long li = (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (li >= total) return;                 // whole workgroup, before any barrier
+    const int bmat = (int)(li / ((long long)tm * tn));
+    const int tile = (int)(li - (long long)bmat * tm * tn);
+    A += bmat * sa;
+    B += bmat * sb;
+    C += bmat * sc;
     __shared__ cdbl As[16][65];
     __shared__ cdbl Bs[16][65];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int l16 = lane & 15, lk = lane >> 4;
     const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
-    const int row0 = blockIdx.x * 64, col0 = blockIdx.y * 64;
+    const int row0 = (tile % tm) * 64, col0 = (tile / tm) * 64;
     dbl4 p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -483,11 +493,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 void zgemm_batched(int m, int n, int k, cdbl alpha, const cdbl* A, int lda, long long sa, const cdbl* B, int ldb,
                    long long sb, bool transB, cdbl beta, cdbl* C, int ldc, long long sc, int batch, hipStream_t st) {
     if (m <= 0 || n <= 0 || batch <= 0) return;
-    dim3 grid((m + 63) / 64, (n + 63) / 64, batch);
+    const long long total = (long long)((m + 63) / 64) * ((n + 63) / 64) * batch;
+    const dim3 grid((unsigned)(8 * ((total + 7) / 8)));
     if (transB)
-        hipLaunchKernelGGL(k_zgemm<true>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc);
+        hipLaunchKernelGGL(k_zgemm<true>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc,
+                           batch);
     else
-        hipLaunchKernelGGL(k_zgemm<false>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc);
+        hipLaunchKernelGGL(k_zgemm<false>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc,
+                           batch);
 }
 
 // Blocked right-looking LU in steps of 64 columns, each factorised as two
