@@ -77,11 +77,27 @@ def device_jy01(x, defs, c):
     return (b * fma(P0, cs, -Q0 * sc), b * fma(P0, sc, Q0 * cs), b * fma(P1, sc, Q1 * cs), b * fma(Q1, sc, -P1 * cs))
 
 
+def device_jy01_touching(x, defs, c):
+    """J0, Y0, J1, Y1 as the assembly kernels of touching panel pairs compute
+    them (bessel_jy01_f<true> in hbem_assembly.hip): below HBEM_TINY_X the entire
+    parts come from the low-degree table fitted on (0, HBEM_TINY_X)."""
+    if x >= defs["HBEM_TINY_X"]:
+        return device_jy01(x, defs, c)
+    two_pi = 2.0 / np.pi
+    y = (0.125 * x) ** 2
+    A, B, C, D = horner4(c, defs["HBEM_TAB_TINY"], defs["HBEM_TINY_DEG"], y)
+    L2 = two_pi * np.log(0.5 * x)
+    J1 = x * B
+    return A, fma(L2, A, C), J1, fma(L2, J1, fma(x, D, -two_pi * (1.0 / x)))
+
+
 def test_table_layout():
     defs, c = _table()
     assert defs["HBEM_TAB_SMALL"] == 0
     assert defs["HBEM_TAB_PQ"] == 4 * (defs["HBEM_SMALL_DEG"] + 1)
     assert defs["HBEM_TAB_SC"] == defs["HBEM_TAB_PQ"] + 4 * (defs["HBEM_PQ_DEG"] + 1)
+    assert defs["HBEM_TAB_TINY"] == defs["HBEM_TAB_SC"] + 4 + 2 * (defs["HBEM_SC_DEG"] + 1)
+    assert defs["HBEM_TAB_SIZE"] == defs["HBEM_TAB_TINY"] + 4 * (defs["HBEM_TINY_DEG"] + 1)
     T = defs["HBEM_TAB_SC"]
     assert c[T] == 2.0 / np.pi and c[T + 1] == np.pi / 2
     # the three parts sum to pi/2 far beyond double precision
@@ -116,3 +132,28 @@ def test_device_bessel_vs_mpmath():
             # entire parts, 1.5e-16 relative for the asymptotic amplitudes
             tol = 5e-14 if x < 8.0 else 2e-15
             assert abs(g - ref) <= tol * max(abs(ref), amp), (name, x, g, ref)
+
+
+def test_device_bessel_touching_pairs_vs_mpmath():
+    """The low-degree table of touching panel pairs against the 50-digit
+    fixture below HBEM_TINY_X, and against the full table: it must not be less
+    accurate than the (0, 8) fit it replaces there."""
+    defs, c = _table()
+    xt = defs["HBEM_TINY_X"]
+    with open(os.path.join(HERE, "golden", "bessel_mpmath.json")) as f:
+        rows = [r for r in json.load(f)["rows"] if 0.0 < r["x"] < xt]
+    assert len(rows) > 5
+    for r in rows:
+        x = r["x"]
+        got = device_jy01_touching(x, defs, c)
+        for name, g in zip(("J0", "Y0", "J1", "Y1"), got):
+            ref = float(r[name])
+            assert abs(g - ref) <= 2e-15 * max(abs(ref), 1.0), (name, x, g, ref)
+    # dense sweep of (0, HBEM_TINY_X) against mpmath directly
+    mp.mp.dps = 30
+    for x in np.concatenate([np.geomspace(1e-5, 1.0, 25), np.linspace(1.0, xt * 0.9999, 25)]):
+        x = float(x)
+        got = device_jy01_touching(x, defs, c)
+        ref = [mp.besselj(0, x), mp.bessely(0, x), mp.besselj(1, x), mp.bessely(1, x)]
+        for g, e in zip(got, ref):
+            assert abs(g - float(e)) <= 2e-15 * max(abs(float(e)), 1.0), (x, g, e)
