@@ -287,3 +287,19 @@ def test_sweep_batches_are_independent(H):
     base = H.sigma_sweep(bd, ks, 1.0, 16.0, R, 2)
     one = H.sigma_sweep(bd, ks[137:138], 1.0, 16.0, R, 2)
     assert abs(one[0] - base[137]) <= 1e-12 * base[137]
+
+
+@pytest.mark.gpu
+def test_rsvd_tall_panels_and_wide_blocks(H, O):
+    """n = 1100: the first LU panels have more than 1024 rows (per-column panel
+    kernel), the later ones run with the panel rows in registers; eight
+    right-hand sides take the one-column-per-sweep path of the adjoint solves.
+    sigma against the oracle with the same test matrix."""
+    rng = np.random.default_rng(11)
+    n = 1100
+    T = np.asfortranarray(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    for nc, q in ((2, 1), (8, 1)):
+        R = H.gaussian_test_matrix(n, nc, 5 + nc)
+        s = H.randomized_svd.sv(T, R, q)
+        so = O.sv(T, R, q)
+        assert abs(s - so) <= SIG_TOL * so, (nc, q, s, so)
