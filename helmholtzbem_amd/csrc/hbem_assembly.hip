@@ -221,20 +221,22 @@ __device__ __forceinline__ void acc_reduce(Acc<D>& A) {
 // second triangle's K factor) and only the entries m >= n of W and V are kept.
 // H0, H1 at kappa r: real positive kappa (the reference's Hankel1Real01
 // path) or any other kappa (its complex path)
+template <bool TINY>
 __device__ __forceinline__ void hankel_at(double kap, double r, cdbl& h0, cdbl& h1) {
     double j0, y0, j1, y1;
-    bessel_jy01(kap * r, j0, y0, j1, y1);
+    bessel_jy01<TINY>(kap * r, j0, y0, j1, y1);
     h0 = mk(j0, y0);
     h1 = mk(j1, y1);
 }
+template <bool TINY>
 __device__ __forceinline__ void hankel_at(cdbl kap, double r, cdbl& h0, cdbl& h1) { hankel01_cplx(r * kap, h0, h1); }
 
-template <int D, typename KT>
+template <int D, typename KT, bool TINY>
 __device__ __forceinline__ void node_accum(Acc<D>& A, KT kap, double r, double r2, double w, const double bi[2],
                                            const double bj[2], double cd, double ct, double bt, double ttp, bool sym,
                                            double cd2) {
     cdbl h0, h1;
-    hankel_at(kap, r, h0, h1);
+    hankel_at<TINY>(kap, r, h0, h1);
     const cdbl hr = r * h1;
     const double rinv = r > 0.0 ? 1.0 / r : 0.0;
     const cdbl h1r = rinv * h1;
@@ -417,8 +419,8 @@ __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
             const double s = cur.xa, t = cur.xb;
             const double w = cur.wa * cur.wb;
             const double bi[2] = {s, 1.0 - s}, bj[2] = {t, 1.0 - t};
-            node_accum<D, KT>(A, kap, cur.o[0], cur.o[1], w, bi, bj, fd * cur.o[2], ft * cur.o[3], fb * cur.o[4],
-                          cur.o[5], false, 0.0);
+            node_accum<D, KT, false>(A, kap, cur.o[0], cur.o[1], w, bi, bj, fd * cur.o[2], ft * cur.o[3],
+                                     fb * cur.o[4], cur.o[5], false, 0.0);
             cur = nxt;
         }
     } else if (CLS == 1) {
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
             const double cdv = (fh & 1) ? 0.0 : cur.o[2];
             const double ctv = (fh & 2) ? 0.0 : cur.o[3];
             const double bt = (fh & 4) ? 0.0 : 2.0 * cur.o[4];
-            node_accum<D, KT>(A, kap, cur.o[0], cur.o[1], w, bi, bj, cdv, ctv, bt, cur.o[5], false, 0.0);
+            node_accum<D, KT, true>(A, kap, cur.o[0], cur.o[1], w, bi, bj, cdv, ctv, bt, cur.o[5], false, 0.0);
             cur = nxt;
         }
     } else {
@@ -455,8 +457,8 @@ __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
             const double t = cur.xa, s = t * (1.0 - cur.xb);
             const double w = t * cur.wa * cur.wb;
             const double bi[2] = {s, 1.0 - s}, bj[2] = {t, 1.0 - t};
-            node_accum<D, KT>(A, kap, cur.o[0], cur.o[1], w, bi, bj, f0 * cur.o[2], 0.0, fb * cur.o[4], cur.o[5], true,
-                          f1 * cur.o[3]);
+            node_accum<D, KT, true>(A, kap, cur.o[0], cur.o[1], w, bi, bj, f0 * cur.o[2], 0.0, fb * cur.o[4],
+                                    cur.o[5], true, f1 * cur.o[3]);
             cur = nxt;
         }
     }

@@ -135,13 +135,17 @@ __device__ __forceinline__ void sincos_med(double x, double& s, double& c) {
 //   x <= 1e6     Hankel asymptotic form with P_nu, x Q_nu from the polynomial
 //                fits of hankel_tables.inc (A&S 9.2.5-9.2.6)
 //   x > 1e6      leading asymptotic term (cbessel.cpp:1847-1856)
+// TINY (touching panel pairs): below x = HBEM_TINY_X the entire parts come
+// from the degree-HBEM_TINY_DEG fit of that range.
+template <bool TINY = false>
 __device__ __forceinline__ void bessel_jy01(double x, double& j0, double& y0, double& j1, double& y1) {
     if (x < 8.0) {
         const double xs = x > 0.0 ? x : 1.0;
         const double h = 0.125 * xs;
         const double y = h * h;
         double A, B, C, D;
-        horner4<HBEM_SMALL_DEG>(bessel_table() + HBEM_TAB_SMALL, y, A, B, C, D);
+        if (TINY && xs < (double)HBEM_TINY_X) horner4<HBEM_TINY_DEG>(bessel_table() + HBEM_TAB_TINY, y, A, B, C, D);
+        else horner4<HBEM_SMALL_DEG>(bessel_table() + HBEM_TAB_SMALL, y, A, B, C, D);
         const double L2 = kTwoOverPi * log(0.5 * xs);
         const double xinv = 1.0 / xs;
         const double J1 = xs * B;
