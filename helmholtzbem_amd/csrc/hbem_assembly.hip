@@ -486,10 +486,10 @@ __global__ __launch_bounds__(256) void k_assemble(AsmParams P) {
 // ---------------------------------------------------------------------------
 constexpr int kRec = 22;     // doubles per staged node record
 constexpr int kStage = 256;  // nodes per staging round
-// general pairs: 16-double records, 128 nodes per round (their LDS leaves room
-// for the row-folded sums, see row_fold)
-template <int CLS> constexpr int rec_len() { return CLS == 2 ? 16 : kRec; }
-template <int CLS> constexpr int stage_len() { return CLS == 2 ? 128 : kStage; }
+// general and adjacent pairs: 16-double records, 128 nodes per round (their
+// LDS leaves room for the row-folded sums, see row_fold)
+template <int CLS> constexpr int rec_len() { return CLS >= 1 ? 16 : kRec; }
+template <int CLS> constexpr int stage_len() { return CLS >= 1 ? 128 : kStage; }
 
 // record: 0 r, 1 r^2, 2 1/r, 3 (2/pi) ln r, 4 1/sqrt r, 5 weight of the curl sum,
 //         6 + 8h + {0,1} K direct, {2,3} K transposed, {4,5} W, {6,7} V coefficient
@@ -501,11 +501,17 @@ template <int CLS> constexpr int stage_len() { return CLS == 2 ? 128 : kStage; }
 //         6 + 5h + 0 K direct, 1 W, 2 V (weight x geometry x b_h(t)),
 //         3 + {0,1} K transposed of the entry q = m + 2h as above --
 // and the kernel applies b_m(s) once per row (row_fold).
+// Adjacent pairs (node half hn, row a at t = xc_a): in half 1 the test point
+// is t, so b_m is constant along a row as for general pairs; in half 0 the
+// trial point is t, so there lane half h takes the test shape b_h(s) instead
+// (it holds entries (h, n), exchanged into (m, h) at the end) and b_n(t) is
+// folded per row.
 template <int CLS>
 __device__ __forceinline__ void stage_node(const AsmParams& P, const double* __restrict__ geo, unsigned f, bool swap,
                                            int nd, double* __restrict__ rec) {
     const double* o = geo + (size_t)nd * kNodeStride;
     double w, bi[2], bj[2], cd, ct, bt;
+    int hn = 0;
     if (CLS == 2) {
         const int ns = P.ns, a = nd / ns, b = nd - a * ns;
         const double s = P.xg[a], t = P.xg[b];
@@ -516,13 +522,14 @@ __device__ __forceinline__ void stage_node(const AsmParams& P, const double* __r
         bt = (f & 4) ? 0.0 : 2.0 * o[4];
     } else if (CLS == 1) {
         const int nc = P.nc, nh = nc * nc;
-        const int h = nd >= nh ? 1 : 0, gg = nd - h * nh, a = gg / nc, b = gg - a * nc;
+        hn = nd >= nh ? 1 : 0;
+        const int gg = nd - hn * nh, a = gg / nc, b = gg - a * nc;
         const double t = P.xc[a], s = t * P.xc[b];
         w = t * P.wc[a] * P.wc[b];
-        const double si = h == 0 ? s : t, sj = h == 0 ? t : s;
+        const double si = hn == 0 ? s : t, sj = hn == 0 ? t : s;
         bi[0] = swap ? si : 1.0 - si; bi[1] = swap ? 1.0 - si : si;
         bj[0] = swap ? 1.0 - sj : sj; bj[1] = swap ? sj : 1.0 - sj;
-        const unsigned fh = f >> (3 * h);
+        const unsigned fh = f >> (3 * hn);
         cd = (fh & 1) ? 0.0 : o[2];
         ct = (fh & 2) ? 0.0 : o[3];
         bt = (fh & 4) ? 0.0 : 2.0 * o[4];
@@ -543,13 +550,14 @@ __device__ __forceinline__ void stage_node(const AsmParams& P, const double* __r
     rec[4] = o[8];
     rec[5] = CLS == 0 ? 2.0 * w : w;
     const double wcd = w * cd, wct = w * ct, wbt = w * bt, wtt = w * o[5];
-    if (CLS == 2) {
+    if (CLS >= 1) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             double* c = rec + 6 + 5 * h;
-            c[0] = wcd * bj[h];
-            c[1] = wbt * bj[h];
-            c[2] = wtt * bj[h];
+            const double own = (CLS == 1 && hn == 0) ? bi[h] : bj[h];   // the shape not folded per row
+            c[0] = wcd * own;
+            c[1] = wbt * own;
+            c[2] = wtt * own;
             c[3] = wct * (bj[0] * bi[h]);
             c[4] = wct * (bj[1] * bi[h]);
         }
@@ -644,8 +652,9 @@ __device__ __forceinline__ void bessel_jy01_f(double x, double y, double L2, dou
 }
 
 // one node's contribution to this lane's sums (entries q = m + 2h).  General
-// pairs: K direct, W and V go to the running row-free sums U in slot m = 1
-// (see row_fold); K transposed to both entries as for the other classes.
+// and adjacent pairs: K direct, W and V go to the running row-free sums U in
+// slot m = 1 (see row_fold); K transposed to both entries as for coinciding
+// pairs.
 template <int CLS, int D>
 __device__ __forceinline__ void node_accum_x(AccX<D>& A, const double* __restrict__ R, int h, double j0, double y0,
                                              double j1, double y1) {
@@ -654,8 +663,8 @@ __device__ __forceinline__ void node_accum_x(AccX<D>& A, const double* __restric
     const cdbl h1r = mk(j1 * rinv, y1 * rinv);
     const cdbl hr = mk(j1 * r, y1 * r);
     const cdbl h0r2 = mk(j0 * r2, y0 * r2);
-    const double* __restrict__ c = R + 6 + (CLS == 2 ? 5 : 8) * h;
-    if (CLS == 2) {
+    const double* __restrict__ c = R + 6 + (CLS >= 1 ? 5 : 8) * h;
+    if (CLS >= 1) {
         const double ukd = c[0], ub = c[1], ug = c[2];
         axpy(A.kd[0][1], ukd, h1r);
         axpy(A.wb[0][1], ub, h0);
@@ -699,13 +708,14 @@ __device__ __forceinline__ void node_accum_x(AccX<D>& A, const double* __restric
     if (D >= 2) axpy(A.wa[2], wa, h0r2);
 }
 
-// General pairs, end of row a of the tensor rule.  With U_a the running sum of
-// a term after row a and T_a = U_a - U_{a-1} the row's own sum, the test shape
-// b_0(s) = s gives sum_a s_a T_a = sum_a (s_a - s_{a+1}) U_a (s_{ns} = 0): one
-// update per row, where the other classes pay one per node.  These nine sums
-// V (K direct, W, V for every order) are touched once per row, so they live in
-// LDS (vacc, one column per thread) and leave the registers to the node loop.
-// b_1 = 1 - s is then U - V (row_finish).
+// General and adjacent pairs, end of row a of the node chunk.  With U_a the
+// running sum of a term after row a (from the start of the chunk) and
+// T_a = U_a - U_{a-1} the chunk's part of row a, the row-constant shape
+// f(a) = b_0 gives sum_a f(a) T_a = sum_{a < a1} (f(a) - f(a+1)) U_a + f(a1) U_a1
+// (a1 the chunk's last row): one update per row, where coinciding pairs pay one
+// per node.  These nine sums V (K direct, W, V for every order) are touched
+// once per row, so they live in LDS (vacc, one column per thread) and leave
+// the registers to the node loop.  b_1 = 1 - b_0 is then U - V (row_finish).
 template <int D>
 __device__ __forceinline__ void row_fold(const AccX<D>& A, double delta, cdbl* vacc) {
     const int t0 = threadIdx.x;
@@ -742,7 +752,7 @@ template <int CLS, int D>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_assemble_x2(AsmParams P) {
     constexpr int REC = rec_len<CLS>(), STG = stage_len<CLS>();
     __shared__ double lds[(STG + 1) * REC];
-    __shared__ cdbl vacc[CLS == 2 ? 3 * (D + 1) * 256 : 1];
+    __shared__ cdbl vacc[CLS >= 1 ? 3 * (D + 1) * 256 : 1];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int h = lane >> 5, j = lane & 31;
     const int nqb = (P.nk + 31) >> 5;        // blocks of 32 slots
@@ -762,8 +772,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const unsigned f = P.flags[e];
     const bool swap = CLS == 1 ? P.ent[3 * e + 2] != 0 : false;
     const int nn = CLS == 2 ? P.ns * P.ns : (CLS == 1 ? 2 * P.nc * P.nc : P.nc * P.nc);
-    const int lo = chunk * P.chunk;
-    const int hi = min(nn, lo + P.chunk);
+    // adjacent pairs: the chunks of each node half are numbered after each other
+    // (a chunk never spans the two halves)
+    int lo = chunk * P.chunk, hi = min(nn, lo + P.chunk), hn = 0;
+    if (CLS == 1) {
+        const int nh = P.nc * P.nc, cph = (nh + P.chunk - 1) / P.chunk;
+        hn = chunk / cph;
+        lo = hn * nh + (chunk - hn * cph) * P.chunk;
+        hi = min(hn * nh + nh, lo + P.chunk);
+    }
     const double* __restrict__ geo = P.ngeo + (size_t)e * nn * kNodeStride;
 
     AccX<D> A;
@@ -774,10 +791,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         for (int m = 0; m < 2; ++m) { A.kd[t][m] = mk(0, 0); A.kt[t][m] = mk(0, 0); A.wb[t][m] = mk(0, 0); A.v[t][m] = mk(0, 0); }
     }
 
-    if (CLS == 2) {
+    if (CLS >= 1) {
 #pragma unroll
         for (int t = 0; t < 3 * (D + 1); ++t) vacc[t * 256 + threadIdx.x] = mk(0, 0);
     }
+    // Row-constant shape value b_0 of row a: general pairs s = x_a (their chunk
+    // is the whole tensor rule, so the last row's fold is x_{ns-1} - 0, see the
+    // node loop); adjacent pairs the test shape at t = xc_a in half 1, the trial
+    // shape at t in half 0, folded after node nd when it ends a row or the chunk.
+    const bool fx = CLS == 1 && (swap != (hn == 1));   // adjacent rows whose b_0 is 1 - t
+    auto row_shape = [&](int a) {
+        const double t = P.xc[a];
+        return fx ? 1.0 - t : t;
+    };
+    auto fold_after = [&](int nd) {
+        const int rl = P.nc;
+        const int g = nd - hn * P.nc * P.nc, a = g / rl;
+        if (nd == hi - 1) row_fold(A, row_shape(a), vacc);
+        else if (g - a * rl == rl - 1) row_fold(A, row_shape(a) - row_shape(a + 1), vacc);
+    };
     for (int base = lo; base < hi; base += STG) {
         const int cnt = min(STG, hi - base);
         __syncthreads();   // the previous round's records are no longer read
@@ -802,17 +834,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 if (CLS == 2) {
                     const int nd = base + p, a = nd / P.ns;
                     if (nd - a * P.ns == P.ns - 1) row_fold(A, P.xg[a] - (a + 1 < P.ns ? P.xg[a + 1] : 0.0), vacc);
+                } else if (CLS == 1) {
+                    fold_after(base + p);
                 }
                 node_accum_x<CLS, D>(A, R0 + REC, h, a1, b1, c1, d1);
                 if (CLS == 2) {
                     const int nd = base + p + 1, a = nd / P.ns;
                     if (nd < hi && nd - a * P.ns == P.ns - 1)
                         row_fold(A, P.xg[a] - (a + 1 < P.ns ? P.xg[a + 1] : 0.0), vacc);
+                } else if (CLS == 1 && base + p + 1 < hi) {
+                    fold_after(base + p + 1);
                 }
             }
         }
     }
-    if (CLS == 2) row_finish(A, vacc);
+    if (CLS >= 1) row_finish(A, vacc);
+    if (CLS == 1 && hn == 0) {
+        // adjacent half 0: lane half h holds P = entry (h, 0) and Q = entry (h, 1)
+        // of K direct, W and V; lane half n takes entries (0, n), (1, n)
+#pragma unroll
+        for (int t = 0; t <= D; ++t) {
+            cdbl* terms[3][2] = {{&A.kd[t][0], &A.kd[t][1]}, {&A.wb[t][0], &A.wb[t][1]}, {&A.v[t][0], &A.v[t][1]}};
+#pragma unroll
+            for (int u = 0; u < 3; ++u) {
+                const cdbl Pv = *terms[u][0], Qv = *terms[u][1];
+                double p0x, p1x, p0y, p1y, q0x, q1x, q0y, q1y;
+                halves(Pv.x, p0x, p1x);
+                halves(Pv.y, p0y, p1y);
+                halves(Qv.x, q0x, q1x);
+                halves(Qv.y, q0y, q1y);
+                *terms[u][0] = h == 0 ? mk(p0x, p0y) : mk(q0x, q0y);
+                *terms[u][1] = h == 0 ? mk(p1x, p1y) : mk(q1x, q1y);
+            }
+        }
+    }
 
     // Sums -> this lane's entries q = m + 2h of the reference's local matrices
     // (same scalings as acc_store)

@@ -191,10 +191,12 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
         P.flags = l.flags;
         P.pidx = l.pidx;
         // general pairs: one chunk; singular pairs: chunks of 2048 nodes (paired-lane
-        // kernel, 8 staging rounds) or 64 x 48 nodes (one wavefront per item)
+        // kernel, 16 staging rounds; for adjacent pairs each of the two node halves
+        // is chunked on its own) or 64 x 48 nodes (one wavefront per item)
         const int chunk_nodes = l.cls == 2 ? l.nodes : (l.G == 0 ? 2048 : (l.G == 1 ? 512 : 64 * 48));
         P.chunk = chunk_nodes;
         P.nchunk = (l.nodes + chunk_nodes - 1) / chunk_nodes;
+        if (l.cls == 1 && l.G == 0) P.nchunk = 2 * ((l.nodes / 2 + chunk_nodes - 1) / chunk_nodes);
         cdbl* part = nullptr;
         if (P.nchunk > 1) part = g->w_part.as<cdbl>((size_t)nk * l.nent * P.nchunk * nslot);
         P.out = P.nchunk > 1 ? part : L;
