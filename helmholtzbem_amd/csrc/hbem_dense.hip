@@ -641,8 +641,8 @@ __device__ __forceinline__ void diag_solve(int n, int r0, int bs, bool lower, bo
 // Right-looking pass with the column-major factor: L (lower, unit) forward or
 // U (upper) backward.  After block b, rows outside it (below / above) take
 // w_i -= sum_j M(i, r0 + j) x_j: a wavefront covers 16 rows x 4 groups of 16
-// columns (lane = row + 16 group), so every thread keeps 16 loads in flight
-// (8 for more than two right-hand sides, within the register budget).
+// columns (lane = row + 16 group), each thread loading its 16 columns in two
+// batches of eight.
 template <int NC>
 __device__ __forceinline__ void subst_right(const cdbl* M, int n, bool lower, cdbl* w, SolveLds& S) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
@@ -666,14 +666,22 @@ __device__ __forceinline__ void subst_right(const cdbl* M, int n, bool lower, cd
             cdbl acc[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) acc[c] = mk(0, 0);
-            if (i < ib) {
-#pragma unroll(NC <= 2 ? 16 : 8)
-                for (int u = 0; u < 16; ++u) {
-                    const int j = 16 * grp + u;
-                    if (j < bs) {
-                        const cdbl t = M[i + (size_t)(r0 + j) * n];
+            {
+                // eight loads at a time from clamped (in-range) addresses, values
+                // beyond the block zeroed afterwards: no branch between the loads
+                const int ic = min(i, ib - 1);
+#pragma unroll 1
+                for (int u0 = 0; u0 < 16; u0 += 8) {
+                    cdbl t[8];
 #pragma unroll
-                        for (int c = 0; c < NC; ++c) cfma(acc[c], t, S.xb[j * kMaxNc + c]);
+                    for (int u = 0; u < 8; ++u)
+                        t[u] = M[ic + (size_t)(r0 + min(16 * grp + u0 + u, bs - 1)) * n];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int j = 16 * grp + u0 + u;
+                        const cdbl tv = j < bs ? t[u] : mk(0, 0);
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) cfma(acc[c], tv, S.xb[j * kMaxNc + c]);
                     }
                 }
             }
@@ -720,17 +728,25 @@ __device__ __forceinline__ void subst_left(const cdbl* M, int n, bool lower, cdb
             for (int q = 0; q < CP; ++q)
 #pragma unroll
                 for (int c = 0; c < NC; ++c) acc[q][c] = mk(0, 0);
-            for (int j = ja + lane; j < jb; j += 64) {
-                cdbl xj[NC];
+            // the same number of sweeps for every lane, loads at clamped
+            // addresses (values beyond the range replaced by zero afterwards)
+            const int T = (jb - ja + 63) >> 6;
+            const int qmax = bs - 1 - 4 * wv - q0;    // last column of the block for this wavefront
+            for (int tt = 0; tt < T; ++tt) {
+                const int j = ja + lane + 64 * tt, jc = min(j, jb - 1);
+                cdbl xj[NC], mv[CP];
 #pragma unroll
-                for (int c = 0; c < NC; ++c) xj[c] = w[j + (size_t)c * n];
+                for (int c = 0; c < NC; ++c) xj[c] = w[jc + (size_t)c * n];
+#pragma unroll
+                for (int q = 0; q < CP; ++q) mv[q] = M[jc + (size_t)(i0 + min(q, qmax)) * n];
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                    if (j >= jb) xj[c] = mk(0, 0);
 #pragma unroll
                 for (int q = 0; q < CP; ++q) {
-                    if (4 * wv + q0 + q < bs) {
-                        const cdbl t = cconj(M[j + (size_t)(i0 + q) * n]);
+                    const cdbl t = q <= qmax ? cconj(mv[q]) : mk(0, 0);
 #pragma unroll
-                        for (int c = 0; c < NC; ++c) cfma(acc[q][c], t, xj[c]);
-                    }
+                    for (int c = 0; c < NC; ++c) cfma(acc[q][c], t, xj[c]);
                 }
             }
 #pragma unroll
