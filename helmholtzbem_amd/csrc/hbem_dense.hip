@@ -221,13 +221,16 @@ __global__ __launch_bounds__(1024) void k_lu_panel_rg(cdbl* __restrict__ A, int 
         const int rem = jb - c0 - w;             // panel columns right of the chunk
         if (rem <= 0) break;
         __syncthreads();
+        // (loads below take clamped, in-range addresses so that they issue
+        // together; what lies outside the range is never used or stored)
         if (tid < rem) {
             cdbl* pc = a + (size_t)(cb + w + tid) * n + cb;
             cdbl x[kPanelChunk];
 #pragma unroll
+            for (int r = 0; r < kPanelChunk; ++r) x[r] = pc[min(r, w - 1)];
+#pragma unroll
             for (int r = 0; r < kPanelChunk; ++r) {
                 if (r < w) {
-                    x[r] = pc[r];
 #pragma unroll
                     for (int u = 0; u < kPanelChunk; ++u)
                         if (u < r) cfms(x[r], L11[r][u], x[u]);
@@ -237,14 +240,22 @@ __global__ __launch_bounds__(1024) void k_lu_panel_rg(cdbl* __restrict__ A, int 
             }
         }
         __syncthreads();
-        if (row < n && row >= cb + w) {
-            for (int cc = 0; cc < rem; ++cc) {
-                cdbl* p = a + row + (size_t)(cb + w + cc) * n;
-                cdbl y = *p;
+        {
+            const bool upd = row < n && row >= cb + w;
+            const int rowc = min(row, n - 1);
+            for (int cc0 = 0; cc0 < rem; cc0 += 8) {
+                cdbl y[8];
 #pragma unroll
-                for (int u = 0; u < kPanelChunk; ++u)
-                    if (u < w) cfms(y, v[u], U12[u][cc]);
-                *p = y;
+                for (int e = 0; e < 8; ++e) y[e] = a[rowc + (size_t)(cb + w + min(cc0 + e, rem - 1)) * n];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    if (cc0 + e < rem) {
+#pragma unroll
+                        for (int u = 0; u < kPanelChunk; ++u)
+                            if (u < w) cfms(y[e], v[u], U12[u][cc0 + e]);
+                        if (upd) a[row + (size_t)(cb + w + cc0 + e) * n] = y[e];
+                    }
+                }
             }
         }
         __syncthreads();
@@ -472,22 +483,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
         __syncthreads();
     }
+    // Epilogue per row block i: the eight C values (beta != 0) are loaded
+    // together from clamped (in-range) addresses, then combined and stored.
     const bool rd = beta.x != 0.0 || beta.y != 0.0;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 2; ++i) {
+        const int r = row0 + wr + 16 * i + l16, rc = min(r, m - 1);
+        cdbl cv[2][4];
+        if (rd) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int c = min(col0 + wc + 16 * j + 4 * v + lk, n - 1);
+                    cv[j][v] = C[rc + (size_t)c * ldc];
+                }
+        }
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = row0 + wr + 16 * i + l16, c = col0 + wc + 16 * j + 4 * v + lk;
+                const int c = col0 + wc + 16 * j + 4 * v + lk;
                 if (r < m && c < n) {
-                    cdbl* p = C + r + (size_t)c * ldc;
                     const double q1 = p1[i][j][v], q2 = p2[i][j][v];
                     cdbl x = alpha * mk(q1 - q2, p3[i][j][v] - q1 - q2);
-                    if (rd) x = x + beta * (*p);
-                    *p = x;
+                    if (rd) x = x + beta * cv[j][v];
+                    C[r + (size_t)c * ldc] = x;
                 }
             }
+    }
 }
 
 void zgemm_batched(int m, int n, int k, cdbl alpha, const cdbl* A, int lda, long long sa, const cdbl* B, int ldb,
