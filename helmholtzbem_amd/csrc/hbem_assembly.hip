@@ -609,21 +609,27 @@ struct AccX {
 // J0, Y0, J1, Y1 of x = kappa r from the node's kappa-free factors.
 //   y = x^2 / 64, L2 = (2/pi) ln(x/2) = lk + lr, 1/x = kinv / r,
 //   1/sqrt(pi x) = ksq / sqrt(r)
-// TINY (touching panel pairs, whose arguments are small): x < HBEM_TINY_X
+// TINY >= 1 (touching panel pairs, whose arguments are small): x < HBEM_TINY_X
 // takes the entire parts from the degree-HBEM_TINY_DEG table fitted on that
-// range instead of the degree-HBEM_SMALL_DEG table of (0, 8).
+// range instead of the degree-HBEM_SMALL_DEG table of (0, 8); TINY = 2
+// (coinciding panels) takes x < HBEM_TINY1_X from the degree-HBEM_TINY1_DEG
+// table of that range first.
 // sin / cos beyond x = 1e6 (never reached by the benchmark's arguments): kept
 // out of line so the library routine's constants do not occupy registers of
 // the callers' loops
 __device__ __noinline__ void sincos_far(double x, double& s, double& c) { sincos(x, &s, &c); }
 
-template <bool TINY>
+template <int TINY>
 __device__ __forceinline__ void bessel_jy01_f(double x, double y, double L2, double xinv, double b, double& j0,
                                               double& y0, double& j1, double& y1) {
     if (x < 8.0) {
         double A, B, C, Dd;
-        if (TINY && x < (double)HBEM_TINY_X) horner4<HBEM_TINY_DEG>(bessel_table() + HBEM_TAB_TINY, y, A, B, C, Dd);
-        else horner4<HBEM_SMALL_DEG>(bessel_table() + HBEM_TAB_SMALL, y, A, B, C, Dd);
+        if (TINY >= 2 && x < (double)HBEM_TINY1_X)
+            horner4<HBEM_TINY1_DEG>(bessel_table() + HBEM_TAB_TINY1, y, A, B, C, Dd);
+        else if (TINY >= 1 && x < (double)HBEM_TINY_X)
+            horner4<HBEM_TINY_DEG>(bessel_table() + HBEM_TAB_TINY, y, A, B, C, Dd);
+        else
+            horner4<HBEM_SMALL_DEG>(bessel_table() + HBEM_TAB_SMALL, y, A, B, C, Dd);
         const bool pos = x > 0.0;
         const double J1 = x * B;
         j0 = pos ? A : 1.0;
@@ -823,8 +829,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 const double r = Rown[0];
                 const double x = kap * r;
                 double j0, y0, j1, y1;
-                bessel_jy01_f<CLS != 2>(x, x * x * (1.0 / 64.0), lk + Rown[3], kinv * Rown[2], ksq * Rown[4], j0, y0, j1,
-                                        y1);
+                bessel_jy01_f<CLS == 2 ? 0 : (CLS == 1 ? 1 : 2)>(x, x * x * (1.0 / 64.0), lk + Rown[3], kinv * Rown[2],
+                                                                 ksq * Rown[4], j0, y0, j1, y1);
                 double a0, a1, b0, b1, c0, c1, d0, d1;
                 halves(j0, a0, a1);
                 halves(y0, b0, b1);

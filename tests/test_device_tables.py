@@ -91,13 +91,27 @@ def device_jy01_touching(x, defs, c):
     return A, fma(L2, A, C), J1, fma(L2, J1, fma(x, D, -two_pi * (1.0 / x)))
 
 
+def device_jy01_coinciding(x, defs, c):
+    """As the assembly kernel of coinciding panel pairs computes them
+    (bessel_jy01_f<2>): below HBEM_TINY1_X from the degree-HBEM_TINY1_DEG table."""
+    if x >= defs["HBEM_TINY1_X"]:
+        return device_jy01_touching(x, defs, c)
+    two_pi = 2.0 / np.pi
+    y = (0.125 * x) ** 2
+    A, B, C, D = horner4(c, defs["HBEM_TAB_TINY1"], defs["HBEM_TINY1_DEG"], y)
+    L2 = two_pi * np.log(0.5 * x)
+    J1 = x * B
+    return A, fma(L2, A, C), J1, fma(L2, J1, fma(x, D, -two_pi * (1.0 / x)))
+
+
 def test_table_layout():
     defs, c = _table()
     assert defs["HBEM_TAB_SMALL"] == 0
     assert defs["HBEM_TAB_PQ"] == 4 * (defs["HBEM_SMALL_DEG"] + 1)
     assert defs["HBEM_TAB_SC"] == defs["HBEM_TAB_PQ"] + 4 * (defs["HBEM_PQ_DEG"] + 1)
     assert defs["HBEM_TAB_TINY"] == defs["HBEM_TAB_SC"] + 4 + 2 * (defs["HBEM_SC_DEG"] + 1)
-    assert defs["HBEM_TAB_SIZE"] == defs["HBEM_TAB_TINY"] + 4 * (defs["HBEM_TINY_DEG"] + 1)
+    assert defs["HBEM_TAB_TINY1"] == defs["HBEM_TAB_TINY"] + 4 * (defs["HBEM_TINY_DEG"] + 1)
+    assert defs["HBEM_TAB_SIZE"] == defs["HBEM_TAB_TINY1"] + 4 * (defs["HBEM_TINY1_DEG"] + 1)
     T = defs["HBEM_TAB_SC"]
     assert c[T] == 2.0 / np.pi and c[T + 1] == np.pi / 2
     # the three parts sum to pi/2 far beyond double precision
@@ -149,11 +163,13 @@ def test_device_bessel_touching_pairs_vs_mpmath():
         for name, g in zip(("J0", "Y0", "J1", "Y1"), got):
             ref = float(r[name])
             assert abs(g - ref) <= 2e-15 * max(abs(ref), 1.0), (name, x, g, ref)
-    # dense sweep of (0, HBEM_TINY_X) against mpmath directly
+    # dense sweep of (0, HBEM_TINY_X) against mpmath directly, for both the
+    # adjacent (x < 2 table) and the coinciding (x < 1 table first) forms
     mp.mp.dps = 30
-    for x in np.concatenate([np.geomspace(1e-5, 1.0, 25), np.linspace(1.0, xt * 0.9999, 25)]):
+    for x in np.concatenate([np.geomspace(1e-5, 1.0, 25), np.linspace(0.5, xt * 0.9999, 25)]):
         x = float(x)
-        got = device_jy01_touching(x, defs, c)
         ref = [mp.besselj(0, x), mp.bessely(0, x), mp.besselj(1, x), mp.bessely(1, x)]
-        for g, e in zip(got, ref):
-            assert abs(g - float(e)) <= 2e-15 * max(abs(float(e)), 1.0), (x, g, e)
+        for fn in (device_jy01_touching, device_jy01_coinciding):
+            got = fn(x, defs, c)
+            for g, e in zip(got, ref):
+                assert abs(g - float(e)) <= 2e-15 * max(abs(float(e)), 1.0), (fn.__name__, x, g, e)
