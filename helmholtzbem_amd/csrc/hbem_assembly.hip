@@ -611,9 +611,8 @@ struct AccX {
 //   1/sqrt(pi x) = ksq / sqrt(r)
 // TINY >= 1 (touching panel pairs, whose arguments are small): x < HBEM_TINY_X
 // takes the entire parts from the degree-HBEM_TINY_DEG table fitted on that
-// range instead of the degree-HBEM_SMALL_DEG table of (0, 8); TINY = 2
-// (coinciding panels) takes x < HBEM_TINY1_X from the degree-HBEM_TINY1_DEG
-// table of that range first.
+// range instead of the degree-HBEM_SMALL_DEG table of (0, 8); TINY = 2 takes
+// x < HBEM_TINY1_X from the degree-HBEM_TINY1_DEG table of that range first.
 // sin / cos beyond x = 1e6 (never reached by the benchmark's arguments): kept
 // out of line so the library routine's constants do not occupy registers of
 // the callers' loops
@@ -829,8 +828,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 const double r = Rown[0];
                 const double x = kap * r;
                 double j0, y0, j1, y1;
-                bessel_jy01_f<CLS == 2 ? 0 : (CLS == 1 ? 1 : 2)>(x, x * x * (1.0 / 64.0), lk + Rown[3], kinv * Rown[2],
-                                                                 ksq * Rown[4], j0, y0, j1, y1);
+                bessel_jy01_f<CLS == 2 ? 0 : 2>(x, x * x * (1.0 / 64.0), lk + Rown[3], kinv * Rown[2], ksq * Rown[4],
+                                                j0, y0, j1, y1);
                 double a0, a1, b0, b1, c0, c1, d0, d1;
                 halves(j0, a0, a1);
                 halves(y0, b0, b1);
