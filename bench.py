@@ -74,8 +74,8 @@ def main():
         print(json.dumps({"copy_calibration_bytes": n * 8, "dispatches": 5}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sigma_min(k) sweep would shard over wavenumbers; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(the sigma_min(k) sweep would shard over wavenumbers)", file=sys.stderr)
         sys.exit(2)
 
     import helmholtzbem_amd as H

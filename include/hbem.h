@@ -39,7 +39,7 @@ extern "C" {
 /* error codes */
 #define HBEM_OK 0
 #define HBEM_EINVAL (-1)     /* invalid argument (reference: std::runtime_error / out_of_range) */
-#define HBEM_ENOTSUP (-2)    /* not supported by this build (e.g. submatrix assembly) */
+#define HBEM_ENOTSUP (-2)    /* not supported by this library (e.g. submatrix assembly) */
 #define HBEM_EDEVICE (-3)    /* HIP runtime error */
 #define HBEM_ENOMEM (-4)     /* allocation failed */
 #define HBEM_ENUMERIC (-5)   /* numerical failure of a solve (reference: std::runtime_error) */

@@ -1,4 +1,4 @@
-// Reference-style driver on the C++ facade: the square of configs[0]
+// Reference-style driver on the C++ facade: the 48-panel square of the reference's small CPU case
 // (roots_brent_square_rsvd.cpp:79-171 pattern) -- mesh from ParametrizedLine::split,
 // BuilderData, SolutionsOperator::gen_sol_op, randomized_svd::sv; also
 // GalerkinBuilder getters and gen_sol_op_1st_der + sv_der.

@@ -1,6 +1,6 @@
 """The C++ facade (include/hbem/hbem.hpp) compiles against the C-ABI library
 with g++ and, on the GPU, reproduces the oracle through a reference-style
-driver (configs[0] square)."""
+driver (the 48-panel square of the reference's small CPU case)."""
 import os
 import subprocess
 

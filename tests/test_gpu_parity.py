@@ -19,7 +19,7 @@ def rel(a, b):
 
 
 CASES = {
-    # square of side 0.5 centred at the origin, 12 line panels per side, order 11
+    # the reference's small CPU case: square of side 0.5 centred at the origin, 12 line panels per side, order 11
     "square48_o11": lambda H: (H.square_mesh(0.5, 12), 11),
     "square_odd_N": lambda H: (H.polygon_mesh([(0, 0), (1, 0), (0.3, 0.8)], 13), 7),
     "circle_arcs": lambda H: (H.circle_mesh(16, 0.7, (0.2, -0.1)), 6),
@@ -191,8 +191,8 @@ def test_sigma_sweep_parity(case, H, O):
     assert abs(dsig_c[0] - so_d[1]) <= 1e-8 * max(1.0, abs(so_d[1]))
 
 
-def test_config0_sigma_exact(H, O):
-    """48-panel square, k=2, c_i=20, c_o=1, order 11, 2 random
+def test_square48_sigma_exact(H, O):
+    """The reference's small CPU case: 48-panel square, k=2, c_i=20, c_o=1, order 11, 2 random
     columns, 2 power iterations: sigma_min vs the oracle with the same R."""
     mesh = H.square_mesh(0.5, 12)
     sp = H.ContinuousSpace1()
