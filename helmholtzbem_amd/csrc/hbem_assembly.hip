@@ -762,10 +762,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const int h = lane >> 5, j = lane & 31;
     const int nqb = (P.nk + 31) >> 5;        // blocks of 32 slots
     const int ngrp = (nqb + 3) >> 2;         // groups of 4 blocks = one workgroup
-    const int grp = blockIdx.x % ngrp;
-    const int rest = blockIdx.x / ngrp;
-    const int chunk = rest % P.nchunk;
-    const int e = rest / P.nchunk;           // < nent by the grid size
+    // Workgroups are dealt to the eight XCDs in turn (id % 8); the grid is
+    // renumbered so that each XCD takes a contiguous run of (pair, chunk, slot
+    // group) indices: the four slot groups of a node chunk then share the
+    // chunk's node table in that XCD's L2 instead of each fetching it from HBM.
+    const long long total = (long long)P.nent * P.nchunk * ngrp;
+    const long long per = (total + 7) >> 3;
+    const long long li = (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (li >= total) return;                 // whole workgroup, before any barrier
+    const int grp = (int)(li % ngrp);
+    const long long rest = li / ngrp;
+    const int chunk = (int)(rest % P.nchunk);
+    const int e = (int)(rest / P.nchunk);
     const int qb = grp * 4 + wv;
     int q = qb * 32 + j;
     const bool active = qb < nqb && q < P.nk;
@@ -1116,7 +1124,7 @@ static void launch_asm_g(const AsmParams& P, int G, hipStream_t st) {
     if (G == 0) {
         // paired-lane kernel: one workgroup per (pair, chunk, 4 blocks of 32 slots)
         const long long groups = ((P.nk + 31) / 32 + 3) / 4;
-        const long long blocks = (long long)P.nent * P.nchunk * groups;
+        const long long blocks = 8 * (((long long)P.nent * P.nchunk * groups + 7) / 8);   // a multiple of the XCD count
         hipLaunchKernelGGL((k_assemble_x2<CLS, D>), dim3((unsigned)blocks), dim3(256), 0, st, P);
         return;
     }
