@@ -32,7 +32,7 @@ PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
 # rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (KiB per dispatch, scaled by
 # the counters' reading of a plain 1 GiB copy), from
 # profiles/k_assemble_x2_general_star400_hbm_traffic.txt
-GENERAL_KERNEL_TRAFFIC_BYTES = 3.7082e10
+GENERAL_KERNEL_TRAFFIC_BYTES = 3.2686e10
 N_PANELS, ORDER, C_I, C_O, NK, K_MIN, K_MAX, NC, Q = 400, 16, 16.0, 1.0, 256, 1.0, 10.0, 2, 2
 
 
