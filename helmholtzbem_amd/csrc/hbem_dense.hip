@@ -3,12 +3,15 @@
 // Restates the Eigen calls of randomized_svd::sv (randsvd.cpp:99-146) for the
 // GPU, batched over independent matrices (one per wavenumber):
 //   * PartialPivLU (randsvd.cpp:104): right-looking blocked LU with partial
-//     pivoting (largest modulus, first index on ties); panel factorisation by
-//     one workgroup per matrix, row interchanges, triangular solve of the
-//     U12 block, trailing update by a tiled complex GEMM;
-//   * lu.solve / lu.adjoint().solve on thin right-hand sides (n x nc): blocked
-//     substitution, the 64 x 64 diagonal blocks by one wavefront with
-//     cross-lane broadcasts, the off-diagonal updates by the whole workgroup;
+//     pivoting (largest modulus, first index on ties) in 64-column steps of two
+//     32-column sub-panels; panel factorisation by one workgroup per matrix
+//     with the panel rows in registers, row interchanges, triangular solve of
+//     the U12 block, trailing update by a complex GEMM on the FP64 matrix cores;
+//   * lu.solve / lu.adjoint().solve on thin right-hand sides (n x nc), reading
+//     the factors in place: blocked substitution, the 64 x 64 diagonal blocks
+//     by one wavefront with cross-lane broadcasts, the off-diagonal updates by
+//     the whole workgroup;
+//   * the projection factor's thin products in compressed-row form;
 //   * HouseholderQR's role (an orthonormal basis of the span): modified
 //     Gram-Schmidt applied twice;
 //   * JacobiSVD's role (largest singular value of the thin n x nc matrix):
