@@ -18,13 +18,18 @@
 // (2/pi) ln r, 1/sqrt(r)}, so the per-wavenumber kernels only evaluate the
 // Hankel functions and accumulate.
 //
-// Work item = (pair, node chunk, wavenumber).  G lanes of a wavefront
-// cooperate on one item, striding over its nodes, and reduce with cross-lane
-// shuffles; 64/G items share a wavefront.  G = 1 (one lane per wavenumber, no
-// reduction; the pair and chunk are then wavefront-uniform) serves batched
-// k-sweeps, G = 16/64 single wavenumbers.  The singular classes split their
-// Nc^2 / 2 Nc^2 nodes into chunks whose partial local matrices are summed in
-// chunk order by k_chunk_sum.
+// Two kernel families:
+//   * k_assemble_x2 (batches of >= 64 wavenumber slots, the k-sweeps): a
+//     wavefront takes one (pair, node chunk) and 32 slots, its two lane halves
+//     evaluating the Hankel functions of two nodes and sharing them, with the
+//     row-constant shape functions of general and adjacent pairs folded once
+//     per row of the quadrature rule (see the comments at the kernel);
+//   * k_assemble (single wavenumbers): work item = (pair, node chunk,
+//     wavenumber); G lanes of a wavefront cooperate on one item, striding over
+//     its nodes, and reduce with cross-lane shuffles; 64/G items share a
+//     wavefront.
+// The singular classes split their Nc^2 / 2 Nc^2 nodes into chunks whose
+// partial local matrices are summed in chunk order by k_chunk_sum.
 //
 // Local slot layout per derivative order t (16 complex):
 //   [0..3] K direct (test on panel i, trial on j), [4..7] K transposed block,
