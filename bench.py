@@ -1,18 +1,21 @@
 """Benchmark of the sigma_min(k) path on one MI355X.
 
-Workload (the configuration the metric is quoted on): smooth star curve
-r(t) = 1 + 0.25 cos 5t split into 400 equal-parameter panels (800 x 800 complex
-transmission matrix), c_i = 16, c_o = 1, Gauss order 16, 256 real k =
-linspace(1, 10); per k the operators are assembled with their first and second
-k-derivatives (FP64), T, T', T'' are formed, sigma_min is estimated by the
-randomized SVD (2 Gaussian columns, 2 subspace iterations) and dsigma/dk is
-evaluated from the singular pair.
+Headline workload ("polygon2000"): a random star-shaped 12-vertex polygon
+(seed 3; angles sorted uniform in [0, 2 pi), radii uniform in [0.5, 1.5])
+split into 2000 line panels in proportion to edge length (a 4000 x 4000
+complex transmission matrix), c_i = 20, c_o = 1, Gauss order 11, 128 real k
+= linspace(5, 40).  Per k: V, K, W for both refraction indices, T, and
+sigma_min by the randomized SVD (2 Gaussian columns, 2 subspace iterations,
+projected as SolutionsOperator does).  One step = one pass over the 128 k.
 
-One step = one pass over the 256-point k grid.  value = sigma_min(k)
-evaluations per second over the timed steps (inputs -- geometry, quadrature,
-projection factor, test matrix -- are resident on the device before timing).
+Second workload ("star400", reported under the key "star400"): the smooth
+star r(t) = 1 + 0.25 cos 5t in 400 equal-parameter panels (800 x 800), c_i =
+16, c_o = 1, order 16, 256 k = linspace(1, 10), operators with first and
+second k-derivatives, sigma_min and dsigma/dk per k.
 
-Prints one JSON line.
+value = sigma_min(k) evaluations per second over the timed steps, inputs
+(geometry, quadrature, projection factor, test matrix) resident on the device
+before timing.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -28,28 +31,118 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
-# HBM bytes per launch of the general-pair assembly kernel at this workload:
-# rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (KiB per dispatch, scaled by
-# the counters' reading of a plain 1 GiB copy), from
-# profiles/k_assemble_x2_general_star400_hbm_traffic.txt
-GENERAL_KERNEL_TRAFFIC_BYTES = 3.2686e10
-N_PANELS, ORDER, C_I, C_O, NK, K_MIN, K_MAX, NC, Q = 400, 16, 16.0, 1.0, 256, 1.0, 10.0, 2, 2
+
+# HBM bytes per launch of the general-pair assembly kernel (rocprofv3
+# FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, KiB per dispatch, scaled by the
+# counters' reading of a plain 1 GiB copy): profiles/*_hbm_traffic.txt
+TRAFFIC_BYTES = {"star400": 3.2686e10, "polygon2000": None}
+
+WORKLOADS = {
+    "polygon2000": dict(panels=2000, order=11, c_i=20.0, c_o=1.0, nk=128, k_min=5.0, k_max=40.0, der=0, seed=3,
+                        nvert=12,
+                        text="random 12-vertex polygon (seed 3), 2000 line panels (4000x4000 complex T), c_i=20, "
+                             "c_o=1, Gauss order 11, 128 k in linspace(5,40), sigma_min per k, rSVD nc=2 q=2, FP64"),
+    "star400": dict(panels=400, order=16, c_i=16.0, c_o=1.0, nk=256, k_min=1.0, k_max=10.0, der=2, seed=1,
+                    text="star r(t)=1+0.25cos5t, 400 panels (800x800 complex T), c_i=16, c_o=1, Gauss order 16, "
+                         "256 k in linspace(1,10), d=2 operators, sigma_min + dsigma/dk per k, rSVD nc=2 q=2, FP64"),
+}
+NC, Q = 2, 2
+PHASES = ("assembly_general", "assembly_adjacent", "assembly_coinciding", "lu", "form_T", "rsvd")
 
 
-def cpu_baseline(mesh, R, threads):
-    """The oracle (CPU FP64 restatement) on a bounded sample: one k of the
-    workload -- gen_sol_op with d = 2 (both refraction indices, T, T', T''
-    projected as the reference does) plus sv -- on `threads` host threads."""
+def make_mesh(H, w):
+    if w.get("nvert") is None:
+        return H.star_mesh(w["panels"])
+    return H.random_polygon_mesh(w["seed"], w["nvert"], w["panels"])
+
+
+def cpu_baseline(name, w, mesh, R, threads):
+    """The oracle (CPU FP64 restatement, "port") on a bounded sample of the
+    workload: one k (the middle of the grid).  Assembly (gen_sol_op, both
+    refraction indices, unprojected) runs on `threads` threads; the dense
+    stage (LU + sv) runs single-threaded as the reference's does per k.
+    Throughput of `threads` cores running different k at once (the
+    reference drivers' parallel loop over k) = threads / (threads x t_asm +
+    t_dense) evaluations per second."""
     from oracle import oracle as O
-    S = O.SolOp(mesh, ORDER)
-    k = 0.5 * (K_MIN + K_MAX)
+    S = O.SolOp(mesh, w["order"], proj=False)
+    k = 0.5 * (w["k_min"] + w["k_max"])
     t0 = time.perf_counter()
-    Ts = S.gen_sol_op(k, C_O, C_I, d=2, nthreads=threads)
-    O.sv(Ts[0], R, Q)
-    dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "sigma_min(k) evals/s", "cores": threads, "kind": "port",
-            "sample": "1 k of the workload (k=5.5): gen_sol_op d=2 (c_i and c_o assembly, T/T'/T'' projected) "
-                      "+ sv; dsigma/dk not included; %.1f s" % dt}
+    T = S.gen_sol_op(k, w["c_o"], w["c_i"], d=0, nthreads=threads)[0]
+    t_asm = time.perf_counter() - t0
+    n = T.shape[0]
+    if n > 2000:
+        # the leading n/2 block, timed and scaled by 8 (LU and solves grow as n^3 / n^2)
+        h = n // 2
+        Th = np.asfortranarray(T[:h, :h])
+        t0 = time.perf_counter()
+        O.sv(Th, np.asfortranarray(R[:h]), Q)
+        t_dense = 8.0 * (time.perf_counter() - t0)
+        dense_txt = "LU + sv (q=2) of the leading %dx%d block of T on 1 thread, x8 for n^3" % (h, h)
+    else:
+        t0 = time.perf_counter()
+        O.sv(T, R, Q)
+        t_dense = time.perf_counter() - t0
+        dense_txt = "LU + sv (q=2) on 1 thread"
+    value = threads / (threads * t_asm + t_dense)
+    return {"value": value, "unit": "sigma_min(k) evals/s", "cores": threads, "kind": "port",
+            "sample": "1 k of %s (k=%g): gen_sol_op d=0 (c_i and c_o assembly, T) on %d threads %.2f s; %s "
+                      "%.1f s; the reference's explicit projection F^-1 T F^-T (16n^3) not timed; value = cores / "
+                      "(cores x t_asm + t_dense)" % (name, k, threads, t_asm, dense_txt, t_dense)}
+
+
+def run_workload(H, name, steps, warmup):
+    w = WORKLOADS[name]
+    mesh = make_mesh(H, w)
+    sp = H.ContinuousSpace1()
+    bd = H.BuilderData(mesh, sp, sp, w["order"])
+    N = w["panels"]
+    n = 2 * N
+    ks = np.linspace(w["k_min"], w["k_max"], w["nk"])
+    R = H.gaussian_test_matrix(n, NC, w["seed"])
+    der = w["der"]
+
+    def sweep():
+        return H.sigma_sweep(bd, ks, w["c_o"], w["c_i"], R, Q, der=der)
+
+    for _ in range(warmup):
+        sweep()
+    H.profile_enable(bd, True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = sweep()            # synchronous: returns after its stream drained
+    elapsed = time.perf_counter() - t0
+    prof = [H.profile_query(bd, i) for i in range(len(PHASES))]
+    H.profile_enable(bd, False)
+    sig = res[0] if der else res
+    if not (np.all(np.isfinite(sig)) and np.all(sig > 0)):
+        raise RuntimeError("non-finite or non-positive sigma in the %s sweep" % name)
+    if der and not np.all(np.isfinite(res[1])):
+        raise RuntimeError("non-finite dsigma/dk in the %s sweep" % name)
+    evals = w["nk"] * steps
+    asm_s = sum(p[0] for p in prof[:3]) * 1e-3
+    pairs_per_k = 2 * N * (N + 1) // 2
+    ms_gen, n_gen, fl_gen = prof[0]
+    avg_launch_s = ms_gen * 1e-3 / max(n_gen, 1)
+    achieved = (fl_gen / max(n_gen, 1)) / avg_launch_s / 1e12 if n_gen else 0.0
+    ms_lu, n_lu, fl_lu = prof[3]
+    lu_tflops = fl_lu / (ms_lu * 1e-3) / 1e12 if ms_lu > 0 else None
+    out = {
+        "value": evals / elapsed,
+        "ms_per_step": elapsed / steps * 1e3,
+        "config": {"workload": w["text"], "panels": N, "order": w["order"], "nk": w["nk"], "global_batch": w["nk"],
+                   "parallelism": "single-gpu"},
+        "panel_pair_evals_per_s": pairs_per_k * evals / asm_s if asm_s > 0 else None,
+        "phase_seconds_per_step": {ph: p[0] * 1e-3 / steps for ph, p in zip(PHASES, prof)},
+        "lu_model_tflops": lu_tflops,
+        "roofline": {"kernel": "k_assemble_x2<general, d=%d> (general panel pairs, %d k x 2 c per launch)"
+                               % (der, w["nk"]),
+                     "bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES[name],
+                     "avg_launch_ms": avg_launch_s * 1e3, "launches": n_gen,
+                     "flops_per_launch": fl_gen / max(n_gen, 1)},
+    }
+    return out, mesh, R
 
 
 def main():
@@ -57,6 +150,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="polygon2000")
+    ap.add_argument("--no-second", action="store_true", help="skip the star400 sweep")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -80,66 +175,33 @@ def main():
 
     import helmholtzbem_amd as H
 
-    mesh = H.star_mesh(N_PANELS)
-    sp = H.ContinuousSpace1()
-    bd = H.BuilderData(mesh, sp, sp, ORDER)
-    n = 2 * N_PANELS
-    ks = np.linspace(K_MIN, K_MAX, NK)
-    R = H.gaussian_test_matrix(n, NC, 1)
-
-    for _ in range(args.warmup):
-        H.sigma_sweep(bd, ks, C_O, C_I, R, Q, der=2)
-    H.profile_enable(bd, True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sig, dsig = H.sigma_sweep(bd, ks, C_O, C_I, R, Q, der=2)   # synchronous: returns after its stream drained
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    ms_gen, n_gen, fl_gen = H.profile_query(bd, 0)
-    ms_adj, n_adj, fl_adj = H.profile_query(bd, 1)
-    ms_coi, n_coi, fl_coi = H.profile_query(bd, 2)
-    ms_lu, n_lu, fl_lu = H.profile_query(bd, 3)
-    H.profile_enable(bd, False)
-    if not (np.all(np.isfinite(sig)) and np.all(np.isfinite(dsig)) and np.all(sig > 0)):
-        raise RuntimeError("non-finite or non-positive sigma in the benchmark sweep")
-
-    evals = NK * args.steps
-    value = evals / elapsed
-    asm_s = (ms_gen + ms_adj + ms_coi) * 1e-3
-    pairs_per_k = 2 * N_PANELS * (N_PANELS + 1) // 2
-    avg_launch_s = ms_gen * 1e-3 / max(n_gen, 1)
-    achieved = (fl_gen / max(n_gen, 1)) / avg_launch_s / 1e12 if n_gen else 0.0
+    head, mesh, R = run_workload(H, args.workload, args.steps, args.warmup)
     out = {
         "metric": "sigma_min(k) evals/s",
-        "value": value,
+        "value": head["value"],
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "star r(t)=1+0.25cos5t, 400 panels (800x800 complex T), c_i=16, c_o=1, "
-                               "Gauss order 16, 256 k in linspace(1,10), d=2 operators, sigma_min + dsigma/dk per k, "
-                               "rSVD nc=2 q=2, FP64",
-                   "panels": N_PANELS, "order": ORDER, "nk": NK, "global_batch": NK, "parallelism": "single-gpu"},
-        "panel_pair_evals_per_s": pairs_per_k * evals / asm_s if asm_s > 0 else None,
-        "phase_seconds_per_step": {"assembly_general": ms_gen * 1e-3 / args.steps,
-                                   "assembly_adjacent": ms_adj * 1e-3 / args.steps,
-                                   "assembly_coinciding": ms_coi * 1e-3 / args.steps,
-                                   "lu": ms_lu * 1e-3 / args.steps},
-        "roofline": {"kernel": "k_assemble_x2<general, d=2> (general panel pairs, 256 k x 2 c per launch)",
-                     "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": GENERAL_KERNEL_TRAFFIC_BYTES,
-                     "avg_launch_ms": avg_launch_s * 1e3, "launches": n_gen,
-                     "flops_per_launch": fl_gen / max(n_gen, 1)},
+        "config": dict(head["config"], name=args.workload),
+        "panel_pair_evals_per_s": head["panel_pair_evals_per_s"],
+        "phase_seconds_per_step": head["phase_seconds_per_step"],
+        "lu_model_tflops": head["lu_model_tflops"],
+        "roofline": head["roofline"],
     }
+    other = "star400" if args.workload == "polygon2000" else "polygon2000"
+    if not args.no_second:
+        sec, _, _ = run_workload(H, other, max(1, args.steps), 1)
+        out[other] = sec
     if not args.no_cpu_baseline:
         threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
-        out["cpu_baseline"] = cpu_baseline(mesh, R, threads)
+        out["cpu_baseline"] = cpu_baseline(args.workload, WORKLOADS[args.workload], mesh, R, threads)
     print(json.dumps(out), flush=True)
 
 

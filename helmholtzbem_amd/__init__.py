@@ -330,6 +330,37 @@ class randomized_svd:
         return gaussian_test_matrix(nr, nc, seed)
 
     @staticmethod
+    def benchmarking(yes):
+        """randsvd.cpp:27-30: record the phase timers of sv / sv_der / sv_der2."""
+        check(lib().hbem_rsvd_benchmarking(1 if yes else 0))
+
+    @staticmethod
+    def _timers():
+        out = np.zeros(4)
+        check(lib().hbem_rsvd_timers(_dptr(out)))
+        return out
+
+    @staticmethod
+    def get_lu_time():
+        return float(randomized_svd._timers()[0])
+
+    @staticmethod
+    def get_qr_time():
+        return float(randomized_svd._timers()[1])
+
+    @staticmethod
+    def get_sub_iter_time():
+        return float(randomized_svd._timers()[2])
+
+    @staticmethod
+    def get_svd_time():
+        return float(randomized_svd._timers()[3])
+
+    @staticmethod
+    def reset_timer():
+        check(lib().hbem_rsvd_reset_timer())
+
+    @staticmethod
     def sv(T, R, q=0):
         T = np.asfortranarray(T, dtype=np.complex128)
         R = np.asfortranarray(R, dtype=np.complex128)
@@ -411,6 +442,15 @@ def sigma_sweep_multi(instances, q=2, der=0, proj=True, max_workers=None):
     with ThreadPoolExecutor(max_workers=workers) as ex:
         futs = [ex.submit(sigma_sweep, bd, ks, c_o, c_i, R, q, der, proj) for bd, ks, c_o, c_i, R in items]
         return [f.result() for f in futs]
+
+
+def sweep_batch_size(npanels, order, d, nc, avail_bytes, nk=0):
+    """Wavenumbers per batch of sigma_sweep for a closed curve of `npanels`
+    panels when `avail_bytes` of device memory are usable (host-only)."""
+    B = ctypes.c_int()
+    check(lib().hbem_sweep_batch_size(int(npanels), int(order), int(d), int(nc), int(avail_bytes), int(nk),
+                                      ctypes.byref(B)))
+    return B.value
 
 
 def profile_enable(builder_data: BuilderData, on=True, proj=True):

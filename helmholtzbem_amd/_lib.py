@@ -18,7 +18,8 @@ EXPORTED = [
     "hbem_last_error", "hbem_init", "hbem_geometry_create", "hbem_geometry_destroy", "hbem_geometry_info",
     "hbem_assemble_dense", "hbem_mass_matrix", "hbem_projection_factor", "hbem_gen_sol_op", "hbem_projection",
     "hbem_rsvd_sv", "hbem_rsvd_sv_der", "hbem_rsvd_sv_der2", "hbem_sigma_sweep", "hbem_profile_enable",
-    "hbem_profile_query", "hbem_assemble_dense_batch", "hbem_tp_A_rhs", "hbem_tp_solve",
+    "hbem_profile_query", "hbem_assemble_dense_batch", "hbem_tp_A_rhs", "hbem_tp_solve", "hbem_sweep_batch_size",
+    "hbem_rsvd_benchmarking", "hbem_rsvd_timers", "hbem_rsvd_reset_timer",
 ]
 
 _lib = None
@@ -57,6 +58,10 @@ def lib():
         L.hbem_profile_query.argtypes = [vp, i, dp, ctypes.POINTER(ctypes.c_longlong), dp]
         L.hbem_tp_A_rhs.argtypes = [vp, d, d, d, d, vp, i, vp, vp]
         L.hbem_tp_solve.argtypes = [vp, d, d, d, d, vp, i, vp]
+        L.hbem_sweep_batch_size.argtypes = [i, i, i, i, ctypes.c_ulonglong, i, ip]
+        L.hbem_rsvd_benchmarking.argtypes = [i]
+        L.hbem_rsvd_timers.argtypes = [dp]
+        L.hbem_rsvd_reset_timer.argtypes = []
         _lib = L
     return _lib
 

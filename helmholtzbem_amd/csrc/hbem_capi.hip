@@ -79,6 +79,102 @@ struct DCsr {
 // Bessel) and accumulation for P1 x P1, three layers, derivative order d.
 double flops_per_node(int d) { return 184.0 + (d == 0 ? 108.0 : d == 1 ? 192.0 : 314.0); }
 
+// Timed sections on a stream.  begin/end record a pair of events around
+// device work; nothing waits for them until resolve(), which the caller runs
+// after the timed work (so timing does not add host synchronisations to it).
+struct EventLog {
+    static constexpr int kSlots = 8;
+    struct Rec { int slot; hipEvent_t a, b; double flops; };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[kSlots] = {};
+    long long n[kSlots] = {};
+    double flops[kSlots] = {};
+    hipEvent_t take() {
+        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t e = nullptr;
+        HIPC(hipEventCreate(&e));
+        return e;
+    }
+    hipEvent_t begin(hipStream_t st) {
+        hipEvent_t e = take();
+        HIPC(hipEventRecord(e, st));
+        return e;
+    }
+    void end(hipStream_t st, int slot, hipEvent_t a, double fl = 0.0) {
+        hipEvent_t b = take();
+        HIPC(hipEventRecord(b, st));
+        pending.push_back({slot, a, b, fl});
+    }
+    void resolve() {
+        for (auto& r : pending) {
+            HIPC(hipEventSynchronize(r.b));
+            float t = 0.f;
+            HIPC(hipEventElapsedTime(&t, r.a, r.b));
+            ms[r.slot] += t;
+            n[r.slot] += 1;
+            flops[r.slot] += r.flops;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        pending.clear();
+    }
+    void reset() {
+        resolve();
+        for (int i = 0; i < kSlots; ++i) { ms[i] = 0; n[i] = 0; flops[i] = 0; }
+    }
+    ~EventLog() {
+        for (auto& r : pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// Profile slots of a geometry context (hbem_profile_query)
+enum { kProfGeneral = 0, kProfAdjacent = 1, kProfCoinciding = 2, kProfLU = 3, kProfFormT = 4, kProfRsvd = 5 };
+// randomized_svd timer slots (randsvd.cpp:27-47): LU, first solve + QR, subspace iterations, last solve + SVD
+enum { kTimLU = 0, kTimQR = 1, kTimIter = 2, kTimSVD = 3 };
+
+// Number of node chunks of the paired-lane kernel per singular pair (chunks of
+// 2048 nodes; adjacent pairs chunk each of their two node halves on its own).
+constexpr int kChunkNodes = 2048;
+long long adj_chunks(int nc) { return 2 * (((long long)nc * nc + kChunkNodes - 1) / kChunkNodes); }
+long long coi_chunks(int nc) { return ((long long)nc * nc + kChunkNodes - 1) / kChunkNodes; }
+
+// Device bytes a sigma sweep needs per wavenumber of a batch: local matrices
+// and chunk partials of two refraction indices, T (factorised in place), T',
+// T'', pivots and the thin blocks of the randomized SVD.
+size_t sweep_bytes_per_k(long long npairs, long long nadj, long long ncoi, int nc_rule, int n, int D, int ncols) {
+    const size_t nslot = 16 * (size_t)(D + 1);
+    const size_t part = (size_t)std::max(nadj * adj_chunks(nc_rule), ncoi * coi_chunks(nc_rule));
+    const size_t nn = (size_t)n * n;
+    return 2 * (size_t)npairs * nslot * sizeof(cdbl) + 2 * part * nslot * sizeof(cdbl) + (size_t)(D + 1) * nn * sizeof(cdbl) +
+           2 * (size_t)n * sizeof(int) + (4 * (size_t)ncols + 4) * n * sizeof(cdbl) + 64;
+}
+
+// Wavenumbers per sweep batch: as many as fit in `avail` bytes, at most 256
+// (the dense stage then has about one matrix per CU; more only adds latency),
+// and balanced so the batches of nk wavenumbers have equal sizes.
+int sweep_batch(size_t per_k, size_t avail, int nk) {
+    long long B = (long long)(avail / std::max<size_t>(per_k, 1));
+    if (B > 256) B = 256;
+    if (B < 1) B = 1;
+    if (nk > 0) {
+        const long long nbatch = (nk + B - 1) / B;
+        B = (nk + nbatch - 1) / nbatch;
+    }
+    return (int)B;
+}
+
+// Free device memory usable by a call that may regrow buffers currently
+// holding `held` bytes, less a reserve of 2 GiB or 3% of the device.
+size_t usable_device_bytes(size_t held) {
+    size_t fr = 0, tot = 0;
+    HIPC(hipMemGetInfo(&fr, &tot));
+    const size_t reserve = std::max<size_t>((size_t)2 << 30, tot / 32);
+    const size_t have = fr + held;
+    return have > reserve ? have - reserve : 0;
+}
+
 }  // namespace
 
 struct hbem_geometry {
@@ -102,12 +198,17 @@ struct hbem_geometry {
     DBuf w_Ao, w_U, w_rhs, w_flag;
     DBuf w_kapi, w_kri, w_L, w_T0, w_T1, w_T2, w_piv, w_perm, w_X, w_Y, w_work, w_R, w_u, w_v, w_a, w_b, w_sig, w_dsig, w_kap,
         w_kr, w_cv, w_qi, w_qo, w_tmp, w_out, w_part;
-    // profiling
+    // profiling (hbem_profile_enable / query)
     bool prof = false;
-    double prof_ms[4] = {0, 0, 0, 0};
-    long long prof_n[4] = {0, 0, 0, 0};
-    double prof_flops[4] = {0, 0, 0, 0};
+    EventLog plog;
     std::mutex mu;
+    // bytes held by the per-batch work buffers of the sweep
+    size_t work_held() const {
+        const DBuf* bs[] = {&w_L, &w_part, &w_T0, &w_T1, &w_T2, &w_piv, &w_perm, &w_X, &w_Y, &w_work, &w_u, &w_v, &w_a, &w_b};
+        size_t s = 0;
+        for (auto* b : bs) s += b->n;
+        return s;
+    }
 };
 
 namespace {
@@ -193,35 +294,21 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
         // general pairs: one chunk; singular pairs: chunks of 2048 nodes (paired-lane
         // kernel, 16 staging rounds; for adjacent pairs each of the two node halves
         // is chunked on its own) or 64 x 48 nodes (one wavefront per item)
-        const int chunk_nodes = l.cls == 2 ? l.nodes : (l.G == 0 ? 2048 : (l.G == 1 ? 512 : 64 * 48));
+        const int chunk_nodes = l.cls == 2 ? l.nodes : (l.G == 0 ? kChunkNodes : (l.G == 1 ? 512 : 64 * 48));
         P.chunk = chunk_nodes;
         P.nchunk = (l.nodes + chunk_nodes - 1) / chunk_nodes;
         if (l.cls == 1 && l.G == 0) P.nchunk = 2 * ((l.nodes / 2 + chunk_nodes - 1) / chunk_nodes);
         cdbl* part = nullptr;
         if (P.nchunk > 1) part = g->w_part.as<cdbl>((size_t)nk * l.nent * P.nchunk * nslot);
         P.out = P.nchunk > 1 ? part : L;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (g->prof) {
-            HIPC(hipEventCreate(&e0));
-            HIPC(hipEventCreate(&e1));
-            HIPC(hipEventRecord(e0, g->st));
-        }
+        // the class's kernel alone is timed: the chunk sums are a separate launch
+        hipEvent_t e0 = g->prof ? g->plog.begin(g->st) : nullptr;
         hbem::launch_assemble(l.cls, D, P, l.G, g->st);
         HIPC(hipGetLastError());
+        if (g->prof) g->plog.end(g->st, l.which, e0, (double)l.nent * nk * l.nodes * flops_per_node(D));
         if (P.nchunk > 1) {
             hbem::launch_chunk_sum(part, l.nent, P.nchunk, nk, nslot, l.pidx, g->npairs, L, g->st);
             HIPC(hipGetLastError());
-        }
-        if (g->prof) {
-            HIPC(hipEventRecord(e1, g->st));
-            HIPC(hipEventSynchronize(e1));
-            float ms = 0.f;
-            HIPC(hipEventElapsedTime(&ms, e0, e1));
-            g->prof_ms[l.which] += ms;
-            g->prof_n[l.which] += 1;
-            g->prof_flops[l.which] += (double)l.nent * nk * l.nodes * flops_per_node(D);
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
         }
     }
     return L;
@@ -242,6 +329,7 @@ struct DenseCtx {
     cdbl* X;             // n x nc x batch
     cdbl* Y;
     cdbl* work;          // 2 n nc batch: thin-solve scratch, then the projected right-hand sides
+    EventLog* tim = nullptr;   // randomized_svd timers (kTim*), or null
 };
 
 // Y <- T_p^{-1} X  or  T_p^{-H} X  (T_p^{-1} = F^T T^{-1} F, T_p^{-H} = F^T T^{-H} F); X is kept.
@@ -263,17 +351,21 @@ void tp_solve(DenseCtx& c, const cdbl* X, cdbl* Y, bool adj) {
 // randomized_svd::sv (randsvd.cpp:99-146) on the batch; X holds R replicated
 // per matrix.  The power iteration alternates between X and Y.
 void rsvd_core(DenseCtx& c, double* d_sigma, cdbl* d_u, cdbl* d_v) {
+    hipEvent_t e = c.tim ? c.tim->begin(c.st) : nullptr;
     tp_solve(c, c.X, c.Y, false);                              // Y = T_p^{-1} R
     hbem::orthonormalize_thin_batched(c.Y, c.n, c.batch, c.nc, c.st);   // Q
+    if (c.tim) { c.tim->end(c.st, kTimQR, e); e = c.tim->begin(c.st); }
     for (int it = 0; it < c.q; ++it) {
         tp_solve(c, c.Y, c.X, true);
         hbem::orthonormalize_thin_batched(c.X, c.n, c.batch, c.nc, c.st);
         tp_solve(c, c.X, c.Y, false);
         hbem::orthonormalize_thin_batched(c.Y, c.n, c.batch, c.nc, c.st);
     }
+    if (c.tim) { c.tim->end(c.st, kTimIter, e); e = c.tim->begin(c.st); }
     // Q = Y, X = T_p^{-H} Q
     tp_solve(c, c.Y, c.X, true);
     hbem::sigma_from_thin_batched(c.X, c.Y, c.n, c.batch, c.nc, d_sigma, d_u, d_v, c.st);
+    if (c.tim) c.tim->end(c.st, kTimSVD, e);
 }
 
 void copy_out_square(const cdbl* d, void* h, size_t count, hipStream_t st) {
@@ -530,7 +622,11 @@ int hbem_assemble_dense_batch(hbem_geometry_t g, const double* ks, int nk, doubl
     const int N = g->N;
     const size_t NN = (size_t)N * N * (d + 1);
     cdbl* out = g->w_out.as<cdbl>(3 * NN);
-    const int B = 256;
+    // one wavenumber slot per k: its local matrices and chunk partials
+    const size_t nslot = 16 * (size_t)(d + 1);
+    const size_t part = (size_t)std::max(g->nadj * adj_chunks(g->nc), g->ncoi * coi_chunks(g->nc));
+    const size_t per = ((size_t)g->npairs + part) * nslot * sizeof(cdbl);
+    const int B = sweep_batch(per, usable_device_bytes(g->w_L.n + g->w_part.n), nk);
     for (int k0 = 0; k0 < nk; k0 += B) {
         const int nb = std::min(B, nk - k0);
         std::vector<double> kap(nb), kr(nb), cv(nb, c);
@@ -688,11 +784,16 @@ int hbem_tp_solve(hbem_geometry_t g, double k_re, double k_im, double c_o, doubl
 
 namespace {
 
-// Standalone dense context for randomized_svd::* on caller matrices.
+// Standalone dense context for randomized_svd::* on caller matrices.  The
+// reference keeps the randomized_svd timers in file-scope statics shared by
+// every call (randsvd.cpp:27-47); here they are this context's event log,
+// filled while benchmarking is on and read when a timer is queried.
 struct Standalone {
     std::mutex mu;
     hipStream_t st = nullptr;
     DBuf T, T1, T2, piv, perm, X, Y, work, u, v, sig, out, W, Wpiv, Wperm, p, p2, ud, tmp;
+    bool bench = false;
+    EventLog tim;
     void init() {
         if (!st) {
             HIPC(hipSetDevice(g_device));
@@ -718,10 +819,13 @@ double standalone_sv(const void* T, int n, const void* R, int nc, int q, bool wa
     HIPC(hipMemcpyAsync(dT, T, nn * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
     int* piv = s.piv.as<int>(n);
     int* perm = s.perm.as<int>(n);
+    hipEvent_t e = s.bench ? s.tim.begin(s.st) : nullptr;
     hbem::lu_factor_batched(dT, n, 1, piv, perm, s.st);
+    if (s.bench) s.tim.end(s.st, kTimLU, e, (8.0 / 3.0) * (double)n * n * n);
     DenseCtx c;
     c.st = s.st; c.n = n; c.batch = 1; c.nc = nc; c.q = q; c.LU = dT; c.ms = (long long)n * n; c.perm = perm;
     c.F = nullptr; c.FT = nullptr;
+    c.tim = s.bench ? &s.tim : nullptr;
     c.X = s.X.as<cdbl>((size_t)n * nc);
     c.Y = s.Y.as<cdbl>((size_t)n * nc);
     c.work = s.work.as<cdbl>(2 * (size_t)n * nc);
@@ -878,16 +982,12 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
     // Batch of B wavenumbers -> 2B slots ordered [c_i for every k | c_o for every k]:
     // each wavefront of the assembly sees a single refraction index and 64
     // consecutive k, so the arguments kappa r of its lanes stay close (fewer
-    // divergent Bessel branches).  B = 256 gives the dense stage (one
+    // divergent Bessel branches).  B <= 256 gives the dense stage (one
     // workgroup per matrix in the panel factorisation and the thin solves)
-    // one matrix per CU.  B is bounded by the working set: local matrices,
-    // chunk partials and T, T', T'' per wavenumber.
-    int B = 256;
-    const size_t nslot = 16 * (size_t)(D + 1);
-    const size_t maxchunks = (size_t)g->nadj * ((2 * (size_t)g->nc * g->nc + 511) / 512);
-    const size_t per = nn * sizeof(cdbl) * (D + 1) +   // T (factorised in place), T', T''
-                       (size_t)2 * ((size_t)g->npairs + maxchunks) * nslot * sizeof(cdbl);
-    while (B > 1 && (size_t)B * per > ((size_t)64 << 30)) B /= 2;
+    // up to one matrix per CU.  B is bounded by the free device memory: local
+    // matrices, chunk partials and T, T', T'' per wavenumber.
+    const size_t per = sweep_bytes_per_k(g->npairs, g->nadj, g->ncoi, g->nc, n, D, nc);
+    const int B = sweep_batch(per, usable_device_bytes(g->work_held()), nk);
     cdbl* dR = g->w_R.as<cdbl>((size_t)n * nc);
     HIPC(hipMemcpyAsync(dR, R, (size_t)n * nc * sizeof(cdbl), hipMemcpyHostToDevice, g->st));
     for (int k0 = 0; k0 < nk; k0 += B) {
@@ -910,22 +1010,16 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
         cdbl* T2 = D >= 2 ? g->w_T2.as<cdbl>(nn * nb) : nullptr;
         const int* dqi = upload(g->w_qi, qi, g->st);
         const int* dqo = upload(g->w_qo, qo, g->st);
+        hipEvent_t e0 = g->prof ? g->plog.begin(g->st) : nullptr;
         hbem::launch_form_T(L, 2 * nb, 16 * (D + 1), dqi, dqo, nb, g->N, D, (const double*)g->d_M.p, T0, T1, T2, g->st);
+        if (g->prof) g->plog.end(g->st, kProfFormT, e0);
         int* piv = g->w_piv.as<int>((size_t)n * nb);
         int* perm = g->w_perm.as<int>((size_t)n * nb);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (g->prof) { HIPC(hipEventCreate(&e0)); HIPC(hipEventCreate(&e1)); HIPC(hipEventRecord(e0, g->st)); }
+        if (g->prof) e0 = g->plog.begin(g->st);
         hbem::lu_factor_batched(T0, n, nb, piv, perm, g->st);
         if (g->prof) {
-            HIPC(hipEventRecord(e1, g->st));
-            HIPC(hipEventSynchronize(e1));
-            float ms = 0.f;
-            HIPC(hipEventElapsedTime(&ms, e0, e1));
-            g->prof_ms[3] += ms;
-            g->prof_n[3] += 1;
-            g->prof_flops[3] += (8.0 / 3.0) * (double)n * n * n * nb;
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
+            g->plog.end(g->st, kProfLU, e0, (8.0 / 3.0) * (double)n * n * n * nb);
+            e0 = g->plog.begin(g->st);
         }
         DenseCtx c;
         c.st = g->st; c.n = n; c.batch = nb; c.nc = nc; c.q = q; c.LU = T0; c.ms = (long long)nn; c.perm = perm;
@@ -955,6 +1049,7 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
             hbem::bilinear_re_batched(T1, (long long)nn, a, bb, n, nb, dd, dd + nb, g->st);
             HIPC(hipMemcpyAsync(dsigma + k0, dd, nb * sizeof(double), hipMemcpyDeviceToHost, g->st));
         }
+        if (g->prof) g->plog.end(g->st, kProfRsvd, e0);
         HIPC(hipGetLastError());
     }
     HIPC(hipStreamSynchronize(g->st));
@@ -964,17 +1059,55 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
 int hbem_profile_enable(hbem_geometry_t g, int enable) {
     API_BEGIN
     if (!g) throw Error(HBEM_EINVAL, "null geometry");
+    std::lock_guard<std::mutex> lk(g->mu);
+    g->plog.reset();
     g->prof = enable != 0;
-    for (int i = 0; i < 4; ++i) { g->prof_ms[i] = 0; g->prof_n[i] = 0; g->prof_flops[i] = 0; }
     API_END
 }
 
 int hbem_profile_query(hbem_geometry_t g, int which, double* ms, long long* launches, double* flops) {
     API_BEGIN
-    if (!g || which < 0 || which > 3) throw Error(HBEM_EINVAL, "invalid argument");
-    if (ms) *ms = g->prof_ms[which];
-    if (launches) *launches = g->prof_n[which];
-    if (flops) *flops = g->prof_flops[which];
+    if (!g || which < 0 || which > kProfRsvd) throw Error(HBEM_EINVAL, "invalid argument");
+    std::lock_guard<std::mutex> lk(g->mu);
+    g->plog.resolve();
+    if (ms) *ms = g->plog.ms[which];
+    if (launches) *launches = g->plog.n[which];
+    if (flops) *flops = g->plog.flops[which];
+    API_END
+}
+
+int hbem_sweep_batch_size(int npanels, int order, int d, int nc, unsigned long long avail_bytes, int nk, int* B) {
+    API_BEGIN
+    if (!B) throw Error(HBEM_EINVAL, "null output");
+    if (npanels < 3 || order < 1 || order > 40 || d < 0 || d > 2 || nc < 1 || nk < 0)
+        throw Error(HBEM_EINVAL, "invalid argument");
+    // a closed curve of N panels: N coinciding, N adjacent pairs
+    const long long N = npanels;
+    const int ncr = order * (order + 1) / 2 - 1;
+    *B = sweep_batch(sweep_bytes_per_k(N * (N + 1) / 2, N, N, ncr, 2 * npanels, d, nc), (size_t)avail_bytes, nk);
+    API_END
+}
+
+int hbem_rsvd_benchmarking(int yes) {
+    API_BEGIN
+    std::lock_guard<std::mutex> lk(g_sa.mu);
+    g_sa.bench = yes != 0;
+    API_END
+}
+
+int hbem_rsvd_timers(double* seconds4) {
+    API_BEGIN
+    if (!seconds4) throw Error(HBEM_EINVAL, "null output");
+    std::lock_guard<std::mutex> lk(g_sa.mu);
+    g_sa.tim.resolve();
+    for (int i = 0; i < 4; ++i) seconds4[i] = g_sa.tim.ms[i] * 1e-3;
+    API_END
+}
+
+int hbem_rsvd_reset_timer(void) {
+    API_BEGIN
+    std::lock_guard<std::mutex> lk(g_sa.mu);
+    g_sa.tim.reset();
     API_END
 }
 

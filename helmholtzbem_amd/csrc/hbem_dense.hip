@@ -113,47 +113,57 @@ __global__ __launch_bounds__(1024) void k_lu_panel(cdbl* __restrict__ A, int n, 
     }
 }
 
-// The same panel factorisation with the rows held in registers, eight panel
-// columns (a chunk) at a time, for panels of at most 1024 rows: thread t owns
-// row j0 + t.  Inside a chunk every column step -- pivot search, exchange of
-// the two rows' chunk values through LDS, scaling, rank-1 update -- works on
-// registers; the interchange of the other panel columns goes to memory, one
-// thread per column.  After the chunk, its rows are stored, U12 = L11^{-1} A12
-// is solved for the panel columns right of it (one thread per column) and
-// those columns receive A22 -= L21 U12 in one pass.  The panel is read and
-// written about three times instead of once per column step.
-constexpr int kPanelChunk = 8;
-
+// The same panel factorisation with the rows held in registers, CW panel
+// columns (a chunk) at a time, for panels of at most 1024 R rows: thread t owns
+// rows j0 + t + 1024 u, u < R.  Inside a chunk every column step -- pivot
+// search, exchange of the two rows' chunk values through LDS, scaling, rank-1
+// update -- works on registers; the interchange of the other panel columns
+// goes to memory, one thread per column.  After the chunk, its rows are
+// stored, U12 = L11^{-1} A12 is solved for the panel columns right of it (one
+// thread per column) and those columns receive A22 -= L21 U12 in one pass.
+// The panel is read and written about jb / CW + 1 times instead of once per
+// column step.  R x CW complex values stay in registers (at most 32 of the
+// 128 VGPRs a thread of a 1024-thread workgroup has).
+template <int R, int CW>
 __global__ __launch_bounds__(1024) void k_lu_panel_rg(cdbl* __restrict__ A, int n, int j0, int jb,
                                                       int* __restrict__ ipiv) {
+    constexpr int YB = R >= 2 ? 4 : 8;           // trailing columns updated per round
     cdbl* a = A + (size_t)blockIdx.x * n * n;
     int* ip = ipiv + (size_t)blockIdx.x * n;
     __shared__ double sv[16];
     __shared__ int si[16];
     __shared__ int spiv;
-    __shared__ cdbl prow[kPanelChunk];          // pivot row (chunk columns)
-    __shared__ cdbl xrow[kPanelChunk];          // the row it displaces
-    __shared__ cdbl L11[kPanelChunk][kPanelChunk];
-    __shared__ cdbl U12[kPanelChunk][32];
+    __shared__ cdbl prow[CW];                   // pivot row (chunk columns)
+    __shared__ cdbl xrow[CW];                   // the row it displaces
+    __shared__ cdbl L11[CW][CW];
+    __shared__ cdbl U12[CW][32];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int row = j0 + tid;
-    for (int c0 = 0; c0 < jb; c0 += kPanelChunk) {
-        const int w = min(kPanelChunk, jb - c0);
+    int rowu[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) rowu[u] = j0 + tid + 1024 * u;
+    for (int c0 = 0; c0 < jb; c0 += CW) {
+        const int w = min(CW, jb - c0);
         const int cb = j0 + c0;                  // first column of the chunk = its first diagonal row
-        const bool own = row < n && row >= cb;
-        cdbl v[kPanelChunk];
+        cdbl v[R][CW];
 #pragma unroll
-        for (int u = 0; u < kPanelChunk; ++u) v[u] = (own && u < w) ? a[row + (size_t)(cb + u) * n] : mk(0, 0);
+        for (int u = 0; u < R; ++u) {
+            const bool own = rowu[u] < n && rowu[u] >= cb;
 #pragma unroll
-        for (int c = 0; c < kPanelChunk; ++c) {
+            for (int e = 0; e < CW; ++e) v[u][e] = (own && e < w) ? a[rowu[u] + (size_t)(cb + e) * n] : mk(0, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
             if (c >= w) break;
             const int col = cb + c;
             double best = -1.0;
             int bi = n;
-            if (own && row >= col) {
-                const double x = cabs2(v[c]);
-                if (x > best) { best = x; bi = row; }
-            }
+            // rows ascending within a thread: strict > keeps the first index on ties
+#pragma unroll
+            for (int u = 0; u < R; ++u)
+                if (rowu[u] < n && rowu[u] >= col) {
+                    const double x = cabs2(v[u][c]);
+                    if (x > best) { best = x; bi = rowu[u]; }
+                }
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) {
                 const double v2 = __shfl_xor(best, o, 64);
@@ -176,13 +186,16 @@ __global__ __launch_bounds__(1024) void k_lu_panel_rg(cdbl* __restrict__ A, int 
             }
             __syncthreads();
             const int piv = spiv;
-            if (row == piv) {
 #pragma unroll
-                for (int u = 0; u < kPanelChunk; ++u) prow[u] = v[u];
-            }
-            if (row == col) {
+            for (int u = 0; u < R; ++u) {
+                if (rowu[u] == piv) {
 #pragma unroll
-                for (int u = 0; u < kPanelChunk; ++u) xrow[u] = v[u];
+                    for (int e = 0; e < CW; ++e) prow[e] = v[u][e];
+                }
+                if (rowu[u] == col) {
+#pragma unroll
+                    for (int e = 0; e < CW; ++e) xrow[e] = v[u][e];
+                }
             }
             if (piv != col && tid < jb && (tid < c0 || tid >= c0 + w)) {
                 cdbl* pc = a + (size_t)(j0 + tid) * n;
@@ -192,33 +205,41 @@ __global__ __launch_bounds__(1024) void k_lu_panel_rg(cdbl* __restrict__ A, int 
             }
             __syncthreads();
             if (piv != col) {
-                if (row == piv) {
 #pragma unroll
-                    for (int u = 0; u < kPanelChunk; ++u) v[u] = xrow[u];
-                } else if (row == col) {
+                for (int u = 0; u < R; ++u) {
+                    if (rowu[u] == piv) {
 #pragma unroll
-                    for (int u = 0; u < kPanelChunk; ++u) v[u] = prow[u];
+                        for (int e = 0; e < CW; ++e) v[u][e] = xrow[e];
+                    } else if (rowu[u] == col) {
+#pragma unroll
+                        for (int e = 0; e < CW; ++e) v[u][e] = prow[e];
+                    }
                 }
             }
             const cdbl d = prow[c];
             const cdbl inv = (d.x == 0.0 && d.y == 0.0) ? mk(0, 0) : crecip(d);
-            if (own && row > col) {
-                const cdbl l = v[c] * inv;
-                v[c] = l;
 #pragma unroll
-                for (int u = c + 1; u < kPanelChunk; ++u)
-                    if (u < w) cfms(v[u], l, prow[u]);
-            }
+            for (int u = 0; u < R; ++u)
+                if (rowu[u] < n && rowu[u] > col) {
+                    const cdbl l = v[u][c] * inv;
+                    v[u][c] = l;
+#pragma unroll
+                    for (int e = c + 1; e < CW; ++e)
+                        if (e < w) cfms(v[u][e], l, prow[e]);
+                }
         }
         // chunk rows back to memory; L11 of the chunk for the U12 solve
-        if (own) {
 #pragma unroll
-            for (int u = 0; u < kPanelChunk; ++u)
-                if (u < w) a[row + (size_t)(cb + u) * n] = v[u];
-            const int r = row - cb;
-            if (r < w) {
+        for (int u = 0; u < R; ++u) {
+            if (rowu[u] < n && rowu[u] >= cb) {
 #pragma unroll
-                for (int u = 0; u < kPanelChunk; ++u) L11[r][u] = v[u];
+                for (int e = 0; e < CW; ++e)
+                    if (e < w) a[rowu[u] + (size_t)(cb + e) * n] = v[u][e];
+                const int r = rowu[u] - cb;
+                if (r < w) {
+#pragma unroll
+                    for (int e = 0; e < CW; ++e) L11[r][e] = v[u][e];
+                }
             }
         }
         const int rem = jb - c0 - w;             // panel columns right of the chunk
@@ -228,34 +249,36 @@ __global__ __launch_bounds__(1024) void k_lu_panel_rg(cdbl* __restrict__ A, int 
         // together; what lies outside the range is never used or stored)
         if (tid < rem) {
             cdbl* pc = a + (size_t)(cb + w + tid) * n + cb;
-            cdbl x[kPanelChunk];
+            cdbl x[CW];
 #pragma unroll
-            for (int r = 0; r < kPanelChunk; ++r) x[r] = pc[min(r, w - 1)];
+            for (int r = 0; r < CW; ++r) x[r] = pc[min(r, w - 1)];
 #pragma unroll
-            for (int r = 0; r < kPanelChunk; ++r) {
+            for (int r = 0; r < CW; ++r) {
                 if (r < w) {
 #pragma unroll
-                    for (int u = 0; u < kPanelChunk; ++u)
-                        if (u < r) cfms(x[r], L11[r][u], x[u]);
+                    for (int e = 0; e < CW; ++e)
+                        if (e < r) cfms(x[r], L11[r][e], x[e]);
                     pc[r] = x[r];
                     U12[r][tid] = x[r];
                 }
             }
         }
         __syncthreads();
-        {
-            const bool upd = row < n && row >= cb + w;
-            const int rowc = min(row, n - 1);
-            for (int cc0 = 0; cc0 < rem; cc0 += 8) {
-                cdbl y[8];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) y[e] = a[rowc + (size_t)(cb + w + min(cc0 + e, rem - 1)) * n];
+        for (int u = 0; u < R; ++u) {
+            const int row = rowu[u];
+            if (row >= n) break;                 // rows ascend with u
+            const bool upd = row >= cb + w;
+            for (int cc0 = 0; cc0 < rem; cc0 += YB) {
+                cdbl y[YB];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
+                for (int e = 0; e < YB; ++e) y[e] = a[row + (size_t)(cb + w + min(cc0 + e, rem - 1)) * n];
+#pragma unroll
+                for (int e = 0; e < YB; ++e) {
                     if (cc0 + e < rem) {
 #pragma unroll
-                        for (int u = 0; u < kPanelChunk; ++u)
-                            if (u < w) cfms(y[e], v[u], U12[u][cc0 + e]);
+                        for (int f = 0; f < CW; ++f)
+                            if (f < w) cfms(y[e], v[u][f], U12[f][cc0 + e]);
                         if (upd) a[row + (size_t)(cb + w + cc0 + e) * n] = y[e];
                     }
                 }
@@ -554,7 +577,10 @@ void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStrea
         else hipLaunchKernelGGL(k_lu_rows<2>, g, t, 0, st, A, n, j0, jb, ipiv);
     };
     auto panel = [&](int j, int jb) {
-        if (n - j <= 1024) hipLaunchKernelGGL(k_lu_panel_rg, dim3(batch), dim3(1024), 0, st, A, n, j, jb, ipiv);
+        const int rows = n - j;
+        if (rows <= 1024) hipLaunchKernelGGL((k_lu_panel_rg<1, 8>), dim3(batch), dim3(1024), 0, st, A, n, j, jb, ipiv);
+        else if (rows <= 2048) hipLaunchKernelGGL((k_lu_panel_rg<2, 6>), dim3(batch), dim3(1024), 0, st, A, n, j, jb, ipiv);
+        else if (rows <= 4096) hipLaunchKernelGGL((k_lu_panel_rg<4, 4>), dim3(batch), dim3(1024), 0, st, A, n, j, jb, ipiv);
         else hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(1024), 0, st, A, n, j, jb, ipiv);
     };
     for (int j0 = 0; j0 < n; j0 += 2 * sb) {

@@ -183,13 +183,33 @@ int hbem_rsvd_sv_der2(const void* T, const void* Td, const void* Tdd, int n, con
 int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i,
                      const void* R, int nc, int q, int want_der, double* sigma, double* dsigma);
 
-/* Kernel timing (HIP events on the context's stream) of the dominant
- * assembly kernel, for the benchmark's roofline figure.  enable != 0 starts
- * recording; query returns the accumulated milliseconds and launch count of
- * kernel `which` (0 = general-pair assembly, 1 = adjacent, 2 = coinciding,
- * 3 = dense LU update) and the algorithmic FP64 flop count they performed. */
+/* Phase timing (HIP events on the context's stream), for the benchmark's
+ * roofline figures.  enable != 0 clears and starts recording; the events are
+ * only read by query (no host synchronisation inside the timed calls).  query
+ * returns the accumulated milliseconds and launch count of phase `which`
+ * (0 = general-pair assembly kernel, 1 = adjacent-pair kernel, 2 =
+ * coinciding-pair kernel, 3 = LU factorisation, 4 = T formation, 5 = the
+ * randomized SVD stage after the LU) and the algorithmic FP64 flop count of
+ * the frozen model (0 where none is defined). */
 int hbem_profile_enable(hbem_geometry_t g, int enable);
 int hbem_profile_query(hbem_geometry_t g, int which, double* ms, long long* launches, double* flops);
+
+/* Wavenumbers per batch that hbem_sigma_sweep uses for a closed curve of
+ * npanels panels at Gauss order `order`, derivative order d and nc random
+ * columns when avail_bytes of device memory are usable and nk wavenumbers are
+ * requested (at most 256, batches balanced).  Host-only: no device is used.
+ * (No counterpart in the reference, whose sweep runs one k per thread.) */
+int hbem_sweep_batch_size(int npanels, int order, int d, int nc, unsigned long long avail_bytes, int nk, int* B);
+
+/* randomized_svd::benchmarking, get_lu_time / get_qr_time / get_sub_iter_time
+ * / get_svd_time, reset_timer (randsvd.hpp:23-35, randsvd.cpp:27-47).  While
+ * benchmarking is on, hbem_rsvd_sv / sv_der / sv_der2 record device events
+ * around the LU, the first solve + orthonormalisation, the subspace
+ * iterations and the last solve + SVD; hbem_rsvd_timers returns the
+ * accumulated seconds of the four phases in that order. */
+int hbem_rsvd_benchmarking(int yes);
+int hbem_rsvd_timers(double* seconds4);
+int hbem_rsvd_reset_timer(void);
 
 #ifdef __cplusplus
 }
