@@ -47,6 +47,27 @@ def _vptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else None
 
 
+def _square(name, T, n=None):
+    """T as a Fortran-ordered complex n x n matrix; ValueError otherwise (the
+    C-ABI reads n^2 entries from the pointer it is given)."""
+    T = np.asfortranarray(T, dtype=np.complex128)
+    if T.ndim != 2 or T.shape[0] != T.shape[1] or T.shape[0] < 1:
+        raise ValueError("%s must be a square matrix, got shape %s" % (name, T.shape))
+    if n is not None and T.shape[0] != n:
+        raise ValueError("%s must be %d x %d, got %s" % (name, n, n, T.shape))
+    return T
+
+
+def _test_matrix(R, n):
+    """R as a Fortran-ordered complex n x nc matrix with 1 <= nc <= min(8, n)."""
+    R = np.asfortranarray(R, dtype=np.complex128)
+    if R.ndim != 2 or R.shape[0] != n:
+        raise ValueError("R must have %d rows, got shape %s" % (n, R.shape))
+    if not 1 <= R.shape[1] <= min(8, n):
+        raise ValueError("R must have between 1 and %d columns, got %d" % (min(8, n), R.shape[1]))
+    return R
+
+
 class LayerType(enum.IntFlag):
     """galerkin_builder.hpp:100-106."""
     NONE = 0
@@ -309,7 +330,7 @@ class SolutionsOperator:
     def projection(self, T):
         if not self.do_projection:
             raise RuntimeError("projection is disabled for this SolutionsOperator")
-        T = np.asfortranarray(T, dtype=np.complex128)
+        T = _square("T", T, 2 * self.builder_data.getNumPanels())
         out = np.zeros_like(T, order="F")
         check(lib().hbem_projection(self._g.h, _vptr(T), _vptr(out)))
         return out
@@ -326,7 +347,8 @@ class randomized_svd:
         """Complex Gaussian test matrix, reproducible from ``seed`` (the
         reference's draws are unseeded, randsvd.cpp:49-69)."""
         if nc is None:
-            return gaussian_test_matrix(nr, 1, seed)[:, 0]
+            # the vector overload omits the matrix form's 1/sqrt2 (randsvd.cpp:60-69)
+            return gaussian_test_matrix(nr, 1, seed)[:, 0] * math.sqrt(2.0)
         return gaussian_test_matrix(nr, nc, seed)
 
     @staticmethod
@@ -362,27 +384,27 @@ class randomized_svd:
 
     @staticmethod
     def sv(T, R, q=0):
-        T = np.asfortranarray(T, dtype=np.complex128)
-        R = np.asfortranarray(R, dtype=np.complex128)
+        T = _square("T", T)
+        R = _test_matrix(R, T.shape[0])
         s = ctypes.c_double()
         check(lib().hbem_rsvd_sv(_vptr(T), T.shape[0], _vptr(R), R.shape[1], int(q), ctypes.byref(s)))
         return s.value
 
     @staticmethod
     def sv_der(T, T_der, R, q=0):
-        T = np.asfortranarray(T, dtype=np.complex128)
-        Td = np.asfortranarray(T_der, dtype=np.complex128)
-        R = np.asfortranarray(R, dtype=np.complex128)
+        T = _square("T", T)
+        Td = _square("T_der", T_der, T.shape[0])
+        R = _test_matrix(R, T.shape[0])
         out = np.zeros(2)
         check(lib().hbem_rsvd_sv_der(_vptr(T), _vptr(Td), T.shape[0], _vptr(R), R.shape[1], int(q), _dptr(out)))
         return out
 
     @staticmethod
     def sv_der2(T, T_der, T_der2, R, q):
-        T = np.asfortranarray(T, dtype=np.complex128)
-        Td = np.asfortranarray(T_der, dtype=np.complex128)
-        Tdd = np.asfortranarray(T_der2, dtype=np.complex128)
-        R = np.asfortranarray(R, dtype=np.complex128)
+        T = _square("T", T)
+        Td = _square("T_der", T_der, T.shape[0])
+        Tdd = _square("T_der2", T_der2, T.shape[0])
+        R = _test_matrix(R, T.shape[0])
         out = np.zeros(3)
         check(lib().hbem_rsvd_sv_der2(_vptr(T), _vptr(Td), _vptr(Tdd), T.shape[0], _vptr(R), R.shape[1], int(q),
                                       _dptr(out)))
@@ -414,9 +436,11 @@ def assemble_dense_batch(builder_data: BuilderData, ks, c, layers=LayerType.ALL,
 def sigma_sweep(builder_data: BuilderData, ks, c_o, c_i, R, q=2, der=0, proj=True):
     """sigma_min(k) for every k (and dsigma/dk when der >= 1; der = 2 assembles
     the operators with their first and second k-derivatives)."""
-    g = builder_data._ctx(proj)
     ks = np.ascontiguousarray(ks, dtype=np.float64)
-    R = np.asfortranarray(R, dtype=np.complex128)
+    if ks.ndim != 1:
+        raise ValueError("ks must be a 1-D array of wavenumbers")
+    R = _test_matrix(R, 2 * builder_data.getNumPanels())
+    g = builder_data._ctx(proj)
     sig = np.zeros(ks.size)
     dsig = np.zeros(ks.size) if der else None
     check(lib().hbem_sigma_sweep(g.h, _dptr(ks), ks.size, float(c_o), float(c_i), _vptr(R), R.shape[1], int(q),

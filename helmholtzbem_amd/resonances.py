@@ -90,14 +90,20 @@ def rtsafe_iter(x1: float, x2: float, tol: float) -> Generator[Tuple[float, bool
     x1, x2 = float(x1), float(x2)
     fl = yield (x1, False)
     fh = yield (x2, False)
-    if fl * fh > 0.0:
-        return (x1 if fl > fh else x2), False, 0
-    if fl == 0.0:
-        return x1, True, 0
-    if fh == 0.0:
-        return x2, True, 0
-    xl, xh = (x1, x2) if fl < 0.0 else (x2, x1)
-    rts = 0.5 * (x1 + x2)
+    if fl == fl and fh == fh:
+        if fl * fh > 0.0:
+            return (x1 if fl > fh else x2), False, 0
+        if fl == 0.0:
+            return x1, True, 0
+        if fh == 0.0:
+            return x2, True, 0
+        xl, xh = (x1, x2) if fl < 0.0 else (x2, x1)
+        rts = 0.5 * (x1 + x2)
+    else:
+        # an end value is NaN: the function's value at NaN is the first guess
+        # (find_roots.cpp:159-162)
+        rts = float((yield (math.nan, False)))
+        xl, xh = x1, x2
     dxold = abs(x2 - x1)
     dx = dxold
     f, df = yield (rts, True)

@@ -11,9 +11,12 @@ of interpolation and smooth curve fitting based on local procedures, J. ACM
     m_j = (y_{j+1} - y_j) / (x_{j+1} - x_j)                      secant slopes
     t_i = (|m_{i+1} - m_i| m_{i-1} + |m_{i-1} - m_{i-2}| m_i)
           / (|m_{i+1} - m_i| + |m_{i-1} - m_{i-2}|)             node slopes
-          (the mean (m_{i-1} + m_i) / 2 when the denominator is 0)
     on [x_j, x_{j+1}], d = x - x_j, h = x_{j+1} - x_j:
     y = y_j + t_j d + (3 m_j - 2 t_j - t_{j+1}) / h d^2 + (t_j + t_{j+1} - 2 m_j) / h^2 d^3
+
+When the denominator of t_i is 0 (two exactly straight runs meet at node i)
+the spline follows GSL's akima: the piece starting at node i is linear with
+slope m_i, and the piece ending there takes m_{i-1} as its right-end slope.
 
 The secants beyond the ends are continued periodically (m_{-1} = m_{n-2},
 m_{-2} = m_{n-3}, m_{n-1} = m_0, m_n = m_1, for data with y_{n-1} = y_0) or
@@ -41,11 +44,29 @@ def _akima_slopes(x: np.ndarray, y: np.ndarray, periodic: bool) -> np.ndarray:
         right = [2.0 * m[-1] - m[-2], 3.0 * m[-1] - 2.0 * m[-2]]
         ext = np.concatenate([left, m, right])
     t = np.empty(n)
+    flat = np.zeros(n, dtype=bool)
     for i in range(n):
         mm2, mm1, m0, mp1 = ext[i], ext[i + 1], ext[i + 2], ext[i + 3]
         w1, w2 = abs(mp1 - m0), abs(mm1 - mm2)
-        t[i] = 0.5 * (mm1 + m0) if w1 + w2 == 0.0 else (w1 * mm1 + w2 * m0) / (w1 + w2)
-    return t
+        flat[i] = w1 + w2 == 0.0
+        t[i] = 0.0 if flat[i] else (w1 * mm1 + w2 * m0) / (w1 + w2)
+    return t, flat
+
+
+def _akima_pieces(x: np.ndarray, v: np.ndarray, periodic: bool):
+    """(b, c, d) of every piece.  Where both weights at node j vanish (two
+    exactly straight runs meet there) the piece from node j is linear with
+    slope m_j, and a piece ending at such a node ends with its own secant
+    slope -- GSL's convention, which keeps both runs straight."""
+    h = np.diff(x)
+    m = np.diff(v) / h
+    t, flat = _akima_slopes(x, v, periodic)
+    tl = t[:-1]
+    tr = np.where(flat[1:], m, t[1:])
+    b = np.where(flat[:-1], m, tl)
+    c = np.where(flat[:-1], 0.0, (3.0 * m - 2.0 * tl - tr) / h)
+    d = np.where(flat[:-1], 0.0, (tl + tr - 2.0 * m) / (h * h))
+    return b, c, d
 
 
 class ComplexSpline:
@@ -57,12 +78,7 @@ class ComplexSpline:
         if x.size != y.size or x.size < (4 if periodic else 5) or np.any(np.diff(x) <= 0):
             raise ValueError("need increasing abscissae and enough points for an Akima spline")
         self.x, self.y, self.periodic = x, y, periodic
-        h = np.diff(x)
-        self.coef = []
-        for part in (y.real, y.imag):
-            tp = _akima_slopes(x, part, periodic)
-            m = np.diff(part) / h
-            self.coef.append((tp[:-1], (3.0 * m - 2.0 * tp[:-1] - tp[1:]) / h, (tp[:-1] + tp[1:] - 2.0 * m) / (h * h)))
+        self.coef = [_akima_pieces(x, part, periodic) for part in (y.real, y.imag)]
 
     def _piece(self, t):
         t = np.asarray(t, dtype=np.float64)

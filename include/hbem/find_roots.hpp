@@ -93,12 +93,20 @@ inline double rtsafe(std::function<double(double)> fct, std::function<hbem::Matr
     root_found = false;
     num_iter = 0;
     const double fl = fct(x1), fh = fct(x2);
-    if (fl * fh > 0.0) return fl > fh ? x1 : x2;
-    if (fl == 0.0) { root_found = true; return x1; }
-    if (fh == 0.0) { root_found = true; return x2; }
-    double xl, xh;
-    if (fl < 0.0) { xl = x1; xh = x2; } else { xh = x1; xl = x2; }
-    double rts = 0.5 * (x1 + x2), dxold = std::fabs(x2 - x1), dx = dxold;
+    double xl, xh, rts;
+    if (fl == fl && fh == fh) {
+        if (fl * fh > 0.0) return fl > fh ? x1 : x2;
+        if (fl == 0.0) { root_found = true; return x1; }
+        if (fh == 0.0) { root_found = true; return x2; }
+        if (fl < 0.0) { xl = x1; xh = x2; } else { xh = x1; xl = x2; }
+        rts = 0.5 * (x1 + x2);
+    } else {
+        // an end value is NaN: fct(NaN) supplies the first guess (find_roots.cpp:159-162)
+        rts = fct(std::numeric_limits<double>::quiet_NaN());
+        xl = x1;
+        xh = x2;
+    }
+    double dxold = std::fabs(x2 - x1), dx = dxold;
     hbem::MatrixXd m = fct_both(rts);
     double f = m(0, 0), df = m(0, 1);
     for (int j = 1; j <= kMaxIt; ++j) {

@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -24,6 +25,7 @@
 #if defined(__has_include)
 #if __has_include(<Eigen/Dense>)
 #include <Eigen/Dense>
+#include <Eigen/Sparse>
 #define HBEM_HAVE_EIGEN 1
 #endif
 #endif
@@ -37,6 +39,8 @@ using VectorXcd = Eigen::VectorXcd;
 using Vector2d = Eigen::Vector2d;
 using Vector3d = Eigen::Vector3d;
 using Vector2cd = Eigen::Vector2cd;
+template <class T>
+using SparseMatrix = Eigen::SparseMatrix<T>;
 #else
 template <class T>
 class Matrix {
@@ -62,6 +66,11 @@ class Matrix {
 using MatrixXcd = Matrix<std::complex<double>>;
 using MatrixXd = Matrix<double>;
 using VectorXcd = Matrix<std::complex<double>>;
+// the element type of GalerkinBuilder::assembleSparse's result map
+template <class T>
+struct SparseMatrix {
+    long rows = 0, cols = 0;
+};
 struct Vector2d { double v[2] = {0, 0}; double& operator()(int i) { return v[i]; } double operator()(int i) const { return v[i]; } };
 struct Vector3d { double v[3] = {0, 0, 0}; double& operator()(int i) { return v[i]; } double operator()(int i) const { return v[i]; } };
 struct Vector2cd {
@@ -390,6 +399,22 @@ class GalerkinBuilder {
     const MatrixXcd& getHypersingular(size_t d = 0) const { return get(1, d); }
     const MatrixXcd& getSingleLayer(size_t d = 0) const { return get(2, d); }
 
+    // Per-call scratch of the reference's pair routines (galerkin_builder.hpp:117-137);
+    // the device path keeps its work space in the geometry context.
+    struct workspace_t {};
+    // GalerkinBuilder::assembleSparse (galerkin_builder.hpp:250-255).  The
+    // reference's body computes nothing and leaves `layers` unchanged
+    // (galerkin_builder.cpp:1020-1048); it is declared here with the same
+    // signature so that code naming it compiles, and it likewise leaves
+    // `layers` unchanged.
+    void assembleSparse(const std::complex<double>& k_in, double c_in,
+                        std::map<LayerType, SparseMatrix<std::complex<double>>>& layers, size_t row, size_t col,
+                        size_t rows, size_t cols, std::vector<size_t>& ai, std::vector<size_t>& aj, size_t i, size_t j,
+                        workspace_t& ws) {
+        (void)k_in; (void)c_in; (void)layers; (void)row; (void)col; (void)rows; (void)cols;
+        (void)ai; (void)aj; (void)i; (void)j; (void)ws;
+    }
+
   private:
     const MatrixXcd& get(int l, size_t d) const {
         if (d > 2) throw std::runtime_error("Invalid order of derivative");
@@ -497,13 +522,21 @@ class SolutionsOperator {
 // ---------------------------------------------------------------------------
 namespace randomized_svd {
 
-inline bool& bench_flag() { static bool b = false; return b; }
-inline void benchmarking(bool yes) { bench_flag() = yes; }
-inline double get_lu_time() { return 0.0; }
-inline double get_qr_time() { return 0.0; }
-inline double get_sub_iter_time() { return 0.0; }
-inline double get_svd_time() { return 0.0; }
-inline void reset_timer() {}
+// randsvd.cpp:27-47: while benchmarking is on, sv / sv_der / sv_der2 record
+// device events around the LU, the first solve + orthonormalisation, the
+// subspace iterations and the last solve + SVD; the getters return the
+// accumulated seconds of each phase.
+inline void benchmarking(bool yes) { hbem::check(hbem_rsvd_benchmarking(yes ? 1 : 0)); }
+inline double timer_seconds(int which) {
+    double t[4];
+    hbem::check(hbem_rsvd_timers(t));
+    return t[which];
+}
+inline double get_lu_time() { return timer_seconds(0); }
+inline double get_qr_time() { return timer_seconds(1); }
+inline double get_sub_iter_time() { return timer_seconds(2); }
+inline double get_svd_time() { return timer_seconds(3); }
+inline void reset_timer() { hbem::check(hbem_rsvd_reset_timer()); }
 
 // Complex Gaussian n x nc test matrix (g1 + i g2)/sqrt2 from a seeded
 // splitmix64 / xoshiro256** stream (the reference's draws are unseeded).
@@ -535,6 +568,16 @@ inline hbem::MatrixXcd randGaussian(int nr, int nc, uint64_t seed = 0) {
             R(i, j) = std::complex<double>(a, b) * std::sqrt(0.5);
         }
     return R;
+}
+
+// randGaussian(int) (randsvd.cpp:60-69): a vector of n entries g1 + i g2,
+// without the 1/sqrt2 of the matrix form, as the reference's vector overload;
+// the draws are those of randGaussian(n, 1) (seed 0).
+inline hbem::VectorXcd randGaussian(int n) {
+    const hbem::MatrixXcd R = randGaussian(n, 1, 0);
+    hbem::VectorXcd v(n, 1);
+    for (int i = 0; i < n; ++i) v(i) = R(i, 0) * std::sqrt(2.0);
+    return v;
 }
 
 inline double sv(const hbem::MatrixXcd& T, const hbem::MatrixXcd& R, int q = 0) {

@@ -79,11 +79,15 @@ class ComplexSpline {
                 ext[n + 2] = 3.0 * m[n - 2] - 2.0 * m[n - 3];
             }
             for (size_t j = 0; j + 1 < n; ++j) ext[j + 2] = m[j];
+            // node slopes; flat[i]: both Akima weights at node i vanish (two
+            // straight runs meet there), where GSL's akima keeps each run exact
             std::vector<double> t(n);
+            std::vector<char> flat(n);
             for (size_t i = 0; i < n; ++i) {
                 const double mm2 = ext[i], mm1 = ext[i + 1], m0 = ext[i + 2], mp1 = ext[i + 3];
                 const double w1 = std::fabs(mp1 - m0), w2 = std::fabs(mm1 - mm2);
-                t[i] = (w1 + w2 == 0.0) ? 0.5 * (mm1 + m0) : (w1 * mm1 + w2 * m0) / (w1 + w2);
+                flat[i] = w1 + w2 == 0.0;
+                t[i] = flat[i] ? 0.0 : (w1 * mm1 + w2 * m0) / (w1 + w2);
             }
             std::vector<double>& b = part ? by_ : bx_;
             std::vector<double>& c = part ? cy_ : cx_;
@@ -92,9 +96,16 @@ class ComplexSpline {
             c.resize(n - 1);
             d.resize(n - 1);
             for (size_t j = 0; j + 1 < n; ++j) {
+                if (flat[j]) {              // a linear piece of slope m_j
+                    b[j] = m[j];
+                    c[j] = 0.0;
+                    d[j] = 0.0;
+                    continue;
+                }
+                const double tr = flat[j + 1] ? m[j] : t[j + 1];   // slope at the right end
                 b[j] = t[j];
-                c[j] = (3.0 * m[j] - 2.0 * t[j] - t[j + 1]) / h[j];
-                d[j] = (t[j] + t[j + 1] - 2.0 * m[j]) / (h[j] * h[j]);
+                c[j] = (3.0 * m[j] - 2.0 * t[j] - tr) / h[j];
+                d[j] = (t[j] + tr - 2.0 * m[j]) / (h[j] * h[j]);
             }
         }
         rows_ = std::make_shared<std::vector<double>>();

@@ -38,5 +38,14 @@ int main() {
         const double m = brent_gsl(c.a, c.b, c.g, f, 1e-7, ic);
         std::printf("%.17g %u\n", m, ic);
     }
+    // rtsafe's NaN-start mode: f(x1) is NaN, so fct(NaN) supplies the first guess
+    {
+        auto f = [](double x) { return x != x ? 1.25 : (x == 1.0 ? std::nan("") : std::sin(x) - 0.9); };
+        auto both = [&](double x) { hbem::MatrixXd m(1, 2); m(0, 0) = f(x); m(0, 1) = std::cos(x); return m; };
+        bool rf;
+        unsigned it = 0;
+        const double r = rtsafe(f, both, 1.0, 2.0, 1e-13, rf, it);
+        std::printf("%.17g %d %u\n", r, (int)rf, it);
+    }
     return 0;
 }

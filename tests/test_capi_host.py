@@ -157,3 +157,51 @@ def test_sweep_batch_size_helper(H):
     assert H.sweep_batch_size(400, 16, 2, 2, 1, 300) == 1
     with pytest.raises(H.HbemError):
         H.sweep_batch_size(2, 11, 0, 2, 1 << 30)
+
+
+def test_wrappers_reject_bad_shapes(H):
+    """The wrappers check shapes before handing raw pointers to the C-ABI
+    (which reads n^2 / n nc entries from them)."""
+    rs = H.randomized_svd
+    T = np.eye(6, dtype=np.complex128)
+    R = np.ones((6, 2), dtype=np.complex128)
+    with pytest.raises(ValueError):
+        rs.sv(np.ones((6, 5)), R, 1)                 # not square
+    with pytest.raises(ValueError):
+        rs.sv(T, np.ones((5, 2)), 1)                 # R rows != n
+    with pytest.raises(ValueError):
+        rs.sv(T, np.ones((6, 9)), 1)                 # too many columns
+    with pytest.raises(ValueError):
+        rs.sv(T, np.ones((6, 0)), 1)                 # no column
+    with pytest.raises(ValueError):
+        rs.sv(T, np.ones(6), 1)                      # 1-D R
+    with pytest.raises(ValueError):
+        rs.sv_der(T, np.eye(5), R, 1)                # T' of another order
+    with pytest.raises(ValueError):
+        rs.sv_der2(T, T, np.ones((6, 7)), R, 1)      # T'' not square
+    with pytest.raises(ValueError):
+        rs.sv_der2(T, T, T, np.ones((7, 2)), 1)
+
+    class FakeBuilder:                               # the sweep validates before touching the device
+        def getNumPanels(self):
+            return 3
+
+        def _ctx(self, proj=True):
+            raise AssertionError("validation must come first")
+    with pytest.raises(ValueError):
+        H.sigma_sweep(FakeBuilder(), np.array([1.0]), 1.0, 4.0, np.ones((3, 2)), 2)    # N rows instead of 2N
+    with pytest.raises(ValueError):
+        H.sigma_sweep(FakeBuilder(), np.ones((2, 2)), 1.0, 4.0, np.ones((6, 2)), 2)    # 2-D ks
+    with pytest.raises(ValueError):
+        H.sigma_sweep(FakeBuilder(), np.array([1.0]), 1.0, 4.0, np.ones((6, 9)), 2)    # nc > 8
+
+
+def test_rsvd_timers_and_vector_gaussian_host(H):
+    rs = H.randomized_svd
+    rs.benchmarking(True)
+    rs.reset_timer()
+    assert (rs.get_lu_time(), rs.get_qr_time(), rs.get_sub_iter_time(), rs.get_svd_time()) == (0, 0, 0, 0)
+    rs.benchmarking(False)
+    v = rs.randGaussian(9)
+    assert v.shape == (9,)
+    assert np.array_equal(v, rs.randGaussian(9, 1)[:, 0] * np.sqrt(2.0))

@@ -39,3 +39,18 @@ def test_facade_driver_matches_oracle(tmp_path, H, O):
     assert abs(der - O.sv_der(T, Td, R, 30)[1]) <= 1e-8 * max(1.0, abs(der))
     K, W, V = O.assemble(mesh, 11, 2.0, 20.0, d=0)
     assert abs(v01 - abs(V[0][0, 1])) <= 1e-12 * abs(V[0][0, 1])
+
+
+def test_facade_host_api(tmp_path):
+    """The vector randGaussian, assembleSparse's declaration and the
+    randomized_svd timers (zero before any timed device work)."""
+    exe = str(tmp_path / "facade_api")
+    cmd = ["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "facade_api.cpp"), "-o", exe, "-L", LIBDIR, "-lhbem",
+           "-Wl,-rpath," + LIBDIR]
+    subprocess.run(cmd, check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    out = subprocess.run([exe], check=True, stdout=subprocess.PIPE, text=True, timeout=60).stdout.split("\n")
+    name, n, err = out[0].split()
+    assert name == "randGaussian_vector" and int(n) == 7 and float(err) <= 1e-15
+    assert out[1] == "assembleSparse 1"
+    assert out[2] == "timers 0 0 0 0"

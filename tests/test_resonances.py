@@ -108,6 +108,32 @@ def test_cpp_header_matches_python(tmp_path):
     for k, (f, a, b, g, _) in enumerate(MIN_CASES):
         m, ic = lines[off + k].split()
         assert (float(m), int(ic)) == RZ.brent_gsl(a, b, g, f, 1e-7), k
+    r, ok, it = lines[off + len(MIN_CASES)].split()
+    f, both = nan_start_case()
+    assert (float(r), bool(int(ok)), int(it)) == RZ.rtsafe(f, both, 1.0, 2.0, 1e-13)
+
+
+def nan_start_case():
+    """f(1) is NaN, so rtsafe asks f(NaN) for its first guess (1.25)."""
+    def f(x):
+        if x != x:
+            return 1.25
+        return math.nan if x == 1.0 else math.sin(x) - 0.9
+    return f, lambda x: (f(x), math.cos(x))
+
+
+def test_rtsafe_nan_start():
+    """find_roots.cpp:159-162: with a NaN end value the first Newton iterate is
+    fct(NaN) and the bracket is [x1, x2] as given."""
+    f, both = nan_start_case()
+    seen = []
+
+    def fct(x):
+        seen.append(x)
+        return f(x)
+    r, ok, it = RZ.rtsafe(fct, both, 1.0, 2.0, 1e-13)
+    assert ok and abs(r - math.asin(0.9)) <= 1e-13
+    assert seen[0] == 1.0 and seen[1] == 2.0 and seen[2] != seen[2]
 
 
 @pytest.mark.gpu

@@ -40,7 +40,7 @@ def test_open_spline_matches_scipy_akima():
     rng = np.random.default_rng(5)
     x = np.cumsum(rng.uniform(0.1, 1.0, 14))
     y = rng.normal(size=14) + 1j * rng.normal(size=14)
-    y[5] = y[6] = y[7]                     # a flat run: zero denominators
+    y[5] = y[6] = y[7]                     # two equal secants (weights zero on one side only)
     s = ComplexSpline(x, y)
     t = np.linspace(x[0], x[-1], 997)
     ref = interpolate.Akima1DInterpolator(x, y.real)(t) + 1j * interpolate.Akima1DInterpolator(x, y.imag)(t)
@@ -236,3 +236,22 @@ def test_device_transmission_on_smooth_circle(O):
     A, rhs = O.tp_A_rhs(mesh, 11, 2.0, 1.0, 4.0, U)
     xo = np.linalg.solve(A, rhs)[:, 0]
     assert np.linalg.norm(x - xo) <= 1e-12 * np.linalg.cond(A) * np.linalg.norm(xo)
+
+
+def test_corner_between_straight_runs_follows_gsl():
+    """Both Akima weights vanish at a node where two exactly straight runs
+    meet.  GSL's akima (the reference's gsl_interp_akima, cspline.cpp:24-25)
+    then makes the piece from that node linear with its secant slope and ends
+    the piece before it with that piece's own secant slope, so both runs stay
+    exactly straight and the corner stays a corner."""
+    x = np.arange(11.0)
+    y = np.where(x <= 5.0, 0.0, x - 5.0) + 1j * np.where(x <= 5.0, 2.0 * x, 10.0)
+    s = ComplexSpline(x, y)
+    t = np.linspace(2.0, 8.0, 601)
+    poly = np.where(t <= 5.0, 0.0, t - 5.0) + 1j * np.where(t <= 5.0, 2.0 * t, 10.0)
+    assert np.abs(s.eval(t) - poly).max() <= 1e-14
+    assert s.eval_der(np.array([4.999]))[0] == 0.0 + 2.0j
+    assert s.eval_der(np.array([5.0]))[0] == 1.0 + 0.0j
+    # the mean-slope convention (scipy) rounds the corner instead
+    ref = interpolate.Akima1DInterpolator(x, y.real)(4.5)
+    assert abs(ref) > 1e-3 and s.eval(np.array([4.5]))[0].real == 0.0
