@@ -1,9 +1,10 @@
 """Many scatterers of uneven size at once (the instance batches of the
-resonance drivers; 64 random polygons with N in [100, 1000] at full size):
-sigma_sweep_multi issues every instance's batched sweep from its own host
-thread onto its own device stream.  The results must be the same bits as
-one instance at a time, and the oracle's sigma_min to the path's bar
-(<= 1e-10 relative with the same test matrix R)."""
+resonance drivers), on a small stand-in: 6 random polygons with N in
+[24, 90], Gauss order 8, 6 wavenumbers each.  sigma_sweep_multi issues every
+instance's batched sweep from its own host thread onto its own device
+stream.  The results must be the same bits as one instance at a time, and
+the oracle's sigma_min to the path's bar (<= 1e-10 relative with the same
+test matrix R)."""
 import numpy as np
 import pytest
 
