@@ -47,6 +47,8 @@ WORKLOADS = {
                          "256 k in linspace(1,10), d=2 operators, sigma_min + dsigma/dk per k, rSVD nc=2 q=2, FP64"),
 }
 NC, Q = 2, 2
+MIXED_REFINE = 3
+MIXED_TOL = 1e-8            # the mixed-precision bar: sigma within 1e-8 relative of FP64
 PHASES = ("assembly_general", "assembly_adjacent", "assembly_coinciding", "lu", "form_T", "rsvd")
 
 
@@ -91,18 +93,26 @@ def cpu_baseline(name, w, mesh, R, threads):
                       "(cores x t_asm + t_dense)" % (name, k, threads, t_asm, dense_txt, t_dense)}
 
 
-def run_workload(H, name, steps, warmup):
+def run_workload(H, name, steps, warmup, mixed=False, bd=None, ref_sigma=None):
+    """Times `steps` sweeps of the workload (FP64, or with the mixed-precision
+    dense stage); with ref_sigma, also the largest relative sigma difference."""
     w = WORKLOADS[name]
     mesh = make_mesh(H, w)
-    sp = H.ContinuousSpace1()
-    bd = H.BuilderData(mesh, sp, sp, w["order"])
+    if bd is None:
+        sp = H.ContinuousSpace1()
+        bd = H.BuilderData(mesh, sp, sp, w["order"])
     N = w["panels"]
     n = 2 * N
     ks = np.linspace(w["k_min"], w["k_max"], w["nk"])
     R = H.gaussian_test_matrix(n, NC, w["seed"])
     der = w["der"]
+    fallback = [0]
 
     def sweep():
+        if mixed:
+            sig, used = H.sigma_sweep_mixed(bd, ks, w["c_o"], w["c_i"], R, Q, refine=MIXED_REFINE)
+            fallback[0] = int(used.sum())
+            return sig
         return H.sigma_sweep(bd, ks, w["c_o"], w["c_i"], R, Q, der=der)
 
     for _ in range(warmup):
@@ -130,6 +140,8 @@ def run_workload(H, name, steps, warmup):
     out = {
         "value": evals / elapsed,
         "ms_per_step": elapsed / steps * 1e3,
+        "sigma": sig,
+        "bd": bd,
         "config": {"workload": w["text"], "panels": N, "order": w["order"], "nk": w["nk"], "global_batch": w["nk"],
                    "parallelism": "single-gpu"},
         "panel_pair_evals_per_s": pairs_per_k * evals / asm_s if asm_s > 0 else None,
@@ -142,6 +154,11 @@ def run_workload(H, name, steps, warmup):
                      "avg_launch_ms": avg_launch_s * 1e3, "launches": n_gen,
                      "flops_per_launch": fl_gen / max(n_gen, 1)},
     }
+    if mixed:
+        out["refine_steps"] = MIXED_REFINE
+        out["fp64_fallbacks"] = fallback[0]
+    if ref_sigma is not None:
+        out["max_rel_sigma_err_vs_fp64"] = float(np.max(np.abs(sig - ref_sigma) / ref_sigma))
     return out, mesh, R
 
 
@@ -153,6 +170,7 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="polygon2000")
     ap.add_argument("--no-second", action="store_true", help="skip the star400 sweep")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision sweep")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 times), the reference "
@@ -175,7 +193,14 @@ def main():
 
     import helmholtzbem_amd as H
 
-    head, mesh, R = run_workload(H, args.workload, args.steps, args.warmup)
+    fp64, mesh, R = run_workload(H, args.workload, args.steps, args.warmup)
+    head = fp64
+    mixed = None
+    if WORKLOADS[args.workload]["der"] == 0 and not args.no_mixed:
+        mixed, _, _ = run_workload(H, args.workload, args.steps, args.warmup, mixed=True, bd=fp64["bd"],
+                                   ref_sigma=fp64["sigma"])
+        if mixed["max_rel_sigma_err_vs_fp64"] <= MIXED_TOL and mixed["value"] > fp64["value"]:
+            head = mixed
     out = {
         "metric": "sigma_min(k) evals/s",
         "value": head["value"],
@@ -187,18 +212,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 LU + f64 refinement (assembly f64)" if head is mixed else "f64",
         "data": "synthetic",
         "config": dict(head["config"], name=args.workload),
         "panel_pair_evals_per_s": head["panel_pair_evals_per_s"],
         "phase_seconds_per_step": head["phase_seconds_per_step"],
         "lu_model_tflops": head["lu_model_tflops"],
         "roofline": head["roofline"],
+        "path": "mixed" if head is mixed else "fp64",
     }
+    strip = lambda d: {k: v for k, v in d.items() if k not in ("sigma", "bd")}
+    out["fp64"] = strip(fp64)
+    if mixed is not None:
+        out["mixed"] = strip(mixed)
+        out["mixed"]["speedup_vs_fp64"] = mixed["value"] / fp64["value"]
     other = "star400" if args.workload == "polygon2000" else "polygon2000"
     if not args.no_second:
         sec, _, _ = run_workload(H, other, max(1, args.steps), 1)
-        out[other] = sec
+        out[other] = strip(sec)
     if not args.no_cpu_baseline:
         threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(args.workload, WORKLOADS[args.workload], mesh, R, threads)

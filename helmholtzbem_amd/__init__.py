@@ -29,7 +29,7 @@ from .geometry import (ParametrizedCircularArc, ParametrizedLine, ParametrizedMe
                        square_mesh, star_curve, star_mesh)
 
 __all__ = ["LayerType", "ContinuousSpace1", "BuilderData", "GalerkinBuilder", "SolutionsOperator",
-           "randomized_svd", "sigma_sweep", "sigma_sweep_multi", "HbemError", "ParametrizedLine", "ParametrizedCircularArc",
+           "randomized_svd", "sigma_sweep", "sigma_sweep_der2", "sigma_sweep_mixed", "sigma_sweep_multi", "HbemError", "ParametrizedLine", "ParametrizedCircularArc",
            "ParametrizedTrigCurve", "ParametrizedMesh", "square_mesh", "star_mesh", "circle_mesh",
            "polygon_mesh", "random_polygon_mesh", "gaussian_test_matrix", "Xoshiro256ss", "star_curve"]
 
@@ -446,6 +446,38 @@ def sigma_sweep(builder_data: BuilderData, ks, c_o, c_i, R, q=2, der=0, proj=Tru
     check(lib().hbem_sigma_sweep(g.h, _dptr(ks), ks.size, float(c_o), float(c_i), _vptr(R), R.shape[1], int(q),
                                  int(der), _dptr(sig), _dptr(dsig) if der else None))
     return (sig, dsig) if der else sig
+
+
+def sigma_sweep_der2(builder_data: BuilderData, ks, c_o, c_i, R, q=2, proj=True):
+    """(sigma, dsigma/dk, d2sigma/dk2) for every k: the operators with their
+    first and second k-derivatives and randomized_svd::sv_der2's bordered
+    system, batched on the device (the evaluations of the Newton refinement)."""
+    ks = np.ascontiguousarray(ks, dtype=np.float64)
+    if ks.ndim != 1:
+        raise ValueError("ks must be a 1-D array of wavenumbers")
+    R = _test_matrix(R, 2 * builder_data.getNumPanels())
+    g = builder_data._ctx(proj)
+    out = [np.zeros(ks.size) for _ in range(3)]
+    check(lib().hbem_sigma_sweep_der2(g.h, _dptr(ks), ks.size, float(c_o), float(c_i), _vptr(R), R.shape[1], int(q),
+                                      *[_dptr(o) for o in out]))
+    return tuple(out)
+
+
+def sigma_sweep_mixed(builder_data: BuilderData, ks, c_o, c_i, R, q=2, refine=3, proj=True):
+    """sigma_min(k) for every k with the dense stage in mixed precision (FP32
+    LU on the matrix cores, `refine` FP64 refinement steps per solve).
+    Returns (sigma, fp64_used): fp64_used[i] is True where the refinement did
+    not converge and k was recomputed with the FP64 factorisation."""
+    ks = np.ascontiguousarray(ks, dtype=np.float64)
+    if ks.ndim != 1:
+        raise ValueError("ks must be a 1-D array of wavenumbers")
+    R = _test_matrix(R, 2 * builder_data.getNumPanels())
+    g = builder_data._ctx(proj)
+    sig = np.zeros(ks.size)
+    used = np.zeros(ks.size, dtype=np.int32)
+    check(lib().hbem_sigma_sweep_mixed(g.h, _dptr(ks), ks.size, float(c_o), float(c_i), _vptr(R), R.shape[1], int(q),
+                                       int(refine), _dptr(sig), used.ctypes.data_as(_ip)))
+    return sig, used.astype(bool)
 
 
 def sigma_sweep_multi(instances, q=2, der=0, proj=True, max_workers=None):

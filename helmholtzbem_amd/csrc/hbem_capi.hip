@@ -12,6 +12,7 @@
 #include "hbem_internal.hpp"
 
 using hbem::cdbl;
+using hbem::cflt;
 
 namespace {
 
@@ -151,6 +152,25 @@ size_t sweep_bytes_per_k(long long npairs, long long nadj, long long ncoi, int n
            2 * (size_t)n * sizeof(int) + (4 * (size_t)ncols + 4) * n * sizeof(cdbl) + 64;
 }
 
+// Extra device bytes per wavenumber of the sigma'' stage: T_p, the bordered
+// 2n x 2n matrix (which is also the projection's scratch), its pivots and
+// the 2n-vectors of the two solves.
+size_t der2_bytes_per_k(int n) {
+    const size_t nn = (size_t)n * n;
+    return 5 * nn * sizeof(cdbl) + 4 * (size_t)n * sizeof(int) + 16 * (size_t)n * sizeof(cdbl) + 64;
+}
+
+// Extra device bytes per wavenumber of the mixed-precision dense stage: the
+// FP32 copy of T (factorised in place) and the refinement's thin blocks.
+size_t mixed_bytes_per_k(int n, int ncols) {
+    return (size_t)n * n * sizeof(cflt) + (2 * (size_t)ncols * n + hbem::residual_part_count(n, ncols, 1)) * sizeof(cdbl) +
+           64;
+}
+
+// Refinement tolerance of the mixed-precision sweep: a k is recomputed in FP64
+// when its last correction exceeds this fraction of the solution.
+constexpr double kMixedTol = 1e-9;
+
 // Wavenumbers per sweep batch: as many as fit in `avail` bytes, at most 256
 // (the dense stage then has about one matrix per CU; more only adds latency),
 // and balanced so the batches of nk wavenumbers have equal sizes.
@@ -193,9 +213,11 @@ struct hbem_geometry {
     DBuf d_cent, d_cpidx, d_cflags, d_cngeo;
     int ngen = 0, nadj = 0, ncoi = 0;
     DBuf d_M, d_Finvc;
-    DCsr s_F, s_FT, s_FinvT;   // F, F^T and Finv^T of the projection
+    DCsr s_F, s_FT, s_FinvT, s_Finv;   // F, F^T, Finv^T and Finv of the projection
     // work space
     DBuf w_Ao, w_U, w_rhs, w_flag;
+    DBuf w_Tp, w_B, w_mv, w_bpiv, w_bperm, w_bvec, w_d2;   // bordered system of sv_der2 (sweeps)
+    DBuf w_T32, w_rb, w_rr, w_rpart, w_flag2;                // mixed-precision dense stage
     DBuf w_kapi, w_kri, w_L, w_T0, w_T1, w_T2, w_piv, w_perm, w_X, w_Y, w_work, w_R, w_u, w_v, w_a, w_b, w_sig, w_dsig, w_kap,
         w_kr, w_cv, w_qi, w_qo, w_tmp, w_out, w_part;
     // profiling (hbem_profile_enable / query)
@@ -204,7 +226,8 @@ struct hbem_geometry {
     std::mutex mu;
     // bytes held by the per-batch work buffers of the sweep
     size_t work_held() const {
-        const DBuf* bs[] = {&w_L, &w_part, &w_T0, &w_T1, &w_T2, &w_piv, &w_perm, &w_X, &w_Y, &w_work, &w_u, &w_v, &w_a, &w_b};
+        const DBuf* bs[] = {&w_L, &w_part, &w_T0, &w_T1, &w_T2, &w_piv, &w_perm, &w_X, &w_Y, &w_work, &w_u, &w_v, &w_a, &w_b,
+                            &w_Tp, &w_B, &w_bpiv, &w_bperm, &w_bvec, &w_T32, &w_rb, &w_rr, &w_rpart};
         size_t s = 0;
         for (auto* b : bs) s += b->n;
         return s;
@@ -330,7 +353,35 @@ struct DenseCtx {
     cdbl* Y;
     cdbl* work;          // 2 n nc batch: thin-solve scratch, then the projected right-hand sides
     EventLog* tim = nullptr;   // randomized_svd timers (kTim*), or null
+    // mixed precision (LU32 set): the factors are FP32 (LU32, stride ms) and
+    // every solve with T is refined `refine` times against T64 (the FP64 T,
+    // stride ms): residual in FP64, correction from the FP32 factors
+    const cflt* LU32 = nullptr;
+    const cdbl* T64 = nullptr;
+    int refine = 0;
+    double rtol = 0.0;   // a matrix whose last correction exceeds rtol of its solution is flagged
+    int* flag = nullptr; // per matrix
+    cdbl* rb = nullptr;  // n nc batch: the right-hand side
+    cdbl* rr = nullptr;  // n nc batch: residual, then correction
+    cdbl* rpart = nullptr;
 };
+
+// Z <- op(T)^{-1} Z with the context's factors (op = T or T^H): FP64 factors,
+// or FP32 factors plus `refine` steps of FP64 iterative refinement.
+void t_solve(DenseCtx& c, cdbl* Z, bool adj) {
+    if (!c.LU32) {
+        hbem::lu_solve_thin_batched(c.LU, c.ms, c.perm, c.n, c.batch, Z, c.nc, adj, c.work, c.st);
+        return;
+    }
+    const size_t cnt = (size_t)c.n * c.nc;
+    HIPC(hipMemcpyAsync(c.rb, Z, sizeof(cdbl) * cnt * c.batch, hipMemcpyDeviceToDevice, c.st));
+    hbem::lu_solve_thin_batched_f32(c.LU32, c.ms, c.perm, c.n, c.batch, Z, c.nc, adj, c.work, c.st);
+    for (int it = 0; it < c.refine; ++it) {
+        hbem::residual_thin_batched(c.T64, c.ms, Z, c.rb, c.rr, c.n, c.nc, c.batch, adj, c.rpart, c.st);
+        hbem::lu_solve_thin_batched_f32(c.LU32, c.ms, c.perm, c.n, c.batch, c.rr, c.nc, adj, c.work, c.st);
+        hbem::refine_update_batched(Z, c.rr, (int)cnt, c.batch, c.rtol, it == c.refine - 1, c.flag, c.st);
+    }
+}
 
 // Y <- T_p^{-1} X  or  T_p^{-H} X  (T_p^{-1} = F^T T^{-1} F, T_p^{-H} = F^T T^{-H} F); X is kept.
 void tp_solve(DenseCtx& c, const cdbl* X, cdbl* Y, bool adj) {
@@ -339,12 +390,12 @@ void tp_solve(DenseCtx& c, const cdbl* X, cdbl* Y, bool adj) {
         cdbl* Z = c.work + (size_t)c.n * cols;
         hbem::csr_apply_batched((const int*)c.F->rp.p, (const int*)c.F->ci.p, (const double*)c.F->v.p, c.n, X, Z, cols,
                                 c.st);
-        hbem::lu_solve_thin_batched(c.LU, c.ms, c.perm, c.n, c.batch, Z, c.nc, adj, c.work, c.st);
+        t_solve(c, Z, adj);
         hbem::csr_apply_batched((const int*)c.FT->rp.p, (const int*)c.FT->ci.p, (const double*)c.FT->v.p, c.n, Z, Y,
                                 cols, c.st);
     } else {
         HIPC(hipMemcpyAsync(Y, X, sizeof(cdbl) * (size_t)c.n * cols, hipMemcpyDeviceToDevice, c.st));
-        hbem::lu_solve_thin_batched(c.LU, c.ms, c.perm, c.n, c.batch, Y, c.nc, adj, c.work, c.st);
+        t_solve(c, Y, adj);
     }
 }
 
@@ -372,11 +423,12 @@ void copy_out_square(const cdbl* d, void* h, size_t count, hipStream_t st) {
     if (h) HIPC(hipMemcpyAsync(h, d, count * sizeof(cdbl), hipMemcpyDeviceToHost, st));
 }
 
-// T_p = Finv T Finv^T for `count` matrices (in place through tmp)
-void project_inplace(hbem_geometry* g, cdbl* T, int count) {
+// T_p = Finv T Finv^T for `count` matrices (in place through tmp: count n^2
+// entries, the context's buffer when null)
+void project_inplace(hbem_geometry* g, cdbl* T, int count, cdbl* tmp = nullptr) {
     const int n = 2 * g->N;
     const long long s = (long long)n * n;
-    cdbl* tmp = g->w_tmp.as<cdbl>((size_t)s * count);
+    if (!tmp) tmp = g->w_tmp.as<cdbl>((size_t)s * count);
     const cdbl* Fi = (const cdbl*)g->d_Finvc.p;
     hbem::zgemm_batched(n, n, n, hbem::mk(1, 0), Fi, n, 0, T, n, s, false, hbem::mk(0, 0), tmp, n, s, count, g->st);
     hbem::zgemm_batched(n, n, n, hbem::mk(1, 0), tmp, n, s, Fi, n, 0, true, hbem::mk(0, 0), T, n, s, count, g->st);
@@ -551,6 +603,7 @@ int hbem_geometry_create(const double* panels, int npanels, const double* coefs,
         g->s_F.build(g->F, n, false, st);
         g->s_FT.build(g->F, n, true, st);
         g->s_FinvT.build(g->Finv, n, true, st);
+        g->s_Finv.build(g->Finv, n, false, st);
         std::vector<cdbl> Fic((size_t)n * n);
         for (size_t e = 0; e < Fic.size(); ++e) Fic[e] = hbem::mk(g->Finv[e], 0.0);
         upload(g->d_Finvc, Fic, st);
@@ -966,27 +1019,87 @@ int hbem_rsvd_sv_der2(const void* T, const void* Td, const void* Tdd, int n, con
     API_END
 }
 
-int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i, const void* R, int nc, int q,
-                     int want_der, double* sigma, double* dsigma) {
-    API_BEGIN
-    if (!g || !ks || !R || !sigma) throw Error(HBEM_EINVAL, "null argument");
-    if (nk < 0) throw Error(HBEM_EINVAL, "negative number of wavenumbers");
+}  // extern "C"
+
+namespace {
+
+// out_head = T_p^(t) x_tail, out_tail = T_p^(t)H x_head for the 2n-vectors X of
+// the batch (2n x nb), with T_p^(t) = F^{-1} Tt F^{-T} (Tt: n x n x nb) or Tt
+// itself without projection.  h: scratch of 4 n nb complex.
+void apply_pair_op(hbem_geometry* g, const cdbl* Tt, const cdbl* X, cdbl* out, int nb, cdbl* h) {
     const int n = 2 * g->N;
-    check_dense_args(ks, n, R, nc, q);
-    if (want_der && !dsigma) throw Error(HBEM_EINVAL, "null dsigma");
-    for (int i = 0; i < nk; ++i) { check_k(ks[i], 0.0, c_i); check_k(ks[i], 0.0, c_o); }
-    std::lock_guard<std::mutex> lk(g->mu);
-    HIPC(hipSetDevice(g->dev));
-    const int D = want_der ? (want_der >= 2 ? 2 : 1) : 0;
+    const long long nn = (long long)n * n, m2 = 2LL * n;
+    cdbl *xh = h, *xt = h + (size_t)n * nb, *z = h + 2 * (size_t)n * nb, *y = h + 3 * (size_t)n * nb;
+    hbem::copy_blocks_batched(X, m2, 0, xh, n, 0, n, nb, g->st);
+    hbem::copy_blocks_batched(X, m2, n, xt, n, 0, n, nb, g->st);
+    for (int part = 0; part < 2; ++part) {
+        const cdbl* src = part == 0 ? xt : xh;
+        const bool adj = part == 1;
+        const long long off = part == 0 ? 0 : n;
+        if (g->proj) {
+            const DCsr& A = g->s_FinvT;
+            const DCsr& Bm = g->s_Finv;
+            hbem::csr_apply_batched((const int*)A.rp.p, (const int*)A.ci.p, (const double*)A.v.p, n, src, z, nb, g->st);
+            hbem::gemv_batched(Tt, nn, z, y, n, nb, adj, g->st);
+            hbem::csr_apply_batched((const int*)Bm.rp.p, (const int*)Bm.ci.p, (const double*)Bm.v.p, n, y, z, nb, g->st);
+            hbem::copy_blocks_batched(z, n, 0, out, m2, off, n, nb, g->st);
+        } else {
+            hbem::gemv_batched(Tt, nn, src, y, n, nb, adj, g->st);
+            hbem::copy_blocks_batched(y, n, 0, out, m2, off, n, nb, g->st);
+        }
+    }
+}
+
+// The sweep shared by hbem_sigma_sweep and hbem_sigma_sweep_der2: sigma for
+// every k, dsigma/dk when dsigma is given (operators of order D >= 1), and
+// d2sigma/dk2 when d2sigma is given (D = 2).
+// sigma of the matrices idx of a batch from their FP64 T (T0, stride n^2) with
+// the FP64 factorisation: the fallback of the mixed-precision sweep
+void sweep_fp64_subset(hbem_geometry* g, const cdbl* T0, const std::vector<int>& idx, const cdbl* dR, int nc, int q,
+                       double* sigma_host) {
+    const int n = 2 * g->N, m = (int)idx.size();
     const size_t nn = (size_t)n * n;
+    if (m == 0) return;
+    cdbl* Tf = g->w_T1.as<cdbl>(nn * m);
+    for (int j = 0; j < m; ++j)
+        HIPC(hipMemcpyAsync(Tf + j * nn, T0 + (size_t)idx[j] * nn, nn * sizeof(cdbl), hipMemcpyDeviceToDevice, g->st));
+    int* piv = g->w_bpiv.as<int>((size_t)n * m);
+    int* perm = g->w_bperm.as<int>((size_t)n * m);
+    hbem::lu_factor_batched(Tf, n, m, piv, perm, g->st);
+    DenseCtx c;
+    c.st = g->st; c.n = n; c.batch = m; c.nc = nc; c.q = q; c.LU = Tf; c.ms = (long long)nn; c.perm = perm;
+    c.F = g->proj ? &g->s_F : nullptr;
+    c.FT = g->proj ? &g->s_FT : nullptr;
+    c.X = g->w_X.as<cdbl>((size_t)n * nc * m);
+    c.Y = g->w_Y.as<cdbl>((size_t)n * nc * m);
+    c.work = g->w_work.as<cdbl>(2 * (size_t)n * nc * m);
+    hbem::replicate_batched(dR, (long long)n * nc, m, c.X, g->st);
+    double* dsig = g->w_d2.as<double>(m);
+    rsvd_core(c, dsig, nullptr, nullptr);
+    std::vector<double> out(m);
+    HIPC(hipMemcpyAsync(out.data(), dsig, m * sizeof(double), hipMemcpyDeviceToHost, g->st));
+    HIPC(hipStreamSynchronize(g->st));
+    for (int j = 0; j < m; ++j) sigma_host[idx[j]] = out[j];
+}
+
+// mixed >= 0 (D = 0 only): the dense stage in mixed precision with `mixed`
+// refinement steps per solve; fp64_used (may be null) marks the k recomputed
+// in FP64.
+void sweep_run(hbem_geometry* g, const double* ks, int nk, double c_o, double c_i, const void* R, int nc, int q, int D,
+               double* sigma, double* dsigma, double* d2sigma, int mixed = -1, int* fp64_used = nullptr) {
+    const int n = 2 * g->N;
+    const size_t nn = (size_t)n * n;
+    const bool mx = mixed >= 0 && D == 0 && !d2sigma;
     // Batch of B wavenumbers -> 2B slots ordered [c_i for every k | c_o for every k]:
     // each wavefront of the assembly sees a single refraction index and 64
     // consecutive k, so the arguments kappa r of its lanes stay close (fewer
     // divergent Bessel branches).  B <= 256 gives the dense stage (one
     // workgroup per matrix in the panel factorisation and the thin solves)
     // up to one matrix per CU.  B is bounded by the free device memory: local
-    // matrices, chunk partials and T, T', T'' per wavenumber.
-    const size_t per = sweep_bytes_per_k(g->npairs, g->nadj, g->ncoi, g->nc, n, D, nc);
+    // matrices, chunk partials and T, T', T'' per wavenumber (and the
+    // bordered systems of the sigma'' stage).
+    const size_t per = sweep_bytes_per_k(g->npairs, g->nadj, g->ncoi, g->nc, n, D, nc) + (d2sigma ? der2_bytes_per_k(n) : 0) +
+                       (mx ? mixed_bytes_per_k(n, nc) : 0);
     const int B = sweep_batch(per, usable_device_bytes(g->work_held()), nk);
     cdbl* dR = g->w_R.as<cdbl>((size_t)n * nc);
     HIPC(hipMemcpyAsync(dR, R, (size_t)n * nc * sizeof(cdbl), hipMemcpyHostToDevice, g->st));
@@ -1015,8 +1128,16 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
         if (g->prof) g->plog.end(g->st, kProfFormT, e0);
         int* piv = g->w_piv.as<int>((size_t)n * nb);
         int* perm = g->w_perm.as<int>((size_t)n * nb);
+        cflt* T32 = nullptr;
         if (g->prof) e0 = g->plog.begin(g->st);
-        hbem::lu_factor_batched(T0, n, nb, piv, perm, g->st);
+        if (mx) {
+            // FP32 copy factorised on the FP32 matrix cores; T0 stays the FP64 T of the residuals
+            T32 = g->w_T32.as<cflt>(nn * nb);
+            hbem::to_f32(T0, (long long)(nn * nb), T32, g->st);
+            hbem::lu_factor_batched_f32(T32, n, nb, piv, perm, g->st);
+        } else {
+            hbem::lu_factor_batched(T0, n, nb, piv, perm, g->st);
+        }
         if (g->prof) {
             g->plog.end(g->st, kProfLU, e0, (8.0 / 3.0) * (double)n * n * n * nb);
             e0 = g->plog.begin(g->st);
@@ -1028,12 +1149,37 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
         c.X = g->w_X.as<cdbl>((size_t)n * nc * nb);
         c.Y = g->w_Y.as<cdbl>((size_t)n * nc * nb);
         c.work = g->w_work.as<cdbl>(2 * (size_t)n * nc * nb);
+        int* flag = nullptr;
+        if (mx) {
+            c.LU32 = T32;
+            c.T64 = T0;
+            c.refine = mixed;
+            c.rtol = kMixedTol;
+            flag = g->w_flag2.as<int>(nb);
+            hbem::fill_int(flag, nb, mixed > 0 ? 0 : 1, g->st);   // no refinement: nothing certifies the solves
+            c.flag = flag;
+            c.rb = g->w_rb.as<cdbl>((size_t)n * nc * nb);
+            c.rr = g->w_rr.as<cdbl>((size_t)n * nc * nb);
+            c.rpart = g->w_rpart.as<cdbl>(hbem::residual_part_count(n, nc, nb));
+        }
         hbem::replicate_batched(dR, (long long)n * nc, nb, c.X, g->st);
         double* dsig = g->w_sig.as<double>(nb);
         cdbl* du = D ? g->w_u.as<cdbl>((size_t)n * nb) : nullptr;
         cdbl* dv = D ? g->w_v.as<cdbl>((size_t)n * nb) : nullptr;
         rsvd_core(c, dsig, du, dv);
         HIPC(hipMemcpyAsync(sigma + k0, dsig, nb * sizeof(double), hipMemcpyDeviceToHost, g->st));
+        if (mx) {
+            // k whose refinement did not converge: sigma again with the FP64 factorisation
+            std::vector<int> hf(nb);
+            HIPC(hipMemcpyAsync(hf.data(), flag, nb * sizeof(int), hipMemcpyDeviceToHost, g->st));
+            HIPC(hipStreamSynchronize(g->st));
+            std::vector<int> idx;
+            for (int b = 0; b < nb; ++b) {
+                if (hf[b]) idx.push_back(b);
+                if (fp64_used) fp64_used[k0 + b] = hf[b] ? 1 : 0;
+            }
+            sweep_fp64_subset(g, T0, idx, dR, nc, q, sigma + k0);
+        }
         if (D) {
             cdbl* a = g->w_a.as<cdbl>((size_t)n * nb);
             cdbl* bb = g->w_b.as<cdbl>((size_t)n * nb);
@@ -1047,12 +1193,88 @@ int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, do
             }
             double* dd = g->w_dsig.as<double>((size_t)nb * (1 + hbem::kBilinearParts));
             hbem::bilinear_re_batched(T1, (long long)nn, a, bb, n, nb, dd, dd + nb, g->st);
-            HIPC(hipMemcpyAsync(dsigma + k0, dd, nb * sizeof(double), hipMemcpyDeviceToHost, g->st));
+            if (dsigma) HIPC(hipMemcpyAsync(dsigma + k0, dd, nb * sizeof(double), hipMemcpyDeviceToHost, g->st));
+        }
+        if (d2sigma) {
+            // randomized_svd::sv_der2's bordered system (randsvd.cpp:171-216) for
+            // every k of the batch, on T_p formed explicitly: T again from the
+            // local matrices (T0 holds its LU factors), projected through the
+            // bordered matrix's storage.
+            const int m2 = 2 * n;
+            cdbl* Tp = g->w_Tp.as<cdbl>(nn * nb);
+            cdbl* Bw = g->w_B.as<cdbl>(4 * nn * nb);
+            hbem::launch_form_T(L, 2 * nb, 16 * (D + 1), dqi, dqo, nb, g->N, 0, (const double*)g->d_M.p, Tp, nullptr,
+                                nullptr, g->st);
+            if (g->proj) project_inplace(g, Tp, nb, Bw);
+            int* mv = g->w_mv.as<int>(nb);
+            cdbl* vec = g->w_bvec.as<cdbl>(16 * (size_t)n * nb);
+            cdbl *X = vec, *P = vec + (size_t)m2 * nb, *Pd = vec + 2 * (size_t)m2 * nb, *P2 = vec + 3 * (size_t)m2 * nb;
+            cdbl *h = vec + 4 * (size_t)m2 * nb, *wk = vec + 6 * (size_t)m2 * nb, *tv = vec + 7 * (size_t)m2 * nb;
+            hbem::border_prep_batched(du, dv, n, nb, mv, X, g->st);
+            hbem::border_build_batched(Tp, n, nb, dsig, mv, X, Bw, g->st);
+            int* bpiv = g->w_bpiv.as<int>((size_t)m2 * nb);
+            int* bperm = g->w_bperm.as<int>((size_t)m2 * nb);
+            hbem::lu_factor_batched(Bw, m2, nb, bpiv, bperm, g->st);
+            apply_pair_op(g, T1, X, P, nb, h);                       // p = [T_p' u_2; T_p'^H u_1]
+            hbem::lu_solve_thin_batched(Bw, 4LL * (long long)nn, bperm, m2, nb, P, 1, false, wk, g->st);
+            hbem::border_negate_pick_batched(P, n, nb, mv, tv, g->st);   // P = u', tv = t
+            apply_pair_op(g, T1, P, Pd, nb, h);                      // [T_p' u'_2; T_p'^H u'_1]
+            apply_pair_op(g, T2, X, P2, nb, h);                      // [T_p'' u_2; T_p''^H u_1]
+            hbem::border_rhs_batched(Pd, P2, P, tv, n, nb, Pd, g->st);
+            hbem::lu_solve_thin_batched(Bw, 4LL * (long long)nn, bperm, m2, nb, Pd, 1, false, wk, g->st);
+            double* d2 = g->w_d2.as<double>(nb);
+            hbem::border_final_batched(Pd, n, nb, mv, d2, g->st);
+            HIPC(hipMemcpyAsync(d2sigma + k0, d2, nb * sizeof(double), hipMemcpyDeviceToHost, g->st));
         }
         if (g->prof) g->plog.end(g->st, kProfRsvd, e0);
         HIPC(hipGetLastError());
     }
     HIPC(hipStreamSynchronize(g->st));
+}
+
+void check_sweep_args(hbem_geometry* g, const double* ks, int nk, double c_o, double c_i, const void* R, int nc, int q,
+                      const double* sigma) {
+    if (!g || !ks || !R || !sigma) throw Error(HBEM_EINVAL, "null argument");
+    if (nk < 0) throw Error(HBEM_EINVAL, "negative number of wavenumbers");
+    check_dense_args(ks, 2 * g->N, R, nc, q);
+    for (int i = 0; i < nk; ++i) { check_k(ks[i], 0.0, c_i); check_k(ks[i], 0.0, c_o); }
+}
+
+}  // namespace
+
+extern "C" {
+
+int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i, const void* R, int nc, int q,
+                     int want_der, double* sigma, double* dsigma) {
+    API_BEGIN
+    check_sweep_args(g, ks, nk, c_o, c_i, R, nc, q, sigma);
+    if (want_der && !dsigma) throw Error(HBEM_EINVAL, "null dsigma");
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPC(hipSetDevice(g->dev));
+    const int D = want_der ? (want_der >= 2 ? 2 : 1) : 0;
+    sweep_run(g, ks, nk, c_o, c_i, R, nc, q, D, sigma, D ? dsigma : nullptr, nullptr);
+    API_END
+}
+
+int hbem_sigma_sweep_der2(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i, const void* R, int nc,
+                          int q, double* sigma, double* dsigma, double* d2sigma) {
+    API_BEGIN
+    check_sweep_args(g, ks, nk, c_o, c_i, R, nc, q, sigma);
+    if (!dsigma || !d2sigma) throw Error(HBEM_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPC(hipSetDevice(g->dev));
+    sweep_run(g, ks, nk, c_o, c_i, R, nc, q, 2, sigma, dsigma, d2sigma);
+    API_END
+}
+
+int hbem_sigma_sweep_mixed(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i, const void* R, int nc,
+                           int q, int refine, double* sigma, int* fp64_used) {
+    API_BEGIN
+    check_sweep_args(g, ks, nk, c_o, c_i, R, nc, q, sigma);
+    if (refine < 0 || refine > 20) throw Error(HBEM_EINVAL, "number of refinement steps must be in [0, 20]");
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPC(hipSetDevice(g->dev));
+    sweep_run(g, ks, nk, c_o, c_i, R, nc, q, 0, sigma, nullptr, nullptr, refine, fp64_used);
     API_END
 }
 

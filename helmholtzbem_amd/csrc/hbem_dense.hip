@@ -36,6 +36,23 @@ __device__ __forceinline__ cdbl shfl_c(cdbl v, int src) {
     return mk(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
 }
 
+// The same operations on single-precision complex values (the factors of the
+// mixed-precision LU).
+__device__ __forceinline__ float cabs2(cflt a) { return a.x * a.x + a.y * a.y; }
+__device__ __forceinline__ cflt operator*(cflt a, cflt b) { return cflt{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ cflt crecip(cflt d) {
+    const float s = 1.0f / (d.x * d.x + d.y * d.y);
+    return cflt{d.x * s, -d.y * s};
+}
+__device__ __forceinline__ void cfms(cflt& acc, cflt a, cflt b) {   // acc -= a * b
+    acc.x = fmaf(-a.x, b.x, fmaf(a.y, b.y, acc.x));
+    acc.y = fmaf(-a.x, b.y, fmaf(-a.y, b.x, acc.y));
+}
+__device__ __forceinline__ cflt shfl_c(cflt v, int src) { return cflt{__shfl(v.x, src, 64), __shfl(v.y, src, 64)}; }
+// widening load of a factor entry for double-precision arithmetic
+__device__ __forceinline__ cdbl to_cdbl(cdbl v) { return v; }
+__device__ __forceinline__ cdbl to_cdbl(cflt v) { return mk((double)v.x, (double)v.y); }
+
 // ---------------------------------------------------------------------------
 // LU with partial pivoting
 // ---------------------------------------------------------------------------
@@ -44,13 +61,14 @@ __device__ __forceinline__ cdbl shfl_c(cdbl v, int src) {
 // ties) by wavefront shuffles and one LDS exchange, the row interchange inside
 // the panel, then the rank-1 update of the panel with one thread per row and
 // the pivot row staged in LDS.
-__global__ __launch_bounds__(1024) void k_lu_panel(cdbl* __restrict__ A, int n, int j0, int jb, int* __restrict__ ipiv) {
-    cdbl* a = A + (size_t)blockIdx.x * n * n;
+template <class C>
+__global__ __launch_bounds__(1024) void k_lu_panel(C* __restrict__ A, int n, int j0, int jb, int* __restrict__ ipiv) {
+    C* a = A + (size_t)blockIdx.x * n * n;
     int* ip = ipiv + (size_t)blockIdx.x * n;
     __shared__ double sv[16];
     __shared__ int si[16];
     __shared__ int spiv;
-    __shared__ cdbl prow[32];
+    __shared__ C prow[32];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int c = 0; c < jb; ++c) {
         const int col = j0 + c;
@@ -84,20 +102,20 @@ __global__ __launch_bounds__(1024) void k_lu_panel(cdbl* __restrict__ A, int n, 
         __syncthreads();
         const int piv = spiv;
         if (tid < jb) {
-            cdbl* pc = a + (size_t)(j0 + tid) * n;
-            const cdbl u = pc[piv];
+            C* pc = a + (size_t)(j0 + tid) * n;
+            const C u = pc[piv];
             if (piv != col) { pc[piv] = pc[col]; pc[col] = u; }
             prow[tid] = u;
         }
         __syncthreads();
-        const cdbl d = prow[c];
-        const cdbl inv = (d.x == 0.0 && d.y == 0.0) ? mk(0, 0) : crecip(d);
+        const C d = prow[c];
+        const C inv = (d.x == 0.0 && d.y == 0.0) ? C{} : crecip(d);
         for (int i = col + 1 + tid; i < n; i += blockDim.x) {
-            const cdbl l = a[i + (size_t)col * n] * inv;
+            const C l = a[i + (size_t)col * n] * inv;
             a[i + (size_t)col * n] = l;
             // eight columns per round, loads issued together
             for (int cc0 = c + 1; cc0 < jb; cc0 += 8) {
-                cdbl v[8];
+                C v[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
                     if (cc0 + u < jb) v[u] = a[i + (size_t)(j0 + cc0 + u) * n];
@@ -124,19 +142,19 @@ __global__ __launch_bounds__(1024) void k_lu_panel(cdbl* __restrict__ A, int n, 
 // The panel is read and written about jb / CW + 1 times instead of once per
 // column step.  R x CW complex values stay in registers (at most 32 of the
 // 128 VGPRs a thread of a 1024-thread workgroup has).
-template <int R, int CW>
-__global__ __launch_bounds__(1024) void k_lu_panel_rg(cdbl* __restrict__ A, int n, int j0, int jb,
+template <class C, int R, int CW>
+__global__ __launch_bounds__(1024) void k_lu_panel_rg(C* __restrict__ A, int n, int j0, int jb,
                                                       int* __restrict__ ipiv) {
     constexpr int YB = R >= 2 ? 4 : 8;           // trailing columns updated per round
-    cdbl* a = A + (size_t)blockIdx.x * n * n;
+    C* a = A + (size_t)blockIdx.x * n * n;
     int* ip = ipiv + (size_t)blockIdx.x * n;
     __shared__ double sv[16];
     __shared__ int si[16];
     __shared__ int spiv;
-    __shared__ cdbl prow[CW];                   // pivot row (chunk columns)
-    __shared__ cdbl xrow[CW];                   // the row it displaces
-    __shared__ cdbl L11[CW][CW];
-    __shared__ cdbl U12[CW][32];
+    __shared__ C prow[CW];                   // pivot row (chunk columns)
+    __shared__ C xrow[CW];                   // the row it displaces
+    __shared__ C L11[CW][CW];
+    __shared__ C U12[CW][32];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     int rowu[R];
 #pragma unroll
@@ -144,12 +162,12 @@ __global__ __launch_bounds__(1024) void k_lu_panel_rg(cdbl* __restrict__ A, int 
     for (int c0 = 0; c0 < jb; c0 += CW) {
         const int w = min(CW, jb - c0);
         const int cb = j0 + c0;                  // first column of the chunk = its first diagonal row
-        cdbl v[R][CW];
+        C v[R][CW];
 #pragma unroll
         for (int u = 0; u < R; ++u) {
             const bool own = rowu[u] < n && rowu[u] >= cb;
 #pragma unroll
-            for (int e = 0; e < CW; ++e) v[u][e] = (own && e < w) ? a[rowu[u] + (size_t)(cb + e) * n] : mk(0, 0);
+            for (int e = 0; e < CW; ++e) v[u][e] = (own && e < w) ? a[rowu[u] + (size_t)(cb + e) * n] : C{};
         }
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
@@ -198,8 +216,8 @@ __global__ __launch_bounds__(1024) void k_lu_panel_rg(cdbl* __restrict__ A, int 
                 }
             }
             if (piv != col && tid < jb && (tid < c0 || tid >= c0 + w)) {
-                cdbl* pc = a + (size_t)(j0 + tid) * n;
-                const cdbl t = pc[piv];
+                C* pc = a + (size_t)(j0 + tid) * n;
+                const C t = pc[piv];
                 pc[piv] = pc[col];
                 pc[col] = t;
             }
@@ -216,12 +234,12 @@ __global__ __launch_bounds__(1024) void k_lu_panel_rg(cdbl* __restrict__ A, int 
                     }
                 }
             }
-            const cdbl d = prow[c];
-            const cdbl inv = (d.x == 0.0 && d.y == 0.0) ? mk(0, 0) : crecip(d);
+            const C d = prow[c];
+            const C inv = (d.x == 0.0 && d.y == 0.0) ? C{} : crecip(d);
 #pragma unroll
             for (int u = 0; u < R; ++u)
                 if (rowu[u] < n && rowu[u] > col) {
-                    const cdbl l = v[u][c] * inv;
+                    const C l = v[u][c] * inv;
                     v[u][c] = l;
 #pragma unroll
                     for (int e = c + 1; e < CW; ++e)
@@ -248,8 +266,8 @@ __global__ __launch_bounds__(1024) void k_lu_panel_rg(cdbl* __restrict__ A, int 
         // (loads below take clamped, in-range addresses so that they issue
         // together; what lies outside the range is never used or stored)
         if (tid < rem) {
-            cdbl* pc = a + (size_t)(cb + w + tid) * n + cb;
-            cdbl x[CW];
+            C* pc = a + (size_t)(cb + w + tid) * n + cb;
+            C x[CW];
 #pragma unroll
             for (int r = 0; r < CW; ++r) x[r] = pc[min(r, w - 1)];
 #pragma unroll
@@ -270,7 +288,7 @@ __global__ __launch_bounds__(1024) void k_lu_panel_rg(cdbl* __restrict__ A, int 
             if (row >= n) break;                 // rows ascend with u
             const bool upd = row >= cb + w;
             for (int cc0 = 0; cc0 < rem; cc0 += YB) {
-                cdbl y[YB];
+                C y[YB];
 #pragma unroll
                 for (int e = 0; e < YB; ++e) y[e] = a[row + (size_t)(cb + w + min(cc0 + e, rem - 1)) * n];
 #pragma unroll
@@ -300,13 +318,13 @@ __global__ __launch_bounds__(1024) void k_lu_panel_rg(cdbl* __restrict__ A, int 
 //   * Solve: a wavefront takes 8 columns, 8 lanes per column holding 4 rows
 //     each in registers; step k broadcasts row k within its column by a lane
 //     permute and updates the rows below it with L11 read from LDS.
-template <int MODE>
-__global__ __launch_bounds__(256) void k_lu_rows(cdbl* __restrict__ A, int n, int j0, int jb,
+template <class C, int MODE>
+__global__ __launch_bounds__(256) void k_lu_rows(C* __restrict__ A, int n, int j0, int jb,
                                                  const int* __restrict__ ipiv) {
-    cdbl* a = A + (size_t)blockIdx.y * n * n;
+    C* a = A + (size_t)blockIdx.y * n * n;
     const int* ip = ipiv + (size_t)blockIdx.y * n;
-    __shared__ cdbl rows[64][33];   // slot t < jb: panel row j0 + t; slot jb + e: pivot row erow[e]
-    __shared__ cdbl L[32][33];
+    __shared__ C rows[64][33];   // slot t < jb: panel row j0 + t; slot jb + e: pivot row erow[e]
+    __shared__ C L[32][33];
     __shared__ int erow[32];
     __shared__ int src[64];
     __shared__ int nes;
@@ -364,11 +382,11 @@ __global__ __launch_bounds__(256) void k_lu_rows(cdbl* __restrict__ A, int n, in
         const int c8 = lane & 7, g = lane >> 3;
         const int cc = 8 * wv + c8;
         const bool act = cc < ncol && colof(cc) >= j0 + jb;
-        cdbl x[4];
+        C x[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int i = g + 8 * u;
-            x[u] = (act && i < jb) ? rows[i][cc] : mk(0, 0);
+            x[u] = (act && i < jb) ? rows[i][cc] : C{};
         }
         // row k = 8 uk + kk sits in register uk of lane kk * 8 + c8
 #pragma unroll
@@ -376,7 +394,7 @@ __global__ __launch_bounds__(256) void k_lu_rows(cdbl* __restrict__ A, int n, in
             for (int kk = 0; kk < 8; ++kk) {
                 const int k = 8 * uk + kk;
                 if (k + 1 >= jb) break;
-                const cdbl xk = shfl_c(x[uk], (kk << 3) | c8);
+                const C xk = shfl_c(x[uk], (kk << 3) | c8);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int i = g + 8 * u;
@@ -553,6 +571,124 @@ void zgemm_batched(int m, int n, int k, cdbl alpha, const cdbl* A, int lda, long
                            batch);
 }
 
+// C -= A B on the FP32 matrix cores for the mixed-precision LU (single-
+// precision complex, column-major, all three with leading dimension ld and
+// batch stride sm): the tiling of k_zgemm -- 64 x 64 tiles of four
+// wavefronts of 32 x 32, k through LDS in steps of 16 with the next step
+// prefetched, the 3M complex product formed transposed, XCD-grouped
+// numbering -- with v_mfma_f32_16x16x4f32, FP32 operands and FP32
+// accumulation.  Element v of lane l of that instruction's result is
+// D(4 (l >> 4) + v, l & 15) (the transposed product's row, i.e. C's column).
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_cgemm32_sub(
+    int m, int n, int k, const cflt* __restrict__ A, const cflt* __restrict__ B, cflt* __restrict__ C, int ld,
+    long long sm, int batch) {
+    const int tm = (m + 63) >> 6, tn = (n + 63) >> 6;
+    const long long total = (long long)tm * tn * batch;
+    const long long per = (total + 7) >> 3;
+    const long long li = (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (li >= total) return;                 // whole workgroup, before any barrier
+    const int bmat = (int)(li / ((long long)tm * tn));
+    const int tile = (int)(li - (long long)bmat * tm * tn);
+    A += bmat * sm;
+    B += bmat * sm;
+    C += bmat * sm;
+    __shared__ cflt As[16][65];
+    __shared__ cflt Bs[16][65];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int l16 = lane & 15, lk = lane >> 4;
+    const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
+    const int row0 = (tile % tm) * 64, col0 = (tile / tm) * 64;
+    flt4 p1[2][2], p2[2][2], p3[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            p1[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+            p2[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+            p3[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+        }
+    cflt pa[4], pb[4];
+    auto fetch = [&](int kb) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u, r = e & 63, kk = e >> 6;
+            pa[u] = (row0 + r < m && kb + kk < k) ? A[(row0 + r) + (size_t)(kb + kk) * ld] : cflt{0.f, 0.f};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u, kk = e & 15, cc = e >> 4;
+            pb[u] = (col0 + cc < n && kb + kk < k) ? B[(kb + kk) + (size_t)(col0 + cc) * ld] : cflt{0.f, 0.f};
+        }
+    };
+    fetch(0);
+    for (int kb = 0; kb < k; kb += 16) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            As[e >> 6][e & 63] = pa[u];
+            Bs[e & 15][e >> 4] = pb[u];
+        }
+        __syncthreads();
+        if (kb + 16 < k) fetch(kb + 16);
+#pragma unroll
+        for (int k4 = 0; k4 < 16; k4 += 4) {
+            cflt a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = As[k4 + lk][wr + 16 * i + l16];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = Bs[k4 + lk][wc + 16 * j + l16];
+            const float as[2] = {a[0].x + a[0].y, a[1].x + a[1].y};
+            const float bs[2] = {b[0].x + b[0].y, b[1].x + b[1].y};
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    p1[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(b[j].x, a[i].x, p1[i][j], 0, 0, 0);
+                    p2[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(b[j].y, a[i].y, p2[i][j], 0, 0, 0);
+                    p3[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bs[j], as[i], p3[i][j], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int r = row0 + wr + 16 * i + l16, rc = min(r, m - 1);
+        cflt cv[2][4];
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = min(col0 + wc + 16 * j + 4 * lk + v, n - 1);
+                cv[j][v] = C[rc + (size_t)c * ld];
+            }
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = col0 + wc + 16 * j + 4 * lk + v;
+                if (r < m && c < n) {
+                    const float q1 = p1[i][j][v], q2 = p2[i][j][v];
+                    C[r + (size_t)c * ld] = cflt{cv[j][v].x - (q1 - q2), cv[j][v].y - (p3[i][j][v] - q1 - q2)};
+                }
+            }
+    }
+}
+
+// C -= A B for the blocked LU: FP64 on k_zgemm, FP32 on k_cgemm32_sub
+static void gemm_sub(int m, int n, int k, const cdbl* A, const cdbl* B, cdbl* C, int ld, long long s, int batch,
+                     hipStream_t st) {
+    zgemm_batched(m, n, k, mk(-1, 0), A, ld, s, B, ld, s, false, mk(1, 0), C, ld, s, batch, st);
+}
+static void gemm_sub(int m, int n, int k, const cflt* A, const cflt* B, cflt* C, int ld, long long s, int batch,
+                     hipStream_t st) {
+    if (m <= 0 || n <= 0 || batch <= 0) return;
+    const long long total = (long long)((m + 63) / 64) * ((n + 63) / 64) * batch;
+    hipLaunchKernelGGL(k_cgemm32_sub, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, m, n, k, A, B, C, ld, s,
+                       batch);
+}
+
 // Blocked right-looking LU in steps of 64 columns, each factorised as two
 // 32-column sub-panels: the second sub-panel, and its future U rows right of
 // it, are brought up to date with the first one's k = 32 product, and the
@@ -564,7 +700,9 @@ void zgemm_batched(int m, int n, int k, cdbl alpha, const cdbl* A, int lda, long
 //                A[j1:j2, j2:] -= L1[j1:j2] U1[:, j2:] with the interchanged L1 rows;
 //                U2 = L22^{-1} A[j1:j2, j2:]
 //   trailing:    A[j2:, j2:] -= [L1 L2][j2:] [U1; U2]
-void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStream_t st) {
+// C = cdbl: the FP64 factorisation; C = cflt: the same steps in FP32.
+template <class C>
+static void lu_factor_impl(C* A, int n, int batch, int* ipiv, int* perm, hipStream_t st) {
     const int sb = 32;
     const long long s = (long long)n * n;
     auto at = [&](int r, int c) { return A + r + (size_t)c * n; };
@@ -572,16 +710,17 @@ void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStrea
         const int nout = n - jb;
         if (nout <= 0) return;
         const dim3 g((nout + 31) / 32, batch), t(256);
-        if (mode == 3) hipLaunchKernelGGL(k_lu_rows<3>, g, t, 0, st, A, n, j0, jb, ipiv);
-        else if (mode == 1) hipLaunchKernelGGL(k_lu_rows<1>, g, t, 0, st, A, n, j0, jb, ipiv);
-        else hipLaunchKernelGGL(k_lu_rows<2>, g, t, 0, st, A, n, j0, jb, ipiv);
+        if (mode == 3) hipLaunchKernelGGL((k_lu_rows<C, 3>), g, t, 0, st, A, n, j0, jb, ipiv);
+        else if (mode == 1) hipLaunchKernelGGL((k_lu_rows<C, 1>), g, t, 0, st, A, n, j0, jb, ipiv);
+        else hipLaunchKernelGGL((k_lu_rows<C, 2>), g, t, 0, st, A, n, j0, jb, ipiv);
     };
     auto panel = [&](int j, int jb) {
         const int rows = n - j;
-        if (rows <= 1024) hipLaunchKernelGGL((k_lu_panel_rg<1, 8>), dim3(batch), dim3(1024), 0, st, A, n, j, jb, ipiv);
-        else if (rows <= 2048) hipLaunchKernelGGL((k_lu_panel_rg<2, 6>), dim3(batch), dim3(1024), 0, st, A, n, j, jb, ipiv);
-        else if (rows <= 4096) hipLaunchKernelGGL((k_lu_panel_rg<4, 4>), dim3(batch), dim3(1024), 0, st, A, n, j, jb, ipiv);
-        else hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(1024), 0, st, A, n, j, jb, ipiv);
+        const dim3 g(batch), t(1024);
+        if (rows <= 1024) hipLaunchKernelGGL((k_lu_panel_rg<C, 1, 8>), g, t, 0, st, A, n, j, jb, ipiv);
+        else if (rows <= 2048) hipLaunchKernelGGL((k_lu_panel_rg<C, 2, 6>), g, t, 0, st, A, n, j, jb, ipiv);
+        else if (rows <= 4096) hipLaunchKernelGGL((k_lu_panel_rg<C, 4, 4>), g, t, 0, st, A, n, j, jb, ipiv);
+        else hipLaunchKernelGGL((k_lu_panel<C>), g, t, 0, st, A, n, j, jb, ipiv);
     };
     for (int j0 = 0; j0 < n; j0 += 2 * sb) {
         const int jb1 = n - j0 < sb ? n - j0 : sb;
@@ -591,21 +730,39 @@ void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStrea
         const int jb2 = n - j1 < sb ? n - j1 : sb;
         const int j2 = j1 + jb2;
         if (jb2 > 0) {
-            zgemm_batched(n - j1, jb2, jb1, mk(-1, 0), at(j1, j0), n, s, at(j0, j1), n, s, false, mk(1, 0), at(j1, j1),
-                          n, s, batch, st);
+            gemm_sub(n - j1, jb2, jb1, at(j1, j0), at(j0, j1), at(j1, j1), n, s, batch, st);
             panel(j1, jb2);
             rows(1, j1, jb2);
             if (n - j2 > 0) {
-                zgemm_batched(jb2, n - j2, jb1, mk(-1, 0), at(j1, j0), n, s, at(j0, j2), n, s, false, mk(1, 0),
-                              at(j1, j2), n, s, batch, st);
+                gemm_sub(jb2, n - j2, jb1, at(j1, j0), at(j0, j2), at(j1, j2), n, s, batch, st);
                 rows(2, j1, jb2);
             }
         }
-        if (n - j2 > 0)
-            zgemm_batched(n - j2, n - j2, j2 - j0, mk(-1, 0), at(j2, j0), n, s, at(j0, j2), n, s, false, mk(1, 0),
-                          at(j2, j2), n, s, batch, st);
+        if (n - j2 > 0) gemm_sub(n - j2, n - j2, j2 - j0, at(j2, j0), at(j0, j2), at(j2, j2), n, s, batch, st);
     }
     hipLaunchKernelGGL(k_lu_perm, dim3(batch), dim3(64), 0, st, n, ipiv, perm);
+}
+
+void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStream_t st) {
+    lu_factor_impl(A, n, batch, ipiv, perm, st);
+}
+
+void lu_factor_batched_f32(cflt* A, int n, int batch, int* ipiv, int* perm, hipStream_t st) {
+    lu_factor_impl(A, n, batch, ipiv, perm, st);
+}
+
+// Single-precision copy of count complex values (round to nearest)
+__global__ __launch_bounds__(256) void k_to_f32(const cdbl* __restrict__ src, long long count, cflt* __restrict__ dst) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < count) {
+        const cdbl v = src[id];
+        dst[id] = cflt{(float)v.x, (float)v.y};
+    }
+}
+
+void to_f32(const cdbl* src, long long count, cflt* dst, hipStream_t st) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, src, count, dst);
 }
 
 // ---------------------------------------------------------------------------
@@ -635,12 +792,13 @@ struct SolveLds {
 // Stage the diagonal block [r0, r0 + bs)^2 of the pass operator with threads
 // t0, t0 + stride, ...: conj = false: op(i, j) = LU(i, j); conj = true:
 // op(i, j) = conj(LU(j, i)).
-__device__ __forceinline__ void stage_diag(const cdbl* M, int n, int r0, int bs, bool conj, cdbl* tile, int t0,
+template <class F>
+__device__ __forceinline__ void stage_diag(const F* M, int n, int r0, int bs, bool conj, cdbl* tile, int t0,
                                            int stride) {
     for (int e = t0; e < 64 * 64; e += stride) {
         const int p = e & 63, q = e >> 6;   // p runs down a column of LU
         const bool in = p < bs && q < bs;
-        const cdbl v = in ? M[(r0 + p) + (size_t)(r0 + q) * n] : mk(0, 0);
+        const cdbl v = in ? to_cdbl(M[(r0 + p) + (size_t)(r0 + q) * n]) : mk(0, 0);
         if (conj) tile[q * kTS + p] = cconj(v);
         else tile[p * kTS + q] = v;
     }
@@ -696,8 +854,8 @@ __device__ __forceinline__ void diag_solve(int n, int r0, int bs, bool lower, bo
 // w_i -= sum_j M(i, r0 + j) x_j: a wavefront covers 16 rows x 4 groups of 16
 // columns (lane = row + 16 group), each thread loading its 16 columns in two
 // batches of eight.
-template <int NC>
-__device__ __forceinline__ void subst_right(const cdbl* M, int n, bool lower, cdbl* w, SolveLds& S) {
+template <int NC, class F>
+__device__ __forceinline__ void subst_right(const F* M, int n, bool lower, cdbl* w, SolveLds& S) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
     const int l16 = lane & 15, grp = lane >> 4;
     const int nblk = (n + 63) / 64;
@@ -725,14 +883,14 @@ __device__ __forceinline__ void subst_right(const cdbl* M, int n, bool lower, cd
                 const int ic = min(i, ib - 1);
 #pragma unroll 1
                 for (int u0 = 0; u0 < 16; u0 += 8) {
-                    cdbl t[8];
+                    F t[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u)
                         t[u] = M[ic + (size_t)(r0 + min(16 * grp + u0 + u, bs - 1)) * n];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
                         const int j = 16 * grp + u0 + u;
-                        const cdbl tv = j < bs ? t[u] : mk(0, 0);
+                        const cdbl tv = j < bs ? to_cdbl(t[u]) : mk(0, 0);
 #pragma unroll
                         for (int c = 0; c < NC; ++c) cfma(acc[c], tv, S.xb[j * kMaxNc + c]);
                     }
@@ -762,8 +920,8 @@ __device__ __forceinline__ void subst_right(const cdbl* M, int n, bool lower, cd
 // (CP of them per sweep down the column, CP = 4 / 2 / 1 for up to 2 / 4 / 8
 // right-hand sides) with its lanes down the column, then the diagonal block is
 // solved.
-template <int NC>
-__device__ __forceinline__ void subst_left(const cdbl* M, int n, bool lower, cdbl* w, SolveLds& S) {
+template <int NC, class F>
+__device__ __forceinline__ void subst_left(const F* M, int n, bool lower, cdbl* w, SolveLds& S) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int nblk = (n + 63) / 64;
     auto rb = [&](int s) { return 64 * (lower ? s : nblk - 1 - s); };
@@ -787,7 +945,8 @@ __device__ __forceinline__ void subst_left(const cdbl* M, int n, bool lower, cdb
             const int qmax = bs - 1 - 4 * wv - q0;    // last column of the block for this wavefront
             for (int tt = 0; tt < T; ++tt) {
                 const int j = ja + lane + 64 * tt, jc = min(j, jb - 1);
-                cdbl xj[NC], mv[CP];
+                cdbl xj[NC];
+                F mv[CP];
 #pragma unroll
                 for (int c = 0; c < NC; ++c) xj[c] = w[jc + (size_t)c * n];
 #pragma unroll
@@ -797,7 +956,7 @@ __device__ __forceinline__ void subst_left(const cdbl* M, int n, bool lower, cdb
                     if (j >= jb) xj[c] = mk(0, 0);
 #pragma unroll
                 for (int q = 0; q < CP; ++q) {
-                    const cdbl t = q <= qmax ? cconj(mv[q]) : mk(0, 0);
+                    const cdbl t = q <= qmax ? cconj(to_cdbl(mv[q])) : mk(0, 0);
 #pragma unroll
                     for (int c = 0; c < NC; ++c) cfma(acc[q][c], t, xj[c]);
                 }
@@ -833,8 +992,8 @@ __device__ __forceinline__ void subst_left(const cdbl* M, int n, bool lower, cdb
 }
 
 // one workgroup of kSolveThreads threads per matrix
-template <int NC>
-__global__ __launch_bounds__(kSolveThreads) void k_solve_thin(const cdbl* __restrict__ LU, long long ms,
+template <class F, int NC>
+__global__ __launch_bounds__(kSolveThreads) void k_solve_thin(const F* __restrict__ LU, long long ms,
                                                               const int* __restrict__ perm, int n,
                                                               cdbl* __restrict__ X, int adj, cdbl* __restrict__ work) {
     __shared__ SolveLds S;
@@ -842,7 +1001,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_thin(const cdbl* __rest
     const int* p = perm + (size_t)b * n;
     cdbl* x = X + (size_t)b * n * NC;
     cdbl* w = work + (size_t)b * n * NC;
-    const cdbl* lu = LU + (size_t)b * ms;
+    const F* lu = LU + (size_t)b * ms;
     const int tid = threadIdx.x;
     if (!adj) {
         // A = P^T L U:  w = P x, L y = w, U z = y
@@ -867,20 +1026,201 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_thin(const cdbl* __rest
     }
 }
 
-void lu_solve_thin_batched(const cdbl* LU, long long ms, const int* perm, int n, int batch, cdbl* X, int nc, bool adj,
-                           cdbl* work, hipStream_t st) {
+template <class F>
+static void solve_thin_impl(const F* LU, long long ms, const int* perm, int n, int batch, cdbl* X, int nc, bool adj,
+                            cdbl* work, hipStream_t st) {
     const int a = adj ? 1 : 0;
     const dim3 g(batch), t(kSolveThreads);
     switch (nc) {
-    case 1: hipLaunchKernelGGL(k_solve_thin<1>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
-    case 2: hipLaunchKernelGGL(k_solve_thin<2>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
-    case 3: hipLaunchKernelGGL(k_solve_thin<3>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
-    case 4: hipLaunchKernelGGL(k_solve_thin<4>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
-    case 5: hipLaunchKernelGGL(k_solve_thin<5>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
-    case 6: hipLaunchKernelGGL(k_solve_thin<6>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
-    case 7: hipLaunchKernelGGL(k_solve_thin<7>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
-    default: hipLaunchKernelGGL(k_solve_thin<8>, g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    case 1: hipLaunchKernelGGL((k_solve_thin<F, 1>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    case 2: hipLaunchKernelGGL((k_solve_thin<F, 2>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    case 3: hipLaunchKernelGGL((k_solve_thin<F, 3>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    case 4: hipLaunchKernelGGL((k_solve_thin<F, 4>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    case 5: hipLaunchKernelGGL((k_solve_thin<F, 5>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    case 6: hipLaunchKernelGGL((k_solve_thin<F, 6>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    case 7: hipLaunchKernelGGL((k_solve_thin<F, 7>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    default: hipLaunchKernelGGL((k_solve_thin<F, 8>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
     }
+}
+
+void lu_solve_thin_batched(const cdbl* LU, long long ms, const int* perm, int n, int batch, cdbl* X, int nc, bool adj,
+                           cdbl* work, hipStream_t st) {
+    solve_thin_impl(LU, ms, perm, n, batch, X, nc, adj, work, st);
+}
+
+void lu_solve_thin_batched_f32(const cflt* LU, long long ms, const int* perm, int n, int batch, cdbl* X, int nc,
+                               bool adj, cdbl* work, hipStream_t st) {
+    solve_thin_impl(LU, ms, perm, n, batch, X, nc, adj, work, st);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 residuals and updates of the mixed-precision iterative refinement
+// ---------------------------------------------------------------------------
+// R = Bv - op(T) Z for the thin blocks of a batch (T: n x n FP64 with stride
+// ts; Z, Bv, R: n x NC with stride n NC).  op = T: a workgroup takes 256 rows
+// and a slice of kResidSlice columns (the slice's Z rows staged in LDS), one
+// thread per row streaming down the columns; the slices' partial sums are
+// added in slice order by k_resid_sum.  op = T^H: a wavefront takes one
+// column i of T, (T^H z)_i = sum_j conj(T(j, i)) z_j, lanes down the column.
+constexpr int kResidSlice = 256;
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_resid_n(const cdbl* __restrict__ T, long long ts, const cdbl* __restrict__ Z,
+                                                 int n, int nslice, cdbl* __restrict__ part) {
+    __shared__ cdbl zs[kResidSlice][NC];
+    const int b = blockIdx.y;
+    const int rb = blockIdx.x / nslice, sl = blockIdx.x - rb * nslice;
+    const int i = rb * 256 + threadIdx.x;
+    const int j0 = sl * kResidSlice, j1 = min(n, j0 + kResidSlice);
+    const cdbl* z = Z + (size_t)b * n * NC;
+    for (int e = threadIdx.x; e < kResidSlice * NC; e += 256) {
+        const int jj = e % kResidSlice, c = e / kResidSlice;
+        zs[jj][c] = j0 + jj < j1 ? z[j0 + jj + (size_t)c * n] : mk(0, 0);
+    }
+    __syncthreads();
+    cdbl acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = mk(0, 0);
+    if (i < n) {
+        const cdbl* t = T + (size_t)b * ts + i + (size_t)j0 * n;
+        int j = j0;
+        for (; j + 4 <= j1; j += 4) {
+            cdbl tv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) tv[u] = t[(size_t)(j - j0 + u) * n];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) cfma(acc[c], tv[u], zs[j - j0 + u][c]);
+        }
+        for (; j < j1; ++j) {
+            const cdbl tv = t[(size_t)(j - j0) * n];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) cfma(acc[c], tv, zs[j - j0][c]);
+        }
+        cdbl* p = part + (((size_t)sl * gridDim.y + b) * NC) * n + i;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) p[(size_t)c * n] = acc[c];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_resid_sum(const cdbl* __restrict__ part, int nslice, int n, int ncb, int batch,
+                                                   const cdbl* __restrict__ Bv, cdbl* __restrict__ R) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per = (long long)n * ncb;
+    if (id >= per * batch) return;
+    cdbl acc = mk(0, 0);
+    for (int sl = 0; sl < nslice; ++sl) acc = acc + part[(size_t)sl * per * batch + id];
+    R[id] = Bv[id] - acc;
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_resid_h(const cdbl* __restrict__ T, long long ts, const cdbl* __restrict__ Z,
+                                                 const cdbl* __restrict__ Bv, cdbl* __restrict__ R, int n) {
+    const int b = blockIdx.y;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const cdbl* col = T + (size_t)b * ts + (size_t)i * n;
+    const cdbl* z = Z + (size_t)b * n * NC;
+    cdbl acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = mk(0, 0);
+    for (int j = lane; j < n; j += 64) {
+        const cdbl tv = cconj(col[j]);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) cfma(acc[c], tv, z[j + (size_t)c * n]);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            acc[c].x += __shfl_xor(acc[c].x, o, 64);
+            acc[c].y += __shfl_xor(acc[c].y, o, 64);
+        }
+    if (lane == 0) {
+        const size_t base = (size_t)b * n * NC + i;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) R[base + (size_t)c * n] = Bv[base + (size_t)c * n] - acc[c];
+    }
+}
+
+template <int NC>
+static void resid_impl(const cdbl* T, long long ts, const cdbl* Z, const cdbl* Bv, cdbl* R, int n, int batch, bool adj,
+                       cdbl* part, hipStream_t st) {
+    if (adj) {
+        hipLaunchKernelGGL(k_resid_h<NC>, dim3((n + 3) / 4, batch), dim3(256), 0, st, T, ts, Z, Bv, R, n);
+        return;
+    }
+    const int nslice = (n + kResidSlice - 1) / kResidSlice, nrb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_resid_n<NC>, dim3(nrb * nslice, batch), dim3(256), 0, st, T, ts, Z, n, nslice, part);
+    const long long tot = (long long)n * NC * batch;
+    hipLaunchKernelGGL(k_resid_sum, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part, nslice, n, NC, batch, Bv,
+                       R);
+}
+
+void residual_thin_batched(const cdbl* T, long long ts, const cdbl* Z, const cdbl* Bv, cdbl* R, int n, int nc,
+                           int batch, bool adj, cdbl* part, hipStream_t st) {
+    if (batch <= 0 || n <= 0) return;
+    switch (nc) {
+    case 1: resid_impl<1>(T, ts, Z, Bv, R, n, batch, adj, part, st); break;
+    case 2: resid_impl<2>(T, ts, Z, Bv, R, n, batch, adj, part, st); break;
+    case 3: resid_impl<3>(T, ts, Z, Bv, R, n, batch, adj, part, st); break;
+    case 4: resid_impl<4>(T, ts, Z, Bv, R, n, batch, adj, part, st); break;
+    case 5: resid_impl<5>(T, ts, Z, Bv, R, n, batch, adj, part, st); break;
+    case 6: resid_impl<6>(T, ts, Z, Bv, R, n, batch, adj, part, st); break;
+    case 7: resid_impl<7>(T, ts, Z, Bv, R, n, batch, adj, part, st); break;
+    default: resid_impl<8>(T, ts, Z, Bv, R, n, batch, adj, part, st); break;
+    }
+}
+
+size_t residual_part_count(int n, int nc, int batch) {
+    return (size_t)((n + kResidSlice - 1) / kResidSlice) * (size_t)n * nc * batch;
+}
+
+// Z += D for the cnt entries of each matrix's block; with `check`, flag[b] = 1
+// when max |D| > tol max |Z| (the refinement has not converged to tol).
+__global__ __launch_bounds__(256) void k_refine_update(cdbl* __restrict__ Z, const cdbl* __restrict__ Dv, int cnt,
+                                                       double tol2, int check, int* __restrict__ flag) {
+    __shared__ double sh[2][4];
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    cdbl* z = Z + (size_t)b * cnt;
+    const cdbl* d = Dv + (size_t)b * cnt;
+    double md = 0.0, mz = 0.0;
+    for (int e = tid; e < cnt; e += 256) {
+        const cdbl dv = d[e];
+        const cdbl zv = z[e] + dv;
+        z[e] = zv;
+        md = fmax(md, cabs2(dv));
+        mz = fmax(mz, cabs2(zv));
+    }
+    if (!check) return;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        md = fmax(md, __shfl_xor(md, o, 64));
+        mz = fmax(mz, __shfl_xor(mz, o, 64));
+    }
+    if (lane == 0) { sh[0][wv] = md; sh[1][wv] = mz; }
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < 4; ++w) { md = fmax(md, sh[0][w]); mz = fmax(mz, sh[1][w]); }
+        if (!(md <= tol2 * mz)) flag[b] = 1;   // also catches NaN
+    }
+}
+
+void refine_update_batched(cdbl* Z, const cdbl* D, int cnt, int batch, double tol, bool check, int* flag,
+                           hipStream_t st) {
+    if (batch <= 0) return;
+    hipLaunchKernelGGL(k_refine_update, dim3(batch), dim3(256), 0, st, Z, D, cnt, tol * tol, check ? 1 : 0, flag);
+}
+
+__global__ void k_fill_int(int* __restrict__ p, int count, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = v;
+}
+
+void fill_int(int* p, int count, int v, hipStream_t st) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_fill_int, dim3((count + 255) / 256), dim3(256), 0, st, p, count, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -1167,6 +1507,177 @@ __global__ void k_wielandt(const cdbl* __restrict__ T, int n, double s, cdbl* __
 void wielandt_build(const cdbl* T, int n, double s, cdbl* W, hipStream_t st) {
     long long tot = 4LL * n * n;
     hipLaunchKernelGGL(k_wielandt, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, T, n, s, W);
+}
+
+// ---------------------------------------------------------------------------
+// The bordered Wielandt system of randomized_svd::sv_der2 (randsvd.cpp:171-216),
+// batched over nb wavenumbers: x = [u; v] (the singular pair of T_p for sigma),
+// m = the first index of the largest |x_l| (5 when none is positive, as the
+// reference's search starts), u_n = x / x_m, and B = W - sigma I with column m
+// replaced by -u_n, W = [[0, T_p], [T_p^H, 0]] (2n x 2n).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_border_prep(const cdbl* __restrict__ U, const cdbl* __restrict__ V, int n,
+                                                     int* __restrict__ mv, cdbl* __restrict__ X) {
+    __shared__ double sbest[4];
+    __shared__ int sidx[4];
+    __shared__ int sm;
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int m2 = 2 * n;
+    const cdbl* u = U + (size_t)b * n;
+    const cdbl* v = V + (size_t)b * n;
+    double best = 0.0;
+    int bi = m2;
+    for (int l = tid; l < m2; l += blockDim.x) {
+        const double a = sqrt(cabs2(l < n ? u[l] : v[l - n]));
+        if (a > best) { best = a; bi = l; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double b2 = __shfl_xor(best, o, 64);
+        const int i2 = __shfl_xor(bi, o, 64);
+        if (b2 > best || (b2 == best && i2 < bi)) { best = b2; bi = i2; }
+    }
+    if (lane == 0) { sbest[wv] = best; sidx[wv] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+            if (sbest[w] > best || (sbest[w] == best && sidx[w] < bi)) { best = sbest[w]; bi = sidx[w]; }
+        sm = bi < m2 ? bi : (5 < m2 ? 5 : 0);
+        mv[b] = sm;
+    }
+    __syncthreads();
+    const int m = sm;
+    const cdbl xm = m < n ? u[m] : v[m - n];
+    const double den = cabs2(xm);
+    const cdbl inv = den > 0.0 ? mk(xm.x / den, -xm.y / den) : mk(0, 0);
+    cdbl* x = X + (size_t)b * m2;
+    for (int l = tid; l < m2; l += blockDim.x) x[l] = (l < n ? u[l] : v[l - n]) * inv;
+}
+
+// B of matrix b in 32 x 32 tiles; the T_p^H block is read through LDS so that
+// both the reads of T_p and the writes of B run down columns.
+__global__ __launch_bounds__(256) void k_border_build(const cdbl* __restrict__ Tp, int n, const double* __restrict__ sigma,
+                                                      const int* __restrict__ mv, const cdbl* __restrict__ X,
+                                                      cdbl* __restrict__ B) {
+    __shared__ cdbl tile[32][33];
+    const int m2 = 2 * n;
+    const int nt = (m2 + 31) / 32;
+    const int b = blockIdx.y;
+    const int ti = blockIdx.x % nt, tj = blockIdx.x / nt;
+    const int i0 = 32 * ti, j0 = 32 * tj;
+    const cdbl* T = Tp + (size_t)b * n * n;
+    cdbl* Bm = B + (size_t)b * m2 * m2;
+    const int r = threadIdx.x & 31, c0 = threadIdx.x >> 5;   // 8 columns per pass
+    const double s = sigma[b];
+    const int m = mv[b];
+    const cdbl* x = X + (size_t)b * m2;
+    // lower-left block (i >= n, j < n): B(i, j) = conj(T_p(j, i - n)); stage T_p(j0 + r, i0 - n + c)
+    // in the tile for every tile that meets the block (a tile may straddle row n or column n)
+    const bool lowleft = i0 + 31 >= n && j0 < n;
+    if (lowleft) {
+        for (int c = c0; c < 32; c += 8) {
+            const int jr = j0 + r, ic = i0 - n + c;
+            tile[c][r] = (jr < n && ic >= 0 && ic < n) ? T[jr + (size_t)ic * n] : mk(0, 0);
+        }
+    }
+    __syncthreads();
+    for (int c = c0; c < 32; c += 8) {
+        const int i = i0 + r, j = j0 + c;
+        if (i >= m2 || j >= m2) continue;
+        cdbl v;
+        if (j == m) v = mk(-x[i].x, -x[i].y);
+        else {
+            if (i < n && j >= n) v = T[i + (size_t)(j - n) * n];
+            else if (i >= n && j < n) v = cconj(tile[r][c]);
+            else v = mk(0, 0);
+            if (i == j) v.x -= s;
+        }
+        Bm[i + (size_t)j * m2] = v;
+    }
+}
+
+// Y <- -Y (the eigenvector derivative u' = -B^{-1} p), t_b = u'_m, then u'_m = 0
+__global__ __launch_bounds__(256) void k_border_negate_pick(cdbl* __restrict__ Y, int m2, int nb, const int* __restrict__ mv,
+                                                            cdbl* __restrict__ t) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)m2 * nb) return;
+    const int b = (int)(id / m2), l = (int)(id - (long long)b * m2);
+    const cdbl y = Y[id];
+    if (l == mv[b]) {
+        t[b] = mk(-y.x, -y.y);
+        Y[id] = mk(0, 0);
+    } else {
+        Y[id] = mk(-y.x, -y.y);
+    }
+}
+
+// out = p2 + 2 (p - t u')
+__global__ __launch_bounds__(256) void k_border_rhs(const cdbl* __restrict__ P, const cdbl* __restrict__ P2,
+                                                   const cdbl* __restrict__ Ud, const cdbl* __restrict__ t, int m2, int nb,
+                                                   cdbl* __restrict__ out) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)m2 * nb) return;
+    const int b = (int)(id / m2);
+    const cdbl q = P[id] - t[b] * Ud[id];
+    out[id] = P2[id] + 2.0 * q;
+}
+
+// d2_b = -Re(y_m) of the second solve
+__global__ void k_border_final(const cdbl* __restrict__ Y2, int m2, int nb, const int* __restrict__ mv,
+                               double* __restrict__ d2) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    d2[b] = -Y2[(size_t)b * m2 + mv[b]].x;
+}
+
+// dst[b dstride + doff + i] = src[b sstride + soff + i], i < len
+__global__ __launch_bounds__(256) void k_copy_blocks(const cdbl* __restrict__ src, long long sstride, long long soff,
+                                                     cdbl* __restrict__ dst, long long dstride, long long doff, int len,
+                                                     int nb) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)len * nb) return;
+    const int b = (int)(id / len), i = (int)(id - (long long)b * len);
+    dst[b * dstride + doff + i] = src[b * sstride + soff + i];
+}
+
+static unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
+
+void border_prep_batched(const cdbl* U, const cdbl* V, int n, int nb, int* mv, cdbl* X, hipStream_t st) {
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(k_border_prep, dim3(nb), dim3(256), 0, st, U, V, n, mv, X);
+}
+
+void border_build_batched(const cdbl* Tp, int n, int nb, const double* sigma, const int* mv, const cdbl* X, cdbl* B,
+                          hipStream_t st) {
+    if (nb <= 0) return;
+    const unsigned nt = (unsigned)((2 * n + 31) / 32);
+    hipLaunchKernelGGL(k_border_build, dim3(nt * nt, nb), dim3(256), 0, st, Tp, n, sigma, mv, X, B);
+}
+
+void border_negate_pick_batched(cdbl* Y, int n, int nb, const int* mv, cdbl* t, hipStream_t st) {
+    const long long tot = 2LL * n * nb;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_border_negate_pick, dim3(blocks_for(tot)), dim3(256), 0, st, Y, 2 * n, nb, mv, t);
+}
+
+void border_rhs_batched(const cdbl* P, const cdbl* P2, const cdbl* Ud, const cdbl* t, int n, int nb, cdbl* out,
+                        hipStream_t st) {
+    const long long tot = 2LL * n * nb;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_border_rhs, dim3(blocks_for(tot)), dim3(256), 0, st, P, P2, Ud, t, 2 * n, nb, out);
+}
+
+void border_final_batched(const cdbl* Y2, int n, int nb, const int* mv, double* d2, hipStream_t st) {
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(k_border_final, dim3(blocks_for(nb)), dim3(256), 0, st, Y2, 2 * n, nb, mv, d2);
+}
+
+void copy_blocks_batched(const cdbl* src, long long sstride, long long soff, cdbl* dst, long long dstride,
+                         long long doff, int len, int nb, hipStream_t st) {
+    const long long tot = (long long)len * nb;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_copy_blocks, dim3(blocks_for(tot)), dim3(256), 0, st, src, sstride, soff, dst, dstride, doff,
+                       len, nb);
 }
 
 }  // namespace hbem

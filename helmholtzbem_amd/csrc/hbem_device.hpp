@@ -17,6 +17,10 @@ namespace hbem {
 struct cdbl {
     double x, y;
 };
+// single-precision complex: the storage of the mixed-precision LU factors
+struct cflt {
+    float x, y;
+};
 
 __host__ __device__ __forceinline__ cdbl mk(double a, double b) { return cdbl{a, b}; }
 __host__ __device__ __forceinline__ cdbl operator+(cdbl a, cdbl b) { return cdbl{a.x + b.x, a.y + b.y}; }

@@ -67,6 +67,21 @@ void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStrea
 // A X = B, adj = true solves A^H X = B.  work: n * nc * batch.
 void lu_solve_thin_batched(const cdbl* LU, long long ms, const int* perm, int n, int batch, cdbl* X, int nc, bool adj,
                            cdbl* work, hipStream_t st);
+// The same blocked LU in single precision (the mixed-precision sweep), its
+// thin solves (FP32 factors read, FP64 arithmetic), and the FP32 copy of T.
+void lu_factor_batched_f32(cflt* A, int n, int batch, int* ipiv, int* perm, hipStream_t st);
+void lu_solve_thin_batched_f32(const cflt* LU, long long ms, const int* perm, int n, int batch, cdbl* X, int nc,
+                               bool adj, cdbl* work, hipStream_t st);
+void to_f32(const cdbl* src, long long count, cflt* dst, hipStream_t st);
+// R = Bv - op(T) Z (op = T or T^H) for n x nc blocks of a batch, T (n x n,
+// stride ts) in FP64; part: residual_part_count(n, nc, batch) complex scratch
+void residual_thin_batched(const cdbl* T, long long ts, const cdbl* Z, const cdbl* Bv, cdbl* R, int n, int nc,
+                           int batch, bool adj, cdbl* part, hipStream_t st);
+size_t residual_part_count(int n, int nc, int batch);
+// Z += D (cnt entries per matrix); check: flag[b] = 1 if max|D| > tol max|Z|
+void refine_update_batched(cdbl* Z, const cdbl* D, int cnt, int batch, double tol, bool check, int* flag,
+                           hipStream_t st);
+void fill_int(int* p, int count, int v, hipStream_t st);
 // *flag = 1 if any of cnt doubles is not finite / a diagonal entry of A is 0
 void flag_nonfinite(const double* x, long long cnt, int* flag, hipStream_t st);
 void flag_zero_diag(const cdbl* A, int n, int* flag, hipStream_t st);
@@ -90,6 +105,22 @@ void bilinear_re_batched(const cdbl* T, long long tstride, const cdbl* a, const 
 void gemv_batched(const cdbl* T, long long tstride, const cdbl* x, cdbl* y, int n, int batch, bool adj, hipStream_t st);
 // W = [[0, T], [T^H, 0]] - s I  (2n x 2n) for a single matrix
 void wielandt_build(const cdbl* T, int n, double s, cdbl* W, hipStream_t st);
+// The bordered system of randomized_svd::sv_der2, batched over nb matrices of
+// order n (systems of order 2n, vectors of 2n entries back to back):
+// X_b = [u_b; v_b] / x_m with m_b the first index of the largest |x_l|;
+// B_b = [[0, Tp_b], [Tp_b^H, 0]] - sigma_b I with column m_b set to -X_b;
+// negate_pick: Y <- -Y, t_b = Y_b[m_b], then Y_b[m_b] = 0;
+// rhs: out = P2 + 2 (P - t Ud); final: d2_b = -Re(Y2_b[m_b]).
+void border_prep_batched(const cdbl* U, const cdbl* V, int n, int nb, int* mv, cdbl* X, hipStream_t st);
+void border_build_batched(const cdbl* Tp, int n, int nb, const double* sigma, const int* mv, const cdbl* X, cdbl* B,
+                          hipStream_t st);
+void border_negate_pick_batched(cdbl* Y, int n, int nb, const int* mv, cdbl* t, hipStream_t st);
+void border_rhs_batched(const cdbl* P, const cdbl* P2, const cdbl* Ud, const cdbl* t, int n, int nb, cdbl* out,
+                        hipStream_t st);
+void border_final_batched(const cdbl* Y2, int n, int nb, const int* mv, double* d2, hipStream_t st);
+// dst[b dstride + doff + i] = src[b sstride + soff + i] for i < len, b < nb
+void copy_blocks_batched(const cdbl* src, long long sstride, long long soff, cdbl* dst, long long dstride,
+                         long long doff, int len, int nb, hipStream_t st);
 // C = alpha * A * op(B) + beta * C   (complex, column-major, batched with strides)
 void zgemm_batched(int m, int n, int k, cdbl alpha, const cdbl* A, int lda, long long sa, const cdbl* B, int ldb,
                    long long sb, bool transB, cdbl beta, cdbl* C, int ldc, long long sc, int batch, hipStream_t st);

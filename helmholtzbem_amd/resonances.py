@@ -204,7 +204,7 @@ def find_minima(builder_data, k_min: float, k_max: float, n_points: int, c_o: fl
     minimum (method "brent_gsl", examples/roots_brent_polygon_rsvd.cpp:288-311;
     tol is then relative) -- with q_refine subspace iterations.  Returns a list
     of (k, sigma(k), iterations) for the brackets that converged."""
-    from . import SolutionsOperator, gaussian_test_matrix, randomized_svd, sigma_sweep
+    from . import gaussian_test_matrix, sigma_sweep, sigma_sweep_der2
     n = 2 * builder_data.getNumPanels()
     if R is None:
         R = gaussian_test_matrix(n, 2, 0)
@@ -218,18 +218,22 @@ def find_minima(builder_data, k_min: float, k_max: float, n_points: int, c_o: fl
             return list(sigma_sweep(builder_data, np.array(xs, dtype=np.float64), c_o, c_i, R, q_refine, der=1)[1])
         res = lockstep([zbrent_iter(a, b, tol) for a, b in brackets], evaluate)
     elif method == "newton":
-        so = SolutionsOperator(builder_data)
-
         def evaluate(reqs):
-            xs = np.array([x for x, _ in reqs], dtype=np.float64)
-            out = []
-            d1 = sigma_sweep(builder_data, xs, c_o, c_i, R, q_refine, der=1)[1]
-            for (x, both), s1 in zip(reqs, d1):
-                if both:
-                    T, Td, Tdd = so.gen_sol_op_2nd_der(x, c_o, c_i)
-                    out.append((float(s1), float(randomized_svd.sv_der2(T, Td, Tdd, R, q_refine)[2])))
-                else:
-                    out.append(float(s1))
+            # requests for (f, f') of all brackets in one batched sigma'' sweep,
+            # the others (f only) in one dsigma/dk sweep
+            out = [None] * len(reqs)
+            both = [i for i, (_, b) in enumerate(reqs) if b]
+            one = [i for i, (_, b) in enumerate(reqs) if not b]
+            if both:
+                xs = np.array([reqs[i][0] for i in both], dtype=np.float64)
+                _, d1, d2 = sigma_sweep_der2(builder_data, xs, c_o, c_i, R, q_refine)
+                for i, a, b in zip(both, d1, d2):
+                    out[i] = (float(a), float(b))
+            if one:
+                xs = np.array([reqs[i][0] for i in one], dtype=np.float64)
+                d1 = sigma_sweep(builder_data, xs, c_o, c_i, R, q_refine, der=1)[1]
+                for i, a in zip(one, d1):
+                    out[i] = float(a)
             return out
         res = lockstep([rtsafe_iter(a, b, tol) for a, b in brackets], evaluate)
     elif method == "brent_gsl":

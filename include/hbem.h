@@ -183,6 +183,33 @@ int hbem_rsvd_sv_der2(const void* T, const void* Td, const void* Tdd, int n, con
 int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i,
                      const void* R, int nc, int q, int want_der, double* sigma, double* dsigma);
 
+/*
+ * The Newton refinement's evaluations batched on the device: for every k,
+ * gen_sol_op_2nd_der's T, T', T'' (gen_sol_op.cpp:154-196) and
+ * randomized_svd::sv_der2 (randsvd.cpp:171-216) -- sigma, dsigma/dk and
+ * d2sigma/dk2, the last from the reference's bordered 2n x 2n system built
+ * from the singular pair of the randomized subspace (as hbem_rsvd_sv_der2).
+ * The reference's Newton drivers make these calls one k at a time
+ * (examples/roots_newton_square.cpp:155-172).  Nothing but the 3 nk results
+ * crosses to the host.
+ */
+int hbem_sigma_sweep_der2(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i, const void* R, int nc,
+                          int q, double* sigma, double* dsigma, double* d2sigma);
+
+/*
+ * hbem_sigma_sweep (sigma only) with the dense stage of randomized_svd::sv
+ * (randsvd.cpp:99-146) in mixed precision: T is formed in FP64 and kept, its
+ * FP32 copy is LU-factorised on the FP32 matrix cores, and every solve of the
+ * randomized iteration is refined `refine` times (0..20) against T in FP64 --
+ * residual in FP64, correction from the FP32 factors in FP64 arithmetic.  A k
+ * whose last correction exceeds 1e-9 of its solution (or with refine = 0) is
+ * recomputed with the FP64 factorisation, and fp64_used[i] (may be NULL) is
+ * set to 1 for it.  The assembly stays FP64.  No counterpart in the
+ * reference, which factorises in FP64 only.
+ */
+int hbem_sigma_sweep_mixed(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i, const void* R, int nc,
+                           int q, int refine, double* sigma, int* fp64_used);
+
 /* Phase timing (HIP events on the context's stream), for the benchmark's
  * roofline figures.  enable != 0 clears and starts recording; the events are
  * only read by query (no host synchronisation inside the timed calls).  query

@@ -227,8 +227,16 @@ def test_error_paths(H):
     with pytest.raises(H.HbemError, match="Invalid order of derivative"):
         gb.assembleDense(1.0, 2.0, H.LayerType.ALL, 3)
     R = H.gaussian_test_matrix(24, 2, 0)
-    with pytest.raises(H.HbemError):
+    with pytest.raises(ValueError):                  # the wrapper checks R before the C-ABI
         H.randomized_svd.sv(np.eye(24), R[:, :0].copy(), 2)
+    # the C-ABI refuses the same call itself
+    from helmholtzbem_amd import _lib
+    import ctypes
+    T = np.eye(24, dtype=np.complex128)
+    s = ctypes.c_double()
+    rc = _lib.lib().hbem_rsvd_sv(ctypes.c_void_p(T.ctypes.data), 24, ctypes.c_void_p(R.ctypes.data), 0, 2,
+                                 ctypes.byref(s))
+    assert rc == _lib.HBEM_EINVAL
 
 
 @pytest.fixture(scope="module")
