@@ -25,11 +25,12 @@ import numpy as np
 from . import _lib
 from ._lib import HbemError, check, lib
 from .geometry import (ParametrizedCircularArc, ParametrizedLine, ParametrizedMesh, ParametrizedTrigCurve,
-                       Xoshiro256ss, circle_mesh, gaussian_test_matrix, polygon_mesh, random_polygon_mesh,
-                       square_mesh, star_curve, star_mesh)
+                       Xoshiro256ss, circle_mesh, gaussian_test_matrix, polygon_mesh, random_polygon_instances,
+                       random_polygon_mesh, square_mesh, star_curve, star_mesh)
 
 __all__ = ["LayerType", "ContinuousSpace1", "BuilderData", "GalerkinBuilder", "SolutionsOperator",
-           "randomized_svd", "sigma_sweep", "sigma_sweep_der2", "sigma_sweep_mixed", "sigma_sweep_multi", "HbemError", "ParametrizedLine", "ParametrizedCircularArc",
+           "randomized_svd", "sigma_sweep", "sigma_sweep_der2", "sigma_sweep_mixed", "sigma_sweep_multi", "sigma_sweep_batched",
+           "random_polygon_instances", "HbemError", "ParametrizedLine", "ParametrizedCircularArc",
            "ParametrizedTrigCurve", "ParametrizedMesh", "square_mesh", "star_mesh", "circle_mesh",
            "polygon_mesh", "random_polygon_mesh", "gaussian_test_matrix", "Xoshiro256ss", "star_curve"]
 
@@ -168,6 +169,14 @@ class BuilderData:
         if self._geo_noproj is None:
             self._geo_noproj = _Geometry(self.mesh, self.order, False, self.flags)
         return self._geo_noproj
+
+    def release_work(self):
+        """Frees the device work buffers of this geometry's contexts (the
+        geometry and its node tables stay); the next call allocates them
+        again."""
+        for g in (self._geo, self._geo_noproj):
+            if g is not None:
+                check(lib().hbem_geometry_release_work(g.h))
 
     def getTestSpaceDimension(self, n_panels=0):
         return self.test_space.getSpaceDim(n_panels or self.getNumPanels())
@@ -480,18 +489,53 @@ def sigma_sweep_mixed(builder_data: BuilderData, ks, c_o, c_i, R, q=2, refine=3,
     return sig, used.astype(bool)
 
 
-def sigma_sweep_multi(instances, q=2, der=0, proj=True, max_workers=None):
+def sigma_sweep_batched(instances, q=2, proj=True, bucket=0):
+    """sigma_min(k) of many scatterers as one batched computation
+    (hbem_sigma_sweep_multi): instances are (builder_data, ks, c_o, c_i, R)
+    tuples with k grids of one length, one quadrature order and one c_o.
+    Instances whose orders 2N round up to the same multiple of `bucket` (0:
+    256) share one assembly launch per pair class and one batched LU and
+    randomized SVD (T padded with the identity).  Returns the list of sigma
+    arrays, equal bit for bit to sigma_sweep of each instance alone."""
+    items = list(instances)
+    if not items:
+        return []
+    nk = len(items[0][1])
+    c_o = float(items[0][2])
+    nc = np.asarray(items[0][4]).shape[1]
+    for bd, ks, co, ci, R in items:
+        if len(ks) != nk or float(co) != c_o or np.asarray(R).shape[1] != nc:
+            raise ValueError("instances must share the k-grid length, c_o and the number of test columns")
+    Rs = [_test_matrix(R, 2 * bd.getNumPanels()) for bd, _, _, _, R in items]
+    geos = [bd._ctx(proj) for bd, *_ in items]
+    hs = (ctypes.c_void_p * len(items))(*[g.h.value for g in geos])
+    ks_all = np.ascontiguousarray(np.concatenate([np.asarray(ks, dtype=np.float64) for _, ks, _, _, _ in items]))
+    ci = np.ascontiguousarray([float(ci) for _, _, _, ci, _ in items], dtype=np.float64)
+    rp = (ctypes.c_void_p * len(items))(*[R.ctypes.data for R in Rs])
+    sig = np.zeros(ks_all.size)
+    check(lib().hbem_sigma_sweep_multi(hs, len(items), _dptr(ks_all), nk, c_o, _dptr(ci), rp, nc, int(q), int(bucket),
+                                       _dptr(sig)))
+    return [sig[i * nk:(i + 1) * nk].copy() for i in range(len(items))]
+
+
+def sigma_sweep_multi(instances, q=2, der=0, proj=True, max_workers=None, bucket=0):
     """sigma_sweep for many independent scatterers at once -- the drivers'
     loop over instances of uneven size.  instances: (builder_data, ks, c_o,
-    c_i, R) tuples.  Every geometry context owns its device buffers and
-    stream, so the sweeps are issued from a pool of host threads (ctypes
-    releases the GIL) and the GPU overlaps them: small instances, whose
-    batched LU and assembly launches do not fill the GPU alone, share it.
-    Returns the list of sigma_sweep results in the order of the instances."""
+    c_i, R) tuples.  Sigma-only sweeps with k grids of one length, one
+    quadrature order and one c_o run as one batched computation
+    (sigma_sweep_batched).  Otherwise every geometry context's sweep is
+    issued from its own host thread (ctypes releases the GIL) onto its own
+    stream, and the GPU overlaps them.  Returns the list of sigma_sweep
+    results in the order of the instances."""
     from concurrent.futures import ThreadPoolExecutor
     items = list(instances)
     if not items:
         return []
+    if der == 0:
+        orders = {bd.getGaussQROrder() for bd, *_ in items}
+        if len(orders) == 1 and len({len(it[1]) for it in items}) == 1 and len({float(it[2]) for it in items}) == 1 \
+                and len({np.asarray(it[4]).shape[1] for it in items}) == 1:
+            return sigma_sweep_batched(items, q, proj, bucket)
     for bd, *_ in items:
         bd._ctx(proj)                          # contexts are created on this thread
     workers = max_workers or min(len(items), 8)

@@ -20,7 +20,8 @@ EXPORTED = [
     "hbem_rsvd_sv", "hbem_rsvd_sv_der", "hbem_rsvd_sv_der2", "hbem_sigma_sweep", "hbem_profile_enable",
     "hbem_profile_query", "hbem_assemble_dense_batch", "hbem_tp_A_rhs", "hbem_tp_solve", "hbem_sweep_batch_size",
     "hbem_rsvd_benchmarking", "hbem_rsvd_timers", "hbem_rsvd_reset_timer",
-    "hbem_sigma_sweep_der2", "hbem_sigma_sweep_mixed",
+    "hbem_sigma_sweep_der2", "hbem_sigma_sweep_mixed", "hbem_sigma_sweep_multi",
+    "hbem_geometry_release_work",
 ]
 
 _lib = None
@@ -65,6 +66,8 @@ def lib():
         L.hbem_rsvd_reset_timer.argtypes = []
         L.hbem_sigma_sweep_der2.argtypes = [vp, dp, i, d, d, vp, i, i, dp, dp, dp]
         L.hbem_sigma_sweep_mixed.argtypes = [vp, dp, i, d, d, vp, i, i, i, dp, ip]
+        L.hbem_geometry_release_work.argtypes = [vp]
+        L.hbem_sigma_sweep_multi.argtypes = [ctypes.POINTER(vp), i, dp, i, d, dp, ctypes.POINTER(vp), i, i, i, dp]
         _lib = L
     return _lib
 

@@ -783,7 +783,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     int q = qb * 32 + j;
     const bool active = qb < nqb && q < P.nk;
     if (q >= P.nk) q = P.nk - 1;
-    const double kap = P.kap[q];
+    const int qi = (P.einst ? P.einst[e] * P.nk : 0) + q;   // the slot's entry in the kappa tables
+    const double kap = P.kap[qi];
     const double lk = kTwoOverPi * log(0.5 * kap);
     const double kinv = 1.0 / kap;
     const double ksq = 1.0 / sqrt(kPi * kap);
@@ -799,7 +800,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         lo = hn * nh + (chunk - hn * cph) * P.chunk;
         hi = min(hn * nh + nh, lo + P.chunk);
     }
-    const double* __restrict__ geo = P.ngeo + (size_t)e * nn * kNodeStride;
+    const double* __restrict__ geo =
+        P.einst ? P.igeo[P.einst[e]] + (size_t)(e - P.ibase[P.einst[e]]) * nn * kNodeStride : P.ngeo + (size_t)e * nn * kNodeStride;
 
     AccX<D> A;
 #pragma unroll
@@ -889,7 +891,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 
     // Sums -> this lane's entries q = m + 2h of the reference's local matrices
     // (same scalings as acc_store)
-    const double kr = P.kr[q], c = P.cv[q];
+    const double kr = P.kr[qi], c = P.cv[qi];
     const double sqc = sqrt(c), k2h = 0.5 * kap * kap;
     cdbl Ko[D + 1][2], Kto[D + 1][2], Wo[D + 1][2], Vo[D + 1][2];
 #pragma unroll
@@ -1032,7 +1034,7 @@ __global__ void k_gather_layers(const cdbl* __restrict__ L, int nk, int nslot, i
 __global__ __launch_bounds__(256) void k_form_T(const cdbl* __restrict__ L, int nk, int nslot,
                                                 const int* __restrict__ qi, const int* __restrict__ qo, int nb, int N,
                                                 int D, const double* __restrict__ M, cdbl* __restrict__ T0,
-                                                cdbl* __restrict__ T1, cdbl* __restrict__ T2) {
+                                                cdbl* __restrict__ T1, cdbl* __restrict__ T2, int ld) {
     __shared__ cdbl sW[8][8][32];   // [column in tile (I, J)][row in tile (I, J)][output]
     __shared__ cdbl sV[8][8][32];
     const int mt = (N + 7) >> 3, nbc = (nb + 31) >> 5;
@@ -1042,7 +1044,7 @@ __global__ __launch_bounds__(256) void k_form_T(const cdbl* __restrict__ L, int 
     const int i = threadIdx.x & 7, jb = threadIdx.x >> 3;
     const int b = bc * 32 + jb;
     const bool act = b < nb;                 // idle threads still reach every barrier
-    const int n2 = 2 * N;
+    const int n2 = ld;                       // leading dimension (2N, or more for a zero-padded T)
     const size_t NN = (size_t)n2 * n2;
     const int a = act ? qi[b] : 0, o = act ? qo[b] : 0;
     const bool mirror = I != J;
@@ -1178,10 +1180,32 @@ void launch_gather_layers(const cdbl* L, int nk, int nslot, int q, int N, int D,
 }
 
 void launch_form_T(const cdbl* L, int nk, int nslot, const int* qi, const int* qo, int nb, int N, int D,
-                   const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st) {
+                   const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st, int ld) {
     const unsigned mt = (unsigned)((N + 7) / 8);
     hipLaunchKernelGGL(k_form_T, dim3(mt * mt * (unsigned)((nb + 31) / 32)), dim3(256), 0, st, L, nk, nslot, qi, qo,
-                       nb, N, D, M, T0, T1, T2);
+                       nb, N, D, M, T0, T1, T2, ld > 0 ? ld : 2 * N);
+}
+
+// The rows and columns n..ld-1 of nb matrices of order ld (stride ld^2): the
+// identity there, so that a matrix of order n padded to ld has the same LU
+// factors on its leading n x n block and the identity below.
+__global__ __launch_bounds__(256) void k_pad_identity(cdbl* __restrict__ T, int n, int ld) {
+    const int b = blockIdx.y;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int pad = ld - n;
+    // padded columns n..ld-1 (all rows), then the padded rows of columns 0..n-1
+    const long long ncolpad = (long long)pad * ld, total = ncolpad + (long long)pad * n;
+    if (id >= total) return;
+    int i, j;
+    if (id < ncolpad) { j = n + (int)(id / ld); i = (int)(id % ld); }
+    else { const long long r = id - ncolpad; j = (int)(r / pad); i = n + (int)(r % pad); }
+    T[(size_t)b * ld * ld + i + (size_t)j * ld] = mk(i == j ? 1.0 : 0.0, 0.0);
+}
+
+void launch_pad_identity(cdbl* T, int n, int ld, int nb, hipStream_t st) {
+    const long long total = (long long)(ld - n) * ld + (long long)(ld - n) * n;
+    if (total <= 0 || nb <= 0) return;
+    hipLaunchKernelGGL(k_pad_identity, dim3((unsigned)((total + 255) / 256), nb), dim3(256), 0, st, T, n, ld);
 }
 
 void launch_form_Ao(const cdbl* L, int nk, int nslot, int o, int N, const double* M, cdbl* A, hipStream_t st) {

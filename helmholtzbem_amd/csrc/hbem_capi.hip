@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -234,6 +235,9 @@ struct hbem_geometry {
     }
 };
 
+// drops the multi-instance graphs that name g (defined with the multi-instance sweep)
+void multi_forget(hbem_geometry* g);
+
 namespace {
 
 int fail(const std::exception& e) {
@@ -349,6 +353,10 @@ struct DenseCtx {
     const int* perm;
     const DCsr* F;       // projection factor, null = no projection
     const DCsr* FT;      // its transpose
+    // or, for a batch of zero-padded matrices of several scatterers, the
+    // factor of each matrix's scatterer (device arrays of batch entries)
+    const hbem::CsrDesc* Fm = nullptr;
+    const hbem::CsrDesc* FTm = nullptr;
     cdbl* X;             // n x nc x batch
     cdbl* Y;
     cdbl* work;          // 2 n nc batch: thin-solve scratch, then the projected right-hand sides
@@ -386,7 +394,12 @@ void t_solve(DenseCtx& c, cdbl* Z, bool adj) {
 // Y <- T_p^{-1} X  or  T_p^{-H} X  (T_p^{-1} = F^T T^{-1} F, T_p^{-H} = F^T T^{-H} F); X is kept.
 void tp_solve(DenseCtx& c, const cdbl* X, cdbl* Y, bool adj) {
     const long long cols = (long long)c.nc * c.batch;
-    if (c.F) {
+    if (c.Fm) {
+        cdbl* Z = c.work + (size_t)c.n * cols;
+        hbem::csr_apply_multi(c.Fm, c.n, X, Z, c.nc, c.batch, c.st);
+        t_solve(c, Z, adj);
+        hbem::csr_apply_multi(c.FTm, c.n, Z, Y, c.nc, c.batch, c.st);
+    } else if (c.F) {
         cdbl* Z = c.work + (size_t)c.n * cols;
         hbem::csr_apply_batched((const int*)c.F->rp.p, (const int*)c.F->ci.p, (const double*)c.F->v.p, c.n, X, Z, cols,
                                 c.st);
@@ -622,10 +635,31 @@ int hbem_geometry_create(const double* panels, int npanels, const double* coefs,
 int hbem_geometry_destroy(hbem_geometry_t g) {
     API_BEGIN
     if (!g) return HBEM_OK;
+    multi_forget(g);
     (void)hipStreamSynchronize(g->st);
     hipStream_t st = g->st;
     delete g;
     (void)hipStreamDestroy(st);
+    API_END
+}
+
+int hbem_geometry_release_work(hbem_geometry_t g) {
+    API_BEGIN
+    if (!g) throw Error(HBEM_EINVAL, "null geometry");
+    multi_forget(g);
+    std::lock_guard<std::mutex> lk(g->mu);
+    HIPC(hipSetDevice(g->dev));
+    HIPC(hipStreamSynchronize(g->st));
+    DBuf* bs[] = {&g->w_Ao, &g->w_U, &g->w_rhs, &g->w_flag, &g->w_Tp, &g->w_B, &g->w_mv, &g->w_bpiv, &g->w_bperm,
+                  &g->w_bvec, &g->w_d2, &g->w_T32, &g->w_rb, &g->w_rr, &g->w_rpart, &g->w_flag2, &g->w_kapi, &g->w_kri,
+                  &g->w_L, &g->w_T0, &g->w_T1, &g->w_T2, &g->w_piv, &g->w_perm, &g->w_X, &g->w_Y, &g->w_work,
+                  &g->w_R, &g->w_u, &g->w_v, &g->w_a, &g->w_b, &g->w_sig, &g->w_dsig, &g->w_kap, &g->w_kr, &g->w_cv,
+                  &g->w_qi, &g->w_qo, &g->w_tmp, &g->w_out, &g->w_part};
+    for (DBuf* b : bs) {
+        if (b->p) (void)hipFree(b->p);
+        b->p = nullptr;
+        b->n = 0;
+    }
     API_END
 }
 
@@ -1275,6 +1309,328 @@ int hbem_sigma_sweep_mixed(hbem_geometry_t g, const double* ks, int nk, double c
     std::lock_guard<std::mutex> lk(g->mu);
     HIPC(hipSetDevice(g->dev));
     sweep_run(g, ks, nk, c_o, c_i, R, nc, q, 0, sigma, nullptr, nullptr, refine, fp64_used);
+    API_END
+}
+
+}  // extern "C"
+
+namespace {
+
+// Work space of hbem_sigma_sweep_multi (one per process; calls are serialised),
+// and the hipGraphs of its device phases, one per subset shape: a subset's
+// device work (entry concatenation, assembly, T, LU, randomized SVD) is
+// captured on first use and replayed by later calls with the same instances,
+// sizes and buffers.  Everything a replay reads from the host side is
+// uploaded before the launch into buffers the graph reads in place.
+struct GraphEntry {
+    std::vector<const void*> key;
+    hipGraphExec_t exec = nullptr;
+};
+struct MultiWork {
+    std::mutex mu;
+    DBuf ent[3], flags[3], pidx[3], einst[3], igeo[3], ibase[3], kap, kr, cv, L, part[3], T, piv, perm, X, Y, work, sig,
+        qi, qo, desc, descT, roff, nrow, Rall;
+    std::vector<GraphEntry> graphs;
+    void drop_graphs() {
+        for (auto& e : graphs) (void)hipGraphExecDestroy(e.exec);
+        graphs.clear();
+    }
+};
+MultiWork g_multi;
+
+}  // namespace
+
+// Graphs that name a geometry being destroyed (or released) are dropped: a
+// later context may reuse its addresses.
+void multi_forget(hbem_geometry* g) {
+    std::lock_guard<std::mutex> lk(g_multi.mu);
+    auto& v = g_multi.graphs;
+    for (size_t i = 0; i < v.size();) {
+        if (std::find(v[i].key.begin(), v[i].key.end(), (const void*)g) != v[i].key.end()) {
+            (void)hipGraphExecDestroy(v[i].exec);
+            v.erase(v.begin() + i);
+        } else {
+            ++i;
+        }
+    }
+}
+
+namespace {
+
+bool graphs_enabled() {
+    const char* e = std::getenv("HBEM_NO_GRAPH");
+    return !(e && e[0] && e[0] != '0');
+}
+
+// One bucket subset of hbem_sigma_sweep_multi: instances `ids` (all padded to
+// order n_pad), their nk wavenumbers each.  The panel pairs of every class of
+// all these instances are assembled by one launch; T of instance i, k j is
+// matrix i_local nk + j of one batch of order n_pad (identity in the
+// padding), factorised and reduced to sigma by the batched dense stage.
+void multi_subset(const std::vector<hbem_geometry*>& G, const std::vector<int>& ids, int n_pad, const double* ks, int nk,
+                  double c_o, const double* c_i, const void* const* R, int nc, int q, double* sigma) {
+    MultiWork& w = g_multi;
+    hbem_geometry* g0 = G[ids[0]];
+    const hipStream_t st = g0->st;
+    const int m = (int)ids.size();
+    const int nslot = 16, nks = 2 * nk;
+    const size_t nn = (size_t)n_pad * n_pad;
+    const int batch = m * nk;
+    // ---- host side: every table the device phase reads, uploaded first ----
+    // slots of instance i: [c_i(i) for every k | c_o for every k]
+    std::vector<double> kap((size_t)m * nks), kr((size_t)m * nks), cv((size_t)m * nks);
+    std::vector<long long> pair_off(m + 1, 0);
+    for (int a = 0; a < m; ++a) {
+        const int i = ids[a];
+        for (int j = 0; j < nk; ++j) {
+            const double k = ks[(size_t)i * nk + j];
+            kap[(size_t)a * nks + j] = k * std::sqrt(c_i[i]); kr[(size_t)a * nks + j] = k; cv[(size_t)a * nks + j] = c_i[i];
+            kap[(size_t)a * nks + nk + j] = k * std::sqrt(c_o); kr[(size_t)a * nks + nk + j] = k; cv[(size_t)a * nks + nk + j] = c_o;
+        }
+        pair_off[a + 1] = pair_off[a] + G[i]->npairs;
+    }
+    const double* dkap = upload(w.kap, kap, st);
+    const double* dkr = upload(w.kr, kr, st);
+    const double* dcv = upload(w.cv, cv, st);
+    struct ClassPlan {
+        long long ne = 0;
+        std::vector<long long> base;
+        int nodes = 0, nchunk = 1;
+    } cp[3];
+    const double* const* digeo[3] = {nullptr, nullptr, nullptr};
+    const long long* dibase[3] = {nullptr, nullptr, nullptr};
+    for (int cls = 0; cls < 3; ++cls) {
+        ClassPlan& c = cp[cls];
+        c.base.assign(m + 1, 0);
+        std::vector<const double*> geo(m);
+        for (int a = 0; a < m; ++a) {
+            hbem_geometry* g = G[ids[a]];
+            const int ne = cls == 2 ? g->ngen : cls == 1 ? g->nadj : g->ncoi;
+            c.base[a + 1] = c.base[a] + ne;
+            geo[a] = (const double*)(cls == 2 ? g->d_gngeo.p : cls == 1 ? g->d_angeo.p : g->d_cngeo.p);
+        }
+        c.ne = c.base[m];
+        c.nodes = cls == 2 ? g0->ns * g0->ns : cls == 1 ? 2 * g0->nc * g0->nc : g0->nc * g0->nc;
+        c.nchunk = cls == 2 ? 1 : cls == 1 ? (int)adj_chunks(g0->nc) : (int)coi_chunks(g0->nc);
+        digeo[cls] = upload(w.igeo[cls], geo, st);
+        dibase[cls] = upload(w.ibase[cls], c.base, st);
+    }
+    std::vector<int> qi(nk), qo(nk);
+    for (int j = 0; j < nk; ++j) { qi[j] = j; qo[j] = nk + j; }
+    const int* dqi = upload(w.qi, qi, st);
+    const int* dqo = upload(w.qo, qo, st);
+    std::vector<hbem::CsrDesc> desc(batch), descT(batch);
+    std::vector<long long> roff(batch);
+    std::vector<int> nrow(batch);
+    std::vector<cdbl> Rall;
+    for (int a = 0; a < m; ++a) {
+        hbem_geometry* g = G[ids[a]];
+        const int n = 2 * g->N;
+        const long long off = (long long)Rall.size();
+        const cdbl* Rh = (const cdbl*)R[ids[a]];
+        Rall.insert(Rall.end(), Rh, Rh + (size_t)n * nc);
+        for (int j = 0; j < nk; ++j) {
+            const int b = a * nk + j;
+            if (g->proj) {
+                desc[b] = {(const int*)g->s_F.rp.p, (const int*)g->s_F.ci.p, (const double*)g->s_F.v.p, n};
+                descT[b] = {(const int*)g->s_FT.rp.p, (const int*)g->s_FT.ci.p, (const double*)g->s_FT.v.p, n};
+            }
+            roff[b] = off;
+            nrow[b] = n;
+        }
+    }
+    const hbem::CsrDesc* dF = g0->proj ? upload(w.desc, desc, st) : nullptr;
+    const hbem::CsrDesc* dFT = g0->proj ? upload(w.descT, descT, st) : nullptr;
+    const cdbl* dRall = upload(w.Rall, Rall, st);
+    const long long* droff = upload(w.roff, roff, st);
+    const int* dnrow = upload(w.nrow, nrow, st);
+    // ---- every device buffer at its final size before the device phase ----
+    cdbl* L = w.L.as<cdbl>((size_t)pair_off[m] * nslot * nks);
+    int* ent[3];
+    unsigned char* fl[3];
+    long long* pidx[3];
+    int* einst[3];
+    cdbl* part[3];
+    for (int cls = 0; cls < 3; ++cls) {
+        const size_t ne = (size_t)std::max<long long>(cp[cls].ne, 1);
+        ent[cls] = w.ent[cls].as<int>(3 * ne);
+        fl[cls] = w.flags[cls].as<unsigned char>(ne);
+        pidx[cls] = w.pidx[cls].as<long long>(ne);
+        einst[cls] = w.einst[cls].as<int>(ne);
+        part[cls] = cp[cls].nchunk > 1 ? w.part[cls].as<cdbl>((size_t)nks * ne * cp[cls].nchunk * nslot) : nullptr;
+    }
+    cdbl* T = w.T.as<cdbl>(nn * batch);
+    int* piv = w.piv.as<int>((size_t)n_pad * batch);
+    int* perm = w.perm.as<int>((size_t)n_pad * batch);
+    cdbl* X = w.X.as<cdbl>((size_t)n_pad * nc * batch);
+    cdbl* Y = w.Y.as<cdbl>((size_t)n_pad * nc * batch);
+    cdbl* work = w.work.as<cdbl>(2 * (size_t)n_pad * nc * batch);
+    double* dsig = w.sig.as<double>(batch);
+    HIPC(hipStreamSynchronize(st));   // uploads done; nothing below reads host memory
+    // ---- device phase ----
+    auto device_phase = [&]() {
+        hbem::AsmParams P;
+        std::memset(&P, 0, sizeof(P));
+        P.xg = (const double*)g0->d_xg.p;
+        P.wg = (const double*)g0->d_wg.p;
+        P.xc = (const double*)g0->d_xc.p;
+        P.wc = (const double*)g0->d_wc.p;
+        P.nk = nks;
+        P.kap = dkap;
+        P.kr = dkr;
+        P.cv = dcv;
+        P.npairs_total = pair_off[m];
+        P.ns = g0->ns;
+        P.nc = g0->nc;
+        P.qflags = g0->flags;
+        for (int cls = 2; cls >= 0; --cls) {
+            const ClassPlan& c = cp[cls];
+            if (c.ne == 0) continue;
+            for (int a = 0; a < m; ++a) {
+                hbem_geometry* g = G[ids[a]];
+                const long long cnt = c.base[a + 1] - c.base[a];
+                if (cnt == 0) continue;
+                const DBuf& de = cls == 2 ? g->d_gent : cls == 1 ? g->d_aent : g->d_cent;
+                const DBuf& df = cls == 2 ? g->d_gflags : cls == 1 ? g->d_aflags : g->d_cflags;
+                const DBuf& dp = cls == 2 ? g->d_gpidx : cls == 1 ? g->d_apidx : g->d_cpidx;
+                hbem::copy_bytes(ent[cls] + 3 * c.base[a], de.p, 3 * cnt * sizeof(int), st);
+                hbem::copy_bytes(fl[cls] + c.base[a], df.p, (size_t)cnt, st);
+                hbem::copy_bytes(pidx[cls] + c.base[a], dp.p, cnt * sizeof(long long), st);
+                hbem::add_offset_ll(pidx[cls] + c.base[a], cnt, pair_off[a], st);
+                hbem::fill_int(einst[cls] + c.base[a], (int)cnt, a, st);
+            }
+            P.igeo = digeo[cls];
+            P.ibase = dibase[cls];
+            P.einst = einst[cls];
+            P.ent = ent[cls];
+            P.flags = fl[cls];
+            P.pidx = pidx[cls];
+            P.nent = (int)c.ne;
+            P.ngeo = nullptr;
+            P.chunk = cls == 2 ? c.nodes : kChunkNodes;
+            P.nchunk = c.nchunk;
+            P.out = c.nchunk > 1 ? part[cls] : L;
+            hbem::launch_assemble(cls, 0, P, 0, st);
+            if (c.nchunk > 1)
+                hbem::launch_chunk_sum(part[cls], (int)c.ne, c.nchunk, nks, nslot, pidx[cls], pair_off[m], L, st);
+        }
+        for (int a = 0; a < m; ++a) {
+            hbem_geometry* g = G[ids[a]];
+            const int n = 2 * g->N;
+            hbem::launch_form_T(L + (size_t)pair_off[a] * nslot * nks, nks, nslot, dqi, dqo, nk, g->N, 0,
+                                (const double*)g->d_M.p, T + (size_t)a * nk * nn, nullptr, nullptr, st, n_pad);
+            if (n < n_pad) hbem::launch_pad_identity(T + (size_t)a * nk * nn, n, n_pad, nk, st);
+        }
+        hbem::lu_factor_batched(T, n_pad, batch, piv, perm, st);
+        DenseCtx c;
+        c.st = st; c.n = n_pad; c.batch = batch; c.nc = nc; c.q = q; c.LU = T; c.ms = (long long)nn; c.perm = perm;
+        c.F = nullptr;
+        c.FT = nullptr;
+        c.Fm = dF;
+        c.FTm = dFT;
+        c.X = X;
+        c.Y = Y;
+        c.work = work;
+        hbem::replicate_padded(dRall, droff, dnrow, n_pad, nc, batch, c.X, st);
+        rsvd_core(c, dsig, nullptr, nullptr);
+    };
+    if (graphs_enabled()) {
+        std::vector<const void*> key;
+        for (int a = 0; a < m; ++a) key.push_back(G[ids[a]]);
+        key.push_back((const void*)(size_t)nk);
+        key.push_back((const void*)(size_t)nc);
+        key.push_back((const void*)(size_t)q);
+        key.push_back((const void*)(size_t)n_pad);
+        const void* bufs[] = {dkap, dkr, dcv, digeo[0], digeo[1], digeo[2], dibase[0], dibase[1], dibase[2], dqi, dqo,
+                              dF, dFT, dRall, droff, dnrow, L, T, piv, perm, X, Y, work, dsig,
+                              ent[0], ent[1], ent[2], fl[0], fl[1], fl[2], pidx[0], pidx[1], pidx[2], einst[0],
+                              einst[1], einst[2], part[0], part[1], part[2]};
+        key.insert(key.end(), std::begin(bufs), std::end(bufs));
+        GraphEntry* ge = nullptr;
+        for (auto& e : w.graphs)
+            if (e.key == key) ge = &e;
+        if (!ge) {
+            hipGraph_t graph = nullptr;
+            HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            device_phase();
+            const hipError_t ec = hipStreamEndCapture(st, &graph);
+            HIPC(ec);
+            hipGraphExec_t exec = nullptr;
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIPC(ei);
+            if (w.graphs.size() >= 16) w.drop_graphs();
+            w.graphs.push_back({key, exec});
+            ge = &w.graphs.back();
+        }
+        HIPC(hipGraphLaunch(ge->exec, st));
+    } else {
+        device_phase();
+    }
+    HIPC(hipGetLastError());
+    std::vector<double> out(batch);
+    HIPC(hipMemcpyAsync(out.data(), dsig, batch * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    for (int a = 0; a < m; ++a)
+        for (int j = 0; j < nk; ++j) sigma[(size_t)ids[a] * nk + j] = out[(size_t)a * nk + j];
+}
+
+}  // namespace
+
+extern "C" {
+
+int hbem_sigma_sweep_multi(const hbem_geometry_t* geoms, int ninst, const double* ks, int nk, double c_o,
+                           const double* c_i, const void* const* R, int nc, int q, int bucket, double* sigma) {
+    API_BEGIN
+    if (!geoms || !ks || !c_i || !R || !sigma) throw Error(HBEM_EINVAL, "null argument");
+    if (ninst < 1 || nk < 1) throw Error(HBEM_EINVAL, "need at least one instance and one wavenumber");
+    if (bucket < 0 || bucket % 32 != 0 || bucket > 4096) throw Error(HBEM_EINVAL, "bucket must be a multiple of 32 in [0, 4096]");
+    const int gran = bucket == 0 ? 256 : bucket;
+    std::vector<hbem_geometry*> G(geoms, geoms + ninst);
+    for (int i = 0; i < ninst; ++i) {
+        if (!G[i] || !R[i]) throw Error(HBEM_EINVAL, "null geometry or test matrix");
+        if (G[i]->ns != G[0]->ns || G[i]->proj != G[0]->proj || G[i]->flags != G[0]->flags || G[i]->dev != G[0]->dev)
+            throw Error(HBEM_EINVAL, "instances must share quadrature order, projection, flags and device");
+        check_dense_args(ks, 2 * G[i]->N, R[i], nc, q);
+        for (int j = 0; j < nk; ++j) { check_k(ks[(size_t)i * nk + j], 0.0, c_i[i]); check_k(ks[(size_t)i * nk + j], 0.0, c_o); }
+    }
+    // every instance's context is locked for the call (in address order), then the work space
+    std::vector<hbem_geometry*> order(G);
+    std::sort(order.begin(), order.end());
+    order.erase(std::unique(order.begin(), order.end()), order.end());
+    std::vector<std::unique_lock<std::mutex>> locks;
+    for (auto* g : order) locks.emplace_back(g->mu);
+    std::lock_guard<std::mutex> lk(g_multi.mu);
+    HIPC(hipSetDevice(G[0]->dev));
+    // buckets of equal padded order, in increasing order; a bucket is split
+    // into subsets whose work space fits the free device memory
+    std::vector<std::pair<int, int>> key(ninst);
+    for (int i = 0; i < ninst; ++i) key[i] = {(2 * G[i]->N + gran - 1) / gran * gran, i};
+    std::sort(key.begin(), key.end());
+    size_t held = 0;
+    {
+        MultiWork& w = g_multi;
+        const DBuf* bs[] = {&w.L, &w.part[0], &w.part[1], &w.part[2], &w.T, &w.piv, &w.perm, &w.X, &w.Y, &w.work};
+        for (auto* b : bs) held += b->n;
+    }
+    const size_t avail = usable_device_bytes(held);
+    size_t a = 0;
+    while (a < key.size()) {
+        const int n_pad = key[a].first;
+        std::vector<int> ids;
+        size_t used = 0;
+        while (a < key.size() && key[a].first == n_pad) {
+            hbem_geometry* g = G[key[a].second];
+            const size_t part = (size_t)std::max(g->nadj * adj_chunks(g->nc), g->ncoi * coi_chunks(g->nc));
+            const size_t need = (size_t)2 * nk * ((size_t)g->npairs + part) * 16 * sizeof(cdbl) +
+                                (size_t)nk * ((size_t)n_pad * n_pad * sizeof(cdbl) + (size_t)n_pad * (8 + (4 * nc + 4) * sizeof(cdbl)));
+            if (!ids.empty() && used + need > avail) break;
+            ids.push_back(key[a].second);
+            used += need;
+            ++a;
+        }
+        multi_subset(G, ids, n_pad, ks, nk, c_o, c_i, R, nc, q, sigma);
+    }
     API_END
 }
 

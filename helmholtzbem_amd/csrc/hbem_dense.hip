@@ -20,10 +20,15 @@
 
 namespace hbem {
 
+// Reciprocal and product with every multiply-add an explicit fma: the LU's
+// arithmetic per matrix entry is then fixed by the source, whichever kernel
+// instantiation (rows per thread, chunk width) performs it, so a matrix
+// factorised inside a larger zero-padded one gives the same bits.
 __device__ __forceinline__ cdbl crecip(cdbl d) {
-    const double s = 1.0 / (d.x * d.x + d.y * d.y);
+    const double s = 1.0 / fma(d.x, d.x, d.y * d.y);
     return mk(d.x * s, -d.y * s);
 }
+__device__ __forceinline__ cdbl cmulx(cdbl a, cdbl b) { return mk(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x)); }
 __device__ __forceinline__ void cfms(cdbl& acc, cdbl a, cdbl b) {   // acc -= a * b
     acc.x = fma(-a.x, b.x, fma(a.y, b.y, acc.x));
     acc.y = fma(-a.x, b.y, fma(-a.y, b.x, acc.y));
@@ -41,8 +46,11 @@ __device__ __forceinline__ cdbl shfl_c(cdbl v, int src) {
 __device__ __forceinline__ float cabs2(cflt a) { return a.x * a.x + a.y * a.y; }
 __device__ __forceinline__ cflt operator*(cflt a, cflt b) { return cflt{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 __device__ __forceinline__ cflt crecip(cflt d) {
-    const float s = 1.0f / (d.x * d.x + d.y * d.y);
+    const float s = 1.0f / fmaf(d.x, d.x, d.y * d.y);
     return cflt{d.x * s, -d.y * s};
+}
+__device__ __forceinline__ cflt cmulx(cflt a, cflt b) {
+    return cflt{fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x)};
 }
 __device__ __forceinline__ void cfms(cflt& acc, cflt a, cflt b) {   // acc -= a * b
     acc.x = fmaf(-a.x, b.x, fmaf(a.y, b.y, acc.x));
@@ -111,7 +119,7 @@ __global__ __launch_bounds__(1024) void k_lu_panel(C* __restrict__ A, int n, int
         const C d = prow[c];
         const C inv = (d.x == 0.0 && d.y == 0.0) ? C{} : crecip(d);
         for (int i = col + 1 + tid; i < n; i += blockDim.x) {
-            const C l = a[i + (size_t)col * n] * inv;
+            const C l = cmulx(a[i + (size_t)col * n], inv);
             a[i + (size_t)col * n] = l;
             // eight columns per round, loads issued together
             for (int cc0 = c + 1; cc0 < jb; cc0 += 8) {
@@ -239,7 +247,7 @@ __global__ __launch_bounds__(1024) void k_lu_panel_rg(C* __restrict__ A, int n, 
 #pragma unroll
             for (int u = 0; u < R; ++u)
                 if (rowu[u] < n && rowu[u] > col) {
-                    const C l = v[u][c] * inv;
+                    const C l = cmulx(v[u][c], inv);
                     v[u][c] = l;
 #pragma unroll
                     for (int e = c + 1; e < CW; ++e)
@@ -689,21 +697,25 @@ static void gemm_sub(int m, int n, int k, const cflt* A, const cflt* B, cflt* C,
                        batch);
 }
 
-// Blocked right-looking LU in steps of 64 columns, each factorised as two
-// 32-column sub-panels: the second sub-panel, and its future U rows right of
-// it, are brought up to date with the first one's k = 32 product, and the
-// trailing matrix receives one k = 64 update per step.  The trailing update
-// reads and writes the trailing matrix once per step -- the traffic that bounds
-// it at these sizes -- so 64-column steps halve it.
-//   sub-panel 1: panel, interchanges everywhere + U1 for every column right of it
-//   sub-panel 2: A22 -= L1 U1 on its columns; panel; its interchanges everywhere;
-//                A[j1:j2, j2:] -= L1[j1:j2] U1[:, j2:] with the interchanged L1 rows;
-//                U2 = L22^{-1} A[j1:j2, j2:]
-//   trailing:    A[j2:, j2:] -= [L1 L2][j2:] [U1; U2]
-// C = cdbl: the FP64 factorisation; C = cflt: the same steps in FP32.
+// Blocked right-looking LU in steps of S 32-column sub-panels (S = 2, or 4
+// above order 2048).  Within a step, sub-panel s is first brought up to date
+// with the step's earlier sub-panels (k = 32 s), factorised, its row
+// interchanges applied to every other column, and its U rows right of it
+// formed; the trailing matrix then receives one k = 32 S update per step.
+// That update reads and writes the trailing matrix once per step -- the
+// traffic that bounds it at these sizes -- so wider steps cut it, at the cost
+// of more small sub-panel products:
+//   s = 0:  panel; interchanges everywhere + U0 for every column right of it
+//   s > 0:  A[js:, js:js+32] -= L[js:, j0:js] U[j0:js, js:js+32]; panel;
+//           interchanges everywhere; A[js:js+32, js+32:] -= L[js:js+32, j0:js] U[j0:js, js+32:];
+//           Us = Lss^{-1} A[js:js+32, js+32:]
+//   then:   A[jE:, jE:] -= L[jE:, j0:jE] U[j0:jE, jE:]
+// C = cdbl: the FP64 factorisation; C = cflt: the same steps in FP32.  The
+// step width depends on n only, so a matrix gives the same bits in any batch.
 template <class C>
 static void lu_factor_impl(C* A, int n, int batch, int* ipiv, int* perm, hipStream_t st) {
     const int sb = 32;
+    const int S = n > 2048 ? 4 : 2;
     const long long s = (long long)n * n;
     auto at = [&](int r, int c) { return A + r + (size_t)c * n; };
     auto rows = [&](int mode, int j0, int jb) {
@@ -722,23 +734,27 @@ static void lu_factor_impl(C* A, int n, int batch, int* ipiv, int* perm, hipStre
         else if (rows <= 4096) hipLaunchKernelGGL((k_lu_panel_rg<C, 4, 4>), g, t, 0, st, A, n, j, jb, ipiv);
         else hipLaunchKernelGGL((k_lu_panel<C>), g, t, 0, st, A, n, j, jb, ipiv);
     };
-    for (int j0 = 0; j0 < n; j0 += 2 * sb) {
-        const int jb1 = n - j0 < sb ? n - j0 : sb;
-        panel(j0, jb1);
-        rows(3, j0, jb1);
-        const int j1 = j0 + jb1;
-        const int jb2 = n - j1 < sb ? n - j1 : sb;
-        const int j2 = j1 + jb2;
-        if (jb2 > 0) {
-            gemm_sub(n - j1, jb2, jb1, at(j1, j0), at(j0, j1), at(j1, j1), n, s, batch, st);
-            panel(j1, jb2);
-            rows(1, j1, jb2);
-            if (n - j2 > 0) {
-                gemm_sub(jb2, n - j2, jb1, at(j1, j0), at(j0, j2), at(j1, j2), n, s, batch, st);
-                rows(2, j1, jb2);
+    for (int j0 = 0; j0 < n; j0 += S * sb) {
+        const int jE = n < j0 + S * sb ? n : j0 + S * sb;
+        for (int sp = 0; sp < S; ++sp) {
+            const int js = j0 + sp * sb;
+            if (js >= n) break;
+            const int jb = n - js < sb ? n - js : sb;
+            const int je = js + jb;
+            if (sp == 0) {
+                panel(js, jb);
+                rows(3, js, jb);
+                continue;
+            }
+            gemm_sub(n - js, jb, js - j0, at(js, j0), at(j0, js), at(js, js), n, s, batch, st);
+            panel(js, jb);
+            rows(1, js, jb);
+            if (n - je > 0) {
+                gemm_sub(jb, n - je, js - j0, at(js, j0), at(j0, je), at(js, je), n, s, batch, st);
+                rows(2, js, jb);
             }
         }
-        if (n - j2 > 0) gemm_sub(n - j2, n - j2, j2 - j0, at(j2, j0), at(j0, j2), at(j2, j2), n, s, batch, st);
+        if (n - jE > 0) gemm_sub(n - jE, n - jE, jE - j0, at(jE, j0), at(j0, jE), at(jE, jE), n, s, batch, st);
     }
     hipLaunchKernelGGL(k_lu_perm, dim3(batch), dim3(64), 0, st, n, ipiv, perm);
 }
@@ -1213,6 +1229,30 @@ void refine_update_batched(cdbl* Z, const cdbl* D, int cnt, int batch, double to
     hipLaunchKernelGGL(k_refine_update, dim3(batch), dim3(256), 0, st, Z, D, cnt, tol * tol, check ? 1 : 0, flag);
 }
 
+// dst[i] = src[i] for nbytes bytes (a kernel, not a runtime copy, so that it
+// can stand in a captured graph like the other steps)
+__global__ __launch_bounds__(256) void k_copy_bytes(unsigned char* __restrict__ dst, const unsigned char* __restrict__ src,
+                                                    long long nbytes) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nbytes) dst[i] = src[i];
+}
+
+void copy_bytes(void* dst, const void* src, size_t nbytes, hipStream_t st) {
+    if (nbytes == 0) return;
+    hipLaunchKernelGGL(k_copy_bytes, dim3((unsigned)((nbytes + 255) / 256)), dim3(256), 0, st, (unsigned char*)dst,
+                       (const unsigned char*)src, (long long)nbytes);
+}
+
+__global__ void k_add_ll(long long* __restrict__ p, long long count, long long off) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] += off;
+}
+
+void add_offset_ll(long long* p, long long count, long long off, hipStream_t st) {
+    if (count <= 0 || off == 0) return;
+    hipLaunchKernelGGL(k_add_ll, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, p, count, off);
+}
+
 __global__ void k_fill_int(int* __restrict__ p, int count, int v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = v;
@@ -1281,6 +1321,52 @@ void csr_apply_batched(const int* rowptr, const int* colidx, const double* val, 
     if (total <= 0) return;
     hipLaunchKernelGGL(k_csr_apply, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, rowptr, colidx, val, n, X,
                        Y, ncols);
+}
+
+__global__ __launch_bounds__(256) void k_csr_apply_multi(const CsrDesc* __restrict__ desc, int n_pad,
+                                                         const cdbl* __restrict__ X, cdbl* __restrict__ Y, int ncols,
+                                                         int batch) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per = (long long)n_pad * ncols;
+    if (id >= per * batch) return;
+    const int b = (int)(id / per);
+    const int i = (int)(id % n_pad);
+    const CsrDesc d = desc[b];
+    cdbl s = mk(0, 0);
+    if (i < d.n) {
+        const cdbl* x = X + (id - i);
+        const int p1 = d.rp[i + 1];
+        for (int p = d.rp[i]; p < p1; ++p) axpy(s, d.v[p], x[d.ci[p]]);
+    }
+    Y[id] = s;
+}
+
+void csr_apply_multi(const CsrDesc* desc, int n_pad, const cdbl* X, cdbl* Y, int ncols, int batch, hipStream_t st) {
+    const long long total = (long long)n_pad * ncols * batch;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_csr_apply_multi, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, desc, n_pad, X, Y,
+                       ncols, batch);
+}
+
+__global__ __launch_bounds__(256) void k_replicate_padded(const cdbl* __restrict__ src, const long long* __restrict__ off,
+                                                          const int* __restrict__ nrow, int n_pad, int nc, int batch,
+                                                          cdbl* __restrict__ X) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per = (long long)n_pad * nc;
+    if (id >= per * batch) return;
+    const int b = (int)(id / per);
+    const long long r = id - (long long)b * per;
+    const int c = (int)(r / n_pad), i = (int)(r % n_pad);
+    const int nb = nrow[b];
+    X[id] = i < nb ? src[off[b] + (long long)c * nb + i] : mk(0, 0);
+}
+
+void replicate_padded(const cdbl* src, const long long* off, const int* nrow, int n_pad, int nc, int batch, cdbl* X,
+                      hipStream_t st) {
+    const long long total = (long long)n_pad * nc * batch;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_replicate_padded, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, src, off, nrow,
+                       n_pad, nc, batch, X);
 }
 
 // dst[b * count + i] = src[i] for every matrix b of the batch

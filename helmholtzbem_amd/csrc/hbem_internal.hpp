@@ -36,6 +36,14 @@ struct AsmParams {
     long long npairs_total;      // N (N + 1) / 2
     cdbl* out;                   // nchunk == 1: local matrices [npairs_total][16 (D+1)][nk];
                                  // else partials [nent][nchunk][16 (D+1)][nk]
+    // several scatterers in one launch (paired-lane kernel only): entry e
+    // belongs to instance i = einst[e], whose nk slots are kap/kr/cv[i nk ..]
+    // and whose node table igeo[i] holds entry e at position e - ibase[i];
+    // pidx is then the pair's index in the concatenation of the instances'
+    // pair lists.  einst null: one instance.
+    const int* einst;
+    const double* const* igeo;
+    const long long* ibase;
     int ns, nc;                  // Gauss / composite rule sizes
     int qflags;                  // HBEM_FLAG_* quirk bits
 };
@@ -53,9 +61,12 @@ void launch_chunk_sum(const cdbl* part, int nent, int nchunk, int nk, int nslot,
 // L: local matrices [pair][slot][wavenumber] of nk wavenumber slots
 void launch_gather_layers(const cdbl* L, int nk, int nslot, int q, int N, int D, int layers, cdbl* K, cdbl* W,
                           cdbl* V, hipStream_t st);
-// T_t for output b is written at T[t] + b * (2N)^2 (t = 0..D)
+// T_t for output b is written at T[t] + b * ld^2 (t = 0..D) with leading
+// dimension ld (0: 2N)
 void launch_form_T(const cdbl* L, int nk, int nslot, const int* qi, const int* qo, int nb, int N, int D,
-                   const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st);
+                   const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st, int ld = 0);
+// identity in rows and columns n..ld-1 of nb matrices of order ld
+void launch_pad_identity(cdbl* T, int n, int ld, int nb, hipStream_t st);
 // A_o = [[-K_o + M/2, V_o], [W_o, K_o^T + M/2]] of wavenumber slot o (2N x 2N)
 void launch_form_Ao(const cdbl* L, int nk, int nslot, int o, int N, const double* M, cdbl* A, hipStream_t st);
 
@@ -82,6 +93,8 @@ size_t residual_part_count(int n, int nc, int batch);
 void refine_update_batched(cdbl* Z, const cdbl* D, int cnt, int batch, double tol, bool check, int* flag,
                            hipStream_t st);
 void fill_int(int* p, int count, int v, hipStream_t st);
+void add_offset_ll(long long* p, long long count, long long off, hipStream_t st);
+void copy_bytes(void* dst, const void* src, size_t nbytes, hipStream_t st);
 // *flag = 1 if any of cnt doubles is not finite / a diagonal entry of A is 0
 void flag_nonfinite(const double* x, long long cnt, int* flag, hipStream_t st);
 void flag_zero_diag(const cdbl* A, int n, int* flag, hipStream_t st);
@@ -89,6 +102,20 @@ void flag_zero_diag(const cdbl* A, int n, int* flag, hipStream_t st);
 // column indices ascending within a row); X, Y: n x ncols, column-major, ld n.
 void csr_apply_batched(const int* rowptr, const int* colidx, const double* val, int n, const cdbl* X, cdbl* Y,
                        long long ncols, hipStream_t st);
+// A sparse real matrix in compressed-row form on the device, and the product
+// Y = S_b X_b for the n_pad x ncols blocks of a batch whose matrix b has its
+// own S_b = desc[b] of order desc[b].n <= n_pad (rows beyond it are zero).
+struct CsrDesc {
+    const int* rp;
+    const int* ci;
+    const double* v;
+    int n;
+};
+void csr_apply_multi(const CsrDesc* desc, int n_pad, const cdbl* X, cdbl* Y, int ncols, int batch, hipStream_t st);
+// X_b (n_pad x nc) = the n_b x nc block at src + off[b] (column-major, ld
+// n_b) padded with zero rows, n_b = nrow[b]
+void replicate_padded(const cdbl* src, const long long* off, const int* nrow, int n_pad, int nc, int batch, cdbl* X,
+                      hipStream_t st);
 // dst = batch copies of the count entries at src, back to back
 void replicate_batched(const cdbl* src, long long count, int batch, cdbl* dst, hipStream_t st);
 void orthonormalize_thin_batched(cdbl* X, int n, int batch, int nc, hipStream_t st);

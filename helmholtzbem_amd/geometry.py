@@ -372,3 +372,24 @@ def polygon_mesh(vertices, npanels: int) -> ParametrizedMesh:
 def random_polygon_mesh(seed: int, nvert: int, npanels: int) -> ParametrizedMesh:
     rng = Xoshiro256ss(seed)
     return polygon_mesh(random_polygon_vertices(rng, nvert), npanels)
+
+
+def random_polygon_instances(seed: int = 2, count: int = 64, nk: int = 32, k_min: float = 2.0, k_max: float = 20.0,
+                             nmin: int = 100, nmax: int = 1000, vmin: int = 6, vmax: int = 16,
+                             ci_min: float = 4.0, ci_max: float = 25.0):
+    """The many-scatterer workload: `count` random star-shaped polygons from
+    one xoshiro256** stream (per instance, in this order: the vertex count m
+    in [vmin, vmax], m sorted angles and m radii as random_polygon_vertices
+    draws them, the panel count N in [nmin, nmax], c_i uniform in
+    [ci_min, ci_max], then nk wavenumbers uniform in [k_min, k_max], sorted).
+    Returns a list of (mesh, c_i, ks)."""
+    rng = Xoshiro256ss(seed)
+    out = []
+    for _ in range(count):
+        m = rng.randint(vmin, vmax)
+        verts = random_polygon_vertices(rng, m)
+        N = rng.randint(nmin, nmax)
+        c_i = rng.uniform(ci_min, ci_max)
+        ks = np.sort(np.array([rng.uniform(k_min, k_max) for _ in range(nk)]))
+        out.append((polygon_mesh(verts, N), c_i, ks))
+    return out

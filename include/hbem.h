@@ -78,6 +78,11 @@ int hbem_init(int device);
 int hbem_geometry_create(const double* panels, int npanels, const double* coefs, int ncoefs,
                          int order, int proj, int flags, hbem_geometry_t* out);
 int hbem_geometry_destroy(hbem_geometry_t g);
+/* Frees the context's per-call work buffers (local matrices, T, factor and
+ * thin blocks; the geometry, node tables and projection factor stay); the
+ * next call allocates them again.  For callers that keep many contexts alive.
+ * (No counterpart in the reference, which allocates per call.) */
+int hbem_geometry_release_work(hbem_geometry_t g);
 /* getNumPanels, getGaussQROrder, getCGaussQROrder, getPanelsAreLines (galerkin_builder.hpp:94-97) */
 int hbem_geometry_info(hbem_geometry_t g, int* npanels, int* order, int* corder, int* lines);
 
@@ -209,6 +214,23 @@ int hbem_sigma_sweep_der2(hbem_geometry_t g, const double* ks, int nk, double c_
  */
 int hbem_sigma_sweep_mixed(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i, const void* R, int nc,
                            int q, int refine, double* sigma, int* fp64_used);
+
+/*
+ * sigma_min(k) for many scatterers at once -- the reference drivers' loops
+ * over instances and wavenumbers (examples/roots_brent_square_rsvd.cpp:167-171,
+ * find_roots.cpp:1615-1700) as one batched computation.  geoms[i] (contexts
+ * with the same quadrature order, projection and flags) is swept over its nk
+ * wavenumbers ks[i nk .. i nk + nk - 1] with inner index c_i[i], outer c_o
+ * and test matrix R[i] (2 N_i x nc, host); sigma[i nk + j] receives the
+ * result.  The instances are grouped by their order 2 N_i rounded up to a
+ * multiple of `bucket` (a multiple of 32; 0 = 256): every panel-pair class of
+ * a group is one assembly launch, and the group's T matrices, padded with the
+ * identity to the common order, are one batched LU and randomized SVD.  The
+ * padding leaves every result equal bit for bit to hbem_sigma_sweep of the
+ * instance alone (nk >= 32, whose sweep uses the same assembly kernel).
+ */
+int hbem_sigma_sweep_multi(const hbem_geometry_t* geoms, int ninst, const double* ks, int nk, double c_o,
+                           const double* c_i, const void* const* R, int nc, int q, int bucket, double* sigma);
 
 /* Phase timing (HIP events on the context's stream), for the benchmark's
  * roofline figures.  enable != 0 clears and starts recording; the events are

@@ -311,3 +311,27 @@ def test_rsvd_tall_panels_and_wide_blocks(H, O):
         s = H.randomized_svd.sv(T, R, q)
         so = O.sv(T, R, q)
         assert abs(s - so) <= SIG_TOL * so, (nc, q, s, so)
+
+
+def test_star400_oracle_parity_at_bench_size(big, H, O):
+    """The 400-panel order-16 star at one k of its 256-point grid, against the
+    oracle's projected T, T': sigma at q = 2 and q = 30 against the oracle's
+    sv (the same randomized iteration, 1e-10); at q = 400, where the
+    iteration has converged (at this k the two smallest singular values are
+    close, and q = 30 is still 6e-6 above sigma_min), sigma against the exact
+    smallest singular value (LAPACK SVD of the oracle's T_p, 1e-10) and
+    dsigma/dk against Re(u^H T_p' v) of that exact singular pair (1e-8)."""
+    mesh, bd = big
+    k = np.linspace(1.0, 10.0, 256)[137]
+    R = H.gaussian_test_matrix(800, 2, 1)
+    T, Td = O.SolOp(mesh, 16).gen_sol_op(k, 1.0, 16.0, d=1, nthreads=16)
+    for q in (2, 30):
+        sig = H.sigma_sweep(bd, np.array([k]), 1.0, 16.0, R, q)
+        so = O.sv(T, R, q)
+        assert abs(sig[0] - so) <= SIG_TOL * so, (q, sig[0], so)
+    sig, dsig = H.sigma_sweep(bd, np.array([k]), 1.0, 16.0, R, 400, der=1)
+    U, S, Vh = np.linalg.svd(T)
+    u, v = U[:, -1], Vh[-1].conj()
+    assert abs(sig[0] - S[-1]) <= SIG_TOL * S[-1], (sig[0], S[-1], S[-2])
+    d_exact = float(np.real(u.conj() @ (Td @ v)))
+    assert abs(dsig[0] - d_exact) <= 1e-8 * max(1.0, abs(d_exact)), (dsig[0], d_exact)
