@@ -19,7 +19,7 @@ EXPORTED = [
     "hbem_assemble_dense", "hbem_mass_matrix", "hbem_projection_factor", "hbem_gen_sol_op", "hbem_projection",
     "hbem_rsvd_sv", "hbem_rsvd_sv_der", "hbem_rsvd_sv_der2", "hbem_sigma_sweep", "hbem_profile_enable",
     "hbem_profile_query", "hbem_assemble_dense_batch", "hbem_tp_A_rhs", "hbem_tp_solve", "hbem_sweep_batch_size",
-    "hbem_rsvd_benchmarking", "hbem_rsvd_timers", "hbem_rsvd_reset_timer",
+    "hbem_rsvd_benchmarking", "hbem_rsvd_timers", "hbem_rsvd_reset_timer", "hbem_rsvd_sv_noalloc",
     "hbem_sigma_sweep_der2", "hbem_sigma_sweep_mixed", "hbem_sigma_sweep_multi",
     "hbem_geometry_release_work",
 ]
@@ -64,6 +64,7 @@ def lib():
         L.hbem_rsvd_benchmarking.argtypes = [i]
         L.hbem_rsvd_timers.argtypes = [dp]
         L.hbem_rsvd_reset_timer.argtypes = []
+        L.hbem_rsvd_sv_noalloc.argtypes = [vp, i, vp, i, i, vp, vp]
         L.hbem_sigma_sweep_der2.argtypes = [vp, dp, i, d, d, vp, i, i, dp, dp, dp]
         L.hbem_sigma_sweep_mixed.argtypes = [vp, dp, i, d, d, vp, i, i, i, dp, ip]
         L.hbem_geometry_release_work.argtypes = [vp]

@@ -186,6 +186,37 @@ int sweep_batch(size_t per_k, size_t avail, int nk) {
     return (int)B;
 }
 
+// Work space of hbem_rsvd_sv_noalloc: a fixed device array, one per GPU (the
+// runtime instantiates file-scope device variables on every device the
+// library runs on, each at its own address).  Layout for order n <= kNaMaxN
+// and nc <= kNaMaxCols columns, each part 256-byte aligned: the LU copy of T
+// (n^2 complex), pivots, permutation (n ints each), X, Y (n nc complex each),
+// thin-solve work (2 n nc complex).
+constexpr int kNaMaxN = 2048;
+constexpr int kNaMaxCols = 8;
+constexpr size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNaBytes = al256(sizeof(cdbl) * (size_t)kNaMaxN * kNaMaxN) + 2 * al256(sizeof(int) * (size_t)kNaMaxN) +
+                            4 * al256(sizeof(cdbl) * (size_t)kNaMaxN * kNaMaxCols);
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+// Device address of `scratch` on device `dev` (which must be current): the
+// array's name is a host-side stand-in, so its address is looked up once per
+// device and only that address is given to copies and kernels.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+void* scratch_on(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw Error(HBEM_EDEVICE, "device index beyond the library's table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Error(HBEM_EDEVICE, "no device address for the work array");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
 // Free device memory usable by a call that may regrow buffers currently
 // holding `held` bytes, less a reserve of 2 GiB or 3% of the device.
 size_t usable_device_bytes(size_t held) {
@@ -501,6 +532,7 @@ int hbem_init(int device) {
     if (device < 0 || device >= n) throw Error(HBEM_EINVAL, "invalid device index");
     g_device = device;
     HIPC(hipSetDevice(device));
+    scratch_on(device);          // resolve hbem_rsvd_sv_noalloc's array on this device now
     API_END
 }
 
@@ -938,6 +970,42 @@ int hbem_rsvd_sv(const void* T, int n, const void* R, int nc, int q, double* sig
     if (!sigma) throw Error(HBEM_EINVAL, "null output");
     std::lock_guard<std::mutex> lk(g_sa.mu);
     *sigma = standalone_sv(T, n, R, nc, q, false);
+    API_END
+}
+
+// The kernels of hbem_rsvd_sv with every work buffer carved out of this
+// device's `scratch`: a device-to-device copy of T, the blocked LU, the
+// randomized iteration, sigma written to the caller's device double.  Only
+// enqueues work on `stream`; nothing here allocates or waits.  Every buffer is
+// written before it is read, so the array needs no clearing between calls.
+int hbem_rsvd_sv_noalloc(const void* T_dev, int n, const void* R_dev, int nc, int q, double* sigma_dev, void* stream) {
+    API_BEGIN
+    check_dense_args(T_dev, n, R_dev, nc, q);
+    if (!sigma_dev) throw Error(HBEM_EINVAL, "null output");
+    if (n > kNaMaxN) throw Error(HBEM_EINVAL, "matrix order exceeds the no-allocation limit of 2048");
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    unsigned char* base = (unsigned char*)scratch_on(dev);
+    const hipStream_t st = (hipStream_t)stream;
+    const size_t nn = (size_t)n * n;
+    cdbl* LU = (cdbl*)base;
+    base += al256(sizeof(cdbl) * nn);
+    int* piv = (int*)base;
+    base += al256(sizeof(int) * (size_t)n);
+    int* perm = (int*)base;
+    base += al256(sizeof(int) * (size_t)n);
+    const size_t thin = al256(sizeof(cdbl) * (size_t)n * nc);
+    DenseCtx c;
+    c.st = st; c.n = n; c.batch = 1; c.nc = nc; c.q = q; c.LU = LU; c.ms = (long long)nn; c.perm = perm;
+    c.F = nullptr; c.FT = nullptr;
+    c.X = (cdbl*)base;
+    c.Y = (cdbl*)(base + thin);
+    c.work = (cdbl*)(base + 2 * thin);
+    HIPC(hipMemcpyAsync(LU, T_dev, nn * sizeof(cdbl), hipMemcpyDeviceToDevice, st));
+    hbem::lu_factor_batched(LU, n, 1, piv, perm, st);
+    HIPC(hipMemcpyAsync(c.X, R_dev, (size_t)n * nc * sizeof(cdbl), hipMemcpyDeviceToDevice, st));
+    rsvd_core(c, sigma_dev, nullptr, nullptr);
+    HIPC(hipGetLastError());
     API_END
 }
 

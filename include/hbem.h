@@ -176,6 +176,22 @@ int hbem_rsvd_sv_der2(const void* T, const void* Td, const void* Tdd, int n, con
                       double* out3);
 
 /*
+ * hbem_rsvd_sv on device-resident inputs without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops).  T_dev (n x n, column-major,
+ * not modified), R_dev (n x nc) and sigma_dev (one double) are device
+ * pointers of the current device; stream is a hipStream_t (NULL: the null
+ * stream).  The work enqueued on the stream (a copy of T, LU, the randomized
+ * iteration) uses a fixed device array of ~66 MiB per GPU, held until the
+ * process ends, so n <= 2048 and nc <= 8; larger sizes return HBEM_EINVAL
+ * before anything is enqueued.  The call returns without waiting; calls that
+ * overlap on one GPU share that array and race.  The array's device address
+ * is looked up by hbem_init(device) or by the first call on a device, so a
+ * caller that captures graphs calls either one beforehand.
+ */
+int hbem_rsvd_sv_noalloc(const void* T_dev, int n, const void* R_dev, int nc, int q, double* sigma_dev, void* stream);
+
+/*
  * Batched sigma_min(k) sweep -- the caller pattern of the reference drivers
  * (std::transform(par_unseq, ks) { gen_sol_op(k, c_o, c_i, T); sv(T, R, q); },
  * examples/roots_brent_square_rsvd.cpp:167-171).  Everything stays on the
