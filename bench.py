@@ -31,11 +31,16 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
+FLOP_PER_NODE = {0: 292.0, 1: 376.0, 2: 498.0}   # the library's algorithmic model (hbem_profile_query)
 
 # HBM bytes per launch of the general-pair assembly kernel (rocprofv3
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, KiB per dispatch, scaled by the
-# counters' reading of a plain 1 GiB copy): profiles/*_hbm_traffic.txt
-TRAFFIC_BYTES = {"star400": 3.2686e10, "polygon2000": None}
+# counters' reading of a plain 1 GiB copy): profiles/k_assemble_x2_general_*.txt
+TRAFFIC_BYTES = {"star400": 3.2686e10, "polygon2000": 1.4829e11}
+# FP64 FLOP the general-pair kernel executes per quadrature node (rocprofv3
+# SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 x 64 lanes, FMA twice, over the nodes of
+# the launch), next to the 292 / 376 / 498 of the algorithmic model at d = 0/1/2
+EXECUTED_FLOP_PER_NODE = {"polygon2000": 243.3, "star400": None}
 
 WORKLOADS = {
     "polygon2000": dict(panels=2000, order=11, c_i=20.0, c_o=1.0, nk=128, k_min=5.0, k_max=40.0, der=0, seed=3,
@@ -47,7 +52,8 @@ WORKLOADS = {
                          "256 k in linspace(1,10), d=2 operators, sigma_min + dsigma/dk per k, rSVD nc=2 q=2, FP64"),
 }
 NC, Q = 2, 2
-MIXED_REFINE = 3
+MIXED_REFINE = 2            # FP64 refinement steps of the last solve of the randomized SVD
+MIXED_REFINE_INNER = 1      # ... and of the solves before it
 MIXED_TOL = 1e-8            # the mixed-precision bar: sigma within 1e-8 relative of FP64
 PHASES = ("assembly_general", "assembly_adjacent", "assembly_coinciding", "lu", "form_T", "rsvd")
 
@@ -110,7 +116,8 @@ def run_workload(H, name, steps, warmup, mixed=False, bd=None, ref_sigma=None):
 
     def sweep():
         if mixed:
-            sig, used = H.sigma_sweep_mixed(bd, ks, w["c_o"], w["c_i"], R, Q, refine=MIXED_REFINE)
+            sig, used = H.sigma_sweep_mixed(bd, ks, w["c_o"], w["c_i"], R, Q, refine=MIXED_REFINE,
+                                            refine_inner=MIXED_REFINE_INNER)
             fallback[0] = int(used.sum())
             return sig
         return H.sigma_sweep(bd, ks, w["c_o"], w["c_i"], R, Q, der=der)
@@ -135,6 +142,8 @@ def run_workload(H, name, steps, warmup, mixed=False, bd=None, ref_sigma=None):
     ms_gen, n_gen, fl_gen = prof[0]
     avg_launch_s = ms_gen * 1e-3 / max(n_gen, 1)
     achieved = (fl_gen / max(n_gen, 1)) / avg_launch_s / 1e12 if n_gen else 0.0
+    ex_node = EXECUTED_FLOP_PER_NODE[name]
+    executed = achieved * ex_node / FLOP_PER_NODE[der] if ex_node else None
     ms_lu, n_lu, fl_lu = prof[3]
     lu_tflops = fl_lu / (ms_lu * 1e-3) / 1e12 if ms_lu > 0 else None
     out = {
@@ -152,10 +161,11 @@ def run_workload(H, name, steps, warmup, mixed=False, bd=None, ref_sigma=None):
                      "bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES[name],
                      "avg_launch_ms": avg_launch_s * 1e3, "launches": n_gen,
-                     "flops_per_launch": fl_gen / max(n_gen, 1)},
+                     "flops_per_launch": fl_gen / max(n_gen, 1),
+                     "executed_tflops": executed, "executed_frac": executed / PEAK_FP64_TFLOPS if executed else None},
     }
     if mixed:
-        out["refine_steps"] = MIXED_REFINE
+        out["refine_steps"] = {"last_solve": MIXED_REFINE, "other_solves": MIXED_REFINE_INNER}
         out["fp64_fallbacks"] = fallback[0]
     if ref_sigma is not None:
         out["max_rel_sigma_err_vs_fp64"] = float(np.max(np.abs(sig - ref_sigma) / ref_sigma))

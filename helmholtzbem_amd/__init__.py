@@ -472,11 +472,13 @@ def sigma_sweep_der2(builder_data: BuilderData, ks, c_o, c_i, R, q=2, proj=True)
     return tuple(out)
 
 
-def sigma_sweep_mixed(builder_data: BuilderData, ks, c_o, c_i, R, q=2, refine=3, proj=True):
+def sigma_sweep_mixed(builder_data: BuilderData, ks, c_o, c_i, R, q=2, refine=2, refine_inner=-1, proj=True):
     """sigma_min(k) for every k with the dense stage in mixed precision (FP32
-    LU on the matrix cores, `refine` FP64 refinement steps per solve).
-    Returns (sigma, fp64_used): fp64_used[i] is True where the refinement did
-    not converge and k was recomputed with the FP64 factorisation."""
+    LU on the matrix cores; FP64 refinement: `refine` steps for the last
+    solve of the randomized SVD, `refine_inner` for the others, -1 = as the
+    last).  Returns (sigma, fp64_used): fp64_used[i] is True where the last
+    solve's refinement could not certify k (or an earlier solve's refinement
+    did not contract) and it was recomputed with the FP64 factorisation."""
     ks = np.ascontiguousarray(ks, dtype=np.float64)
     if ks.ndim != 1:
         raise ValueError("ks must be a 1-D array of wavenumbers")
@@ -485,21 +487,25 @@ def sigma_sweep_mixed(builder_data: BuilderData, ks, c_o, c_i, R, q=2, refine=3,
     sig = np.zeros(ks.size)
     used = np.zeros(ks.size, dtype=np.int32)
     check(lib().hbem_sigma_sweep_mixed(g.h, _dptr(ks), ks.size, float(c_o), float(c_i), _vptr(R), R.shape[1], int(q),
-                                       int(refine), _dptr(sig), used.ctypes.data_as(_ip)))
+                                       int(refine), int(refine_inner), _dptr(sig), used.ctypes.data_as(_ip)))
     return sig, used.astype(bool)
 
 
-def sigma_sweep_batched(instances, q=2, proj=True, bucket=0):
+def sigma_sweep_batched(instances, q=2, proj=True, bucket=0, tune_cache=None):
     """sigma_min(k) of many scatterers as one batched computation
     (hbem_sigma_sweep_multi): instances are (builder_data, ks, c_o, c_i, R)
     tuples with k grids of one length, one quadrature order and one c_o.
     Instances whose orders 2N round up to the same multiple of `bucket` (0:
-    256) share one assembly launch per pair class and one batched LU and
-    randomized SVD (T padded with the identity).  Returns the list of sigma
-    arrays, equal bit for bit to sigma_sweep of each instance alone."""
+    256; "auto": the autotuned choice, see helmholtzbem_amd.autotune, cached
+    in `tune_cache` or the default cache file) share one assembly launch per
+    pair class and one batched LU and randomized SVD (T padded with the
+    identity).  Returns the list of sigma arrays."""
     items = list(instances)
     if not items:
         return []
+    if bucket == "auto":
+        from . import autotune
+        bucket = autotune.tune_bucket(items, q, proj, cache=tune_cache or autotune.DEFAULT_CACHE)
     nk = len(items[0][1])
     c_o = float(items[0][2])
     nc = np.asarray(items[0][4]).shape[1]

@@ -66,7 +66,7 @@ def lib():
         L.hbem_rsvd_reset_timer.argtypes = []
         L.hbem_rsvd_sv_noalloc.argtypes = [vp, i, vp, i, i, vp, vp]
         L.hbem_sigma_sweep_der2.argtypes = [vp, dp, i, d, d, vp, i, i, dp, dp, dp]
-        L.hbem_sigma_sweep_mixed.argtypes = [vp, dp, i, d, d, vp, i, i, i, dp, ip]
+        L.hbem_sigma_sweep_mixed.argtypes = [vp, dp, i, d, d, vp, i, i, i, i, dp, ip]
         L.hbem_geometry_release_work.argtypes = [vp]
         L.hbem_sigma_sweep_multi.argtypes = [ctypes.POINTER(vp), i, dp, i, d, dp, ctypes.POINTER(vp), i, i, i, dp]
         _lib = L

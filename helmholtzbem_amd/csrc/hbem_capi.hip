@@ -169,8 +169,9 @@ size_t mixed_bytes_per_k(int n, int ncols) {
 }
 
 // Refinement tolerance of the mixed-precision sweep: a k is recomputed in FP64
-// when its last correction exceeds this fraction of the solution.
-constexpr double kMixedTol = 1e-9;
+// when the estimated error left after a refined solve (the last correction
+// times the observed contraction rate) exceeds this fraction of the solution.
+constexpr double kMixedTol = 1e-10;
 
 // Wavenumbers per sweep batch: as many as fit in `avail` bytes, at most 256
 // (the dense stage then has about one matrix per CU; more only adds latency),
@@ -249,7 +250,7 @@ struct hbem_geometry {
     // work space
     DBuf w_Ao, w_U, w_rhs, w_flag;
     DBuf w_Tp, w_B, w_mv, w_bpiv, w_bperm, w_bvec, w_d2;   // bordered system of sv_der2 (sweeps)
-    DBuf w_T32, w_rb, w_rr, w_rpart, w_flag2;                // mixed-precision dense stage
+    DBuf w_T32, w_rb, w_rr, w_rpart, w_flag2, w_rprev;       // mixed-precision dense stage
     DBuf w_kapi, w_kri, w_L, w_T0, w_T1, w_T2, w_piv, w_perm, w_X, w_Y, w_work, w_R, w_u, w_v, w_a, w_b, w_sig, w_dsig, w_kap,
         w_kr, w_cv, w_qi, w_qo, w_tmp, w_out, w_part;
     // profiling (hbem_profile_enable / query)
@@ -259,7 +260,7 @@ struct hbem_geometry {
     // bytes held by the per-batch work buffers of the sweep
     size_t work_held() const {
         const DBuf* bs[] = {&w_L, &w_part, &w_T0, &w_T1, &w_T2, &w_piv, &w_perm, &w_X, &w_Y, &w_work, &w_u, &w_v, &w_a, &w_b,
-                            &w_Tp, &w_B, &w_bpiv, &w_bperm, &w_bvec, &w_T32, &w_rb, &w_rr, &w_rpart};
+                            &w_Tp, &w_B, &w_bpiv, &w_bperm, &w_bvec, &w_T32, &w_rb, &w_rr, &w_rpart, &w_rprev};
         size_t s = 0;
         for (auto* b : bs) s += b->n;
         return s;
@@ -397,7 +398,10 @@ struct DenseCtx {
     // stride ms): residual in FP64, correction from the FP32 factors
     const cflt* LU32 = nullptr;
     const cdbl* T64 = nullptr;
-    int refine = 0;
+    int refine = 0;         // steps of the final solve of the randomized SVD
+    int refine_inner = 0;   // steps of the solves before it
+    bool last_solve = false;
+    double* rprev = nullptr;   // per matrix: the previous correction's size
     double rtol = 0.0;   // a matrix whose last correction exceeds rtol of its solution is flagged
     int* flag = nullptr; // per matrix
     cdbl* rb = nullptr;  // n nc batch: the right-hand side
@@ -413,12 +417,17 @@ void t_solve(DenseCtx& c, cdbl* Z, bool adj) {
         return;
     }
     const size_t cnt = (size_t)c.n * c.nc;
-    HIPC(hipMemcpyAsync(c.rb, Z, sizeof(cdbl) * cnt * c.batch, hipMemcpyDeviceToDevice, c.st));
+    const int steps = c.last_solve ? c.refine : c.refine_inner;
+    if (steps > 0) HIPC(hipMemcpyAsync(c.rb, Z, sizeof(cdbl) * cnt * c.batch, hipMemcpyDeviceToDevice, c.st));
     hbem::lu_solve_thin_batched_f32(c.LU32, c.ms, c.perm, c.n, c.batch, Z, c.nc, adj, c.work, c.st);
-    for (int it = 0; it < c.refine; ++it) {
+    // the final solve, whose vectors give sigma, must reach the tolerance; the
+    // solves before it only move the subspace and must contract
+    const int check_last = c.last_solve ? 2 : 1;
+    for (int it = 0; it < steps; ++it) {
         hbem::residual_thin_batched(c.T64, c.ms, Z, c.rb, c.rr, c.n, c.nc, c.batch, adj, c.rpart, c.st);
         hbem::lu_solve_thin_batched_f32(c.LU32, c.ms, c.perm, c.n, c.batch, c.rr, c.nc, adj, c.work, c.st);
-        hbem::refine_update_batched(Z, c.rr, (int)cnt, c.batch, c.rtol, it == c.refine - 1, c.flag, c.st);
+        hbem::refine_update_batched(Z, c.rr, (int)cnt, c.batch, c.rtol, it == steps - 1 ? check_last : 0, it == 0,
+                                    c.rprev, c.flag, c.st);
     }
 }
 
@@ -458,7 +467,9 @@ void rsvd_core(DenseCtx& c, double* d_sigma, cdbl* d_u, cdbl* d_v) {
     }
     if (c.tim) { c.tim->end(c.st, kTimIter, e); e = c.tim->begin(c.st); }
     // Q = Y, X = T_p^{-H} Q
+    c.last_solve = true;
     tp_solve(c, c.Y, c.X, true);
+    c.last_solve = false;
     hbem::sigma_from_thin_batched(c.X, c.Y, c.n, c.batch, c.nc, d_sigma, d_u, d_v, c.st);
     if (c.tim) c.tim->end(c.st, kTimSVD, e);
 }
@@ -683,7 +694,7 @@ int hbem_geometry_release_work(hbem_geometry_t g) {
     HIPC(hipSetDevice(g->dev));
     HIPC(hipStreamSynchronize(g->st));
     DBuf* bs[] = {&g->w_Ao, &g->w_U, &g->w_rhs, &g->w_flag, &g->w_Tp, &g->w_B, &g->w_mv, &g->w_bpiv, &g->w_bperm,
-                  &g->w_bvec, &g->w_d2, &g->w_T32, &g->w_rb, &g->w_rr, &g->w_rpart, &g->w_flag2, &g->w_kapi, &g->w_kri,
+                  &g->w_bvec, &g->w_d2, &g->w_T32, &g->w_rb, &g->w_rr, &g->w_rpart, &g->w_flag2, &g->w_rprev, &g->w_kapi, &g->w_kri,
                   &g->w_L, &g->w_T0, &g->w_T1, &g->w_T2, &g->w_piv, &g->w_perm, &g->w_X, &g->w_Y, &g->w_work,
                   &g->w_R, &g->w_u, &g->w_v, &g->w_a, &g->w_b, &g->w_sig, &g->w_dsig, &g->w_kap, &g->w_kr, &g->w_cv,
                   &g->w_qi, &g->w_qo, &g->w_tmp, &g->w_out, &g->w_part};
@@ -1188,7 +1199,8 @@ void sweep_fp64_subset(hbem_geometry* g, const cdbl* T0, const std::vector<int>&
 // refinement steps per solve; fp64_used (may be null) marks the k recomputed
 // in FP64.
 void sweep_run(hbem_geometry* g, const double* ks, int nk, double c_o, double c_i, const void* R, int nc, int q, int D,
-               double* sigma, double* dsigma, double* d2sigma, int mixed = -1, int* fp64_used = nullptr) {
+               double* sigma, double* dsigma, double* d2sigma, int mixed = -1, int* fp64_used = nullptr,
+               int mixed_inner = -1) {
     const int n = 2 * g->N;
     const size_t nn = (size_t)n * n;
     const bool mx = mixed >= 0 && D == 0 && !d2sigma;
@@ -1256,10 +1268,12 @@ void sweep_run(hbem_geometry* g, const double* ks, int nk, double c_o, double c_
             c.LU32 = T32;
             c.T64 = T0;
             c.refine = mixed;
+            c.refine_inner = mixed_inner < 0 ? mixed : mixed_inner;
             c.rtol = kMixedTol;
             flag = g->w_flag2.as<int>(nb);
             hbem::fill_int(flag, nb, mixed > 0 ? 0 : 1, g->st);   // no refinement: nothing certifies the solves
             c.flag = flag;
+            c.rprev = g->w_rprev.as<double>(nb);
             c.rb = g->w_rb.as<cdbl>((size_t)n * nc * nb);
             c.rr = g->w_rr.as<cdbl>((size_t)n * nc * nb);
             c.rpart = g->w_rpart.as<cdbl>(hbem::residual_part_count(n, nc, nb));
@@ -1370,13 +1384,14 @@ int hbem_sigma_sweep_der2(hbem_geometry_t g, const double* ks, int nk, double c_
 }
 
 int hbem_sigma_sweep_mixed(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i, const void* R, int nc,
-                           int q, int refine, double* sigma, int* fp64_used) {
+                           int q, int refine, int refine_inner, double* sigma, int* fp64_used) {
     API_BEGIN
     check_sweep_args(g, ks, nk, c_o, c_i, R, nc, q, sigma);
-    if (refine < 0 || refine > 20) throw Error(HBEM_EINVAL, "number of refinement steps must be in [0, 20]");
+    if (refine < 0 || refine > 20 || refine_inner > 20)
+        throw Error(HBEM_EINVAL, "number of refinement steps must be in [0, 20]");
     std::lock_guard<std::mutex> lk(g->mu);
     HIPC(hipSetDevice(g->dev));
-    sweep_run(g, ks, nk, c_o, c_i, R, nc, q, 0, sigma, nullptr, nullptr, refine, fp64_used);
+    sweep_run(g, ks, nk, c_o, c_i, R, nc, q, 0, sigma, nullptr, nullptr, refine, fp64_used, refine_inner);
     API_END
 }
 

@@ -1193,41 +1193,55 @@ size_t residual_part_count(int n, int nc, int batch) {
     return (size_t)((n + kResidSlice - 1) / kResidSlice) * (size_t)n * nc * batch;
 }
 
-// Z += D for the cnt entries of each matrix's block; with `check`, flag[b] = 1
-// when max |D| > tol max |Z| (the refinement has not converged to tol).
+// Z += D for the cnt entries of each matrix's block.  prev[b] keeps max |D|^2
+// of the step before (first: max |Z|^2 before the update, the size of the
+// unrefined solution), so |D| / |D_prev| estimates the contraction rate and
+// rate |D| the error left after this update.  check = 1: flag[b] = 1 unless
+// the rate is at most 1/2 (the refinement contracts); check = 2: also unless
+// rate max|D| <= tol max|Z|.  NaN fails both.
 __global__ __launch_bounds__(256) void k_refine_update(cdbl* __restrict__ Z, const cdbl* __restrict__ Dv, int cnt,
-                                                       double tol2, int check, int* __restrict__ flag) {
-    __shared__ double sh[2][4];
+                                                       double tol2, int check, int first, double* __restrict__ prev,
+                                                       int* __restrict__ flag) {
+    __shared__ double sh[3][4];
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     cdbl* z = Z + (size_t)b * cnt;
     const cdbl* d = Dv + (size_t)b * cnt;
-    double md = 0.0, mz = 0.0;
+    double md = 0.0, mzo = 0.0, mz = 0.0;
     for (int e = tid; e < cnt; e += 256) {
-        const cdbl dv = d[e];
-        const cdbl zv = z[e] + dv;
+        const cdbl dv = d[e], zo = z[e];
+        const cdbl zv = zo + dv;
         z[e] = zv;
         md = fmax(md, cabs2(dv));
+        mzo = fmax(mzo, cabs2(zo));
         mz = fmax(mz, cabs2(zv));
     }
-    if (!check) return;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         md = fmax(md, __shfl_xor(md, o, 64));
+        mzo = fmax(mzo, __shfl_xor(mzo, o, 64));
         mz = fmax(mz, __shfl_xor(mz, o, 64));
     }
-    if (lane == 0) { sh[0][wv] = md; sh[1][wv] = mz; }
+    if (lane == 0) { sh[0][wv] = md; sh[1][wv] = mzo; sh[2][wv] = mz; }
     __syncthreads();
     if (tid == 0) {
-        for (int w = 1; w < 4; ++w) { md = fmax(md, sh[0][w]); mz = fmax(mz, sh[1][w]); }
-        if (!(md <= tol2 * mz)) flag[b] = 1;   // also catches NaN
+        for (int w = 1; w < 4; ++w) { md = fmax(md, sh[0][w]); mzo = fmax(mzo, sh[1][w]); mz = fmax(mz, sh[2][w]); }
+        const double mp = first ? mzo : prev[b];
+        if (check) {
+            const double rate2 = mp > 0.0 ? md / mp : (md > 0.0 ? 1.0 : 0.0);
+            const bool ok = rate2 <= 0.25 && (check == 1 || rate2 * md <= tol2 * mz);   // NaN fails too
+            if (!ok) flag[b] = 1;
+        }
+        prev[b] = md;
     }
 }
 
-void refine_update_batched(cdbl* Z, const cdbl* D, int cnt, int batch, double tol, bool check, int* flag,
-                           hipStream_t st) {
+void refine_update_batched(cdbl* Z, const cdbl* D, int cnt, int batch, double tol, int check, bool first, double* prev,
+                           int* flag, hipStream_t st) {
     if (batch <= 0) return;
-    hipLaunchKernelGGL(k_refine_update, dim3(batch), dim3(256), 0, st, Z, D, cnt, tol * tol, check ? 1 : 0, flag);
+    hipLaunchKernelGGL(k_refine_update, dim3(batch), dim3(256), 0, st, Z, D, cnt, tol * tol, check,
+                       first ? 1 : 0, prev, flag);
 }
+
 
 // dst[i] = src[i] for nbytes bytes (a kernel, not a runtime copy, so that it
 // can stand in a captured graph like the other steps)

@@ -89,9 +89,12 @@ void to_f32(const cdbl* src, long long count, cflt* dst, hipStream_t st);
 void residual_thin_batched(const cdbl* T, long long ts, const cdbl* Z, const cdbl* Bv, cdbl* R, int n, int nc,
                            int batch, bool adj, cdbl* part, hipStream_t st);
 size_t residual_part_count(int n, int nc, int batch);
-// Z += D (cnt entries per matrix); check: flag[b] = 1 if max|D| > tol max|Z|
-void refine_update_batched(cdbl* Z, const cdbl* D, int cnt, int batch, double tol, bool check, int* flag,
-                           hipStream_t st);
+// Z += D (cnt entries per matrix); prev[b]: max|D|^2 of the previous step
+// (first: taken from Z before the update); check 1: flag[b] = 1 unless the
+// estimated contraction is <= 1/2; check 2: also unless the estimated
+// remaining error <= tol max|Z|; check 0: no flag
+void refine_update_batched(cdbl* Z, const cdbl* D, int cnt, int batch, double tol, int check, bool first, double* prev,
+                           int* flag, hipStream_t st);
 void fill_int(int* p, int count, int v, hipStream_t st);
 void add_offset_ll(long long* p, long long count, long long off, hipStream_t st);
 void copy_bytes(void* dst, const void* src, size_t nbytes, hipStream_t st);

@@ -221,15 +221,21 @@ int hbem_sigma_sweep_der2(hbem_geometry_t g, const double* ks, int nk, double c_
  * hbem_sigma_sweep (sigma only) with the dense stage of randomized_svd::sv
  * (randsvd.cpp:99-146) in mixed precision: T is formed in FP64 and kept, its
  * FP32 copy is LU-factorised on the FP32 matrix cores, and every solve of the
- * randomized iteration is refined `refine` times (0..20) against T in FP64 --
- * residual in FP64, correction from the FP32 factors in FP64 arithmetic.  A k
- * whose last correction exceeds 1e-9 of its solution (or with refine = 0) is
- * recomputed with the FP64 factorisation, and fp64_used[i] (may be NULL) is
- * set to 1 for it.  The assembly stays FP64.  No counterpart in the
+ * randomized iteration is refined against T in FP64 -- residual in FP64,
+ * correction from the FP32 factors in FP64 arithmetic: the last solve
+ * `refine` times (0..20), the ones before it `refine_inner` times (-1: as the
+ * last).  The refinement of the last solve certifies each k: when the error it
+ * leaves (its last correction times the contraction rate observed between its
+ * corrections) exceeds 1e-10 of the solution, the rate exceeds 1/2, or refine
+ * = 0, the k is recomputed with the FP64 factorisation and fp64_used[i] (may
+ * be NULL) is set to 1; so is it when the refinement of an earlier solve does
+ * not contract (its last correction above half the one before it, or above
+ * half the unrefined solution after one step).  The assembly stays FP64.  No
+ * counterpart in the
  * reference, which factorises in FP64 only.
  */
 int hbem_sigma_sweep_mixed(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i, const void* R, int nc,
-                           int q, int refine, double* sigma, int* fp64_used);
+                           int q, int refine, int refine_inner, double* sigma, int* fp64_used);
 
 /*
  * sigma_min(k) for many scatterers at once -- the reference drivers' loops

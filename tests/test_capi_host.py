@@ -205,3 +205,27 @@ def test_rsvd_timers_and_vector_gaussian_host(H):
     v = rs.randGaussian(9)
     assert v.shape == (9,)
     assert np.array_equal(v, rs.randGaussian(9, 1)[:, 0] * np.sqrt(2.0))
+
+
+def test_autotune_signature_and_cache(tmp_path, H):
+    """The tuning cache key depends on the sizes that decide the work, not on
+    the order of the instances; a cached choice is returned without timing."""
+    from helmholtzbem_amd import autotune
+
+    class B:
+        def __init__(self, n):
+            self.n = n
+
+        def getNumPanels(self):
+            return self.n
+
+        def getGaussQROrder(self):
+            return 12
+    R2 = np.zeros((10, 2))
+    a = [(B(5), np.zeros(32), 1.0, 4.0, R2), (B(7), np.zeros(32), 1.0, 5.0, R2)]
+    b = list(reversed(a))
+    assert autotune.signature(a, 2, True) == autotune.signature(b, 2, True)
+    assert autotune.signature(a, 2, True) != autotune.signature(a, 3, True)
+    cache = str(tmp_path / "tune.json")
+    autotune._store(cache, {autotune.signature(a, 2, True): {"bucket": 128}})
+    assert autotune.tune_bucket(a, 2, True, cache=cache) == 128

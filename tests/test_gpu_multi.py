@@ -72,3 +72,17 @@ def test_graph_replays_equal_eager(workload, H, monkeypatch):
         bd.release_work()
         for x, y in zip(got, eager):
             assert np.array_equal(x, y), rep
+
+
+def test_autotuned_bucket(workload, H, tmp_path):
+    """The autotuner times the candidate granularities on the instances and
+    caches its choice; any choice gives the same bits."""
+    from helmholtzbem_amd import autotune
+    sub = workload[:8]
+    cache = str(tmp_path / "tune.json")
+    g = autotune.tune_bucket(sub, Q, cache=cache, repeats=1)
+    assert g in autotune.CANDIDATES
+    a = H.sigma_sweep_batched(sub, Q, bucket="auto", tune_cache=cache)
+    b = H.sigma_sweep_batched(sub, Q, bucket=64)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
