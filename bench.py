@@ -172,12 +172,106 @@ def run_workload(H, name, steps, warmup, mixed=False, bd=None, ref_sigma=None):
     return out, mesh, R
 
 
+def run_many_scatterers(H, steps, warmup):
+    """The many-scatterer workload: 64 random star-shaped polygons (seed 2, 6-16
+    vertices, N in [100, 1000], c_i in [4, 25]), 32 k each in [2, 20], order
+    12, c_o = 1, sigma only (q = 2).  One step = the whole set in one batched
+    computation (hbem_sigma_sweep_multi, default bucket granularity, hipGraph
+    replay after the first call); then one pass of per-instance sweeps (one
+    hbem_sigma_sweep per instance) for comparison, with a bitwise check."""
+    sp = H.ContinuousSpace1()
+    items = []
+    for i, (mesh, c_i, ks) in enumerate(H.random_polygon_instances(2)):
+        bd = H.BuilderData(mesh, sp, sp, 12)
+        items.append((bd, ks, 1.0, c_i, H.gaussian_test_matrix(2 * mesh.getNumPanels(), NC, 100 + i)))
+    evals = sum(len(it[1]) for it in items)
+    for _ in range(warmup):
+        H.sigma_sweep_batched(items, Q)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batched = H.sigma_sweep_batched(items, Q)
+    elapsed = time.perf_counter() - t0
+    print("many scatterers: batched %.3f s per step" % (elapsed / steps), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    same = True
+    for (bd, ks, c_o, c_i, R), s in zip(items, batched):
+        alone = H.sigma_sweep(bd, ks, c_o, c_i, R, Q)
+        bd.release_work()
+        same = same and bool(np.array_equal(alone, s))
+    t_serial = time.perf_counter() - t0
+    panels = [it[0].getNumPanels() for it in items]
+    return {"metric": "sigma_min(k) evals/s", "value": evals * steps / elapsed, "unit": "evals/s", "n_gpus": 1,
+            "steps": steps, "warmup": warmup, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "64 random polygons (seed 2), N in [%d, %d], order 12, 32 k each in [2, 20], "
+                                   "c_i in [4, 25], c_o = 1, rSVD nc=2 q=2, one batched computation"
+                                   % (min(panels), max(panels)), "instances": len(items), "global_batch": evals,
+                       "name": "many_scatterers"},
+            "per_instance_sweeps": {"value": evals / t_serial, "seconds": t_serial},
+            "speedup_vs_per_instance": (evals * steps / elapsed) / (evals / t_serial),
+            "bitwise_equal_to_per_instance": same}
+
+
+def run_resonance(H, steps, warmup):
+    """The batched resonance search: the unit square in 1000 line panels and a
+    random 10-vertex polygon (seed 4) in 1000 panels, order 11, c_i = 20, c_o =
+    1.  Per instance: a 4096-point grid on [5, 50] swept in mixed precision and
+    in FP64 (sigma within 1e-8, same brackets), then Newton on every bracket
+    with the batched sigma'' (FP64, q_refine = 20).  One step = both instances."""
+    from helmholtzbem_amd import resonances as RZ
+    sp = H.ContinuousSpace1()
+    meshes = [("square1000", H.square_mesh(1.0, 250, center=(0.5, 0.5))), ("polygon1000", H.random_polygon_mesh(4, 10, 1000))]
+    ks = np.linspace(5.0, 50.0, 4096)
+    res = {}
+    t_grid = t_newton = 0.0
+    for name, mesh in meshes:
+        bd = H.BuilderData(mesh, sp, sp, 11)
+        R = H.gaussian_test_matrix(2000, NC, 4)
+        for _ in range(warmup):
+            H.sigma_sweep_mixed(bd, ks[:256], 1.0, 20.0, R, Q, refine=MIXED_REFINE, refine_inner=MIXED_REFINE_INNER)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sig_m, used = H.sigma_sweep_mixed(bd, ks, 1.0, 20.0, R, Q, refine=MIXED_REFINE,
+                                              refine_inner=MIXED_REFINE_INNER)
+        tg = (time.perf_counter() - t0) / steps
+        t0 = time.perf_counter()
+        sig64 = H.sigma_sweep(bd, ks, 1.0, 20.0, R, Q)
+        t64 = time.perf_counter() - t0
+        br_m, br_64 = RZ.bracket_minima(ks, sig_m), RZ.bracket_minima(ks, sig64)
+        print("resonance %s: grid mixed %.2f s, FP64 %.2f s, %d brackets" % (name, tg, t64, len(br_m)),
+              file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        roots = RZ.find_minima(bd, 5.0, 50.0, 4096, 1.0, 20.0, R=R, q=Q, q_refine=20, tol=1e-11, method="newton",
+                               grid_sigma=sig_m)
+        tn = time.perf_counter() - t0
+        print("resonance %s: Newton %.2f s, %d roots" % (name, tn, len(roots)), file=sys.stderr, flush=True)
+        t_grid += tg
+        t_newton += tn
+        res[name] = {"grid_seconds_mixed": tg, "grid_seconds_fp64": t64, "brackets": len(br_m),
+                     "brackets_equal_fp64": br_m == br_64,
+                     "max_rel_sigma_err_vs_fp64": float(np.max(np.abs(sig_m - sig64) / sig64)),
+                     "fp64_fallbacks": int(used.sum()), "newton_seconds": tn, "roots": len(roots),
+                     "newton_iterations_max": max((it for _, _, it in roots), default=0)}
+        bd.release_work()
+    evals = 2 * ks.size
+    return {"metric": "sigma_min(k) evals/s", "value": evals / t_grid, "unit": "evals/s", "n_gpus": 1,
+            "steps": steps, "warmup": warmup, "ms_per_step": (t_grid + t_newton) * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32 LU + f64 refinement (grid), f64 (Newton)",
+            "data": "synthetic",
+            "config": {"workload": "unit square and random 10-vertex polygon (seed 4), 1000 line panels each, order "
+                                   "11, c_i=20, c_o=1, 4096 k in linspace(5,50) per instance, then Newton with "
+                                   "batched d2sigma/dk2 on every bracket", "global_batch": evals,
+                       "name": "resonance1000"},
+            "newton_seconds": t_newton, "instances": res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="polygon2000")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["many_scatterers", "resonance1000"],
+                    default="polygon2000")
     ap.add_argument("--no-second", action="store_true", help="skip the star400 sweep")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision sweep")
@@ -203,6 +297,12 @@ def main():
 
     import helmholtzbem_amd as H
 
+    if args.workload == "many_scatterers":
+        print(json.dumps(run_many_scatterers(H, args.steps, args.warmup)), flush=True)
+        return
+    if args.workload == "resonance1000":
+        print(json.dumps(run_resonance(H, args.steps, args.warmup)), flush=True)
+        return
     fp64, mesh, R = run_workload(H, args.workload, args.steps, args.warmup)
     head = fp64
     mixed = None

@@ -527,9 +527,10 @@ def sigma_sweep_batched(instances, q=2, proj=True, bucket=0, tune_cache=None):
 def sigma_sweep_multi(instances, q=2, der=0, proj=True, max_workers=None, bucket=0):
     """sigma_sweep for many independent scatterers at once -- the drivers'
     loop over instances of uneven size.  instances: (builder_data, ks, c_o,
-    c_i, R) tuples.  Sigma-only sweeps with k grids of one length, one
-    quadrature order and one c_o run as one batched computation
-    (sigma_sweep_batched).  Otherwise every geometry context's sweep is
+    c_i, R) tuples.  Sigma-only sweeps with k grids of one length (>= 32,
+    real positive), one quadrature order and one c_o run as one batched
+    computation (sigma_sweep_batched), whose results are then the same bits
+    as sigma_sweep's.  Otherwise every geometry context's sweep is
     issued from its own host thread (ctypes releases the GIL) onto its own
     stream, and the GPU overlaps them.  Returns the list of sigma_sweep
     results in the order of the instances."""
@@ -540,7 +541,8 @@ def sigma_sweep_multi(instances, q=2, der=0, proj=True, max_workers=None, bucket
     if der == 0:
         orders = {bd.getGaussQROrder() for bd, *_ in items}
         if len(orders) == 1 and len({len(it[1]) for it in items}) == 1 and len({float(it[2]) for it in items}) == 1 \
-                and len({np.asarray(it[4]).shape[1] for it in items}) == 1:
+                and len({np.asarray(it[4]).shape[1] for it in items}) == 1 and len(items[0][1]) >= 32 \
+                and all(np.all(np.asarray(it[1]) > 0) for it in items):
             return sigma_sweep_batched(items, q, proj, bucket)
     for bd, *_ in items:
         bd._ctx(proj)                          # contexts are created on this thread

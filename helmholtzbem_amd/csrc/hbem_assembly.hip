@@ -935,8 +935,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             }
         }
     }
-    if (active) {
-        const size_t rec = P.nchunk == 1 ? (size_t)P.pidx[e] : (size_t)e * P.nchunk + chunk;
+    const size_t rec = P.nchunk == 1 ? (size_t)P.pidx[e] : (size_t)e * P.nchunk + chunk;
+    if (P.tdiff) {
+        // Waves 2w (inner slot block) and 2w + 1 (outer) of the workgroup hold
+        // the same 32 wavenumbers.  They form the differences T needs (K: inner
+        // - outer, W and V: outer - inner): the inner wave entry m = 0 of its
+        // lane half, the outer wave m = 1, each written once per wavenumber.
+        // Both waves put their entries in the staging LDS (two layers per
+        // round) and read their own and the partner's entry m back, so no
+        // register array is indexed by the wave's parity (that would place
+        // it in scratch memory).  Every thread reaches the barriers.
+        static_assert((STG + 1) * REC >= 256 * 4 * 2, "exchange needs 16 KB of staging LDS");
+        const int mine = wv & 1;
+        const int kt = ((qb >> 1) << 5) + j;
+        const bool kact = qb < nqb && kt < P.nkt;
+        const size_t nkt = (size_t)P.nkt;
+        cdbl* out = P.out + rec * (16 * (D + 1)) * nkt + kt;
+        cdbl* xch = reinterpret_cast<cdbl*>(lds);
+        const int qq = mine + 2 * h;
+#pragma unroll
+        for (int t = 0; t <= D; ++t) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {   // 0: K direct, K transposed; 1: W, V
+                __syncthreads();   // the staging records (or the previous round) are no longer read
+                cdbl* x = xch + threadIdx.x * 4;
+                x[0] = half ? Wo[t][0] : Ko[t][0];
+                x[1] = half ? Wo[t][1] : Ko[t][1];
+                x[2] = half ? Vo[t][0] : Kto[t][0];
+                x[3] = half ? Vo[t][1] : Kto[t][1];
+                __syncthreads();
+                const cdbl* own = xch + threadIdx.x * 4 + mine;
+                const cdbl* par = xch + (threadIdx.x ^ 64) * 4 + mine;   // partner wave, same lane
+                const cdbl in0 = mine ? par[0] : own[0], ou0 = mine ? own[0] : par[0];
+                const cdbl in1 = mine ? par[2] : own[2], ou1 = mine ? own[2] : par[2];
+                if (kact) {
+                    if (half == 0) {
+                        out[(16 * t + qq) * nkt] = in0 - ou0;
+                        out[(16 * t + 4 + qq) * nkt] = CLS == 0 ? mk(0, 0) : in1 - ou1;
+                    } else {
+                        out[(16 * t + 8 + qq) * nkt] = ou0 - in0;
+                        out[(16 * t + 12 + qq) * nkt] = ou1 - in1;
+                    }
+                }
+            }
+        }
+    } else if (active) {
         const size_t nk = P.nk;
         cdbl* out = P.out + rec * (16 * (D + 1)) * nk + q;
 #pragma unroll
@@ -1019,8 +1062,10 @@ __global__ void k_gather_layers(const cdbl* __restrict__ L, int nk, int nslot, i
 
 // Transmission matrix (assemble_operator, gen_sol_op.cpp:59-74, unprojected):
 // T = [[K_i - K_o + M, V_o - V_i], [W_o - W_i, K_o - K_i + M^T]], M only for t = 0.
-// Output b uses wavenumber slots qi[b] (inner c_i), qo[b] (outer c_o); T_t of
-// output b is written at Tt + b (2N)^2.
+// Output b uses wavenumber slots qi[b] (inner c_i), qo[b] (outer c_o); with qo
+// null, L holds the differences per wavenumber (K_i - K_o, W_o - W_i, V_o - V_i
+// at slot qi[b]; the paired-lane kernel's tdiff layout).  T_t of output b is
+// written at Tt + b ld^2.
 // Workgroup = one pair of 8 x 8 tiles (I, J), I >= J, of the N x N blocks for
 // 32 outputs b; thread (i, jb) = (threadIdx & 7, threadIdx >> 3) forms the
 // four block entries (m, n), (m + N, n + N), (m, n + N), (m + N, n) of row
@@ -1046,7 +1091,8 @@ __global__ __launch_bounds__(256) void k_form_T(const cdbl* __restrict__ L, int 
     const bool act = b < nb;                 // idle threads still reach every barrier
     const int n2 = ld;                       // leading dimension (2N, or more for a zero-padded T)
     const size_t NN = (size_t)n2 * n2;
-    const int a = act ? qi[b] : 0, o = act ? qo[b] : 0;
+    const bool diff = qo == nullptr;
+    const int a = act ? qi[b] : 0, o = act && !diff ? qo[b] : 0;
     const bool mirror = I != J;
     for (int t = 0; t <= D; ++t) {
         cdbl* Tt = (t == 0 ? T0 : (t == 1 ? T1 : T2)) + (size_t)(act ? b : 0) * NN;
@@ -1054,12 +1100,20 @@ __global__ __launch_bounds__(256) void k_form_T(const cdbl* __restrict__ L, int 
         for (int c = 0; c < 8; ++c) {
             const int n = 8 * J + c;
             if (!act || m >= N || n >= N) continue;
-            const cdbl Ki = op_entry(L, nk, nslot, a, 0, t, m, n, N), Ko = op_entry(L, nk, nslot, o, 0, t, m, n, N);
-            const cdbl Wi = op_entry(L, nk, nslot, a, 1, t, m, n, N), Wo = op_entry(L, nk, nslot, o, 1, t, m, n, N);
-            const cdbl Vi = op_entry(L, nk, nslot, a, 2, t, m, n, N), Vo = op_entry(L, nk, nslot, o, 2, t, m, n, N);
-            cdbl k00 = Ki - Ko, k11 = Ko - Ki;
+            cdbl k00, k11, w, v;
+            if (diff) {
+                k00 = op_entry(L, nk, nslot, a, 0, t, m, n, N);
+                k11 = mk(-k00.x, -k00.y);
+                w = op_entry(L, nk, nslot, a, 1, t, m, n, N);
+                v = op_entry(L, nk, nslot, a, 2, t, m, n, N);
+            } else {
+                const cdbl Ki = op_entry(L, nk, nslot, a, 0, t, m, n, N), Ko = op_entry(L, nk, nslot, o, 0, t, m, n, N);
+                const cdbl Wi = op_entry(L, nk, nslot, a, 1, t, m, n, N), Wo = op_entry(L, nk, nslot, o, 1, t, m, n, N);
+                const cdbl Vi = op_entry(L, nk, nslot, a, 2, t, m, n, N), Vo = op_entry(L, nk, nslot, o, 2, t, m, n, N);
+                k00 = Ki - Ko; k11 = Ko - Ki;
+                w = Wo - Wi; v = Vo - Vi;
+            }
             if (t == 0) { k00.x += M[m + (size_t)n * N]; k11.x += M[n + (size_t)m * N]; }
-            const cdbl w = Wo - Wi, v = Vo - Vi;
             Tt[m + (size_t)n * n2] = k00;
             Tt[(m + N) + (size_t)(n + N) * n2] = k11;
             Tt[m + (size_t)(n + N) * n2] = v;
@@ -1072,8 +1126,14 @@ __global__ __launch_bounds__(256) void k_form_T(const cdbl* __restrict__ L, int 
         for (int c = 0; c < 8; ++c) {
             const int col = 8 * I + c;
             if (!act || r >= N || col >= N) continue;
-            const cdbl Ki = op_entry(L, nk, nslot, a, 0, t, r, col, N), Ko = op_entry(L, nk, nslot, o, 0, t, r, col, N);
-            cdbl k00 = Ki - Ko, k11 = Ko - Ki;
+            cdbl k00, k11;
+            if (diff) {
+                k00 = op_entry(L, nk, nslot, a, 0, t, r, col, N);
+                k11 = mk(-k00.x, -k00.y);
+            } else {
+                const cdbl Ki = op_entry(L, nk, nslot, a, 0, t, r, col, N), Ko = op_entry(L, nk, nslot, o, 0, t, r, col, N);
+                k00 = Ki - Ko; k11 = Ko - Ki;
+            }
             if (t == 0) { k00.x += M[r + (size_t)col * N]; k11.x += M[col + (size_t)r * N]; }
             Tt[r + (size_t)col * n2] = k00;
             Tt[(r + N) + (size_t)(col + N) * n2] = k11;

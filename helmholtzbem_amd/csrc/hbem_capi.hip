@@ -295,16 +295,42 @@ void check_k(double k_re, double k_im, double c) {
 // k_rp of initialize_constants: the real Hankel path
 bool real_positive(double k_re, double k_im) { return k_im == 0.0 && k_re > 0.0; }
 
+// Slot tables of the tdiff layout (AsmParams::tdiff) for nb wavenumbers ks
+// with refraction indices c_i (inner) and c_o: blocks of 32 slots, block 2w
+// inner and block 2w + 1 outer for wavenumbers 32w .. 32w + 31; slots past
+// the last wavenumber repeat it (computed, never written).
+void tdiff_slots(const double* ks, int nb, double c_i, double c_o, std::vector<double>& kap, std::vector<double>& kr,
+                 std::vector<double>& cv) {
+    const int nblk = (nb + 31) / 32;
+    kap.resize(64 * (size_t)nblk);
+    kr.resize(kap.size());
+    cv.resize(kap.size());
+    for (size_t q = 0; q < kap.size(); ++q) {
+        const int blk = (int)(q >> 5), kt = std::min(nb - 1, (blk >> 1) * 32 + (int)(q & 31));
+        const double c = (blk & 1) ? c_o : c_i;
+        kap[q] = ks[kt] * std::sqrt(c);
+        kr[q] = ks[kt];
+        cv[q] = c;
+    }
+}
+
 // Assemble the local matrices of all pairs for wavenumber slots (kap, kr, cv)
 // with imaginary parts (kapi, kri) when any slot is not real positive; returns
 // the local buffer.  Wide real batches run on the paired-lane kernel, the rest
-// on node-parallel groups of lanes per item.
+// on node-parallel groups of lanes per item.  nkt > 0: the slots are in the
+// tdiff layout (tdiff_slots) for nkt wavenumbers, the paired-lane kernel runs
+// whatever the width, and the buffer holds the differences T is formed from,
+// nkt columns wide.
 cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const std::vector<double>& kr,
                       const std::vector<double>& cv, int D, const std::vector<double>& kapi = {},
-                      const std::vector<double>& kri = {}) {
+                      const std::vector<double>& kri = {}, int nkt = 0) {
     const int nk = (int)kap.size();
     const int nslot = 16 * (D + 1);
-    cdbl* L = g->w_L.as<cdbl>((size_t)nk * g->npairs * nslot);
+    const bool tdiff = nkt > 0;
+    if (tdiff && (!kapi.empty() || nk % 64 != 0 || 2 * nkt > nk || 2 * nkt <= nk - 64))
+        throw Error(HBEM_EINVAL, "tdiff layout needs real slots in blocks of 64 for its wavenumbers");
+    const int nko = tdiff ? nkt : nk;   // columns of the output layouts
+    cdbl* L = g->w_L.as<cdbl>((size_t)nko * g->npairs * nslot);
     hbem::AsmParams P;
     std::memset(&P, 0, sizeof(P));
     P.xg = (const double*)g->d_xg.p;
@@ -324,10 +350,12 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
     P.ns = g->ns;
     P.nc = g->nc;
     P.qflags = g->flags;
+    P.tdiff = tdiff ? 1 : 0;
+    P.nkt = nkt;
     // lanes per item: G = 0 selects the paired-lane kernel for wide batches
     // (32 wavenumbers x 2 nodes per wavefront); node groups of G lanes otherwise
-    const int Gg = cplx ? 16 : (nk >= 64 ? 0 : (nk >= 16 ? 4 : 16));
-    const int Gs = (nk >= 64 && !cplx) ? 0 : 64;
+    const int Gg = cplx ? 16 : (nk >= 64 || tdiff ? 0 : (nk >= 16 ? 4 : 16));
+    const int Gs = ((nk >= 64 || tdiff) && !cplx) ? 0 : 64;
     struct Launch {
         int cls, which, G, nent, nodes;
         const int* ent;
@@ -358,7 +386,7 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
         P.nchunk = (l.nodes + chunk_nodes - 1) / chunk_nodes;
         if (l.cls == 1 && l.G == 0) P.nchunk = 2 * ((l.nodes / 2 + chunk_nodes - 1) / chunk_nodes);
         cdbl* part = nullptr;
-        if (P.nchunk > 1) part = g->w_part.as<cdbl>((size_t)nk * l.nent * P.nchunk * nslot);
+        if (P.nchunk > 1) part = g->w_part.as<cdbl>((size_t)nko * l.nent * P.nchunk * nslot);
         P.out = P.nchunk > 1 ? part : L;
         // the class's kernel alone is timed: the chunk sums are a separate launch
         hipEvent_t e0 = g->prof ? g->plog.begin(g->st) : nullptr;
@@ -366,7 +394,7 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
         HIPC(hipGetLastError());
         if (g->prof) g->plog.end(g->st, l.which, e0, (double)l.nent * nk * l.nodes * flops_per_node(D));
         if (P.nchunk > 1) {
-            hbem::launch_chunk_sum(part, l.nent, P.nchunk, nk, nslot, l.pidx, g->npairs, L, g->st);
+            hbem::launch_chunk_sum(part, l.nent, P.nchunk, nko, nslot, l.pidx, g->npairs, L, g->st);
             HIPC(hipGetLastError());
         }
     }
@@ -1204,10 +1232,10 @@ void sweep_run(hbem_geometry* g, const double* ks, int nk, double c_o, double c_
     const int n = 2 * g->N;
     const size_t nn = (size_t)n * n;
     const bool mx = mixed >= 0 && D == 0 && !d2sigma;
-    // Batch of B wavenumbers -> 2B slots ordered [c_i for every k | c_o for every k]:
-    // each wavefront of the assembly sees a single refraction index and 64
-    // consecutive k, so the arguments kappa r of its lanes stay close (fewer
-    // divergent Bessel branches).  B <= 256 gives the dense stage (one
+    // Batch of B wavenumbers -> slots in blocks of 32 consecutive k of one
+    // refraction index: each wavefront of the assembly sees a single index
+    // and neighbouring k, so the arguments kappa r of its lanes stay close
+    // (fewer divergent Bessel branches).  B <= 256 gives the dense stage (one
     // workgroup per matrix in the panel factorisation and the thin solves)
     // up to one matrix per CU.  B is bounded by the free device memory: local
     // matrices, chunk partials and T, T', T'' per wavenumber (and the
@@ -1217,28 +1245,42 @@ void sweep_run(hbem_geometry* g, const double* ks, int nk, double c_o, double c_
     const int B = sweep_batch(per, usable_device_bytes(g->work_held()), nk);
     cdbl* dR = g->w_R.as<cdbl>((size_t)n * nc);
     HIPC(hipMemcpyAsync(dR, R, (size_t)n * nc * sizeof(cdbl), hipMemcpyHostToDevice, g->st));
+    // Sweeps of >= 32 real positive wavenumbers run every batch on the
+    // paired-lane kernel in the tdiff layout (half the local-matrix bytes),
+    // so a k's bits do not depend on the batch size the free memory allows.
+    bool allpos_all = true;
+    for (int j = 0; j < nk; ++j) allpos_all = allpos_all && ks[j] > 0.0;
+    const bool tdiff = allpos_all && nk >= 32;
     for (int k0 = 0; k0 < nk; k0 += B) {
         const int nb = std::min(B, nk - k0);
-        std::vector<double> kap(2 * nb), kr(2 * nb), cv(2 * nb);
-        std::vector<int> qi(nb), qo(nb);
-        for (int b = 0; b < nb; ++b) {
-            const double k = ks[k0 + b];
-            kap[b] = k * std::sqrt(c_i); kr[b] = k; cv[b] = c_i;
-            kap[nb + b] = k * std::sqrt(c_o); kr[nb + b] = k; cv[nb + b] = c_o;
-            qi[b] = b;
-            qo[b] = nb + b;
-        }
         bool allpos = true;
         for (int b = 0; b < nb; ++b) allpos = allpos && ks[k0 + b] > 0.0;
-        const std::vector<double> zero(allpos ? 0 : 2 * nb, 0.0);
-        cdbl* L = assemble_locals(g, kap, kr, cv, D, zero, zero);
+        std::vector<double> kap, kr, cv;
+        std::vector<int> qi(nb), qo(nb);
+        if (tdiff) {
+            tdiff_slots(ks + k0, nb, c_i, c_o, kap, kr, cv);
+            for (int b = 0; b < nb; ++b) qi[b] = b;
+        } else {
+            // inner slots 0..nb-1, outer nb..2nb-1
+            kap.resize(2 * nb); kr.resize(2 * nb); cv.resize(2 * nb);
+            for (int b = 0; b < nb; ++b) {
+                const double k = ks[k0 + b];
+                kap[b] = k * std::sqrt(c_i); kr[b] = k; cv[b] = c_i;
+                kap[nb + b] = k * std::sqrt(c_o); kr[nb + b] = k; cv[nb + b] = c_o;
+                qi[b] = b;
+                qo[b] = nb + b;
+            }
+        }
+        const std::vector<double> zero(allpos ? 0 : kap.size(), 0.0);
+        cdbl* L = assemble_locals(g, kap, kr, cv, D, zero, zero, tdiff ? nb : 0);
+        const int lcols = tdiff ? nb : 2 * nb;   // columns of the local buffer
         cdbl* T0 = g->w_T0.as<cdbl>(nn * nb);
         cdbl* T1 = D >= 1 ? g->w_T1.as<cdbl>(nn * nb) : nullptr;
         cdbl* T2 = D >= 2 ? g->w_T2.as<cdbl>(nn * nb) : nullptr;
         const int* dqi = upload(g->w_qi, qi, g->st);
-        const int* dqo = upload(g->w_qo, qo, g->st);
+        const int* dqo = tdiff ? nullptr : upload(g->w_qo, qo, g->st);
         hipEvent_t e0 = g->prof ? g->plog.begin(g->st) : nullptr;
-        hbem::launch_form_T(L, 2 * nb, 16 * (D + 1), dqi, dqo, nb, g->N, D, (const double*)g->d_M.p, T0, T1, T2, g->st);
+        hbem::launch_form_T(L, lcols, 16 * (D + 1), dqi, dqo, nb, g->N, D, (const double*)g->d_M.p, T0, T1, T2, g->st);
         if (g->prof) g->plog.end(g->st, kProfFormT, e0);
         int* piv = g->w_piv.as<int>((size_t)n * nb);
         int* perm = g->w_perm.as<int>((size_t)n * nb);
@@ -1319,7 +1361,7 @@ void sweep_run(hbem_geometry* g, const double* ks, int nk, double c_o, double c_
             const int m2 = 2 * n;
             cdbl* Tp = g->w_Tp.as<cdbl>(nn * nb);
             cdbl* Bw = g->w_B.as<cdbl>(4 * nn * nb);
-            hbem::launch_form_T(L, 2 * nb, 16 * (D + 1), dqi, dqo, nb, g->N, 0, (const double*)g->d_M.p, Tp, nullptr,
+            hbem::launch_form_T(L, lcols, 16 * (D + 1), dqi, dqo, nb, g->N, 0, (const double*)g->d_M.p, Tp, nullptr,
                                 nullptr, g->st);
             if (g->proj) project_inplace(g, Tp, nb, Bw);
             int* mv = g->w_mv.as<int>(nb);
@@ -1412,7 +1454,7 @@ struct GraphEntry {
 struct MultiWork {
     std::mutex mu;
     DBuf ent[3], flags[3], pidx[3], einst[3], igeo[3], ibase[3], kap, kr, cv, L, part[3], T, piv, perm, X, Y, work, sig,
-        qi, qo, desc, descT, roff, nrow, Rall;
+        qi, desc, descT, roff, nrow, Rall;
     std::vector<GraphEntry> graphs;
     void drop_graphs() {
         for (auto& e : graphs) (void)hipGraphExecDestroy(e.exec);
@@ -1456,20 +1498,22 @@ void multi_subset(const std::vector<hbem_geometry*>& G, const std::vector<int>& 
     hbem_geometry* g0 = G[ids[0]];
     const hipStream_t st = g0->st;
     const int m = (int)ids.size();
-    const int nslot = 16, nks = 2 * nk;
+    const int nslot = 16, nks = 64 * ((nk + 31) / 32);   // tdiff slots per instance
     const size_t nn = (size_t)n_pad * n_pad;
     const int batch = m * nk;
     // ---- host side: every table the device phase reads, uploaded first ----
-    // slots of instance i: [c_i(i) for every k | c_o for every k]
-    std::vector<double> kap((size_t)m * nks), kr((size_t)m * nks), cv((size_t)m * nks);
+    // slots of instance i: its nk wavenumbers in the tdiff layout (blocks of
+    // 32 with c_i(i), then c_o); the local buffer holds the T differences, nk
+    // columns
+    std::vector<double> kap, kr, cv;
     std::vector<long long> pair_off(m + 1, 0);
     for (int a = 0; a < m; ++a) {
         const int i = ids[a];
-        for (int j = 0; j < nk; ++j) {
-            const double k = ks[(size_t)i * nk + j];
-            kap[(size_t)a * nks + j] = k * std::sqrt(c_i[i]); kr[(size_t)a * nks + j] = k; cv[(size_t)a * nks + j] = c_i[i];
-            kap[(size_t)a * nks + nk + j] = k * std::sqrt(c_o); kr[(size_t)a * nks + nk + j] = k; cv[(size_t)a * nks + nk + j] = c_o;
-        }
+        std::vector<double> ka, kb, kc;
+        tdiff_slots(ks + (size_t)i * nk, nk, c_i[i], c_o, ka, kb, kc);
+        kap.insert(kap.end(), ka.begin(), ka.end());
+        kr.insert(kr.end(), kb.begin(), kb.end());
+        cv.insert(cv.end(), kc.begin(), kc.end());
         pair_off[a + 1] = pair_off[a] + G[i]->npairs;
     }
     const double* dkap = upload(w.kap, kap, st);
@@ -1498,10 +1542,9 @@ void multi_subset(const std::vector<hbem_geometry*>& G, const std::vector<int>& 
         digeo[cls] = upload(w.igeo[cls], geo, st);
         dibase[cls] = upload(w.ibase[cls], c.base, st);
     }
-    std::vector<int> qi(nk), qo(nk);
-    for (int j = 0; j < nk; ++j) { qi[j] = j; qo[j] = nk + j; }
+    std::vector<int> qi(nk);
+    for (int j = 0; j < nk; ++j) qi[j] = j;
     const int* dqi = upload(w.qi, qi, st);
-    const int* dqo = upload(w.qo, qo, st);
     std::vector<hbem::CsrDesc> desc(batch), descT(batch);
     std::vector<long long> roff(batch);
     std::vector<int> nrow(batch);
@@ -1528,7 +1571,7 @@ void multi_subset(const std::vector<hbem_geometry*>& G, const std::vector<int>& 
     const long long* droff = upload(w.roff, roff, st);
     const int* dnrow = upload(w.nrow, nrow, st);
     // ---- every device buffer at its final size before the device phase ----
-    cdbl* L = w.L.as<cdbl>((size_t)pair_off[m] * nslot * nks);
+    cdbl* L = w.L.as<cdbl>((size_t)pair_off[m] * nslot * nk);
     int* ent[3];
     unsigned char* fl[3];
     long long* pidx[3];
@@ -1540,7 +1583,7 @@ void multi_subset(const std::vector<hbem_geometry*>& G, const std::vector<int>& 
         fl[cls] = w.flags[cls].as<unsigned char>(ne);
         pidx[cls] = w.pidx[cls].as<long long>(ne);
         einst[cls] = w.einst[cls].as<int>(ne);
-        part[cls] = cp[cls].nchunk > 1 ? w.part[cls].as<cdbl>((size_t)nks * ne * cp[cls].nchunk * nslot) : nullptr;
+        part[cls] = cp[cls].nchunk > 1 ? w.part[cls].as<cdbl>((size_t)nk * ne * cp[cls].nchunk * nslot) : nullptr;
     }
     cdbl* T = w.T.as<cdbl>(nn * batch);
     int* piv = w.piv.as<int>((size_t)n_pad * batch);
@@ -1566,6 +1609,8 @@ void multi_subset(const std::vector<hbem_geometry*>& G, const std::vector<int>& 
         P.ns = g0->ns;
         P.nc = g0->nc;
         P.qflags = g0->flags;
+        P.tdiff = 1;
+        P.nkt = nk;
         for (int cls = 2; cls >= 0; --cls) {
             const ClassPlan& c = cp[cls];
             if (c.ne == 0) continue;
@@ -1595,12 +1640,12 @@ void multi_subset(const std::vector<hbem_geometry*>& G, const std::vector<int>& 
             P.out = c.nchunk > 1 ? part[cls] : L;
             hbem::launch_assemble(cls, 0, P, 0, st);
             if (c.nchunk > 1)
-                hbem::launch_chunk_sum(part[cls], (int)c.ne, c.nchunk, nks, nslot, pidx[cls], pair_off[m], L, st);
+                hbem::launch_chunk_sum(part[cls], (int)c.ne, c.nchunk, nk, nslot, pidx[cls], pair_off[m], L, st);
         }
         for (int a = 0; a < m; ++a) {
             hbem_geometry* g = G[ids[a]];
             const int n = 2 * g->N;
-            hbem::launch_form_T(L + (size_t)pair_off[a] * nslot * nks, nks, nslot, dqi, dqo, nk, g->N, 0,
+            hbem::launch_form_T(L + (size_t)pair_off[a] * nslot * nk, nk, nslot, dqi, nullptr, nk, g->N, 0,
                                 (const double*)g->d_M.p, T + (size_t)a * nk * nn, nullptr, nullptr, st, n_pad);
             if (n < n_pad) hbem::launch_pad_identity(T + (size_t)a * nk * nn, n, n_pad, nk, st);
         }
@@ -1624,7 +1669,7 @@ void multi_subset(const std::vector<hbem_geometry*>& G, const std::vector<int>& 
         key.push_back((const void*)(size_t)nc);
         key.push_back((const void*)(size_t)q);
         key.push_back((const void*)(size_t)n_pad);
-        const void* bufs[] = {dkap, dkr, dcv, digeo[0], digeo[1], digeo[2], dibase[0], dibase[1], dibase[2], dqi, dqo,
+        const void* bufs[] = {dkap, dkr, dcv, digeo[0], digeo[1], digeo[2], dibase[0], dibase[1], dibase[2], dqi,
                               dF, dFT, dRall, droff, dnrow, L, T, piv, perm, X, Y, work, dsig,
                               ent[0], ent[1], ent[2], fl[0], fl[1], fl[2], pidx[0], pidx[1], pidx[2], einst[0],
                               einst[1], einst[2], part[0], part[1], part[2]};
@@ -1705,7 +1750,7 @@ int hbem_sigma_sweep_multi(const hbem_geometry_t* geoms, int ninst, const double
         while (a < key.size() && key[a].first == n_pad) {
             hbem_geometry* g = G[key[a].second];
             const size_t part = (size_t)std::max(g->nadj * adj_chunks(g->nc), g->ncoi * coi_chunks(g->nc));
-            const size_t need = (size_t)2 * nk * ((size_t)g->npairs + part) * 16 * sizeof(cdbl) +
+            const size_t need = (size_t)nk * ((size_t)g->npairs + part) * 16 * sizeof(cdbl) +
                                 (size_t)nk * ((size_t)n_pad * n_pad * sizeof(cdbl) + (size_t)n_pad * (8 + (4 * nc + 4) * sizeof(cdbl)));
             if (!ids.empty() && used + need > avail) break;
             ids.push_back(key[a].second);

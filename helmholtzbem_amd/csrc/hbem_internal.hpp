@@ -46,6 +46,13 @@ struct AsmParams {
     const long long* ibase;
     int ns, nc;                  // Gauss / composite rule sizes
     int qflags;                  // HBEM_FLAG_* quirk bits
+    // paired-lane kernel only: the slots come in blocks of 32 (nk a multiple
+    // of 64), block 2w holding wavenumbers 32w .. 32w + 31 with the inner
+    // refraction index and block 2w + 1 the same ones with the outer; out
+    // receives per wavenumber the differences T is formed from (K_i - K_o,
+    // W_o - W_i, V_o - V_i) with nkt (the wavenumbers) in place of nk in the
+    // layouts above
+    int tdiff, nkt;
 };
 
 // assembly (hbem_assembly.hip)
@@ -62,7 +69,8 @@ void launch_chunk_sum(const cdbl* part, int nent, int nchunk, int nk, int nslot,
 void launch_gather_layers(const cdbl* L, int nk, int nslot, int q, int N, int D, int layers, cdbl* K, cdbl* W,
                           cdbl* V, hipStream_t st);
 // T_t for output b is written at T[t] + b * ld^2 (t = 0..D) with leading
-// dimension ld (0: 2N)
+// dimension ld (0: 2N); qo null: L is in the tdiff layout (AsmParams) and
+// output b reads wavenumber qi[b] of it
 void launch_form_T(const cdbl* L, int nk, int nslot, const int* qi, const int* qo, int nb, int N, int D,
                    const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st, int ld = 0);
 // identity in rows and columns n..ld-1 of nb matrices of order ld

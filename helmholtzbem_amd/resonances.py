@@ -195,21 +195,28 @@ def bracket_minima(ks: Sequence[float], sig: Sequence[float]) -> List[Tuple[floa
 
 
 def find_minima(builder_data, k_min: float, k_max: float, n_points: int, c_o: float, c_i: float, R=None,
-                q: int = 2, q_refine: int = 20, tol: float = 1e-10, method: str = "brent"):
+                q: int = 2, q_refine: int = 20, tol: float = 1e-10, method: str = "brent", grid_sigma=None):
     """Local minima of sigma_min(k) on [k_min, k_max], the resonance search of
     the reference drivers: a batched sweep of n_points wavenumbers (rSVD with q
     iterations) brackets the minima, then every bracket is refined at once --
     Brent on dsigma/dk (method "brent"), safeguarded Newton with d2sigma/dk2
     (method "newton") or Brent's minimiser on sigma itself from the grid
     minimum (method "brent_gsl", examples/roots_brent_polygon_rsvd.cpp:288-311;
-    tol is then relative) -- with q_refine subspace iterations.  Returns a list
-    of (k, sigma(k), iterations) for the brackets that converged."""
+    tol is then relative) -- with q_refine subspace iterations.  grid_sigma:
+    the grid's sigma values when the caller has them (e.g. from the
+    mixed-precision sweep), instead of the FP64 sweep.  Returns a list of (k,
+    sigma(k), iterations) for the brackets that converged."""
     from . import gaussian_test_matrix, sigma_sweep, sigma_sweep_der2
     n = 2 * builder_data.getNumPanels()
     if R is None:
         R = gaussian_test_matrix(n, 2, 0)
     ks = np.linspace(k_min, k_max, n_points)
-    sig = sigma_sweep(builder_data, ks, c_o, c_i, R, q)
+    if grid_sigma is None:
+        sig = sigma_sweep(builder_data, ks, c_o, c_i, R, q)
+    else:
+        sig = np.asarray(grid_sigma, dtype=np.float64)
+        if sig.shape != ks.shape:
+            raise ValueError("grid_sigma must hold one value per grid point")
     brackets = bracket_minima(ks, sig)
     if not brackets:
         return []

@@ -6,13 +6,16 @@ two ways of splitting a polygon into line panels that the reference drivers
 use before the sigma_min(k) path.
 
 File format (``read_array``, scatterer.cpp:29-72): one row per line, numbers
-separated by commas; all whitespace is removed first, empty lines are skipped,
-a token that is not a number makes the whole read fail.  Rows shorter than the
-widest row are padded with zeros.  ``read`` requires exactly two columns
+separated by commas; all whitespace is removed first, empty lines are skipped.
+A token is read as std::stod reads it: the longest prefix strtod accepts (so
+"1.5abc" is 1.5); a token with no such prefix, or out of the double range,
+makes the whole read fail.  Rows shorter than the widest row are padded with
+zeros.  ``read`` requires exactly two columns
 (x, y) and multiplies by ``scale`` (scatterer.cpp:74-81).
 """
 from __future__ import annotations
 
+import ctypes
 import math
 from typing import List, Optional, Tuple
 
@@ -22,6 +25,24 @@ from .geometry import ParametrizedLine, ParametrizedMesh, ParametrizedPolynomial
 from .spline import ComplexSpline
 
 _MAX_ITER_ALPHA = 5000
+
+
+_libc = ctypes.CDLL(None, use_errno=True)
+_libc.strtod.restype = ctypes.c_double
+_libc.strtod.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p)]
+
+
+def _stod(tok: str) -> Optional[float]:
+    """std::stod: strtod on the token, None where stod throws (nothing
+    converted: invalid_argument; ERANGE: out_of_range)."""
+    buf = ctypes.create_string_buffer(tok.encode())
+    end = ctypes.c_char_p()
+    ctypes.set_errno(0)
+    v = _libc.strtod(buf, ctypes.byref(end))
+    used = ctypes.cast(end, ctypes.c_void_p).value - ctypes.addressof(buf)
+    if used == 0 or ctypes.get_errno() != 0:
+        return None
+    return float(v)
 
 
 def _cround(v: float) -> float:
@@ -49,10 +70,10 @@ def read_array(fname: str) -> Optional[np.ndarray]:
             toks.pop()          # std::getline yields no field after a trailing comma
         row = []
         for tok in toks:
-            try:
-                row.append(float(tok))
-            except ValueError:
+            v = _stod(tok)
+            if v is None:
                 return None
+            row.append(v)
         rows.append(row)
         width = max(width, len(row))
     if not rows:

@@ -199,7 +199,10 @@ int hbem_rsvd_sv_noalloc(const void* T_dev, int n, const void* R_dev, int nc, in
  * operators with their first k-derivatives, want_der = 2 with the first and
  * second (gen_sol_op_2nd_der's operator set); in both cases
  * dsigma/dk = Re(u^H T' v) is returned in dsigma (may be NULL when want_der = 0).
- * R: n x nc test matrix (n = 2N), uploaded once per call.
+ * R: n x nc test matrix (n = 2N), uploaded once per call.  The wavenumbers run
+ * in batches sized to the free device memory; a sweep of >= 32 real positive
+ * k assembles every batch with the same kernel, so each result is the same
+ * bit for bit whatever the batch size (and in any other such sweep).
  */
 int hbem_sigma_sweep(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i,
                      const void* R, int nc, int q, int want_der, double* sigma, double* dsigma);

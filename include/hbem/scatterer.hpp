@@ -52,14 +52,12 @@ inline bool read_array(const std::string& fname, std::vector<std::vector<double>
         std::vector<double> row;
         std::istringstream iss(s);
         while (std::getline(iss, tok, ',')) {
-            size_t used = 0;
             double v;
             try {
-                v = std::stod(tok, &used);
+                v = std::stod(tok);   // the longest numeric prefix, as the reference reads it
             } catch (const std::exception&) {
                 return false;
             }
-            if (used != tok.size()) return false;
             row.push_back(v);
         }
         ncols = std::max(ncols, row.size());

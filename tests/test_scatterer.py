@@ -43,6 +43,13 @@ def test_read_format_rules(tmp_path):
     f.write_text("1,2\n3\n")
     assert np.array_equal(S.read_array(str(f)), [[1, 2], [3, 0]])
     assert S.read_array(str(tmp_path / "missing.dat")) is None
+    # tokens are read as std::stod reads them (scatterer.cpp:45-50): the longest
+    # numeric prefix; nothing numeric, an empty field or an out-of-range value fails
+    f.write_text("1.5abc,2e1x\n-.25,0x10\n")
+    assert np.array_equal(S.read_array(str(f)), [[1.5, 20.0], [-0.25, 16.0]])
+    for bad in ("1,,2\n", "1,e5\n", "1,1e999\n"):
+        f.write_text(bad)
+        assert S.read_array(str(f)) is None, bad
 
 
 def _check_tiling(x, y, panels):
