@@ -36,7 +36,7 @@ FLOP_PER_NODE = {0: 292.0, 1: 376.0, 2: 498.0}   # the library's algorithmic mod
 # HBM bytes per launch of the general-pair assembly kernel (rocprofv3
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, KiB per dispatch, scaled by the
 # counters' reading of a plain 1 GiB copy): profiles/k_assemble_x2_general_*.txt
-TRAFFIC_BYTES = {"star400": 3.2686e10, "polygon2000": 1.4829e11}
+TRAFFIC_BYTES = {"star400": 1.7076e10, "polygon2000": 8.2862e10}
 # FP64 FLOP the general-pair kernel executes per quadrature node (rocprofv3
 # SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 x 64 lanes, FMA twice, over the nodes of
 # the launch), next to the 292 / 376 / 498 of the algorithmic model at d = 0/1/2

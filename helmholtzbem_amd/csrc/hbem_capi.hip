@@ -143,14 +143,25 @@ long long adj_chunks(int nc) { return 2 * (((long long)nc * nc + kChunkNodes - 1
 long long coi_chunks(int nc) { return ((long long)nc * nc + kChunkNodes - 1) / kChunkNodes; }
 
 // Device bytes a sigma sweep needs per wavenumber of a batch: local matrices
-// and chunk partials of two refraction indices, T (factorised in place), T',
+// and chunk partials (lcols columns per k: 1 in the T-difference layout, 2
+// for the two refraction indices otherwise), T (factorised in place), T',
 // T'', pivots and the thin blocks of the randomized SVD.
-size_t sweep_bytes_per_k(long long npairs, long long nadj, long long ncoi, int nc_rule, int n, int D, int ncols) {
+size_t sweep_bytes_per_k(long long npairs, long long nadj, long long ncoi, int nc_rule, int n, int D, int ncols,
+                         int lcols) {
     const size_t nslot = 16 * (size_t)(D + 1);
     const size_t part = (size_t)std::max(nadj * adj_chunks(nc_rule), ncoi * coi_chunks(nc_rule));
     const size_t nn = (size_t)n * n;
-    return 2 * (size_t)npairs * nslot * sizeof(cdbl) + 2 * part * nslot * sizeof(cdbl) + (size_t)(D + 1) * nn * sizeof(cdbl) +
+    return (size_t)lcols * ((size_t)npairs + part) * nslot * sizeof(cdbl) + (size_t)(D + 1) * nn * sizeof(cdbl) +
            2 * (size_t)n * sizeof(int) + (4 * (size_t)ncols + 4) * n * sizeof(cdbl) + 64;
+}
+
+// Sweeps of >= 32 real positive wavenumbers run every batch on the paired-lane
+// kernel in the T-difference layout (AsmParams::tdiff), so a k's bits do not
+// depend on the batch size the free memory allows.
+bool sweep_tdiff(const double* ks, int nk) {
+    bool allpos = nk >= 32;
+    for (int j = 0; j < nk && allpos; ++j) allpos = ks[j] > 0.0;
+    return allpos;
 }
 
 // Extra device bytes per wavenumber of the sigma'' stage: T_p, the bordered
@@ -949,7 +960,7 @@ namespace {
 struct Standalone {
     std::mutex mu;
     hipStream_t st = nullptr;
-    DBuf T, T1, T2, piv, perm, X, Y, work, u, v, sig, out, W, Wpiv, Wperm, p, p2, ud, tmp;
+    DBuf T, T1, T2, piv, perm, X, Y, work, u, v, sig, out, W, Wpiv, Wperm, p, p2, ud, tmp, Wq, cpn, cps, cpl;
     bool bench = false;
     EventLog tim;
     void init() {
@@ -960,6 +971,49 @@ struct Standalone {
     }
 };
 Standalone g_sa;
+
+// The eigenvector of the Wielandt matrix W = [[0, T], [T^H, 0]] for sigma as
+// randomized_svd::sv_der / sv_der2 take it: the last column of Q of the
+// column-pivoted QR of W - sigma I (randsvd.cpp:160-162).  That column is
+// orthogonal to every column of the Hermitian W - sigma I but the last
+// pivot p, so it is (W - sigma I)^{-1} e_p normalised (up to a phase, which
+// neither sigma' nor sigma'' sees): the pivot order comes from
+// colpiv_last_pivot_batched, the solve from the blocked LU.  T is the
+// caller's host matrix; s.T is overwritten.
+void reference_wielandt_vector(Standalone& s, const void* T, int n, double sigma, std::vector<cdbl>& x) {
+    const size_t nn = (size_t)n * n;
+    const int m2 = 2 * n;
+    const size_t mm = (size_t)m2 * m2;
+    cdbl* dT = s.T.as<cdbl>(nn);
+    HIPC(hipMemcpyAsync(dT, T, nn * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
+    cdbl* Wq = s.Wq.as<cdbl>(mm);
+    hbem::wielandt_build(dT, n, sigma, Wq, s.st);
+    int* dl = s.cpl.as<int>(1);
+    hbem::colpiv_last_pivot_batched(Wq, (long long)mm, m2, 1, s.cpn.as<double>(2 * (size_t)m2),
+                                    s.cps.as<int>(2 * (size_t)m2), dl, s.st);
+    int p = 0;
+    HIPC(hipMemcpyAsync(&p, dl, sizeof(int), hipMemcpyDeviceToHost, s.st));
+    HIPC(hipStreamSynchronize(s.st));
+    if (p < 0 || p >= m2) throw Error(HBEM_ENUMERIC, "pivoted QR of the Wielandt matrix failed");
+    hbem::wielandt_build(dT, n, sigma, Wq, s.st);
+    int* wpiv = s.Wpiv.as<int>(m2);
+    int* wperm = s.Wperm.as<int>(m2);
+    hbem::lu_factor_batched(Wq, m2, 1, wpiv, wperm, s.st);
+    std::vector<cdbl> e(m2, hbem::mk(0, 0));
+    e[p] = hbem::mk(1, 0);
+    cdbl* de = s.ud.as<cdbl>(m2);
+    HIPC(hipMemcpyAsync(de, e.data(), m2 * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
+    cdbl* wk = s.work.as<cdbl>(2 * (size_t)m2);
+    hbem::lu_solve_thin_batched(Wq, (long long)mm, wperm, m2, 1, de, 1, false, wk, s.st);
+    x.resize(m2);
+    HIPC(hipMemcpyAsync(x.data(), de, m2 * sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
+    HIPC(hipStreamSynchronize(s.st));
+    double nrm = 0.0;
+    for (const auto& z : x) nrm += z.x * z.x + z.y * z.y;
+    nrm = std::sqrt(nrm);
+    if (!(nrm > 0.0) || !std::isfinite(nrm)) throw Error(HBEM_ENUMERIC, "singular Wielandt matrix");
+    for (auto& z : x) z = hbem::mk(z.x / nrm, z.y / nrm);
+}
 
 void check_dense_args(const void* T, int n, const void* R, int nc, int q) {
     if (!T || !R) throw Error(HBEM_EINVAL, "null matrix");
@@ -1056,22 +1110,30 @@ int hbem_rsvd_sv_der(const void* T, const void* Td, int n, const void* R, int nc
     Standalone& s = g_sa;
     double sigma = standalone_sv(T, n, R, nc, q, true);
     const size_t nn = (size_t)n * n;
+    const int m2 = 2 * n;
     cdbl* dT1 = s.T1.as<cdbl>(nn);
     HIPC(hipMemcpyAsync(dT1, Td, nn * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
-    double* dout = s.out.as<double>(1 + hbem::kBilinearParts);
-    hbem::bilinear_re_batched(dT1, (long long)nn, (const cdbl*)s.u.p, (const cdbl*)s.v.p, n, 1, dout, dout + 1, s.st);
-    HIPC(hipGetLastError());
-    double der = 0.0;
-    HIPC(hipMemcpyAsync(&der, dout, sizeof(double), hipMemcpyDeviceToHost, s.st));
+    std::vector<cdbl> x;
+    reference_wielandt_vector(s, T, n, sigma, x);
+    // sigma' = Re(x^H [T' x_2; T'^H x_1]) (randsvd.cpp:163-167)
+    cdbl* dx = s.p.as<cdbl>(m2);
+    cdbl* dp = s.p2.as<cdbl>(m2);
+    HIPC(hipMemcpyAsync(dx, x.data(), m2 * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
+    hbem::gemv_batched(dT1, 0, dx + n, dp, n, 1, false, s.st);
+    hbem::gemv_batched(dT1, 0, dx, dp + n, n, 1, true, s.st);
+    std::vector<cdbl> hp(m2);
+    HIPC(hipMemcpyAsync(hp.data(), dp, m2 * sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
     HIPC(hipStreamSynchronize(s.st));
+    double der = 0.0;
+    for (int l = 0; l < m2; ++l) der += x[l].x * hp[l].x + x[l].y * hp[l].y;
     out2[0] = sigma;
     out2[1] = der;
     API_END
 }
 
 // randomized_svd::sv_der2 (randsvd.cpp:171-216): the eigenvector x of the
-// Wielandt matrix W = [[0,T],[T^H,0]] for sigma is [u; v]/sqrt2 from the
-// randomized subspace; the bordered system follows the reference.
+// Wielandt matrix W = [[0,T],[T^H,0]] for sigma as the reference takes it
+// (reference_wielandt_vector); the bordered system follows the reference.
 int hbem_rsvd_sv_der2(const void* T, const void* Td, const void* Tdd, int n, const void* R, int nc, int q,
                       double* out3) {
     API_BEGIN
@@ -1086,15 +1148,20 @@ int hbem_rsvd_sv_der2(const void* T, const void* Td, const void* Tdd, int n, con
     cdbl* dT2 = s.T2.as<cdbl>(nn);
     HIPC(hipMemcpyAsync(dT1, Td, nn * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
     HIPC(hipMemcpyAsync(dT2, Tdd, nn * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
-    double* dout = s.out.as<double>(1 + hbem::kBilinearParts);
-    hbem::bilinear_re_batched(dT1, (long long)nn, (const cdbl*)s.u.p, (const cdbl*)s.v.p, n, 1, dout, dout + 1, s.st);
-    double der = 0.0;
-    HIPC(hipMemcpyAsync(&der, dout, sizeof(double), hipMemcpyDeviceToHost, s.st));
-    // x = [u; v] / sqrt2, normalised by its largest-modulus entry
-    std::vector<cdbl> x(m2);
-    HIPC(hipMemcpyAsync(x.data(), s.u.p, n * sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
-    HIPC(hipMemcpyAsync(x.data() + n, s.v.p, n * sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
+    std::vector<cdbl> x;
+    reference_wielandt_vector(s, T, n, sigma, x);
+    // sigma' = Re(x^H [T' x_2; T'^H x_1]); then u = x normalised by its
+    // largest-modulus entry (randsvd.cpp:185-198)
+    cdbl* dx = s.p.as<cdbl>(m2);
+    cdbl* dpx = s.p2.as<cdbl>(m2);
+    HIPC(hipMemcpyAsync(dx, x.data(), m2 * sizeof(cdbl), hipMemcpyHostToDevice, s.st));
+    hbem::gemv_batched(dT1, 0, dx + n, dpx, n, 1, false, s.st);
+    hbem::gemv_batched(dT1, 0, dx, dpx + n, n, 1, true, s.st);
+    std::vector<cdbl> hpx(m2);
+    HIPC(hipMemcpyAsync(hpx.data(), dpx, m2 * sizeof(cdbl), hipMemcpyDeviceToHost, s.st));
     HIPC(hipStreamSynchronize(s.st));
+    double der = 0.0;
+    for (int l = 0; l < m2; ++l) der += x[l].x * hpx[l].x + x[l].y * hpx[l].y;
     double best = 0.0;
     int m = 5 < m2 ? 5 : 0;
     for (int l = 0; l < m2; ++l) {
@@ -1240,17 +1307,13 @@ void sweep_run(hbem_geometry* g, const double* ks, int nk, double c_o, double c_
     // up to one matrix per CU.  B is bounded by the free device memory: local
     // matrices, chunk partials and T, T', T'' per wavenumber (and the
     // bordered systems of the sigma'' stage).
-    const size_t per = sweep_bytes_per_k(g->npairs, g->nadj, g->ncoi, g->nc, n, D, nc) + (d2sigma ? der2_bytes_per_k(n) : 0) +
+    const bool tdiff = sweep_tdiff(ks, nk);
+    const size_t per = sweep_bytes_per_k(g->npairs, g->nadj, g->ncoi, g->nc, n, D, nc, tdiff ? 1 : 2) +
+                       (d2sigma ? der2_bytes_per_k(n) : 0) +
                        (mx ? mixed_bytes_per_k(n, nc) : 0);
     const int B = sweep_batch(per, usable_device_bytes(g->work_held()), nk);
     cdbl* dR = g->w_R.as<cdbl>((size_t)n * nc);
     HIPC(hipMemcpyAsync(dR, R, (size_t)n * nc * sizeof(cdbl), hipMemcpyHostToDevice, g->st));
-    // Sweeps of >= 32 real positive wavenumbers run every batch on the
-    // paired-lane kernel in the tdiff layout (half the local-matrix bytes),
-    // so a k's bits do not depend on the batch size the free memory allows.
-    bool allpos_all = true;
-    for (int j = 0; j < nk; ++j) allpos_all = allpos_all && ks[j] > 0.0;
-    const bool tdiff = allpos_all && nk >= 32;
     for (int k0 = 0; k0 < nk; k0 += B) {
         const int nb = std::min(B, nk - k0);
         bool allpos = true;
@@ -1790,7 +1853,8 @@ int hbem_sweep_batch_size(int npanels, int order, int d, int nc, unsigned long l
     // a closed curve of N panels: N coinciding, N adjacent pairs
     const long long N = npanels;
     const int ncr = order * (order + 1) / 2 - 1;
-    *B = sweep_batch(sweep_bytes_per_k(N * (N + 1) / 2, N, N, ncr, 2 * npanels, d, nc), (size_t)avail_bytes, nk);
+    *B = sweep_batch(sweep_bytes_per_k(N * (N + 1) / 2, N, N, ncr, 2 * npanels, d, nc, nk >= 32 ? 1 : 2),
+                     (size_t)avail_bytes, nk);
     API_END
 }
 

@@ -1610,6 +1610,137 @@ void wielandt_build(const cdbl* T, int n, double s, cdbl* W, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
+// Pivot order of a column-pivoted Householder QR, as the reference's
+// W.colPivHouseholderQr() in randomized_svd::sv_der / sv_der2
+// (randsvd.cpp:160-162, 183-184), with the xGEQP3 rule: at step k the
+// remaining column of largest squared norm (rows k..m-1 after the earlier
+// reflections), the first slot on ties; the column norms are recomputed
+// after every step.  Only the order is kept: the reference's vector, the last
+// column of Q, is orthogonal to every column but the last pivot p, so for the
+// Hermitian W - sigma I it is the direction of (W - sigma I)^{-1} e_p, which
+// the caller forms with the LU.
+// One launch per step, one workgroup per remaining slot of every matrix;
+// slot order and norms are double-buffered by step parity, so no launch
+// reads what it writes.  A (m x m per matrix, stride ms) is overwritten.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();                       // red is free (earlier sums read)
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(256) void k_cp_norms(const cdbl* __restrict__ A, long long ms, int m,
+                                                 double* __restrict__ nrm, int* __restrict__ slot) {
+    __shared__ double red[4];
+    const int c = blockIdx.x, b = blockIdx.y;
+    const cdbl* col = A + (size_t)b * ms + (size_t)c * m;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < m; i += 256) v += cabs2(col[i]);
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) {
+        nrm[(size_t)b * m + c] = v;
+        slot[(size_t)b * m + c] = c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cp_step(cdbl* __restrict__ A, long long ms, int m, int k,
+                                                const double* __restrict__ nin, double* __restrict__ nout,
+                                                const int* __restrict__ sin_, int* __restrict__ sout,
+                                                int* __restrict__ last) {
+    __shared__ double red[4];
+    __shared__ double bv[256];
+    __shared__ int bs[256];
+    const int tid = threadIdx.x, b = blockIdx.y;
+    const int s = k + blockIdx.x;          // this workgroup's slot
+    A += (size_t)b * ms;
+    nin += (size_t)b * m;
+    nout += (size_t)b * m;
+    sin_ += (size_t)b * m;
+    sout += (size_t)b * m;
+    // pivot: the first slot of largest norm among k..m-1 (each thread scans
+    // its slots in increasing order; the tree keeps the lower slot on ties)
+    double best = -1.0;
+    int bslot = m;
+    for (int t = k + tid; t < m; t += 256) {
+        const double v = nin[sin_[t]];
+        if (v > best) { best = v; bslot = t; }
+    }
+    bv[tid] = best;
+    bs[tid] = bslot;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            const double v2 = bv[tid + w];
+            const int s2 = bs[tid + w];
+            if (v2 > bv[tid] || (v2 == bv[tid] && s2 < bs[tid])) { bv[tid] = v2; bs[tid] = s2; }
+        }
+        __syncthreads();
+    }
+    const int ps = bs[0] < m ? bs[0] : k;
+    const int pc = sin_[ps];
+    if (s == k) {
+        // the pivot's workgroup: the next slot order (slots k and ps exchanged)
+        for (int t = tid; t < m; t += 256) sout[t] = t == k ? pc : (t == ps ? sin_[k] : sin_[t]);
+        if (k == m - 1 && tid == 0) last[b] = pc;
+        return;
+    }
+    const int c = s == ps ? sin_[k] : sin_[s];
+    const cdbl* pv = A + (size_t)pc * m;
+    cdbl* cv = A + (size_t)c * m;
+    // reflector of the pivot column, rows k..m-1: v = x - beta e_k,
+    // beta = -(alpha / |alpha|) |x|, tau = 2 / |v|^2
+    double xs = 0.0;
+    for (int i = k + tid; i < m; i += 256) xs += cabs2(pv[i]);
+    const double xn = sqrt(block_sum(xs, red));
+    const cdbl alpha = pv[k];
+    const double aa = sqrt(cabs2(alpha));
+    const cdbl beta = aa == 0.0 ? mk(-xn, 0.0) : mk(-alpha.x / aa * xn, -alpha.y / aa * xn);
+    const cdbl vk = alpha - beta;
+    double vs = 0.0;
+    for (int i = k + 1 + tid; i < m; i += 256) vs += cabs2(pv[i]);
+    const double vn = cabs2(vk) + block_sum(vs, red);
+    const double tau = vn == 0.0 ? 0.0 : 2.0 / vn;
+    // s_c = tau v^H c; c -= s_c v over rows k..m-1
+    double dr = 0.0, di = 0.0;
+    for (int i = k + tid; i < m; i += 256) {
+        const cdbl v = i == k ? vk : pv[i];
+        const cdbl x = cv[i];
+        dr += v.x * x.x + v.y * x.y;
+        di += v.x * x.y - v.y * x.x;
+    }
+    const double sr = tau * block_sum(dr, red);
+    const double si = tau * block_sum(di, red);
+    double ns = 0.0;
+    for (int i = k + tid; i < m; i += 256) {
+        const cdbl v = i == k ? vk : pv[i];
+        cdbl x = cv[i];
+        x.x -= sr * v.x - si * v.y;
+        x.y -= sr * v.y + si * v.x;
+        cv[i] = x;
+        if (i > k) ns += cabs2(x);
+    }
+    ns = block_sum(ns, red);
+    if (tid == 0) nout[c] = ns;
+}
+
+void colpiv_last_pivot_batched(cdbl* A, long long ms, int m, int batch, double* nrm2, int* slot2, int* last,
+                               hipStream_t st) {
+    if (m <= 0 || batch <= 0) return;
+    double* nb[2] = {nrm2, nrm2 + (size_t)m * batch};
+    int* sb[2] = {slot2, slot2 + (size_t)m * batch};
+    hipLaunchKernelGGL(k_cp_norms, dim3(m, batch), dim3(256), 0, st, (const cdbl*)A, ms, m, nb[0], sb[0]);
+    for (int k = 0; k < m; ++k) {
+        const int cur = k & 1;
+        hipLaunchKernelGGL(k_cp_step, dim3(m - k, batch), dim3(256), 0, st, A, ms, m, k, (const double*)nb[cur],
+                           nb[cur ^ 1], (const int*)sb[cur], sb[cur ^ 1], last);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // The bordered Wielandt system of randomized_svd::sv_der2 (randsvd.cpp:171-216),
 // batched over nb wavenumbers: x = [u; v] (the singular pair of T_p for sigma),
 // m = the first index of the largest |x_l| (5 when none is positive, as the

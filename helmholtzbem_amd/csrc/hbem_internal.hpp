@@ -143,6 +143,11 @@ void bilinear_re_batched(const cdbl* T, long long tstride, const cdbl* a, const 
 void gemv_batched(const cdbl* T, long long tstride, const cdbl* x, cdbl* y, int n, int batch, bool adj, hipStream_t st);
 // W = [[0, T], [T^H, 0]] - s I  (2n x 2n) for a single matrix
 void wielandt_build(const cdbl* T, int n, double s, cdbl* W, hipStream_t st);
+// last pivot of the column-pivoted Householder QR (xGEQP3 rule) of each m x m
+// matrix of A (stride ms, overwritten) into last[b]; nrm2: 2 m batch doubles,
+// slot2: 2 m batch ints of scratch
+void colpiv_last_pivot_batched(cdbl* A, long long ms, int m, int batch, double* nrm2, int* slot2, int* last,
+                               hipStream_t st);
 // The bordered system of randomized_svd::sv_der2, batched over nb matrices of
 // order n (systems of order 2n, vectors of 2n entries back to back):
 // X_b = [u_b; v_b] / x_m with m_b the first index of the largest |x_l|;

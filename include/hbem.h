@@ -166,9 +166,14 @@ int hbem_tp_solve(hbem_geometry_t g, double k_re, double k_im, double c_o, doubl
  * randomized_svd::sv / sv_der / sv_der2 (randsvd.hpp:23-30,
  * randsvd.cpp:99-216) on host matrices; the factorisations run on the GPU.
  * out: sigma (sv), {sigma, sigma'} (sv_der), {sigma, sigma', sigma''} (sv_der2).
- * sv_der computes sigma' = Re(u^H T' v) from the singular pair of the
- * randomized subspace; sv_der2 solves the reference's bordered system
- * (randsvd.cpp:190-214) for sigma''.
+ * sv_der and sv_der2 take the eigenvector of the Wielandt matrix
+ * [[0, T], [T^H, 0]] as the reference does, the last column of Q of the
+ * column-pivoted QR of W - sigma I (randsvd.cpp:160-162): the pivot order of
+ * that QR is computed on the GPU and the column itself, (W - sigma I)^{-1}
+ * e_p for the last pivot p, by the blocked LU, so sigma' agrees with the
+ * reference at any q, not only once the subspace has converged (the batched
+ * sweeps use the subspace singular pair instead, see hbem_sigma_sweep).
+ * sv_der2 then solves the reference's bordered system (randsvd.cpp:190-214).
  */
 int hbem_rsvd_sv(const void* T, int n, const void* R, int nc, int q, double* sigma);
 int hbem_rsvd_sv_der(const void* T, const void* Td, int n, const void* R, int nc, int q, double* out2);
@@ -198,7 +203,10 @@ int hbem_rsvd_sv_noalloc(const void* T_dev, int n, const void* R_dev, int nc, in
  * device; only the nk results come back.  want_der = 1 assembles the
  * operators with their first k-derivatives, want_der = 2 with the first and
  * second (gen_sol_op_2nd_der's operator set); in both cases
- * dsigma/dk = Re(u^H T' v) is returned in dsigma (may be NULL when want_der = 0).
+ * dsigma/dk = Re(u^H T' v) is returned in dsigma (may be NULL when want_der = 0),
+ * (u, v) the singular pair of the randomized subspace: the same as
+ * hbem_rsvd_sv_der's once the subspace has converged, within
+ * O((sigma - sigma_min) / gap) of it at small q.
  * R: n x nc test matrix (n = 2N), uploaded once per call.  The wavenumbers run
  * in batches sized to the free device memory; a sweep of >= 32 real positive
  * k assembles every batch with the same kernel, so each result is the same
@@ -271,7 +279,9 @@ int hbem_profile_query(hbem_geometry_t g, int which, double* ms, long long* laun
 /* Wavenumbers per batch that hbem_sigma_sweep uses for a closed curve of
  * npanels panels at Gauss order `order`, derivative order d and nc random
  * columns when avail_bytes of device memory are usable and nk wavenumbers are
- * requested (at most 256, batches balanced).  Host-only: no device is used.
+ * requested (at most 256, batches balanced; nk >= 32 real positive k use the
+ * T-difference layout, half the local-matrix bytes).  Host-only: no device
+ * is used.
  * (No counterpart in the reference, whose sweep runs one k per thread.) */
 int hbem_sweep_batch_size(int npanels, int order, int d, int nc, unsigned long long avail_bytes, int nk, int* B);
 

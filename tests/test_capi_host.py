@@ -131,23 +131,25 @@ def test_layer_type_and_space(H):
 
 def test_sweep_batch_size_helper(H):
     """Wavenumbers per sweep batch from the usable device memory: the bytes per
-    k are the local matrices of two refraction indices (2 x N(N+1)/2 x
-    16(d+1) complex), the chunk partials of the singular pairs, T..T^(d) and
-    the thin blocks; at most 256 per batch, batches balanced."""
+    k are the local matrices (N(N+1)/2 x 16(d+1) complex per column: one
+    column of T differences for sweeps of >= 32 k, one per refraction index
+    otherwise), the chunk partials of the singular pairs (as many columns),
+    T..T^(d) and the thin blocks; at most 256 per batch, batches balanced."""
     N, order = 2000, 11
     nc_rule = order * (order + 1) // 2 - 1
     for d in (0, 1, 2):
         nslot = 16 * (d + 1)
-        loc = 2 * (N * (N + 1) // 2) * nslot * 16
         chunks = 2 * -(-nc_rule * nc_rule // 2048)
-        part = 2 * N * chunks * nslot * 16
         n = 2 * N
-        per = loc + part + (d + 1) * n * n * 16 + 2 * n * 4 + (4 * 2 + 4) * n * 16 + 64
-        for avail_gib in (16, 64, 270):
-            avail = avail_gib << 30
-            bmax = min(256, avail // per)
-            nb = -(-128 // bmax)
-            assert H.sweep_batch_size(N, order, d, 2, avail, 128) == -(-128 // nb)
+        for nk, lcols in ((128, 1), (20, 2)):
+            loc = lcols * (N * (N + 1) // 2) * nslot * 16
+            part = lcols * N * chunks * nslot * 16
+            per = loc + part + (d + 1) * n * n * 16 + 2 * n * 4 + (4 * 2 + 4) * n * 16 + 64
+            for avail_gib in (16, 64, 270):
+                avail = avail_gib << 30
+                bmax = max(1, min(256, avail // per))
+                nb = -(-nk // bmax)
+                assert H.sweep_batch_size(N, order, d, 2, avail, nk) == -(-nk // nb), (d, nk, avail_gib)
     # the 288 GB device holds the whole 128-k grid at d = 0 in one batch
     assert H.sweep_batch_size(N, order, 0, 2, 270 << 30, 128) == 128
     # at d = 2 the old fixed 64 GiB bound gave 16; the device memory gives 64

@@ -159,16 +159,19 @@ def test_rsvd_parity(case, H, O):
         s = H.randomized_svd.sv(T, R, q)
         so = O.sv(T, R, q)
         assert abs(s - so) <= SIG_TOL * so, (name, nc, q, s, so)
-    # derivatives: at converged subspace iteration the singular pair is unique
+    # derivatives: the reference's Wielandt vector (last column of Q of the
+    # column-pivoted QR of W - sigma I) at the drivers' q = 2, where it is
+    # not the subspace's singular pair, and at converged q = 30
     R = H.gaussian_test_matrix(n, 2, 3)
-    d1 = H.randomized_svd.sv_der(T, Td, R, 30)
-    d1o = O.sv_der(T, Td, R, 30)
-    assert abs(d1[0] - d1o[0]) <= SIG_TOL * d1o[0]
-    assert abs(d1[1] - d1o[1]) <= 1e-8 * max(1.0, abs(d1o[1])), (name, d1, d1o)
-    d2 = H.randomized_svd.sv_der2(T, Td, Tdd, R, 30)
-    d2o = O.sv_der2(T, Td, Tdd, R, 30)
-    assert abs(d2[1] - d2o[1]) <= 1e-8 * max(1.0, abs(d2o[1]))
-    assert abs(d2[2] - d2o[2]) <= 1e-6 * max(1.0, abs(d2o[2])), (name, d2, d2o)
+    for q in (2, 30):
+        d1 = H.randomized_svd.sv_der(T, Td, R, q)
+        d1o = O.sv_der(T, Td, R, q)
+        assert abs(d1[0] - d1o[0]) <= SIG_TOL * d1o[0]
+        assert abs(d1[1] - d1o[1]) <= 1e-8 * max(1.0, abs(d1o[1])), (name, q, d1, d1o)
+        d2 = H.randomized_svd.sv_der2(T, Td, Tdd, R, q)
+        d2o = O.sv_der2(T, Td, Tdd, R, q)
+        assert abs(d2[1] - d2o[1]) <= 1e-8 * max(1.0, abs(d2o[1])), (name, q, d2, d2o)
+        assert abs(d2[2] - d2o[2]) <= 1e-6 * max(1.0, abs(d2o[2])), (name, q, d2, d2o)
 
 
 def test_sigma_sweep_parity(case, H, O):
@@ -335,3 +338,28 @@ def test_star400_oracle_parity_at_bench_size(big, H, O):
     assert abs(sig[0] - S[-1]) <= SIG_TOL * S[-1], (sig[0], S[-1], S[-2])
     d_exact = float(np.real(u.conj() @ (Td @ v)))
     assert abs(dsig[0] - d_exact) <= 1e-8 * max(1.0, abs(d_exact)), (dsig[0], d_exact)
+
+
+def test_star400_dsigma_estimators_at_q2(big, H, O):
+    """dsigma/dk at the drivers' q = 2 on the 400-panel order-16 star, at two k
+    of its 256-point grid (indices 17 and 243, where the two estimators differ
+    most).  randomized_svd::sv_der (the per-k API: the reference's pivoted-QR
+    Wielandt vector) against the oracle's restatement of the same method:
+    1e-8.  The batched sweep's estimator (the subspace's singular pair) is
+    stated against it: measured 6.1e-4 and 2.5e-2 relative here; the bound
+    below (5e-2) keeps the statement honest without pinning the rounding."""
+    mesh, bd = big
+    grid = np.linspace(1.0, 10.0, 256)
+    R = H.gaussian_test_matrix(800, 2, 1)
+    so = H.SolutionsOperator(bd)
+    for i in (17, 243):
+        k = grid[i]
+        T, Td = so.gen_sol_op_1st_der(k, 1.0, 16.0)
+        api = H.randomized_svd.sv_der(T, Td, R, 2)
+        ref = O.sv_der(T, Td, R, 2)
+        assert abs(api[1] - ref[1]) <= 1e-8 * max(1.0, abs(ref[1])), (k, api, ref)
+        _, dsig = H.sigma_sweep(bd, np.array([k]), 1.0, 16.0, R, 2, der=1)
+        rel = abs(dsig[0] - ref[1]) / abs(ref[1])
+        print("k %.4f dsigma: reference method %.10e, sweep (subspace pair) %.10e, rel diff %.2e"
+              % (k, ref[1], dsig[0], rel), flush=True)
+        assert rel <= 5e-2
