@@ -39,8 +39,9 @@ FLOP_PER_NODE = {0: 292.0, 1: 376.0, 2: 498.0}   # the library's algorithmic mod
 TRAFFIC_BYTES = {"star400": 1.7076e10, "polygon2000": 8.2862e10}
 # FP64 FLOP the general-pair kernel executes per quadrature node (rocprofv3
 # SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 x 64 lanes, FMA twice, over the nodes of
-# the launch), next to the 292 / 376 / 498 of the algorithmic model at d = 0/1/2
-EXECUTED_FLOP_PER_NODE = {"polygon2000": 243.3, "star400": None}
+# the launch), next to the 292 / 376 / 498 of the algorithmic model at d = 0/1/2:
+# profiles/k_assemble_x2_general_polygon2000_counters.txt, k_assemble_x2_star400_executed_flops.txt
+EXECUTED_FLOP_PER_NODE = {"polygon2000": 243.3, "star400": 337.6}
 
 WORKLOADS = {
     "polygon2000": dict(panels=2000, order=11, c_i=20.0, c_o=1.0, nk=128, k_min=5.0, k_max=40.0, der=0, seed=3,
