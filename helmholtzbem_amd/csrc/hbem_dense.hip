@@ -458,6 +458,23 @@ __global__ void k_lu_perm(int n, const int* __restrict__ ipiv, int* __restrict__
 // in that XCD's L2 instead of being fetched by all eight.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+// Tile (row block tr, column block tc) of tile index t in a tm x tn grid of
+// one matrix, numbered in groups of kTileGroup row blocks: consecutive indices
+// run down a group's rows before moving one column to the right.  The ~64
+// workgroups an XCD holds at a time then cover an 8 x 8 square of tiles and
+// share its 8 row blocks of A and 8 column blocks of B in that XCD's L2 (down
+// whole columns of tiles they shared only the B block, and A's panel was read
+// again for every column).  A tile's arithmetic does not depend on its number.
+constexpr int kTileGroup = 8;
+__device__ __forceinline__ void grouped_tile(int t, int tm, int tn, int& tr, int& tc) {
+    const int per_group = kTileGroup * tn;
+    const int g = t / per_group, first = g * kTileGroup;
+    const int rows = min(kTileGroup, tm - first);
+    const int r = t - g * per_group;
+    tr = first + r % rows;
+    tc = r / rows;
+}
+
 template <bool TB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_zgemm(int m, int n, int k, cdbl alpha, const cdbl* __restrict__ A, int lda,
                                                long long sa, const cdbl* __restrict__ B, int ldb, long long sb,
@@ -477,7 +494,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int l16 = lane & 15, lk = lane >> 4;
     const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
-    const int row0 = (tile % tm) * 64, col0 = (tile / tm) * 64;
+    int tr, tc;
+    grouped_tile(tile, tm, tn, tr, tc);
+    const int row0 = tr * 64, col0 = tc * 64;
     dbl4 p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -607,7 +626,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int l16 = lane & 15, lk = lane >> 4;
     const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
-    const int row0 = (tile % tm) * 64, col0 = (tile / tm) * 64;
+    int tr, tc;
+    grouped_tile(tile, tm, tn, tr, tc);
+    const int row0 = tr * 64, col0 = tc * 64;
     flt4 p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)

@@ -86,3 +86,22 @@ def test_autotuned_bucket(workload, H, tmp_path):
     b = H.sigma_sweep_batched(sub, Q, bucket=64)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def test_batched_k_count_not_a_multiple_of_32(H):
+    """40 wavenumbers per instance: the slot blocks of 32 are padded (the
+    padded slots are computed and never written); three small polygons of
+    uneven size, batched == each alone bit for bit."""
+    sp = H.ContinuousSpace1()
+    rng = np.random.default_rng(11)
+    items = []
+    for i, N in enumerate((37, 52, 90)):
+        mesh = H.random_polygon_mesh(20 + i, 7, N)
+        bd = H.BuilderData(mesh, sp, sp, ORDER)
+        ks = np.sort(rng.uniform(2.0, 12.0, 40))
+        items.append((bd, ks, 1.0, 6.0 + i, H.gaussian_test_matrix(2 * N, 2, 40 + i)))
+    batched = H.sigma_sweep_batched(items, Q, bucket=64)
+    for (bd, ks, c_o, c_i, R), s in zip(items, batched):
+        alone = H.sigma_sweep(bd, ks, c_o, c_i, R, Q)
+        assert s.shape == (40,) and np.all(np.isfinite(s)) and np.all(s > 0)
+        assert np.array_equal(s, alone), np.abs(s - alone).max()
