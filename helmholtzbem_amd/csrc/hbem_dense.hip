@@ -604,11 +604,12 @@ void zgemm_batched(int m, int n, int k, cdbl alpha, const cdbl* A, int lda, long
 // wavefronts of 32 x 32, k through LDS in steps of 16 with the next step
 // prefetched, the 3M complex product formed transposed, XCD-grouped
 // numbering -- with v_mfma_f32_16x16x4f32, FP32 operands and FP32
-// accumulation.  Element v of lane l of that instruction's result is
+// accumulation, at four wavefronts per SIMD (108 VGPRs; against two, the
+// polygon2000 FP32 LU took 241 instead of 258 ms, same bits).  Element v of lane l of that instruction's result is
 // D(4 (l >> 4) + v, l & 15) (the transposed product's row, i.e. C's column).
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_cgemm32_sub(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_cgemm32_sub(
     int m, int n, int k, const cflt* __restrict__ A, const cflt* __restrict__ B, cflt* __restrict__ C, int ld,
     long long sm, int batch) {
     const int tm = (m + 63) >> 6, tn = (n + 63) >> 6;
@@ -888,13 +889,15 @@ __device__ __forceinline__ void diag_solve(int n, int r0, int bs, bool lower, bo
 
 // Right-looking pass with the column-major factor: L (lower, unit) forward or
 // U (upper) backward.  After block b, rows outside it (below / above) take
-// w_i -= sum_j M(i, r0 + j) x_j: a wavefront covers 16 rows x 4 groups of 16
-// columns (lane = row + 16 group), each thread loading its 16 columns in two
-// batches of eight.
+// w_i -= sum_j M(i, r0 + j) x_j: a wavefront covers RW rows x 64 / RW groups
+// of columns (lane = row + RW group), each thread loading its columns in
+// batches of eight.  RW = 16 for FP64 factors, 32 for FP32 ones, so that a
+// load instruction reads runs of 256 bytes down the columns either way.
 template <int NC, class F>
 __device__ __forceinline__ void subst_right(const F* M, int n, bool lower, cdbl* w, SolveLds& S) {
+    constexpr int RW = sizeof(F) == sizeof(cflt) ? 32 : 16, CW = RW;   // rows per wavefront, columns per group
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
-    const int l16 = lane & 15, grp = lane >> 4;
+    const int l16 = lane % RW, grp = lane / RW;
     const int nblk = (n + 63) / 64;
     auto rb = [&](int s) { return 64 * (lower ? s : nblk - 1 - s); };   // first row of the s-th block solved
     auto bsz = [&](int s) { return min(64, n - rb(s)); };
@@ -909,7 +912,7 @@ __device__ __forceinline__ void subst_right(const F* M, int n, bool lower, cdbl*
         }
         __syncthreads();
         const int ia = lower ? r0 + bs : 0, ib = lower ? n : r0;
-        for (int i0 = ia + 16 * wv; i0 < ib; i0 += 16 * nwv) {
+        for (int i0 = ia + RW * wv; i0 < ib; i0 += RW * nwv) {
             const int i = i0 + l16;
             cdbl acc[NC];
 #pragma unroll
@@ -919,14 +922,14 @@ __device__ __forceinline__ void subst_right(const F* M, int n, bool lower, cdbl*
                 // beyond the block zeroed afterwards: no branch between the loads
                 const int ic = min(i, ib - 1);
 #pragma unroll 1
-                for (int u0 = 0; u0 < 16; u0 += 8) {
+                for (int u0 = 0; u0 < CW; u0 += 8) {
                     F t[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u)
-                        t[u] = M[ic + (size_t)(r0 + min(16 * grp + u0 + u, bs - 1)) * n];
+                        t[u] = M[ic + (size_t)(r0 + min(CW * grp + u0 + u, bs - 1)) * n];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        const int j = 16 * grp + u0 + u;
+                        const int j = CW * grp + u0 + u;
                         const cdbl tv = j < bs ? to_cdbl(t[u]) : mk(0, 0);
 #pragma unroll
                         for (int c = 0; c < NC; ++c) cfma(acc[c], tv, S.xb[j * kMaxNc + c]);
@@ -934,10 +937,12 @@ __device__ __forceinline__ void subst_right(const F* M, int n, bool lower, cdbl*
                 }
             }
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                acc[c].x += __shfl_xor(acc[c].x, 16, 64); acc[c].y += __shfl_xor(acc[c].y, 16, 64);
-                acc[c].x += __shfl_xor(acc[c].x, 32, 64); acc[c].y += __shfl_xor(acc[c].y, 32, 64);
-            }
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int o = RW; o < 64; o <<= 1) {
+                    acc[c].x += __shfl_xor(acc[c].x, o, 64);
+                    acc[c].y += __shfl_xor(acc[c].y, o, 64);
+                }
             if (grp == 0 && i < ib) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
