@@ -1079,7 +1079,8 @@ __global__ void k_gather_layers(const cdbl* __restrict__ L, int nk, int nslot, i
 __global__ __launch_bounds__(256) void k_form_T(const cdbl* __restrict__ L, int nk, int nslot,
                                                 const int* __restrict__ qi, const int* __restrict__ qo, int nb, int N,
                                                 int D, const double* __restrict__ M, cdbl* __restrict__ T0,
-                                                cdbl* __restrict__ T1, cdbl* __restrict__ T2, int ld) {
+                                                cdbl* __restrict__ T1, cdbl* __restrict__ T2, int ld,
+                                                cflt* __restrict__ T32) {
     __shared__ cdbl sW[8][8][32];   // [column in tile (I, J)][row in tile (I, J)][output]
     __shared__ cdbl sV[8][8][32];
     const int mt = (N + 7) >> 3, nbc = (nb + 31) >> 5;
@@ -1096,6 +1097,13 @@ __global__ __launch_bounds__(256) void k_form_T(const cdbl* __restrict__ L, int 
     const bool mirror = I != J;
     for (int t = 0; t <= D; ++t) {
         cdbl* Tt = (t == 0 ? T0 : (t == 1 ? T1 : T2)) + (size_t)(act ? b : 0) * NN;
+        // T and, when asked, its FP32 copy for the mixed-precision LU (each
+        // entry rounded to nearest) in the same pass
+        cflt* Ts = t == 0 && T32 ? T32 + (size_t)(act ? b : 0) * NN : nullptr;
+        auto put = [&](size_t idx, cdbl v) {
+            Tt[idx] = v;
+            if (Ts) Ts[idx] = cflt{(float)v.x, (float)v.y};
+        };
         const int m = 8 * I + i;
         for (int c = 0; c < 8; ++c) {
             const int n = 8 * J + c;
@@ -1114,10 +1122,10 @@ __global__ __launch_bounds__(256) void k_form_T(const cdbl* __restrict__ L, int 
                 w = Wo - Wi; v = Vo - Vi;
             }
             if (t == 0) { k00.x += M[m + (size_t)n * N]; k11.x += M[n + (size_t)m * N]; }
-            Tt[m + (size_t)n * n2] = k00;
-            Tt[(m + N) + (size_t)(n + N) * n2] = k11;
-            Tt[m + (size_t)(n + N) * n2] = v;
-            Tt[(m + N) + (size_t)n * n2] = w;
+            put(m + (size_t)n * n2, k00);
+            put((m + N) + (size_t)(n + N) * n2, k11);
+            put(m + (size_t)(n + N) * n2, v);
+            put((m + N) + (size_t)n * n2, w);
             if (mirror) { sW[c][i][jb] = w; sV[c][i][jb] = v; }
         }
         if (!mirror) continue;
@@ -1135,10 +1143,10 @@ __global__ __launch_bounds__(256) void k_form_T(const cdbl* __restrict__ L, int 
                 k00 = Ki - Ko; k11 = Ko - Ki;
             }
             if (t == 0) { k00.x += M[r + (size_t)col * N]; k11.x += M[col + (size_t)r * N]; }
-            Tt[r + (size_t)col * n2] = k00;
-            Tt[(r + N) + (size_t)(col + N) * n2] = k11;
-            Tt[r + (size_t)(col + N) * n2] = sV[i][c][jb];   // V(r, col) = V(col, r)
-            Tt[(r + N) + (size_t)col * n2] = sW[i][c][jb];
+            put(r + (size_t)col * n2, k00);
+            put((r + N) + (size_t)(col + N) * n2, k11);
+            put(r + (size_t)(col + N) * n2, sV[i][c][jb]);   // V(r, col) = V(col, r)
+            put((r + N) + (size_t)col * n2, sW[i][c][jb]);
         }
         __syncthreads();                     // the next order overwrites sW / sV
     }
@@ -1240,10 +1248,10 @@ void launch_gather_layers(const cdbl* L, int nk, int nslot, int q, int N, int D,
 }
 
 void launch_form_T(const cdbl* L, int nk, int nslot, const int* qi, const int* qo, int nb, int N, int D,
-                   const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st, int ld) {
+                   const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st, int ld, cflt* T32) {
     const unsigned mt = (unsigned)((N + 7) / 8);
     hipLaunchKernelGGL(k_form_T, dim3(mt * mt * (unsigned)((nb + 31) / 32)), dim3(256), 0, st, L, nk, nslot, qi, qo,
-                       nb, N, D, M, T0, T1, T2, ld > 0 ? ld : 2 * N);
+                       nb, N, D, M, T0, T1, T2, ld > 0 ? ld : 2 * N, T32);
 }
 
 // The rows and columns n..ld-1 of nb matrices of order ld (stride ld^2): the

@@ -1342,17 +1342,17 @@ void sweep_run(hbem_geometry* g, const double* ks, int nk, double c_o, double c_
         cdbl* T2 = D >= 2 ? g->w_T2.as<cdbl>(nn * nb) : nullptr;
         const int* dqi = upload(g->w_qi, qi, g->st);
         const int* dqo = tdiff ? nullptr : upload(g->w_qo, qo, g->st);
+        // mixed precision: T formed in FP64 (kept for the residuals) and in FP32
+        // (factorised on the FP32 matrix cores) in one pass
+        cflt* T32 = mx ? g->w_T32.as<cflt>(nn * nb) : nullptr;
         hipEvent_t e0 = g->prof ? g->plog.begin(g->st) : nullptr;
-        hbem::launch_form_T(L, lcols, 16 * (D + 1), dqi, dqo, nb, g->N, D, (const double*)g->d_M.p, T0, T1, T2, g->st);
+        hbem::launch_form_T(L, lcols, 16 * (D + 1), dqi, dqo, nb, g->N, D, (const double*)g->d_M.p, T0, T1, T2, g->st, 0,
+                            T32);
         if (g->prof) g->plog.end(g->st, kProfFormT, e0);
         int* piv = g->w_piv.as<int>((size_t)n * nb);
         int* perm = g->w_perm.as<int>((size_t)n * nb);
-        cflt* T32 = nullptr;
         if (g->prof) e0 = g->plog.begin(g->st);
         if (mx) {
-            // FP32 copy factorised on the FP32 matrix cores; T0 stays the FP64 T of the residuals
-            T32 = g->w_T32.as<cflt>(nn * nb);
-            hbem::to_f32(T0, (long long)(nn * nb), T32, g->st);
             hbem::lu_factor_batched_f32(T32, n, nb, piv, perm, g->st);
         } else {
             hbem::lu_factor_batched(T0, n, nb, piv, perm, g->st);

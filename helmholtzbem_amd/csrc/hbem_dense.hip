@@ -789,20 +789,6 @@ void lu_factor_batched_f32(cflt* A, int n, int batch, int* ipiv, int* perm, hipS
     lu_factor_impl(A, n, batch, ipiv, perm, st);
 }
 
-// Single-precision copy of count complex values (round to nearest)
-__global__ __launch_bounds__(256) void k_to_f32(const cdbl* __restrict__ src, long long count, cflt* __restrict__ dst) {
-    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id < count) {
-        const cdbl v = src[id];
-        dst[id] = cflt{(float)v.x, (float)v.y};
-    }
-}
-
-void to_f32(const cdbl* src, long long count, cflt* dst, hipStream_t st) {
-    if (count <= 0) return;
-    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, src, count, dst);
-}
-
 // ---------------------------------------------------------------------------
 // Thin triangular solves
 // ---------------------------------------------------------------------------

@@ -70,9 +70,10 @@ void launch_gather_layers(const cdbl* L, int nk, int nslot, int q, int N, int D,
                           cdbl* V, hipStream_t st);
 // T_t for output b is written at T[t] + b * ld^2 (t = 0..D) with leading
 // dimension ld (0: 2N); qo null: L is in the tdiff layout (AsmParams) and
-// output b reads wavenumber qi[b] of it
+// output b reads wavenumber qi[b] of it; T32 (may be null): T_0 again in
+// FP32, each entry rounded to nearest
 void launch_form_T(const cdbl* L, int nk, int nslot, const int* qi, const int* qo, int nb, int N, int D,
-                   const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st, int ld = 0);
+                   const double* M, cdbl* T0, cdbl* T1, cdbl* T2, hipStream_t st, int ld = 0, cflt* T32 = nullptr);
 // identity in rows and columns n..ld-1 of nb matrices of order ld
 void launch_pad_identity(cdbl* T, int n, int ld, int nb, hipStream_t st);
 // A_o = [[-K_o + M/2, V_o], [W_o, K_o^T + M/2]] of wavenumber slot o (2N x 2N)
@@ -91,7 +92,6 @@ void lu_solve_thin_batched(const cdbl* LU, long long ms, const int* perm, int n,
 void lu_factor_batched_f32(cflt* A, int n, int batch, int* ipiv, int* perm, hipStream_t st);
 void lu_solve_thin_batched_f32(const cflt* LU, long long ms, const int* perm, int n, int batch, cdbl* X, int nc,
                                bool adj, cdbl* work, hipStream_t st);
-void to_f32(const cdbl* src, long long count, cflt* dst, hipStream_t st);
 // R = Bv - op(T) Z (op = T or T^H) for n x nc blocks of a batch, T (n x n,
 // stride ts) in FP64; part: residual_part_count(n, nc, batch) complex scratch
 void residual_thin_batched(const cdbl* T, long long ts, const cdbl* Z, const cdbl* Bv, cdbl* R, int n, int nc,
