@@ -37,6 +37,7 @@ FLOP_PER_NODE = {0: 292.0, 1: 376.0, 2: 498.0}   # the library's algorithmic mod
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, KiB per dispatch, scaled by the
 # counters' reading of a plain 1 GiB copy): profiles/k_assemble_x2_general_*.txt
 TRAFFIC_BYTES = {"star400": 1.7076e10, "polygon2000": 8.2862e10}
+TRAFFIC_BYTES_FAR = {}
 # FP64 FLOP the general-pair kernel executes per quadrature node (rocprofv3
 # SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 x 64 lanes, FMA twice, over the nodes of
 # the launch), next to the 292 / 376 / 498 of the algorithmic model at d = 0/1/2:
@@ -47,10 +48,10 @@ WORKLOADS = {
     "polygon2000": dict(panels=2000, order=11, c_i=20.0, c_o=1.0, nk=128, k_min=5.0, k_max=40.0, der=0, seed=3,
                         nvert=12,
                         text="random 12-vertex polygon (seed 3), 2000 line panels (4000x4000 complex T), c_i=20, "
-                             "c_o=1, Gauss order 11, 128 k in linspace(5,40), sigma_min per k, rSVD nc=2 q=2, FP64"),
+                             "c_o=1, Gauss order 11, 128 k in linspace(5,40), sigma_min per k, rSVD nc=2 q=2"),
     "star400": dict(panels=400, order=16, c_i=16.0, c_o=1.0, nk=256, k_min=1.0, k_max=10.0, der=2, seed=1,
                     text="star r(t)=1+0.25cos5t, 400 panels (800x800 complex T), c_i=16, c_o=1, Gauss order 16, "
-                         "256 k in linspace(1,10), d=2 operators, sigma_min + dsigma/dk per k, rSVD nc=2 q=2, FP64"),
+                         "256 k in linspace(1,10), d=2 operators, sigma_min + dsigma/dk per k, rSVD nc=2 q=2"),
 }
 NC, Q = 2, 2
 MIXED_REFINE = 2            # FP64 refinement steps of the last solve of the randomized SVD
@@ -100,14 +101,15 @@ def cpu_baseline(name, w, mesh, R, threads):
                       "(cores x t_asm + t_dense)" % (name, k, threads, t_asm, dense_txt, t_dense)}
 
 
-def run_workload(H, name, steps, warmup, mixed=False, bd=None, ref_sigma=None):
+def run_workload(H, name, steps, warmup, mixed=False, bd=None, ref_sigma=None, far=False):
     """Times `steps` sweeps of the workload (FP64, or with the mixed-precision
-    dense stage); with ref_sigma, also the largest relative sigma difference."""
+    dense stage; far: on a context with the lower-order rules on separated
+    pairs); with ref_sigma, also the largest relative sigma difference."""
     w = WORKLOADS[name]
     mesh = make_mesh(H, w)
     if bd is None:
         sp = H.ContinuousSpace1()
-        bd = H.BuilderData(mesh, sp, sp, w["order"])
+        bd = H.BuilderData(mesh, sp, sp, w["order"], flags=H.FLAG_FAR_FIELD if far else 0)
     N = w["panels"]
     n = 2 * N
     ks = np.linspace(w["k_min"], w["k_max"], w["nk"])
@@ -157,14 +159,18 @@ def run_workload(H, name, steps, warmup, mixed=False, bd=None, ref_sigma=None):
         "panel_pair_evals_per_s": pairs_per_k * evals / asm_s if asm_s > 0 else None,
         "phase_seconds_per_step": {ph: p[0] * 1e-3 / steps for ph, p in zip(PHASES, prof)},
         "lu_model_tflops": lu_tflops,
-        "roofline": {"kernel": "k_assemble_x2<general, d=%d> (general panel pairs, %d k x 2 c per launch)"
-                               % (der, w["nk"]),
+        "roofline": {"kernel": "k_assemble_x2<general, d=%d> (general panel pairs, %d k x 2 c per launch%s)"
+                               % (der, w["nk"], "; one launch per separation class, Gauss orders 11 / 9 / 7 / 6" if far else ""),
                      "bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES[name],
+                     "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": (TRAFFIC_BYTES_FAR if far else TRAFFIC_BYTES).get(name),
                      "avg_launch_ms": avg_launch_s * 1e3, "launches": n_gen,
                      "flops_per_launch": fl_gen / max(n_gen, 1),
                      "executed_tflops": executed, "executed_frac": executed / PEAK_FP64_TFLOPS if executed else None},
     }
+    if far:
+        out["far_field"] = "general pairs closer than one panel diameter at Gauss order 11, one to two diameters " \
+                           "apart at 9, two to eight at 7, farther at 6 (kappa d <= 0.65 over the grid)"
     if mixed:
         out["refine_steps"] = {"last_solve": MIXED_REFINE, "other_solves": MIXED_REFINE_INNER}
         out["fp64_fallbacks"] = fallback[0]
@@ -306,12 +312,17 @@ def main():
         return
     fp64, mesh, R = run_workload(H, args.workload, args.steps, args.warmup)
     head = fp64
-    mixed = None
+    mixed = mixed_far = None
     if WORKLOADS[args.workload]["der"] == 0 and not args.no_mixed:
         mixed, _, _ = run_workload(H, args.workload, args.steps, args.warmup, mixed=True, bd=fp64["bd"],
                                    ref_sigma=fp64["sigma"])
-        if mixed["max_rel_sigma_err_vs_fp64"] <= MIXED_TOL and mixed["value"] > fp64["value"]:
-            head = mixed
+        fp64["bd"].release_work()
+        mixed_far, _, _ = run_workload(H, args.workload, args.steps, args.warmup, mixed=True, far=True,
+                                       ref_sigma=fp64["sigma"])
+        mixed_far["bd"].release_work()
+        for cand in (mixed, mixed_far):
+            if cand["max_rel_sigma_err_vs_fp64"] <= MIXED_TOL and cand["value"] > head["value"]:
+                head = cand
     out = {
         "metric": "sigma_min(k) evals/s",
         "value": head["value"],
@@ -323,20 +334,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 LU + f64 refinement (assembly f64)" if head is mixed else "f64",
+        "dtype": "f64" if head is fp64 else ("f32 LU + f64 refinement (assembly f64%s)"
+                                              % (", lower-order rules on separated pairs" if head is mixed_far else "")),
         "data": "synthetic",
         "config": dict(head["config"], name=args.workload),
         "panel_pair_evals_per_s": head["panel_pair_evals_per_s"],
         "phase_seconds_per_step": head["phase_seconds_per_step"],
         "lu_model_tflops": head["lu_model_tflops"],
         "roofline": head["roofline"],
-        "path": "mixed" if head is mixed else "fp64",
+        "path": "mixed_far" if head is mixed_far else ("mixed" if head is mixed else "fp64"),
     }
     strip = lambda d: {k: v for k, v in d.items() if k not in ("sigma", "bd")}
     out["fp64"] = strip(fp64)
-    if mixed is not None:
-        out["mixed"] = strip(mixed)
-        out["mixed"]["speedup_vs_fp64"] = mixed["value"] / fp64["value"]
+    for key, leg in (("mixed", mixed), ("mixed_far", mixed_far)):
+        if leg is not None:
+            out[key] = strip(leg)
+            out[key]["speedup_vs_fp64"] = leg["value"] / fp64["value"]
     other = "star400" if args.workload == "polygon2000" else "polygon2000"
     if not args.no_second:
         sec, _, _ = run_workload(H, other, max(1, args.steps), 1)

@@ -35,6 +35,7 @@ __all__ = ["LayerType", "ContinuousSpace1", "BuilderData", "GalerkinBuilder", "S
            "polygon_mesh", "random_polygon_mesh", "gaussian_test_matrix", "Xoshiro256ss", "star_curve"]
 
 FLAG_QUIRK_SLP_COINC_DER = 1
+FLAG_FAR_FIELD = 2          # lower-order Gauss rules on separated panel pairs (HBEM_FLAG_FAR_FIELD)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -477,9 +478,11 @@ def sigma_sweep_mixed(builder_data: BuilderData, ks, c_o, c_i, R, q=2, refine=2,
     LU on the matrix cores; FP64 refinement: `refine` steps for the last
     solve of the randomized SVD, `refine_inner` for the others, -1 = as the
     last; the defaults 2 / 1 held sigma within 2e-10 of FP64 on the 2000-panel
-    polygon's 128-k grid).  Returns (sigma, fp64_used): fp64_used[i] is True where the last
-    solve's refinement could not certify k (or an earlier solve's refinement
-    did not contract) and it was recomputed with the FP64 factorisation."""
+    polygon's 128-k grid).  The assembly is FP64, with the lower-order rules
+    on separated pairs when builder_data was made with FLAG_FAR_FIELD.
+    Returns (sigma, fp64_used): fp64_used[i] is True where the last solve's
+    refinement could not certify k (or an earlier solve's refinement did not
+    contract) and it was recomputed with the FP64 factorisation."""
     ks = np.ascontiguousarray(ks, dtype=np.float64)
     if ks.ndim != 1:
         raise ValueError("ks must be a 1-D array of wavenumbers")

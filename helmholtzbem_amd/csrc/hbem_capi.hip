@@ -4,6 +4,8 @@
 #include <complex>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -256,6 +258,19 @@ struct hbem_geometry {
     DBuf d_aent, d_apidx, d_aflags, d_angeo;
     DBuf d_cent, d_cpidx, d_cflags, d_cngeo;
     int ngen = 0, nadj = 0, ncoi = 0;
+    // General pairs in order of separation (the distance of the panels'
+    // bounding discs in panel diameters): sep[0] pairs closer than one
+    // diameter (or on a spline panel), sep[1] at one to two, sep[2] at two to
+    // eight, sep[3] farther apart (see far_order); dmax is the largest panel
+    // diameter.
+    int sep[4] = {0, 0, 0, 0};
+    double dmax = 0.0;
+    // node tables of the separated classes at lower Gauss orders, built on
+    // first use: key 64 cls + order
+    struct FarSet {
+        DBuf xg, wg, gtab, flags, ngeo;
+    };
+    std::map<int, std::unique_ptr<FarSet>> far;
     DBuf d_M, d_Finvc;
     DCsr s_F, s_FT, s_FinvT, s_Finv;   // F, F^T, Finv^T and Finv of the projection
     // work space
@@ -325,13 +340,56 @@ void tdiff_slots(const double* ks, int nb, double c_i, double c_o, std::vector<d
     }
 }
 
+// Gauss order of the tensor rule for the separated general pairs of class cls
+// (1: one to two panel diameters apart, 2: two to eight, 3: farther) when
+// kappa times the largest panel diameter is at most kd: the lowest order whose
+// single- and double-layer local matrices stay within 1e-13 of their largest
+// entry of the 11-point rule's for straight panels at the class's least
+// separation, end-on, side by side or at any angle (the rule's error falls as
+// (separation)^(-2m) and with the oscillation kappa d as (kappa d)^(2m) /
+// (2m)!); ns when no lower order qualifies.
+int far_order(double kd, int cls, int ns) {
+    static const double edge[4] = {0.65, 1.25, 2.5, 5.0};
+    static const int order[4][3] = {{9, 7, 6}, {9, 7, 7}, {9, 8, 8}, {10, 10, 10}};
+    for (int t = 0; t < 4; ++t)
+        if (kd <= edge[t]) return std::min(order[t][cls - 1], ns);
+    return ns;
+}
+
+// The node tables of separation class cls (1 to 3) at Gauss order m < ns,
+// built on first use (one setup launch per table).
+hbem_geometry::FarSet& far_set(hbem_geometry* g, int cls, int m) {
+    auto& slot = g->far[64 * cls + m];
+    if (slot) return *slot;
+    auto f = std::make_unique<hbem_geometry::FarSet>();
+    std::vector<double> xg(m), wg(m);
+    hbem::gauss_rule(0.0, 1.0, m, xg.data(), wg.data());
+    const double* dxg = upload(f->xg, xg, g->st);
+    upload(f->wg, wg, g->st);
+    double* gtab = f->gtab.as<double>((size_t)g->N * m * 5);
+    hbem::launch_setup_gauss_table((const double*)g->d_panels.p, (const double*)g->d_coefs.p, g->N, dxg, m, gtab, g->st);
+    int off = 0;
+    for (int s = 0; s < cls; ++s) off += g->sep[s];
+    const int cnt = g->sep[cls];
+    f->flags.get(std::max(1, cnt));
+    double* ngeo = f->ngeo.as<double>(std::max<size_t>(1, (size_t)cnt * m * m * hbem::kNodeStride));
+    if (cnt > 0)
+        hbem::launch_setup_general((const int*)g->d_gent.p + 3 * (size_t)off, cnt, gtab, m, ngeo,
+                                   (unsigned char*)f->flags.p, g->st);
+    HIPC(hipGetLastError());
+    slot = std::move(f);
+    return *slot;
+}
+
 // Assemble the local matrices of all pairs for wavenumber slots (kap, kr, cv)
 // with imaginary parts (kapi, kri) when any slot is not real positive; returns
 // the local buffer.  Wide real batches run on the paired-lane kernel, the rest
 // on node-parallel groups of lanes per item.  nkt > 0: the slots are in the
 // tdiff layout (tdiff_slots) for nkt wavenumbers, the paired-lane kernel runs
 // whatever the width, and the buffer holds the differences T is formed from,
-// nkt columns wide.
+// nkt columns wide.  Contexts with HBEM_FLAG_FAR_FIELD and real slots: the
+// separated general pairs take the lower Gauss orders of far_order for the
+// batch's largest kappa, each separation class its own launch.
 cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const std::vector<double>& kr,
                       const std::vector<double>& cv, int D, const std::vector<double>& kapi = {},
                       const std::vector<double>& kri = {}, int nkt = 0) {
@@ -373,15 +431,42 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
         const double* ngeo;
         const unsigned char* flags;
         const long long* pidx;
+        int ns;                       // general pairs: order of the tensor rule
+        const double *xg, *wg;
     };
-    Launch ls[3] = {
-        {2, 0, Gg, g->ngen, g->ns * g->ns, (const int*)g->d_gent.p, (const double*)g->d_gngeo.p,
-         (const unsigned char*)g->d_gflags.p, (const long long*)g->d_gpidx.p},
-        {1, 1, Gs, g->nadj, 2 * g->nc * g->nc, (const int*)g->d_aent.p, (const double*)g->d_angeo.p,
-         (const unsigned char*)g->d_aflags.p, (const long long*)g->d_apidx.p},
-        {0, 2, Gs, g->ncoi, g->nc * g->nc, (const int*)g->d_cent.p, (const double*)g->d_cngeo.p,
-         (const unsigned char*)g->d_cflags.p, (const long long*)g->d_cpidx.p},
-    };
+    const double* dxg = (const double*)g->d_xg.p;
+    const double* dwg = (const double*)g->d_wg.p;
+    std::vector<Launch> ls;
+    const int* gent = (const int*)g->d_gent.p;
+    const long long* gpidx = (const long long*)g->d_gpidx.p;
+    const unsigned char* gflags = (const unsigned char*)g->d_gflags.p;
+    const double* gngeo = (const double*)g->d_gngeo.p;
+    if ((g->flags & HBEM_FLAG_FAR_FIELD) && !cplx) {
+        double kmax = 0.0;
+        for (double v : kap) kmax = std::max(kmax, std::fabs(v));
+        int off = 0;
+        for (int s = 0; s < 4; ++s) {
+            const int cnt = g->sep[s];
+            const int m = s == 0 ? g->ns : far_order(kmax * g->dmax, s, g->ns);
+            if (cnt > 0 && m < g->ns) {
+                auto& f = far_set(g, s, m);
+                ls.push_back({2, 0, Gg, cnt, m * m, gent + 3 * (size_t)off, (const double*)f.ngeo.p,
+                              (const unsigned char*)f.flags.p, gpidx + off, m, (const double*)f.xg.p,
+                              (const double*)f.wg.p});
+            } else if (cnt > 0) {
+                ls.push_back({2, 0, Gg, cnt, g->ns * g->ns, gent + 3 * (size_t)off,
+                              gngeo + (size_t)off * g->ns * g->ns * hbem::kNodeStride, gflags + off, gpidx + off, g->ns,
+                              dxg, dwg});
+            }
+            off += cnt;
+        }
+    } else {
+        ls.push_back({2, 0, Gg, g->ngen, g->ns * g->ns, gent, gngeo, gflags, gpidx, g->ns, dxg, dwg});
+    }
+    ls.push_back({1, 1, Gs, g->nadj, 2 * g->nc * g->nc, (const int*)g->d_aent.p, (const double*)g->d_angeo.p,
+                  (const unsigned char*)g->d_aflags.p, (const long long*)g->d_apidx.p, g->ns, dxg, dwg});
+    ls.push_back({0, 2, Gs, g->ncoi, g->nc * g->nc, (const int*)g->d_cent.p, (const double*)g->d_cngeo.p,
+                  (const unsigned char*)g->d_cflags.p, (const long long*)g->d_cpidx.p, g->ns, dxg, dwg});
     for (auto& l : ls) {
         if (l.nent == 0) continue;
         P.nent = l.nent;
@@ -389,6 +474,9 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
         P.ngeo = l.ngeo;
         P.flags = l.flags;
         P.pidx = l.pidx;
+        P.ns = l.ns;
+        P.xg = l.xg;
+        P.wg = l.wg;
         // general pairs: one chunk; singular pairs: chunks of 2048 nodes (paired-lane
         // kernel, 16 staging rounds; for adjacent pairs each of the two node halves
         // is chunked on its own) or 64 x 48 nodes (one wavefront per item)
@@ -593,6 +681,7 @@ int hbem_geometry_create(const double* panels, int npanels, const double* coefs,
     if (!out) throw Error(HBEM_EINVAL, "null output handle");
     if (npanels < 3) throw Error(HBEM_EINVAL, "a closed P1 mesh needs at least 3 panels");
     if (order < 1 || order > 40) throw Error(HBEM_EINVAL, "quadrature order out of range");
+    if (flags & ~(HBEM_FLAG_QUIRK_SLP_COINC_DER | HBEM_FLAG_FAR_FIELD)) throw Error(HBEM_EINVAL, "unknown geometry flags");
     for (int i = 0; i < npanels; ++i) {
         int t = (int)panels[8 * i];
         if (t < 0 || t > 3 || panels[8 * i] != (double)t) throw Error(HBEM_EINVAL, "invalid panel type");
@@ -632,8 +721,21 @@ int hbem_geometry_create(const double* panels, int npanels, const double* coefs,
     gent.reserve(3 * (size_t)g->npairs);
     gpi.reserve((size_t)g->npairs);
     const double* cptr = g->coefs.empty() ? nullptr : g->coefs.data();
-    std::vector<double> ends(4 * (size_t)N);
+    std::vector<double> ends(4 * (size_t)N), disc(3 * (size_t)N);
     hbem::panel_endpoints(g->panels.data(), cptr, N, ends.data());
+    hbem::panel_bounds(g->panels.data(), cptr, N, disc.data());
+    for (int i = 0; i < N; ++i) g->dmax = std::max(g->dmax, 2.0 * disc[3 * (size_t)i + 2]);
+    // general pairs by separation eta = (distance of the discs) / (larger
+    // diameter), a lower bound of the panels' distance in diameters
+    std::vector<int> sent[4];
+    std::vector<long long> spi[4];
+    auto separation = [&](int i, int j) {
+        if ((int)g->panels[8 * (size_t)i] == 3 || (int)g->panels[8 * (size_t)j] == 3) return 0;
+        const double* a = &disc[3 * (size_t)i];
+        const double* b = &disc[3 * (size_t)j];
+        const double eta = (std::hypot(a[0] - b[0], a[1] - b[1]) - a[2] - b[2]) / (2.0 * std::max(a[2], b[2]));
+        return eta >= 8.0 ? 3 : (eta >= 2.0 ? 2 : (eta >= 1.0 ? 1 : 0));
+    };
     const double tol = 100.0 * hbem::kDblEps;
     for (int i = 0; i < N; ++i)
         for (int j = 0; j <= i; ++j) {
@@ -645,8 +747,18 @@ int hbem_geometry_create(const double* panels, int npanels, const double* coefs,
                 aent.push_back(i); aent.push_back(j); aent.push_back(0); api.push_back(p);
             } else if (std::hypot(ei[0] - ej[2], ei[1] - ej[3]) < tol) {          // start(i) == end(j)
                 aent.push_back(i); aent.push_back(j); aent.push_back(1); api.push_back(p);
-            } else { gent.push_back(i); gent.push_back(j); gent.push_back(0); gpi.push_back(p); }
+            } else {
+                const int s = separation(i, j);
+                sent[s].push_back(i); sent[s].push_back(j); sent[s].push_back(0); spi[s].push_back(p);
+            }
         }
+    for (int s = 0; s < 4; ++s) {
+        g->sep[s] = (int)spi[s].size();
+        gent.insert(gent.end(), sent[s].begin(), sent[s].end());
+        gpi.insert(gpi.end(), spi[s].begin(), spi[s].end());
+        std::vector<int>().swap(sent[s]);
+        std::vector<long long>().swap(spi[s]);
+    }
     g->ngen = (int)gpi.size();
     g->nadj = (int)api.size();
     g->ncoi = (int)cpi.size();
@@ -1782,6 +1894,8 @@ int hbem_sigma_sweep_multi(const hbem_geometry_t* geoms, int ninst, const double
         if (!G[i] || !R[i]) throw Error(HBEM_EINVAL, "null geometry or test matrix");
         if (G[i]->ns != G[0]->ns || G[i]->proj != G[0]->proj || G[i]->flags != G[0]->flags || G[i]->dev != G[0]->dev)
             throw Error(HBEM_EINVAL, "instances must share quadrature order, projection, flags and device");
+        if (G[i]->flags & HBEM_FLAG_FAR_FIELD)
+            throw Error(HBEM_EINVAL, "the multi-instance sweep takes contexts without HBEM_FLAG_FAR_FIELD");
         check_dense_args(ks, 2 * G[i]->N, R[i], nc, q);
         for (int j = 0; j < nk; ++j) { check_k(ks[(size_t)i * nk + j], 0.0, c_i[i]); check_k(ks[(size_t)i * nk + j], 0.0, c_o); }
     }

@@ -179,4 +179,10 @@ void invert_projection_factor(const double* F, const int* tr, int n, double* Fin
 // (operator()(-1 / +1), galerkin_builder.cpp:189-201): 4 doubles per panel
 void panel_endpoints(const double* panels, const double* coefs, int N, double* ends);
 
+// A disc holding each panel: cr[3i], cr[3i + 1] its centre (the panel's point
+// at s = 1/2), cr[3i + 2] its radius (half the length of a line panel; for a
+// curved panel the largest distance of 65 equally spaced points from the
+// centre, plus 2%)
+void panel_bounds(const double* panels, const double* coefs, int N, double* cr);
+
 }  // namespace hbem

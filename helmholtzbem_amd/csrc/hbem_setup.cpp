@@ -151,6 +151,23 @@ void panel_endpoints(const double* panels, const double* coefs, int N, double* e
     }
 }
 
+void panel_bounds(const double* panels, const double* coefs, int N, double* cr) {
+    for (int i = 0; i < N; ++i) {
+        const double* p = panels + 8 * i;
+        const std::complex<double> c = host_point(p, coefs, 0.5);
+        double rad = 0.0;
+        if ((int)p[0] == 0) {
+            rad = 0.5 * std::hypot(p[3] - p[1], p[4] - p[2]);
+        } else {
+            for (int s = 0; s <= 64; ++s) rad = std::max(rad, std::abs(host_point(p, coefs, s / 64.0) - c));
+            rad *= 1.02;
+        }
+        cr[3 * i] = c.real();
+        cr[3 * i + 1] = c.imag();
+        cr[3 * i + 2] = rad;
+    }
+}
+
 // mass_matrix::GalerkinMatrix with the rule chosen by SolutionsOperator
 // (mass_matrix.cpp:6-47, gen_sol_op.cpp:30-33).  P1: dof(0,p) = (p+1)%N, dof(1,p) = p;
 // shapes b0 = t, b1 = 1 - t; |gamma'| = sqrt(x^2 + y^2).

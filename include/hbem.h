@@ -56,6 +56,18 @@ extern "C" {
  * entries).  Off by default: the default is the symmetric copy of the same
  * derivative order, which is what the k-derivative of V requires. */
 #define HBEM_FLAG_QUIRK_SLP_COINC_DER 1
+/* Lower-order quadrature on separated panel pairs.  Every assembly through the
+ * context at real positive wavenumbers (matrices, T, sweeps) integrates the
+ * general panel pairs whose bounding discs lie at least one panel diameter
+ * apart with a tensor Gauss rule of 6-10 points a side instead of the
+ * quadrature order, the order chosen per batch from the separation class (one
+ * to two diameters, two to eight, more) and kappa times the largest panel
+ * diameter so that each pair's local matrices stay within 1e-13 of their
+ * largest entry of the 11-point rule's.  Pairs on spline panels, complex
+ * wavenumbers, and kappa d > 5 keep the full rule.  Off by default: the
+ * reference integrates every general pair with the same rule
+ * (galerkin_builder.cpp:308-327).  Not accepted by hbem_sigma_sweep_multi. */
+#define HBEM_FLAG_FAR_FIELD 2
 
 typedef struct hbem_geometry* hbem_geometry_t;
 
@@ -241,9 +253,10 @@ int hbem_sigma_sweep_der2(hbem_geometry_t g, const double* ks, int nk, double c_
  * = 0, the k is recomputed with the FP64 factorisation and fp64_used[i] (may
  * be NULL) is set to 1; so is it when the refinement of an earlier solve does
  * not contract (its last correction above half the one before it, or above
- * half the unrefined solution after one step).  The assembly stays FP64.  No
- * counterpart in the
- * reference, which factorises in FP64 only.
+ * half the unrefined solution after one step).  The assembly stays FP64 (on
+ * a context created with HBEM_FLAG_FAR_FIELD, with its lower-order rules on
+ * separated pairs).  No counterpart in the reference, which factorises in
+ * FP64 only.
  */
 int hbem_sigma_sweep_mixed(hbem_geometry_t g, const double* ks, int nk, double c_o, double c_i, const void* R, int nc,
                            int q, int refine, int refine_inner, double* sigma, int* fp64_used);

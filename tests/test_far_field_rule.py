@@ -1,0 +1,85 @@
+"""The order table of the lower-order rules on separated panel pairs
+(HBEM_FLAG_FAR_FIELD), checked independently of the library with scipy's
+Hankel functions: for two straight panels of diameter d at the least
+separation of each class -- one diameter (class 1), two (class 2) and eight
+(class 3) between their bounding discs, end-on, side by side and at random
+angles -- and kappa d at both edges of each tier, the tensor Gauss
+rule of the table's order gives single- and double-layer local matrices
+within 1e-13 of their largest entry of the 11-point rule's.  One order lower
+fails the bound at the first tier's upper edge, so the table's orders are
+not loose there.
+"""
+import numpy as np
+import pytest
+from scipy.special import hankel1, roots_legendre
+
+# (upper edge of kappa d, orders for classes 1, 2, 3) as the library's far_order
+TABLE = ((0.65, 9, 7, 6), (1.25, 9, 7, 7), (2.5, 9, 8, 8), (5.0, 10, 10, 10))
+CLASSES = [(1, 1.0), (2, 2.0), (3, 8.0)]
+FULL = 11
+
+
+def rule(n):
+    x, w = roots_legendre(n)
+    return 0.5 * (x + 1.0), 0.5 * w
+
+
+def local(a0, a1, b0, b1, n, kap):
+    """P1 single- and double-layer local matrices of panels [a0, a1] (test) and
+    [b0, b1] (trial) with the n x n tensor Gauss rule."""
+    s, w = rule(n)
+    ha, hb = np.linalg.norm(a1 - a0), np.linalg.norm(b1 - b0)
+    pa = a0[None, :] + s[:, None] * (a1 - a0)[None, :]
+    pb = b0[None, :] + s[:, None] * (b1 - b0)[None, :]
+    v = pa[:, None, :] - pb[None, :, :]
+    r = np.linalg.norm(v, axis=2)
+    nb = np.array([(b1 - b0)[1], -(b1 - b0)[0]]) / hb
+    ww = w[:, None] * w[None, :] * ha * hb
+    sh = np.stack([s, 1.0 - s])
+    V = sh @ (hankel1(0, kap * r) * ww) @ sh.T
+    K = sh @ (hankel1(1, kap * r) / r * (v @ nb) * ww) @ sh.T
+    return V, K
+
+
+def configs(eta, rng):
+    """Panel pairs of diameter 1 whose bounding discs are eta diameters apart."""
+    a0, a1 = np.array([-0.5, 0.0]), np.array([0.5, 0.0])
+    dist = 1.0 + eta                      # centre distance: eta + two radii
+    out = [(a0, a1, a0 + [dist, 0.0], a1 + [dist, 0.0]),          # end-on
+           (a0, a1, a0 + [0.0, dist], a1 + [0.0, dist])]          # side by side
+    for _ in range(6):
+        phi, psi = rng.uniform(0, 2 * np.pi, 2)
+        c = dist * np.array([np.cos(phi), np.sin(phi)])
+        u = 0.5 * np.array([np.cos(psi), np.sin(psi)])
+        out.append((a0, a1, c - u, c + u))
+    return out
+
+
+def worst(order, eta, kd, rng):
+    err = 0.0
+    for a0, a1, b0, b1 in configs(eta, rng):
+        Vf, Kf = local(a0, a1, b0, b1, FULL, kd)
+        Vm, Km = local(a0, a1, b0, b1, order, kd)
+        err = max(err, np.abs(Vm - Vf).max() / np.abs(Vf).max())
+        if np.abs(Kf).max() > 1e-8 * np.abs(Vf).max():
+            err = max(err, np.abs(Km - Kf).max() / np.abs(Kf).max())
+    return err
+
+
+@pytest.mark.parametrize("tier", range(len(TABLE)))
+@pytest.mark.parametrize("cls,eta", CLASSES)
+@pytest.mark.parametrize("edge", ["lower", "upper"])
+def test_table_order_meets_bound(tier, cls, eta, edge):
+    kd = TABLE[tier][0]
+    if edge == "lower":
+        kd = TABLE[tier - 1][0] if tier else 0.05
+    m = TABLE[tier][cls]
+    e = worst(m, eta, kd, np.random.default_rng(10 * tier + cls))
+    assert e <= 1e-13, (kd, cls, m, e)
+
+
+@pytest.mark.parametrize("cls,eta", CLASSES)
+def test_one_order_lower_fails_at_the_first_tier(cls, eta):
+    kd = TABLE[0][0]
+    m = TABLE[0][cls] - 1
+    assert worst(m, eta, kd, np.random.default_rng(3)) > 1e-13

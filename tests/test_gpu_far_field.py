@@ -1,0 +1,113 @@
+"""Lower-order Gauss rules on separated panel pairs (HBEM_FLAG_FAR_FIELD).
+
+Bars, per assertion:
+* the assembled K, W, V and their k-derivatives (d = 0, 1, 2) of a context
+  with the flag within 1e-12 relative Frobenius of the oracle's full-order
+  matrices -- the plain path's matrix bar -- on a polygon, a trigonometric
+  star and a circle of arcs, at kappa d in each tier of the order table
+  (d the largest panel diameter), and measurably different from the plain
+  context's (the lower-order rules ran);
+* at kappa d > 5 every pair keeps the full rule: the matrices equal the plain
+  context's bit for bit;
+* sigma_min of the FP64 sweep within 1e-10, and of the mixed-precision sweep
+  within 1e-8, of the plain context's FP64 sweep;
+* the multi-instance sweep rejects such contexts, and unknown flags are
+  rejected.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ORDER = 11
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def dmax(mesh):
+    """Largest panel chord (the panel diameter of a line panel; panels take
+    their parameter s in [0, 1] through p[s])."""
+    out = 0.0
+    for p in mesh.getPanels():
+        a, b = np.array(p[0.0], dtype=float), np.array(p[1.0], dtype=float)
+        out = max(out, float(np.hypot(*(b - a))))
+    return out
+
+
+GEOMS = {
+    "polygon120": lambda H: H.random_polygon_mesh(2, 9, 120),
+    "star96": lambda H: H.star_mesh(96),
+    "circle64": lambda H: H.circle_mesh(64, 0.7, (0.2, -0.1)),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(GEOMS))
+def pair(request, H):
+    mesh = GEOMS[request.param](H)
+    sp = H.ContinuousSpace1()
+    plain = H.BuilderData(mesh, sp, sp, ORDER)
+    far = H.BuilderData(mesh, sp, sp, ORDER, flags=H.FLAG_FAR_FIELD)
+    return request.param, mesh, plain, far
+
+
+@pytest.mark.parametrize("kd", [0.9, 1.9, 3.8])
+def test_far_matrices_match_oracle(pair, kd, H, O):
+    name, mesh, plain, far = pair
+    c = 20.0
+    k = kd / (dmax(mesh) * np.sqrt(c))
+    gf, gp = H.GalerkinBuilder(far), H.GalerkinBuilder(plain)
+    gf.assembleDense(k, c, H.LayerType.ALL, 2)
+    gp.assembleDense(k, c, H.LayerType.ALL, 2)
+    K, W, V = O.assemble(mesh, ORDER, k, c, d=2)
+    diff = 0.0
+    for t in range(3):
+        for get, ref, lab in ((gf.getDoubleLayer, K, "K"), (gf.getHypersingular, W, "W"), (gf.getSingleLayer, V, "V")):
+            assert rel(get(t), ref[t]) <= 1e-12, (name, kd, lab, t, rel(get(t), ref[t]))
+        diff = max(diff, rel(gf.getSingleLayer(t), gp.getSingleLayer(t)))
+    print(name, "kappa d", kd, "far vs full rule", diff, flush=True)
+    assert 0.0 < diff <= 1e-12, (name, kd, diff)
+
+
+def test_full_rule_beyond_the_table(pair, H):
+    name, mesh, plain, far = pair
+    c = 20.0
+    k = 8.0 / (dmax(mesh) * np.sqrt(c))
+    gf, gp = H.GalerkinBuilder(far), H.GalerkinBuilder(plain)
+    gf.assembleDense(k, c, H.LayerType.ALL, 1)
+    gp.assembleDense(k, c, H.LayerType.ALL, 1)
+    for t in range(2):
+        assert np.array_equal(gf.getDoubleLayer(t), gp.getDoubleLayer(t)), (name, t)
+        assert np.array_equal(gf.getHypersingular(t), gp.getHypersingular(t)), (name, t)
+        assert np.array_equal(gf.getSingleLayer(t), gp.getSingleLayer(t)), (name, t)
+
+
+def test_far_sweeps_match_plain_fp64(H):
+    """A 300-panel polygon, 64 k in [2, 12], c_i = 20: the far context's FP64
+    and mixed sweeps against the plain context's FP64 sweep."""
+    mesh = H.random_polygon_mesh(5, 11, 300)
+    sp = H.ContinuousSpace1()
+    plain = H.BuilderData(mesh, sp, sp, ORDER)
+    far = H.BuilderData(mesh, sp, sp, ORDER, flags=H.FLAG_FAR_FIELD)
+    ks = np.linspace(2.0, 12.0, 64)
+    R = H.gaussian_test_matrix(600, 2, 7)
+    ref = H.sigma_sweep(plain, ks, 1.0, 20.0, R, 2)
+    s64 = H.sigma_sweep(far, ks, 1.0, 20.0, R, 2)
+    smx, used = H.sigma_sweep_mixed(far, ks, 1.0, 20.0, R, 2)
+    e64 = np.max(np.abs(s64 - ref) / ref)
+    emx = np.max(np.abs(smx - ref) / ref)
+    print("far FP64 vs plain", e64, "far mixed vs plain", emx, "fallbacks", int(used.sum()), flush=True)
+    assert e64 <= 1e-10
+    assert emx <= 1e-8
+
+
+def test_multi_rejects_far_contexts_and_unknown_flags(H):
+    mesh = H.random_polygon_mesh(3, 7, 40)
+    sp = H.ContinuousSpace1()
+    far = H.BuilderData(mesh, sp, sp, ORDER, flags=H.FLAG_FAR_FIELD)
+    ks = np.linspace(2.0, 5.0, 32)
+    with pytest.raises(H.HbemError, match="FAR_FIELD"):
+        H.sigma_sweep_batched([(far, ks, 1.0, 5.0, H.gaussian_test_matrix(80, 2, 1))], 2)
+    with pytest.raises(H.HbemError, match="unknown geometry flags"):
+        H.BuilderData(mesh, sp, sp, ORDER, flags=4)
