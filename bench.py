@@ -31,13 +31,21 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
+PEAK_FP32_MFMA_TFLOPS = 157.3   # MI355X dense FP32 on the matrix cores
 FLOP_PER_NODE = {0: 292.0, 1: 376.0, 2: 498.0}   # the library's algorithmic model (hbem_profile_query)
 
 # HBM bytes per launch of the general-pair assembly kernel (rocprofv3
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, KiB per dispatch, scaled by the
 # counters' reading of a plain 1 GiB copy): profiles/k_assemble_x2_general_*.txt
 TRAFFIC_BYTES = {"star400": 1.7076e10, "polygon2000": 8.2862e10}
-TRAFFIC_BYTES_FAR = {}
+# ... on a context with the lower-order rules on separated pairs: the four
+# launches of a sweep (one per separation class) averaged, 7.066e10 B per sweep
+# (profiles/k_cgemm32_sub_polygon2000_far_traffic.txt)
+TRAFFIC_BYTES_FAR = {"polygon2000": 1.7665e10}
+# HBM bytes per launch of the LU's matrix-core updates, averaged over the 217
+# launches of one polygon2000 sweep (the same counters and factors; 4.268e11 B
+# per sweep against 3.900e11 B of algorithmic C, A and B traffic)
+GEMM_TRAFFIC_BYTES = {("polygon2000", "f32"): 1.9666e9}
 # FP64 FLOP the general-pair kernel executes per quadrature node (rocprofv3
 # SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 x 64 lanes, FMA twice, over the nodes of
 # the launch), next to the 292 / 376 / 498 of the algorithmic model at d = 0/1/2:
@@ -133,6 +141,7 @@ def run_workload(H, name, steps, warmup, mixed=False, bd=None, ref_sigma=None, f
         res = sweep()            # synchronous: returns after its stream drained
     elapsed = time.perf_counter() - t0
     prof = [H.profile_query(bd, i) for i in range(len(PHASES))]
+    ms_mm, n_mm, fl_mm = H.profile_query(bd, 6)       # the LU's matrix-core launches
     H.profile_enable(bd, False)
     sig = res[0] if der else res
     if not (np.all(np.isfinite(sig)) and np.all(sig > 0)):
@@ -159,15 +168,30 @@ def run_workload(H, name, steps, warmup, mixed=False, bd=None, ref_sigma=None, f
         "panel_pair_evals_per_s": pairs_per_k * evals / asm_s if asm_s > 0 else None,
         "phase_seconds_per_step": {ph: p[0] * 1e-3 / steps for ph, p in zip(PHASES, prof)},
         "lu_model_tflops": lu_tflops,
-        "roofline": {"kernel": "k_assemble_x2<general, d=%d> (general panel pairs, %d k x 2 c per launch%s)"
+        "roofline_assembly": {"kernel": "k_assemble_x2<general, d=%d> (general panel pairs, %d k x 2 c per launch%s)"
                                % (der, w["nk"], "; one launch per separation class, Gauss orders 11 / 9 / 7 / 6" if far else ""),
                      "bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": (TRAFFIC_BYTES_FAR if far else TRAFFIC_BYTES).get(name),
                      "avg_launch_ms": avg_launch_s * 1e3, "launches": n_gen,
                      "flops_per_launch": fl_gen / max(n_gen, 1),
-                     "executed_tflops": executed, "executed_frac": executed / PEAK_FP64_TFLOPS if executed else None},
+                     "executed_tflops": executed, "executed_frac": executed / PEAK_FP64_TFLOPS if executed else None,
+                     "seconds_per_step": ms_gen * 1e-3 / steps},
     }
+    if n_mm:
+        f32 = mixed
+        peak = PEAK_FP32_MFMA_TFLOPS if f32 else PEAK_FP64_TFLOPS
+        ach = fl_mm / (ms_mm * 1e-3) / 1e12
+        out["roofline_lu_gemm"] = {
+            "kernel": ("k_cgemm32_sub (FP32 3M complex updates of the LU, v_mfma_f32_16x16x4f32)" if f32 else
+                       "k_zgemm (FP64 3M complex updates of the LU, v_mfma_f64_16x16x4f64)"),
+            "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+            "traffic": GEMM_TRAFFIC_BYTES.get((name, "f32" if f32 else "f64")),
+            "avg_launch_ms": ms_mm / n_mm, "launches": n_mm, "flops_per_launch": fl_mm / n_mm,
+            "seconds_per_step": ms_mm * 1e-3 / steps}
+    # the roofline of the kernel with the larger share of the step
+    cands = [v for v in (out["roofline_assembly"], out.get("roofline_lu_gemm")) if v]
+    out["roofline"] = max(cands, key=lambda v: v["seconds_per_step"])
     if far:
         out["far_field"] = "general pairs closer than one panel diameter at Gauss order 11, one to two diameters " \
                            "apart at 9, two to eight at 7, farther at 6 (kappa d <= 0.65 over the grid)"
@@ -281,7 +305,10 @@ def main():
                     default="polygon2000")
     ap.add_argument("--no-second", action="store_true", help="skip the star400 sweep")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision sweep")
+    ap.add_argument("--legs", default="fp64,mixed,mixed_far",
+                    help="sweeps to time: fp64 (the plain path), mixed (FP32 LU + FP64 refinement), mixed_far "
+                         "(the same on a context with the lower-order rules on separated pairs); the headline is "
+                         "the fastest within the 1e-8 bar of the fp64 leg")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 times), the reference "
@@ -310,17 +337,29 @@ def main():
     if args.workload == "resonance1000":
         print(json.dumps(run_resonance(H, args.steps, args.warmup)), flush=True)
         return
-    fp64, mesh, R = run_workload(H, args.workload, args.steps, args.warmup)
-    head = fp64
-    mixed = mixed_far = None
-    if WORKLOADS[args.workload]["der"] == 0 and not args.no_mixed:
-        mixed, _, _ = run_workload(H, args.workload, args.steps, args.warmup, mixed=True, bd=fp64["bd"],
-                                   ref_sigma=fp64["sigma"])
+    legs = [x for x in args.legs.split(",") if x]
+    if not legs or any(x not in ("fp64", "mixed", "mixed_far") for x in legs):
+        ap.error("--legs takes a comma list of fp64, mixed, mixed_far")
+    if WORKLOADS[args.workload]["der"] != 0:
+        legs = ["fp64"]          # the mixed dense stage computes sigma only
+    fp64 = mixed = mixed_far = None
+    mesh = R = None
+    if "fp64" in legs:
+        fp64, mesh, R = run_workload(H, args.workload, args.steps, args.warmup)
+    ref = fp64["sigma"] if fp64 else None
+    if "mixed" in legs:
+        mixed, mesh, R = run_workload(H, args.workload, args.steps, args.warmup, mixed=True,
+                                      bd=fp64["bd"] if fp64 else None, ref_sigma=ref)
+        mixed["bd"].release_work()
+    if fp64:
         fp64["bd"].release_work()
-        mixed_far, _, _ = run_workload(H, args.workload, args.steps, args.warmup, mixed=True, far=True,
-                                       ref_sigma=fp64["sigma"])
+    if "mixed_far" in legs:
+        mixed_far, mesh, R = run_workload(H, args.workload, args.steps, args.warmup, mixed=True, far=True,
+                                          ref_sigma=ref)
         mixed_far["bd"].release_work()
-        for cand in (mixed, mixed_far):
+    head = fp64 or mixed or mixed_far
+    for cand in (mixed, mixed_far):
+        if cand is not None and cand is not head and fp64 is not None:
             if cand["max_rel_sigma_err_vs_fp64"] <= MIXED_TOL and cand["value"] > head["value"]:
                 head = cand
     out = {
@@ -345,11 +384,11 @@ def main():
         "path": "mixed_far" if head is mixed_far else ("mixed" if head is mixed else "fp64"),
     }
     strip = lambda d: {k: v for k, v in d.items() if k not in ("sigma", "bd")}
-    out["fp64"] = strip(fp64)
-    for key, leg in (("mixed", mixed), ("mixed_far", mixed_far)):
+    for key, leg in (("fp64", fp64), ("mixed", mixed), ("mixed_far", mixed_far)):
         if leg is not None:
             out[key] = strip(leg)
-            out[key]["speedup_vs_fp64"] = leg["value"] / fp64["value"]
+            if fp64 is not None and leg is not fp64:
+                out[key]["speedup_vs_fp64"] = leg["value"] / fp64["value"]
     other = "star400" if args.workload == "polygon2000" else "polygon2000"
     if not args.no_second:
         sec, _, _ = run_workload(H, other, max(1, args.steps), 1)

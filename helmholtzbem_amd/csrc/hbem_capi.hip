@@ -133,8 +133,19 @@ struct EventLog {
     }
 };
 
+// The LU's matrix-core launches recorded into an event log slot
+struct GemmTimer : hbem::LaunchTimer {
+    EventLog* log;
+    int slot;
+    hipEvent_t e = nullptr;
+    GemmTimer(EventLog* l, int s) : log(l), slot(s) {}
+    void begin(hipStream_t st) override { e = log->begin(st); }
+    void end(hipStream_t st, double flops) override { log->end(st, slot, e, flops); }
+};
+
 // Profile slots of a geometry context (hbem_profile_query)
-enum { kProfGeneral = 0, kProfAdjacent = 1, kProfCoinciding = 2, kProfLU = 3, kProfFormT = 4, kProfRsvd = 5 };
+enum { kProfGeneral = 0, kProfAdjacent = 1, kProfCoinciding = 2, kProfLU = 3, kProfFormT = 4, kProfRsvd = 5,
+       kProfGemm = 6 };
 // randomized_svd timer slots (randsvd.cpp:27-47): LU, first solve + QR, subspace iterations, last solve + SVD
 enum { kTimLU = 0, kTimQR = 1, kTimIter = 2, kTimSVD = 3 };
 
@@ -1464,10 +1475,11 @@ void sweep_run(hbem_geometry* g, const double* ks, int nk, double c_o, double c_
         int* piv = g->w_piv.as<int>((size_t)n * nb);
         int* perm = g->w_perm.as<int>((size_t)n * nb);
         if (g->prof) e0 = g->plog.begin(g->st);
+        GemmTimer gt(&g->plog, kProfGemm);
         if (mx) {
-            hbem::lu_factor_batched_f32(T32, n, nb, piv, perm, g->st);
+            hbem::lu_factor_batched_f32(T32, n, nb, piv, perm, g->st, g->prof ? &gt : nullptr);
         } else {
-            hbem::lu_factor_batched(T0, n, nb, piv, perm, g->st);
+            hbem::lu_factor_batched(T0, n, nb, piv, perm, g->st, g->prof ? &gt : nullptr);
         }
         if (g->prof) {
             g->plog.end(g->st, kProfLU, e0, (8.0 / 3.0) * (double)n * n * n * nb);
@@ -1950,7 +1962,7 @@ int hbem_profile_enable(hbem_geometry_t g, int enable) {
 
 int hbem_profile_query(hbem_geometry_t g, int which, double* ms, long long* launches, double* flops) {
     API_BEGIN
-    if (!g || which < 0 || which > kProfRsvd) throw Error(HBEM_EINVAL, "invalid argument");
+    if (!g || which < 0 || which > kProfGemm) throw Error(HBEM_EINVAL, "invalid argument");
     std::lock_guard<std::mutex> lk(g->mu);
     g->plog.resolve();
     if (ms) *ms = g->plog.ms[which];

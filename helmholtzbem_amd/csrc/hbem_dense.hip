@@ -719,11 +719,15 @@ static void gemm_sub(int m, int n, int k, const cflt* A, const cflt* B, cflt* C,
                        batch);
 }
 
-// Blocked right-looking LU in steps of S 32-column sub-panels (S = 2, or 4
-// above order 2048).  Within a step, sub-panel s is first brought up to date
-// with the step's earlier sub-panels (k = 32 s), factorised, its row
-// interchanges applied to every other column, and its U rows right of it
-// formed; the trailing matrix then receives one k = 32 S update per step.
+// Blocked right-looking LU in steps of S 32-column sub-panels (S = 2; above
+// order 2048, 8 for FP64 factors and 4 for FP32 ones: at order 4000 the
+// FP64 LU took 391 ms at S = 8 against 412 at S = 4 for 128 matrices, while
+// the FP32 LU at S = 8 left the mixed-precision sigma ten times farther from
+// FP64 -- 2e-9 against 2e-10 -- for 14 of its 230 ms).  Within a step,
+// sub-panel s is first brought up to date with the step's earlier sub-panels
+// (k = 32 s), factorised, its row interchanges applied to every other column,
+// and its U rows right of it formed; the trailing matrix then receives one
+// k = 32 S update per step.
 // That update reads and writes the trailing matrix once per step -- the
 // traffic that bounds it at these sizes -- so wider steps cut it, at the cost
 // of more small sub-panel products:
@@ -733,13 +737,19 @@ static void gemm_sub(int m, int n, int k, const cflt* A, const cflt* B, cflt* C,
 //           Us = Lss^{-1} A[js:js+32, js+32:]
 //   then:   A[jE:, jE:] -= L[jE:, j0:jE] U[j0:jE, jE:]
 // C = cdbl: the FP64 factorisation; C = cflt: the same steps in FP32.  The
-// step width depends on n only, so a matrix gives the same bits in any batch.
+// step width depends on n and the precision only, so a matrix gives the same
+// bits in any batch.
 template <class C>
-static void lu_factor_impl(C* A, int n, int batch, int* ipiv, int* perm, hipStream_t st) {
+static void lu_factor_impl(C* A, int n, int batch, int* ipiv, int* perm, hipStream_t st, LaunchTimer* tm) {
     const int sb = 32;
-    const int S = n > 2048 ? 4 : 2;
+    const int S = n > 2048 ? (sizeof(C) == sizeof(cdbl) ? 8 : 4) : 2;
     const long long s = (long long)n * n;
     auto at = [&](int r, int c) { return A + r + (size_t)c * n; };
+    auto update = [&](int m, int nn, int k, const C* a, const C* b, C* c) {
+        if (tm) tm->begin(st);
+        gemm_sub(m, nn, k, a, b, c, n, s, batch, st);
+        if (tm) tm->end(st, 6.0 * m * nn * k * batch);
+    };
     auto rows = [&](int mode, int j0, int jb) {
         const int nout = n - jb;
         if (nout <= 0) return;
@@ -768,25 +778,25 @@ static void lu_factor_impl(C* A, int n, int batch, int* ipiv, int* perm, hipStre
                 rows(3, js, jb);
                 continue;
             }
-            gemm_sub(n - js, jb, js - j0, at(js, j0), at(j0, js), at(js, js), n, s, batch, st);
+            update(n - js, jb, js - j0, at(js, j0), at(j0, js), at(js, js));
             panel(js, jb);
             rows(1, js, jb);
             if (n - je > 0) {
-                gemm_sub(jb, n - je, js - j0, at(js, j0), at(j0, je), at(js, je), n, s, batch, st);
+                update(jb, n - je, js - j0, at(js, j0), at(j0, je), at(js, je));
                 rows(2, js, jb);
             }
         }
-        if (n - jE > 0) gemm_sub(n - jE, n - jE, jE - j0, at(jE, j0), at(j0, jE), at(jE, jE), n, s, batch, st);
+        if (n - jE > 0) update(n - jE, n - jE, jE - j0, at(jE, j0), at(j0, jE), at(jE, jE));
     }
     hipLaunchKernelGGL(k_lu_perm, dim3(batch), dim3(64), 0, st, n, ipiv, perm);
 }
 
-void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStream_t st) {
-    lu_factor_impl(A, n, batch, ipiv, perm, st);
+void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStream_t st, LaunchTimer* gemm) {
+    lu_factor_impl(A, n, batch, ipiv, perm, st, gemm);
 }
 
-void lu_factor_batched_f32(cflt* A, int n, int batch, int* ipiv, int* perm, hipStream_t st) {
-    lu_factor_impl(A, n, batch, ipiv, perm, st);
+void lu_factor_batched_f32(cflt* A, int n, int batch, int* ipiv, int* perm, hipStream_t st, LaunchTimer* gemm) {
+    lu_factor_impl(A, n, batch, ipiv, perm, st, gemm);
 }
 
 // ---------------------------------------------------------------------------

@@ -81,7 +81,16 @@ void launch_form_Ao(const cdbl* L, int nk, int nslot, int o, int N, const double
 
 // dense (hbem_dense.hip); all batched over `batch` matrices of order n with
 // stride n*n (column-major, lda = n) unless noted.
-void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStream_t st);
+// Optional timing of the LU's matrix-core launches (the trailing and
+// sub-panel updates C -= A B): begin before and end after each launch, with
+// the real FLOP its 3M complex product performs (three real m x n x k
+// products, 6 m n k per matrix).
+struct LaunchTimer {
+    virtual void begin(hipStream_t st) = 0;
+    virtual void end(hipStream_t st, double flops) = 0;
+    virtual ~LaunchTimer() = default;
+};
+void lu_factor_batched(cdbl* A, int n, int batch, int* ipiv, int* perm, hipStream_t st, LaunchTimer* gemm = nullptr);
 // X (n x nc, stride n*nc per matrix) <- op(A)^{-1} X with A = P^T L U from
 // lu_factor_batched (factors of matrix b at LU + b * ms); adj = false solves
 // A X = B, adj = true solves A^H X = B.  work: n * nc * batch.
@@ -89,7 +98,8 @@ void lu_solve_thin_batched(const cdbl* LU, long long ms, const int* perm, int n,
                            cdbl* work, hipStream_t st);
 // The same blocked LU in single precision (the mixed-precision sweep), its
 // thin solves (FP32 factors read, FP64 arithmetic), and the FP32 copy of T.
-void lu_factor_batched_f32(cflt* A, int n, int batch, int* ipiv, int* perm, hipStream_t st);
+void lu_factor_batched_f32(cflt* A, int n, int batch, int* ipiv, int* perm, hipStream_t st,
+                           LaunchTimer* gemm = nullptr);
 void lu_solve_thin_batched_f32(const cflt* LU, long long ms, const int* perm, int n, int batch, cdbl* X, int nc,
                                bool adj, cdbl* work, hipStream_t st);
 // R = Bv - op(T) Z (op = T or T^H) for n x nc blocks of a batch, T (n x n,

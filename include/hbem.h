@@ -284,8 +284,11 @@ int hbem_sigma_sweep_multi(const hbem_geometry_t* geoms, int ninst, const double
  * returns the accumulated milliseconds and launch count of phase `which`
  * (0 = general-pair assembly kernel, 1 = adjacent-pair kernel, 2 =
  * coinciding-pair kernel, 3 = LU factorisation, 4 = T formation, 5 = the
- * randomized SVD stage after the LU) and the algorithmic FP64 flop count of
- * the frozen model (0 where none is defined). */
+ * randomized SVD stage after the LU, 6 = the LU's matrix-core launches, its
+ * trailing and sub-panel updates) and the algorithmic flop count (slots 0-2:
+ * the frozen FP64 model of the assembly; 3: 8 n^3 / 3 per matrix; 6: the 3M
+ * products' 6 m n k per update and matrix, in the precision of the
+ * factorisation; 0 where none is defined). */
 int hbem_profile_enable(hbem_geometry_t g, int enable);
 int hbem_profile_query(hbem_geometry_t g, int which, double* ms, long long* launches, double* flops);
 
