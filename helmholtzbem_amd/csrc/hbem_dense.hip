@@ -884,19 +884,22 @@ __device__ __forceinline__ void diag_solve(int n, int r0, int bs, bool lower, bo
 }
 
 // Right-looking pass with the column-major factor: L (lower, unit) forward or
-// U (upper) backward.  After block b, rows outside it (below / above) take
-// w_i -= sum_j M(i, r0 + j) x_j: a wavefront covers RW rows x 64 / RW groups
+// U (upper) backward, over the rows [pb, pe) (the whole matrix, or one panel
+// of the panelled form).  After block b, the rows of the range outside it
+// (below / above) take w_i -= sum_j M(i, r0 + j) x_j: a wavefront covers RW
+// rows x 64 / RW groups
 // of columns (lane = row + RW group), each thread loading its columns in
 // batches of eight.  RW = 16 for FP64 factors, 32 for FP32 ones, so that a
 // load instruction reads runs of 256 bytes down the columns either way.
-template <int NC, class F>
-__device__ __forceinline__ void subst_right(const F* M, int n, bool lower, cdbl* w, SolveLds& S) {
-    constexpr int RW = sizeof(F) == sizeof(cflt) ? 32 : 16, CW = RW;   // rows per wavefront, columns per group
+template <int NC, class F, int RWT = 0>
+__device__ __forceinline__ void subst_right(const F* M, int n, bool lower, cdbl* w, SolveLds& S, int pb, int pe) {
+    // rows per wavefront, columns per group (RWT: chosen by the caller)
+    constexpr int RW = RWT ? RWT : (sizeof(F) == sizeof(cflt) ? 32 : 16), CW = RW;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
     const int l16 = lane % RW, grp = lane / RW;
-    const int nblk = (n + 63) / 64;
-    auto rb = [&](int s) { return 64 * (lower ? s : nblk - 1 - s); };   // first row of the s-th block solved
-    auto bsz = [&](int s) { return min(64, n - rb(s)); };
+    const int nblk = (pe - pb + 63) / 64;
+    auto rb = [&](int s) { return pb + 64 * (lower ? s : nblk - 1 - s); };   // first row of the s-th block solved
+    auto bsz = [&](int s) { return min(64, pe - rb(s)); };
     stage_diag(M, n, rb(0), bsz(0), false, S.tile[0], tid, blockDim.x);
     for (int s = 0; s < nblk; ++s) {
         const int r0 = rb(s), bs = bsz(s);
@@ -907,7 +910,7 @@ __device__ __forceinline__ void subst_right(const F* M, int n, bool lower, cdbl*
             stage_diag(M, n, rb(s + 1), bsz(s + 1), false, S.tile[(s + 1) & 1], tid - 64, blockDim.x - 64);
         }
         __syncthreads();
-        const int ia = lower ? r0 + bs : 0, ib = lower ? n : r0;
+        const int ia = lower ? r0 + bs : pb, ib = lower ? pe : r0;
         for (int i0 = ia + RW * wv; i0 < ib; i0 += RW * nwv) {
             const int i = i0 + l16;
             cdbl acc[NC];
@@ -952,22 +955,23 @@ __device__ __forceinline__ void subst_right(const F* M, int n, bool lower, cdbl*
 }
 
 // Left-looking pass with the conjugate transpose of the factor: U^H (lower)
-// forward or L^H (upper, unit) backward.  Block b first takes
-// w_i -= sum_j conj(M(j, i)) w_j over the finished unknowns j (above the block
-// for U^H, below it for L^H): wavefront v sums columns i = r0 + 4 v .. + 3
+// forward or L^H (upper, unit) backward, over the rows [pb, pe).  Block b
+// first takes w_i -= sum_j conj(M(j, i)) w_j over the finished unknowns j of
+// the range (above the block for U^H, below it for L^H; those outside the
+// range are gathered beforehand): wavefront v sums columns i = r0 + 4 v .. + 3
 // (CP of them per sweep down the column, CP = 4 / 2 / 1 for up to 2 / 4 / 8
 // right-hand sides) with its lanes down the column, then the diagonal block is
 // solved.
-template <int NC, class F>
-__device__ __forceinline__ void subst_left(const F* M, int n, bool lower, cdbl* w, SolveLds& S) {
+template <int NC, class F, int TW = 1>
+__device__ __forceinline__ void subst_left(const F* M, int n, bool lower, cdbl* w, SolveLds& S, int pb, int pe) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int nblk = (n + 63) / 64;
-    auto rb = [&](int s) { return 64 * (lower ? s : nblk - 1 - s); };
-    auto bsz = [&](int s) { return min(64, n - rb(s)); };
+    const int nblk = (pe - pb + 63) / 64;
+    auto rb = [&](int s) { return pb + 64 * (lower ? s : nblk - 1 - s); };
+    auto bsz = [&](int s) { return min(64, pe - rb(s)); };
     stage_diag(M, n, rb(0), bsz(0), true, S.tile[0], tid, blockDim.x);
     for (int s = 0; s < nblk; ++s) {
         const int r0 = rb(s), bs = bsz(s);
-        const int ja = lower ? 0 : r0 + bs, jb = lower ? r0 : n;
+        const int ja = lower ? pb : r0 + bs, jb = lower ? r0 : pe;
         constexpr int CP = NC <= 2 ? 4 : (NC <= 4 ? 2 : 1);
         for (int q0 = 0; q0 < 4; q0 += CP) {
             if (4 * wv + q0 >= bs) break;        // wave-uniform
@@ -977,26 +981,34 @@ __device__ __forceinline__ void subst_left(const F* M, int n, bool lower, cdbl* 
             for (int q = 0; q < CP; ++q)
 #pragma unroll
                 for (int c = 0; c < NC; ++c) acc[q][c] = mk(0, 0);
-            // the same number of sweeps for every lane, loads at clamped
-            // addresses (values beyond the range replaced by zero afterwards)
+            // the same number of sweeps for every lane, TW sweeps' loads in
+            // flight at once, at clamped addresses (values beyond the range
+            // replaced by zero afterwards; sweeps past the last add zeros)
             const int T = (jb - ja + 63) >> 6;
             const int qmax = bs - 1 - 4 * wv - q0;    // last column of the block for this wavefront
-            for (int tt = 0; tt < T; ++tt) {
-                const int j = ja + lane + 64 * tt, jc = min(j, jb - 1);
-                cdbl xj[NC];
-                F mv[CP];
+            for (int tt = 0; tt < T; tt += TW) {
+                cdbl xj[TW][NC];
+                F mv[TW][CP];
 #pragma unroll
-                for (int c = 0; c < NC; ++c) xj[c] = w[jc + (size_t)c * n];
+                for (int h = 0; h < TW; ++h) {
+                    const int jc = min(ja + lane + 64 * (tt + h), jb - 1);
 #pragma unroll
-                for (int q = 0; q < CP; ++q) mv[q] = M[jc + (size_t)(i0 + min(q, qmax)) * n];
+                    for (int c = 0; c < NC; ++c) xj[h][c] = w[jc + (size_t)c * n];
 #pragma unroll
-                for (int c = 0; c < NC; ++c)
-                    if (j >= jb) xj[c] = mk(0, 0);
+                    for (int q = 0; q < CP; ++q) mv[h][q] = M[jc + (size_t)(i0 + min(q, qmax)) * n];
+                }
 #pragma unroll
-                for (int q = 0; q < CP; ++q) {
-                    const cdbl t = q <= qmax ? cconj(to_cdbl(mv[q])) : mk(0, 0);
+                for (int h = 0; h < TW; ++h) {
+                    const bool in = ja + lane + 64 * (tt + h) < jb;
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) cfma(acc[q][c], t, xj[c]);
+                    for (int c = 0; c < NC; ++c)
+                        if (!in) xj[h][c] = mk(0, 0);
+#pragma unroll
+                    for (int q = 0; q < CP; ++q) {
+                        const cdbl t = q <= qmax ? cconj(to_cdbl(mv[h][q])) : mk(0, 0);
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) cfma(acc[q][c], t, xj[h][c]);
+                    }
                 }
             }
 #pragma unroll
@@ -1048,15 +1060,15 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_thin(const F* __restric
             w[e] = x[p[i] + (size_t)c * n];
         }
         __syncthreads();
-        subst_right<NC>(lu, n, true, w, S);
-        subst_right<NC>(lu, n, false, w, S);
+        subst_right<NC>(lu, n, true, w, S, 0, n);
+        subst_right<NC>(lu, n, false, w, S, 0, n);
         for (int e = tid; e < n * NC; e += blockDim.x) x[e] = w[e];
     } else {
         // A^H = U^H L^H P:  U^H y = x, L^H z = y, x = P^T z
         for (int e = tid; e < n * NC; e += blockDim.x) w[e] = x[e];
         __syncthreads();
-        subst_left<NC>(lu, n, true, w, S);
-        subst_left<NC>(lu, n, false, w, S);
+        subst_left<NC>(lu, n, true, w, S, 0, n);
+        subst_left<NC>(lu, n, false, w, S, 0, n);
         for (int e = tid; e < n * NC; e += blockDim.x) {
             const int i = e % n, c = e / n;
             x[p[i] + (size_t)c * n] = w[e];
@@ -1064,20 +1076,207 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_thin(const F* __restric
     }
 }
 
+// Panelled form of the thin solves for orders above 2 kSolvePanel: the
+// passes run panel by panel of kSolvePanel rows.  A panel's triangle is solved
+// by one workgroup per matrix (k_solve_panel: the blocked substitution above,
+// restricted to the panel); the panel's coupling to the other rows runs on
+// many workgroups per matrix: a right-looking pass then updates every row
+// after the panel with the panel's columns (k_panel_update), a left-looking
+// pass first gathers into the panel's rows the sums over the unknowns finished
+// before it (k_panel_gather).  Each entry of the factors is still read once,
+// but the bulk of the reads no longer waits on one workgroup per matrix.
+constexpr int kSolvePanel = 512;
+
+// w = P x (A x = b) or w = x (A^H x = b)
+__global__ void k_thin_in(const int* __restrict__ perm, int n, int nc, const cdbl* __restrict__ X, int adj,
+                          cdbl* __restrict__ work) {
+    const int b = blockIdx.y;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n * nc) return;
+    const int i = (int)(e % n), c = (int)(e / n);
+    const cdbl* x = X + (size_t)b * n * nc;
+    work[(size_t)b * n * nc + e] = adj ? x[e] : x[perm[(size_t)b * n + i] + (size_t)c * n];
+}
+
+// x = w (A x = b) or x = P^T w (A^H x = b)
+__global__ void k_thin_out(const int* __restrict__ perm, int n, int nc, cdbl* __restrict__ X, int adj,
+                           const cdbl* __restrict__ work) {
+    const int b = blockIdx.y;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n * nc) return;
+    const int i = (int)(e % n), c = (int)(e / n);
+    cdbl* x = X + (size_t)b * n * nc;
+    const cdbl v = work[(size_t)b * n * nc + e];
+    if (adj) x[perm[(size_t)b * n + i] + (size_t)c * n] = v;
+    else x[e] = v;
+}
+
+// one workgroup of kSolveThreads threads per matrix: pass 0 (L, or U^H) or
+// pass 1 (U, or L^H) over the panel [pb, pe)
+template <class F, int NC>
+__global__ __launch_bounds__(kSolveThreads) void k_solve_panel(const F* __restrict__ LU, long long ms, int n, int pass,
+                                                               int adj, cdbl* __restrict__ work, int pb, int pe) {
+    __shared__ SolveLds S;
+    const int b = blockIdx.x;
+    cdbl* w = work + (size_t)b * n * NC;
+    const F* lu = LU + (size_t)b * ms;
+    // inside a panel the updates cover at most kSolvePanel - 64 rows: 16 rows
+    // per wavefront (more wavefronts busy, fewer load rounds each) and two
+    // sweeps' loads in flight in the left-looking sums
+    if (!adj) subst_right<NC, F, 16>(lu, n, pass == 0, w, S, pb, pe);
+    else subst_left<NC, F, NC <= 4 ? 2 : 1>(lu, n, pass == 0, w, S, pb, pe);
+}
+
+// Right-looking coupling of panel [pb, pe): every row i after it in the pass
+// (i >= pe for L, i < pb for U) takes w_i -= sum_{j in panel} M(i, j) w_j, the
+// sum in column order.  A thread per row (consecutive rows: the loads run down
+// the columns), LR column loads in flight, the panel's unknowns in LDS.
+template <class F, int NC>
+__global__ __launch_bounds__(256) void k_panel_update(const F* __restrict__ LU, long long ms, int n, int lower,
+                                                      cdbl* __restrict__ work, int pb, int pe) {
+    constexpr int LR = sizeof(F) == sizeof(cflt) ? 16 : 8;
+    __shared__ cdbl xs[kSolvePanel * NC];
+    const int b = blockIdx.y, tid = threadIdx.x;
+    const F* M = LU + (size_t)b * ms;
+    cdbl* w = work + (size_t)b * n * NC;
+    const int pw = pe - pb;
+    for (int e = tid; e < pw * NC; e += blockDim.x) {
+        const int j = e % pw, c = e / pw;
+        xs[j * NC + c] = w[pb + j + (size_t)c * n];
+    }
+    __syncthreads();
+    const int r_lo = lower ? pe : 0, r_hi = lower ? n : pb;
+    const int i = r_lo + blockIdx.x * blockDim.x + tid;
+    if (i >= r_hi) return;
+    cdbl acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = mk(0, 0);
+    const F* mr = M + i + (size_t)pb * n;
+    for (int j0 = 0; j0 < pw; j0 += LR) {
+        F t[LR];
+#pragma unroll
+        for (int u = 0; u < LR; ++u) t[u] = mr[(size_t)min(j0 + u, pw - 1) * n];
+#pragma unroll
+        for (int u = 0; u < LR; ++u) {
+            if (j0 + u < pw) {
+                const cdbl tv = to_cdbl(t[u]);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) cfma(acc[c], tv, xs[(j0 + u) * NC + c]);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        cdbl& y = w[i + (size_t)c * n];
+        y = y - acc[c];
+    }
+}
+
+// Left-looking coupling of panel [pb, pe): every row i of the panel takes
+// w_i -= sum_j conj(M(j, i)) w_j over the unknowns finished before it (j < pb
+// for U^H, j >= pe for L^H).  A wavefront per four columns, lanes down the
+// column, two sweeps' loads in flight; the lane sums are reduced by shuffles
+// in a fixed order.
+template <class F, int NC>
+__global__ __launch_bounds__(256) void k_panel_gather(const F* __restrict__ LU, long long ms, int n, int lower,
+                                                      cdbl* __restrict__ work, int pb, int pe) {
+    const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int i0 = pb + (blockIdx.x * 4 + wv) * 4;
+    if (i0 >= pe) return;                     // whole wavefront; no barrier in this kernel
+    const F* M = LU + (size_t)b * ms;
+    cdbl* w = work + (size_t)b * n * NC;
+    const int ja = lower ? 0 : pe, jb = lower ? pb : n;
+    const int qmax = min(4, pe - i0) - 1;
+    cdbl acc[4][NC];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[q][c] = mk(0, 0);
+    const int T = (jb - ja + 63) >> 6;
+    for (int tt = 0; tt < T; tt += 2) {
+        cdbl xj[2][NC];
+        F mv[2][4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int jc = min(ja + lane + 64 * (tt + h), jb - 1);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) xj[h][c] = w[jc + (size_t)c * n];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) mv[h][q] = M[jc + (size_t)(i0 + min(q, qmax)) * n];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const bool in = ja + lane + 64 * (tt + h) < jb;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const cdbl t = (in && q <= qmax) ? cconj(to_cdbl(mv[h][q])) : mk(0, 0);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) cfma(acc[q][c], t, xj[h][c]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                acc[q][c].x += __shfl_xor(acc[q][c].x, o, 64);
+                acc[q][c].y += __shfl_xor(acc[q][c].y, o, 64);
+            }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (lane == q && q <= qmax) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                cdbl& y = w[i0 + q + (size_t)c * n];
+                y = y - acc[q][c];
+            }
+        }
+}
+
+template <class F, int NC>
+static void solve_thin_nc(const F* LU, long long ms, const int* perm, int n, int batch, cdbl* X, bool adj, cdbl* work,
+                          hipStream_t st) {
+    const int a = adj ? 1 : 0;
+    if (n <= 2 * kSolvePanel) {
+        hipLaunchKernelGGL((k_solve_thin<F, NC>), dim3(batch), dim3(kSolveThreads), 0, st, LU, ms, perm, n, X, a, work);
+        return;
+    }
+    const dim3 gio((unsigned)(((long long)n * NC + 255) / 256), (unsigned)batch);
+    hipLaunchKernelGGL(k_thin_in, gio, dim3(256), 0, st, perm, n, NC, X, a, work);
+    const int np = (n + kSolvePanel - 1) / kSolvePanel;
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool lower = pass == 0;
+        for (int t = 0; t < np; ++t) {
+            const int p = lower ? t : np - 1 - t;
+            const int pb = p * kSolvePanel, pe = min(n, pb + kSolvePanel);
+            if (adj && (lower ? pb > 0 : pe < n))
+                hipLaunchKernelGGL((k_panel_gather<F, NC>), dim3((unsigned)((pe - pb + 15) / 16), (unsigned)batch),
+                                   dim3(256), 0, st, LU, ms, n, lower ? 1 : 0, work, pb, pe);
+            hipLaunchKernelGGL((k_solve_panel<F, NC>), dim3(batch), dim3(kSolveThreads), 0, st, LU, ms, n, pass, a,
+                               work, pb, pe);
+            const int rows = lower ? n - pe : pb;
+            if (!adj && rows > 0)
+                hipLaunchKernelGGL((k_panel_update<F, NC>), dim3((unsigned)((rows + 255) / 256), (unsigned)batch),
+                                   dim3(256), 0, st, LU, ms, n, lower ? 1 : 0, work, pb, pe);
+        }
+    }
+    hipLaunchKernelGGL(k_thin_out, gio, dim3(256), 0, st, perm, n, NC, X, a, work);
+}
+
 template <class F>
 static void solve_thin_impl(const F* LU, long long ms, const int* perm, int n, int batch, cdbl* X, int nc, bool adj,
                             cdbl* work, hipStream_t st) {
-    const int a = adj ? 1 : 0;
-    const dim3 g(batch), t(kSolveThreads);
     switch (nc) {
-    case 1: hipLaunchKernelGGL((k_solve_thin<F, 1>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
-    case 2: hipLaunchKernelGGL((k_solve_thin<F, 2>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
-    case 3: hipLaunchKernelGGL((k_solve_thin<F, 3>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
-    case 4: hipLaunchKernelGGL((k_solve_thin<F, 4>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
-    case 5: hipLaunchKernelGGL((k_solve_thin<F, 5>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
-    case 6: hipLaunchKernelGGL((k_solve_thin<F, 6>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
-    case 7: hipLaunchKernelGGL((k_solve_thin<F, 7>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
-    default: hipLaunchKernelGGL((k_solve_thin<F, 8>), g, t, 0, st, LU, ms, perm, n, X, a, work); break;
+    case 1: solve_thin_nc<F, 1>(LU, ms, perm, n, batch, X, adj, work, st); break;
+    case 2: solve_thin_nc<F, 2>(LU, ms, perm, n, batch, X, adj, work, st); break;
+    case 3: solve_thin_nc<F, 3>(LU, ms, perm, n, batch, X, adj, work, st); break;
+    case 4: solve_thin_nc<F, 4>(LU, ms, perm, n, batch, X, adj, work, st); break;
+    case 5: solve_thin_nc<F, 5>(LU, ms, perm, n, batch, X, adj, work, st); break;
+    case 6: solve_thin_nc<F, 6>(LU, ms, perm, n, batch, X, adj, work, st); break;
+    case 7: solve_thin_nc<F, 7>(LU, ms, perm, n, batch, X, adj, work, st); break;
+    default: solve_thin_nc<F, 8>(LU, ms, perm, n, batch, X, adj, work, st); break;
     }
 }
 
