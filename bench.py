@@ -38,10 +38,10 @@ FLOP_PER_NODE = {0: 292.0, 1: 376.0, 2: 498.0}   # the library's algorithmic mod
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, KiB per dispatch, scaled by the
 # counters' reading of a plain 1 GiB copy): profiles/k_assemble_x2_general_*.txt
 TRAFFIC_BYTES = {"star400": 1.7076e10, "polygon2000": 8.2862e10}
-# ... on a context with the lower-order rules on separated pairs: the four
-# launches of a sweep (one per separation class) averaged, 7.066e10 B per sweep
-# (profiles/k_cgemm32_sub_polygon2000_far_traffic.txt)
-TRAFFIC_BYTES_FAR = {"polygon2000": 1.7665e10}
+# ... on a context with the lower-order rules on separated pairs: the
+# launches of a sweep (one per separation class and run of k blocks sharing an
+# order) averaged (profiles/k_cgemm32_sub_polygon2000_far_traffic.txt)
+TRAFFIC_BYTES_FAR = {}
 # HBM bytes per launch of the LU's matrix-core updates, averaged over the 217
 # launches of one polygon2000 sweep (the same counters and factors; 4.268e11 B
 # per sweep against 3.900e11 B of algorithmic C, A and B traffic)
@@ -169,7 +169,7 @@ def run_workload(H, name, steps, warmup, mixed=False, bd=None, ref_sigma=None, f
         "phase_seconds_per_step": {ph: p[0] * 1e-3 / steps for ph, p in zip(PHASES, prof)},
         "lu_model_tflops": lu_tflops,
         "roofline_assembly": {"kernel": "k_assemble_x2<general, d=%d> (general panel pairs, %d k x 2 c per launch%s)"
-                               % (der, w["nk"], "; one launch per separation class, Gauss orders 11 / 9 / 7 / 6" if far else ""),
+                               % (der, w["nk"], "; one launch per separation class and run of k blocks of one order, Gauss orders 11 to 5" if far else ""),
                      "bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": (TRAFFIC_BYTES_FAR if far else TRAFFIC_BYTES).get(name),
@@ -194,7 +194,8 @@ def run_workload(H, name, steps, warmup, mixed=False, bd=None, ref_sigma=None, f
     out["roofline"] = max(cands, key=lambda v: v["seconds_per_step"])
     if far:
         out["far_field"] = "general pairs closer than one panel diameter at Gauss order 11, one to two diameters " \
-                           "apart at 9, two to eight at 7, farther at 6 (kappa d <= 0.65 over the grid)"
+                           "apart at 9, two to eight at 8, eight to sixteen at 6, farther at 5 for the k blocks " \
+                           "with kappa d <= 0.4 and 6 for the others"
     if mixed:
         out["refine_steps"] = {"last_solve": MIXED_REFINE, "other_solves": MIXED_REFINE_INNER}
         out["fp64_fallbacks"] = fallback[0]

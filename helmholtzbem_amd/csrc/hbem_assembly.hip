@@ -947,7 +947,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         // it in scratch memory).  Every thread reaches the barriers.
         static_assert((STG + 1) * REC >= 256 * 4 * 2, "exchange needs 16 KB of staging LDS");
         const int mine = wv & 1;
-        const int kt = ((qb >> 1) << 5) + j;
+        const int kt = P.kt0 + ((qb >> 1) << 5) + j;
         const bool kact = qb < nqb && kt < P.nkt;
         const size_t nkt = (size_t)P.nkt;
         cdbl* out = P.out + rec * (16 * (D + 1)) * nkt + kt;

@@ -271,10 +271,11 @@ struct hbem_geometry {
     int ngen = 0, nadj = 0, ncoi = 0;
     // General pairs in order of separation (the distance of the panels'
     // bounding discs in panel diameters): sep[0] pairs closer than one
-    // diameter (or on a spline panel), sep[1] at one to two, sep[2] at two to
-    // eight, sep[3] farther apart (see far_order); dmax is the largest panel
-    // diameter.
-    int sep[4] = {0, 0, 0, 0};
+    // diameter (or on a trigonometric or spline panel), sep[1] at one to two, sep[2] at two to
+    // eight, sep[3] at eight to sixteen, sep[4] farther apart (see far_order);
+    // dmax is the largest panel diameter.
+    static constexpr int kSep = 5;
+    int sep[kSep] = {0, 0, 0, 0, 0};
     double dmax = 0.0;
     // node tables of the separated classes at lower Gauss orders, built on
     // first use: key 64 cls + order
@@ -352,22 +353,23 @@ void tdiff_slots(const double* ks, int nb, double c_i, double c_o, std::vector<d
 }
 
 // Gauss order of the tensor rule for the separated general pairs of class cls
-// (1: one to two panel diameters apart, 2: two to eight, 3: farther) when
-// kappa times the largest panel diameter is at most kd: the lowest order whose
-// single- and double-layer local matrices stay within 1e-13 of their largest
-// entry of the 11-point rule's for straight panels at the class's least
-// separation, end-on, side by side or at any angle (the rule's error falls as
-// (separation)^(-2m) and with the oscillation kappa d as (kappa d)^(2m) /
-// (2m)!); ns when no lower order qualifies.
+// (1: one to two panel diameters apart, 2: two to eight, 3: eight to sixteen,
+// 4: farther) when kappa times the largest panel diameter is at most kd: the
+// lowest order whose single- and double-layer local matrices stay within 1e-13
+// of their largest entry of the 11-point rule's -- with a margin of two, 5e-14
+// -- for straight panels at the class's least separation, end-on, side by side
+// and at 42 random angles, at both ends of the kd interval (the rule's error
+// falls as (separation)^(-2m) and with the oscillation kappa d as
+// (kappa d)^(2m) / (2m)!); ns when no lower order qualifies.
 int far_order(double kd, int cls, int ns) {
-    static const double edge[4] = {0.65, 1.25, 2.5, 5.0};
-    static const int order[4][3] = {{9, 7, 6}, {9, 7, 7}, {9, 8, 8}, {10, 10, 10}};
-    for (int t = 0; t < 4; ++t)
+    static const double edge[5] = {0.4, 0.65, 1.25, 2.5, 5.0};
+    static const int order[5][4] = {{9, 8, 6, 5}, {9, 8, 6, 6}, {9, 8, 7, 7}, {9, 8, 8, 8}, {10, 10, 10, 10}};
+    for (int t = 0; t < 5; ++t)
         if (kd <= edge[t]) return std::min(order[t][cls - 1], ns);
     return ns;
 }
 
-// The node tables of separation class cls (1 to 3) at Gauss order m < ns,
+// The node tables of separation class cls (1 to 4) at Gauss order m < ns,
 // built on first use (one setup launch per table).
 hbem_geometry::FarSet& far_set(hbem_geometry* g, int cls, int m) {
     auto& slot = g->far[64 * cls + m];
@@ -399,8 +401,10 @@ hbem_geometry::FarSet& far_set(hbem_geometry* g, int cls, int m) {
 // tdiff layout (tdiff_slots) for nkt wavenumbers, the paired-lane kernel runs
 // whatever the width, and the buffer holds the differences T is formed from,
 // nkt columns wide.  Contexts with HBEM_FLAG_FAR_FIELD and real slots: the
-// separated general pairs take the lower Gauss orders of far_order for the
-// batch's largest kappa, each separation class its own launch.
+// separated general pairs take the lower Gauss orders of far_order, chosen for
+// the largest kappa of each block of 32 wavenumbers in the tdiff layout (of
+// the batch otherwise); a launch per separation class and run of consecutive
+// blocks with the same order.
 cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const std::vector<double>& kr,
                       const std::vector<double>& cv, int D, const std::vector<double>& kapi = {},
                       const std::vector<double>& kri = {}, int nkt = 0) {
@@ -417,15 +421,12 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
     P.wg = (const double*)g->d_wg.p;
     P.xc = (const double*)g->d_xc.p;
     P.wc = (const double*)g->d_wc.p;
-    P.nk = nk;
-    P.kap = upload(g->w_kap, kap, g->st);
-    P.kr = upload(g->w_kr, kr, g->st);
-    P.cv = upload(g->w_cv, cv, g->st);
+    const double* dkap = upload(g->w_kap, kap, g->st);
+    const double* dkr = upload(g->w_kr, kr, g->st);
+    const double* dcv = upload(g->w_cv, cv, g->st);
     const bool cplx = !kapi.empty();
-    if (cplx) {
-        P.kapi = upload(g->w_kapi, kapi, g->st);
-        P.kri = upload(g->w_kri, kri, g->st);
-    }
+    const double* dkapi = cplx ? upload(g->w_kapi, kapi, g->st) : nullptr;
+    const double* dkri = cplx ? upload(g->w_kri, kri, g->st) : nullptr;
     P.npairs_total = g->npairs;
     P.ns = g->ns;
     P.nc = g->nc;
@@ -444,6 +445,7 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
         const long long* pidx;
         int ns;                       // general pairs: order of the tensor rule
         const double *xg, *wg;
+        int q0, nq, kt0;              // slots [q0, q0 + nq) of the batch; first wavenumber column (tdiff)
     };
     const double* dxg = (const double*)g->d_xg.p;
     const double* dwg = (const double*)g->d_wg.p;
@@ -453,31 +455,44 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
     const unsigned char* gflags = (const unsigned char*)g->d_gflags.p;
     const double* gngeo = (const double*)g->d_gngeo.p;
     if ((g->flags & HBEM_FLAG_FAR_FIELD) && !cplx) {
-        double kmax = 0.0;
-        for (double v : kap) kmax = std::max(kmax, std::fabs(v));
+        // units of slots that share an order: blocks of 32 wavenumbers (64
+        // slots) in the tdiff layout, the whole batch otherwise
+        const int uslots = tdiff ? 64 : nk, nunit = nk / uslots;
+        std::vector<double> ukd(nunit, 0.0);
+        for (int u = 0; u < nunit; ++u)
+            for (int q = u * uslots; q < (u + 1) * uslots; ++q) ukd[u] = std::max(ukd[u], std::fabs(kap[q]) * g->dmax);
         int off = 0;
-        for (int s = 0; s < 4; ++s) {
+        for (int s = 0; s < hbem_geometry::kSep; ++s) {
             const int cnt = g->sep[s];
-            const int m = s == 0 ? g->ns : far_order(kmax * g->dmax, s, g->ns);
-            if (cnt > 0 && m < g->ns) {
-                auto& f = far_set(g, s, m);
-                ls.push_back({2, 0, Gg, cnt, m * m, gent + 3 * (size_t)off, (const double*)f.ngeo.p,
-                              (const unsigned char*)f.flags.p, gpidx + off, m, (const double*)f.xg.p,
-                              (const double*)f.wg.p});
-            } else if (cnt > 0) {
-                ls.push_back({2, 0, Gg, cnt, g->ns * g->ns, gent + 3 * (size_t)off,
-                              gngeo + (size_t)off * g->ns * g->ns * hbem::kNodeStride, gflags + off, gpidx + off, g->ns,
-                              dxg, dwg});
+            auto order_of = [&](int u) { return s == 0 ? g->ns : far_order(ukd[u], s, g->ns); };
+            for (int u = 0; u < nunit && cnt > 0;) {
+                const int m = order_of(u);
+                int v = u + 1;
+                while (v < nunit && order_of(v) == m) ++v;
+                Launch l{2, 0, Gg, cnt, g->ns * g->ns, gent + 3 * (size_t)off,
+                         gngeo + (size_t)off * g->ns * g->ns * hbem::kNodeStride, gflags + off, gpidx + off, g->ns,
+                         dxg, dwg, u * uslots, (v - u) * uslots, tdiff ? 32 * u : 0};
+                if (m < g->ns) {
+                    auto& f = far_set(g, s, m);
+                    l.nodes = m * m;
+                    l.ngeo = (const double*)f.ngeo.p;
+                    l.flags = (const unsigned char*)f.flags.p;
+                    l.ns = m;
+                    l.xg = (const double*)f.xg.p;
+                    l.wg = (const double*)f.wg.p;
+                }
+                ls.push_back(l);
+                u = v;
             }
             off += cnt;
         }
     } else {
-        ls.push_back({2, 0, Gg, g->ngen, g->ns * g->ns, gent, gngeo, gflags, gpidx, g->ns, dxg, dwg});
+        ls.push_back({2, 0, Gg, g->ngen, g->ns * g->ns, gent, gngeo, gflags, gpidx, g->ns, dxg, dwg, 0, nk, 0});
     }
     ls.push_back({1, 1, Gs, g->nadj, 2 * g->nc * g->nc, (const int*)g->d_aent.p, (const double*)g->d_angeo.p,
-                  (const unsigned char*)g->d_aflags.p, (const long long*)g->d_apidx.p, g->ns, dxg, dwg});
+                  (const unsigned char*)g->d_aflags.p, (const long long*)g->d_apidx.p, g->ns, dxg, dwg, 0, nk, 0});
     ls.push_back({0, 2, Gs, g->ncoi, g->nc * g->nc, (const int*)g->d_cent.p, (const double*)g->d_cngeo.p,
-                  (const unsigned char*)g->d_cflags.p, (const long long*)g->d_cpidx.p, g->ns, dxg, dwg});
+                  (const unsigned char*)g->d_cflags.p, (const long long*)g->d_cpidx.p, g->ns, dxg, dwg, 0, nk, 0});
     for (auto& l : ls) {
         if (l.nent == 0) continue;
         P.nent = l.nent;
@@ -488,6 +503,15 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
         P.ns = l.ns;
         P.xg = l.xg;
         P.wg = l.wg;
+        P.nk = l.nq;
+        P.kap = dkap + l.q0;
+        P.kr = dkr + l.q0;
+        P.cv = dcv + l.q0;
+        if (cplx) {
+            P.kapi = dkapi + l.q0;
+            P.kri = dkri + l.q0;
+        }
+        P.kt0 = l.kt0;
         // general pairs: one chunk; singular pairs: chunks of 2048 nodes (paired-lane
         // kernel, 16 staging rounds; for adjacent pairs each of the two node halves
         // is chunked on its own) or 64 x 48 nodes (one wavefront per item)
@@ -502,7 +526,7 @@ cdbl* assemble_locals(hbem_geometry* g, const std::vector<double>& kap, const st
         hipEvent_t e0 = g->prof ? g->plog.begin(g->st) : nullptr;
         hbem::launch_assemble(l.cls, D, P, l.G, g->st);
         HIPC(hipGetLastError());
-        if (g->prof) g->plog.end(g->st, l.which, e0, (double)l.nent * nk * l.nodes * flops_per_node(D));
+        if (g->prof) g->plog.end(g->st, l.which, e0, (double)l.nent * l.nq * l.nodes * flops_per_node(D));
         if (P.nchunk > 1) {
             hbem::launch_chunk_sum(part, l.nent, P.nchunk, nko, nslot, l.pidx, g->npairs, L, g->st);
             HIPC(hipGetLastError());
@@ -738,14 +762,17 @@ int hbem_geometry_create(const double* panels, int npanels, const double* coefs,
     for (int i = 0; i < N; ++i) g->dmax = std::max(g->dmax, 2.0 * disc[3 * (size_t)i + 2]);
     // general pairs by separation eta = (distance of the discs) / (larger
     // diameter), a lower bound of the panels' distance in diameters
-    std::vector<int> sent[4];
-    std::vector<long long> spi[4];
+    std::vector<int> sent[hbem_geometry::kSep];
+    std::vector<long long> spi[hbem_geometry::kSep];
+    // (line and arc panels only: a trigonometric or spline panel's |gamma'| has
+    // complex branch points a few panel lengths away, which bound every rule
+    // on the panel whatever the separation)
     auto separation = [&](int i, int j) {
-        if ((int)g->panels[8 * (size_t)i] == 3 || (int)g->panels[8 * (size_t)j] == 3) return 0;
+        if ((int)g->panels[8 * (size_t)i] >= 2 || (int)g->panels[8 * (size_t)j] >= 2) return 0;
         const double* a = &disc[3 * (size_t)i];
         const double* b = &disc[3 * (size_t)j];
         const double eta = (std::hypot(a[0] - b[0], a[1] - b[1]) - a[2] - b[2]) / (2.0 * std::max(a[2], b[2]));
-        return eta >= 8.0 ? 3 : (eta >= 2.0 ? 2 : (eta >= 1.0 ? 1 : 0));
+        return eta >= 16.0 ? 4 : (eta >= 8.0 ? 3 : (eta >= 2.0 ? 2 : (eta >= 1.0 ? 1 : 0)));
     };
     const double tol = 100.0 * hbem::kDblEps;
     for (int i = 0; i < N; ++i)
@@ -763,7 +790,7 @@ int hbem_geometry_create(const double* panels, int npanels, const double* coefs,
                 sent[s].push_back(i); sent[s].push_back(j); sent[s].push_back(0); spi[s].push_back(p);
             }
         }
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < hbem_geometry::kSep; ++s) {
         g->sep[s] = (int)spi[s].size();
         gent.insert(gent.end(), sent[s].begin(), sent[s].end());
         gpi.insert(gpi.end(), spi[s].begin(), spi[s].end());

@@ -51,8 +51,9 @@ struct AsmParams {
     // refraction index and block 2w + 1 the same ones with the outer; out
     // receives per wavenumber the differences T is formed from (K_i - K_o,
     // W_o - W_i, V_o - V_i) with nkt (the wavenumbers) in place of nk in the
-    // layouts above
-    int tdiff, nkt;
+    // layouts above; kt0: the first wavenumber of the launch's slots in those
+    // layouts (a launch over a range of slot blocks)
+    int tdiff, nkt, kt0;
 };
 
 // assembly (hbem_assembly.hip)

@@ -58,15 +58,17 @@ extern "C" {
 #define HBEM_FLAG_QUIRK_SLP_COINC_DER 1
 /* Lower-order quadrature on separated panel pairs.  Every assembly through the
  * context at real positive wavenumbers (matrices, T, sweeps) integrates the
- * general panel pairs whose bounding discs lie at least one panel diameter
- * apart with a tensor Gauss rule of 6-10 points a side instead of the
- * quadrature order, the order chosen per batch from the separation class (one
- * to two diameters, two to eight, more) and kappa times the largest panel
- * diameter so that each pair's local matrices stay within 1e-13 of their
- * largest entry of the 11-point rule's.  Pairs on spline panels, complex
- * wavenumbers, and kappa d > 5 keep the full rule.  Off by default: the
- * reference integrates every general pair with the same rule
- * (galerkin_builder.cpp:308-327).  Not accepted by hbem_sigma_sweep_multi. */
+ * general pairs of line and arc panels whose bounding discs lie at least one
+ * panel diameter apart with a tensor Gauss rule of 5-10 points a side instead
+ * of the quadrature order, the order chosen from the separation class (one to
+ * two diameters, two to eight, eight to sixteen, more) and kappa times the
+ * largest panel diameter -- per block of 32 wavenumbers in the sweeps, per
+ * batch otherwise -- so that each pair's local matrices stay within 1e-13 of
+ * their largest entry of the 11-point rule's.  Pairs on trigonometric-curve
+ * or spline panels, complex wavenumbers, and kappa d > 5 keep the full rule.
+ * Off by default: the reference integrates every general pair with the same
+ * rule (galerkin_builder.cpp:308-327).  Not accepted by
+ * hbem_sigma_sweep_multi. */
 #define HBEM_FLAG_FAR_FIELD 2
 
 typedef struct hbem_geometry* hbem_geometry_t;

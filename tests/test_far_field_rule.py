@@ -1,21 +1,22 @@
 """The order table of the lower-order rules on separated panel pairs
 (HBEM_FLAG_FAR_FIELD), checked independently of the library with scipy's
 Hankel functions: for two straight panels of diameter d at the least
-separation of each class -- one diameter (class 1), two (class 2) and eight
-(class 3) between their bounding discs, end-on, side by side and at random
-angles -- and kappa d at both edges of each tier, the tensor Gauss
-rule of the table's order gives single- and double-layer local matrices
-within 1e-13 of their largest entry of the 11-point rule's.  One order lower
-fails the bound at the first tier's upper edge, so the table's orders are
-not loose there.
+separation of each class -- one diameter (class 1), two (class 2), eight
+(class 3) and sixteen (class 4) between their bounding discs, end-on, side
+by side and at 42 random angles -- and kappa d at both edges of each tier,
+the tensor Gauss rule of the table's order gives single- and double-layer
+local matrices within 1e-13 of their largest entry of the 11-point rule's.
+The table was chosen with a margin of two (5e-14); one order lower misses
+that margin at the first tier's upper edge, so its orders are not loose
+there.
 """
 import numpy as np
 import pytest
 from scipy.special import hankel1, roots_legendre
 
-# (upper edge of kappa d, orders for classes 1, 2, 3) as the library's far_order
-TABLE = ((0.65, 9, 7, 6), (1.25, 9, 7, 7), (2.5, 9, 8, 8), (5.0, 10, 10, 10))
-CLASSES = [(1, 1.0), (2, 2.0), (3, 8.0)]
+# (upper edge of kappa d, orders for classes 1, 2, 3, 4) as the library's far_order
+TABLE = ((0.4, 9, 8, 6, 5), (0.65, 9, 8, 6, 6), (1.25, 9, 8, 7, 7), (2.5, 9, 8, 8, 8), (5.0, 10, 10, 10, 10))
+CLASSES = [(1, 1.0), (2, 2.0), (3, 8.0), (4, 16.0)]
 FULL = 11
 
 
@@ -55,7 +56,11 @@ def configs(eta, rng):
     return out
 
 
-def worst(order, eta, kd, rng):
+def worst(order, eta, kd, rng=None):
+    """Largest error over the end-on and side-by-side pairs and 6 random
+    angles per seed, seeds 1000..1006 unless rng is given."""
+    if rng is None:
+        return max(worst(order, eta, kd, np.random.default_rng(1000 + s)) for s in range(7))
     err = 0.0
     for a0, a1, b0, b1 in configs(eta, rng):
         Vf, Kf = local(a0, a1, b0, b1, FULL, kd)
@@ -74,12 +79,12 @@ def test_table_order_meets_bound(tier, cls, eta, edge):
     if edge == "lower":
         kd = TABLE[tier - 1][0] if tier else 0.05
     m = TABLE[tier][cls]
-    e = worst(m, eta, kd, np.random.default_rng(10 * tier + cls))
+    e = worst(m, eta, kd)
     assert e <= 1e-13, (kd, cls, m, e)
 
 
 @pytest.mark.parametrize("cls,eta", CLASSES)
-def test_one_order_lower_fails_at_the_first_tier(cls, eta):
+def test_one_order_lower_misses_the_margin_at_the_first_tier(cls, eta):
     kd = TABLE[0][0]
     m = TABLE[0][cls] - 1
-    assert worst(m, eta, kd, np.random.default_rng(3)) > 1e-13
+    assert worst(m, eta, kd) > 5e-14

@@ -3,10 +3,11 @@
 Bars, per assertion:
 * the assembled K, W, V and their k-derivatives (d = 0, 1, 2) of a context
   with the flag within 1e-12 relative Frobenius of the oracle's full-order
-  matrices -- the plain path's matrix bar -- on a polygon, a trigonometric
-  star and a circle of arcs, at kappa d in each tier of the order table
-  (d the largest panel diameter), and measurably different from the plain
-  context's (the lower-order rules ran);
+  matrices -- the plain path's matrix bar -- on a polygon (line panels) and a
+  circle of arcs, at kappa d in each tier of the order table (d the largest
+  panel diameter), and measurably different from the plain context's (the
+  lower-order rules ran); on a trigonometric star, whose panels keep the full
+  rule, equal to the plain context's bit for bit;
 * at kappa d > 5 every pair keeps the full rule: the matrices equal the plain
   context's bit for bit;
 * sigma_min of the FP64 sweep within 1e-10, and of the mixed-precision sweep
@@ -52,7 +53,7 @@ def pair(request, H):
     return request.param, mesh, plain, far
 
 
-@pytest.mark.parametrize("kd", [0.9, 1.9, 3.8])
+@pytest.mark.parametrize("kd", [0.3, 0.9, 1.9, 3.8])
 def test_far_matrices_match_oracle(pair, kd, H, O):
     name, mesh, plain, far = pair
     c = 20.0
@@ -67,7 +68,10 @@ def test_far_matrices_match_oracle(pair, kd, H, O):
             assert rel(get(t), ref[t]) <= 1e-12, (name, kd, lab, t, rel(get(t), ref[t]))
         diff = max(diff, rel(gf.getSingleLayer(t), gp.getSingleLayer(t)))
     print(name, "kappa d", kd, "far vs full rule", diff, flush=True)
-    assert 0.0 < diff <= 1e-12, (name, kd, diff)
+    if name == "star96":
+        assert diff == 0.0, (name, kd, diff)      # trigonometric panels keep the full rule
+    else:
+        assert 0.0 < diff <= 1e-12, (name, kd, diff)
 
 
 def test_full_rule_beyond_the_table(pair, H):
