@@ -41,7 +41,7 @@ TRAFFIC_BYTES = {"star400": 1.7076e10, "polygon2000": 8.2862e10}
 # ... on a context with the lower-order rules on separated pairs: the
 # launches of a sweep (one per separation class and run of k blocks sharing an
 # order) averaged (profiles/k_cgemm32_sub_polygon2000_far_traffic.txt)
-TRAFFIC_BYTES_FAR = {}
+TRAFFIC_BYTES_FAR = {"polygon2000": 1.2371e10}     # 7.423e10 B per sweep over 6 launches
 # HBM bytes per launch of the LU's matrix-core updates, averaged over the 217
 # launches of one polygon2000 sweep (the same counters and factors; 4.268e11 B
 # per sweep against 3.900e11 B of algorithmic C, A and B traffic)
