@@ -394,6 +394,20 @@ hbem_geometry::FarSet& far_set(hbem_geometry* g, int cls, int m) {
     return *slot;
 }
 
+// Contexts with HBEM_FLAG_FAR_FIELD: the node tables every order far_order
+// gives for these kappa builds now (a batch's order is the order of one of its
+// kappa), so that batch sizing sees the memory they take.
+void prepare_far(hbem_geometry* g, const std::vector<double>& kap) {
+    if (!(g->flags & HBEM_FLAG_FAR_FIELD)) return;
+    for (int s = 1; s < hbem_geometry::kSep; ++s) {
+        if (g->sep[s] == 0) continue;
+        for (double v : kap) {
+            const int m = far_order(std::fabs(v) * g->dmax, s, g->ns);
+            if (m < g->ns) far_set(g, s, m);
+        }
+    }
+}
+
 // Assemble the local matrices of all pairs for wavenumber slots (kap, kr, cv)
 // with imaginary parts (kapi, kri) when any slot is not real positive; returns
 // the local buffer.  Wide real batches run on the paired-lane kernel, the rest
@@ -945,6 +959,11 @@ int hbem_assemble_dense_batch(hbem_geometry_t g, const double* ks, int nk, doubl
     const size_t nslot = 16 * (size_t)(d + 1);
     const size_t part = (size_t)std::max(g->nadj * adj_chunks(g->nc), g->ncoi * coi_chunks(g->nc));
     const size_t per = ((size_t)g->npairs + part) * nslot * sizeof(cdbl);
+    {
+        std::vector<double> all(nk);
+        for (int i = 0; i < nk; ++i) all[i] = ks[i] * sc;
+        prepare_far(g, all);
+    }
     const int B = sweep_batch(per, usable_device_bytes(g->w_L.n + g->w_part.n), nk);
     for (int k0 = 0; k0 < nk; k0 += B) {
         const int nb = std::min(B, nk - k0);
@@ -1461,6 +1480,12 @@ void sweep_run(hbem_geometry* g, const double* ks, int nk, double c_o, double c_
     const size_t per = sweep_bytes_per_k(g->npairs, g->nadj, g->ncoi, g->nc, n, D, nc, tdiff ? 1 : 2) +
                        (d2sigma ? der2_bytes_per_k(n) : 0) +
                        (mx ? mixed_bytes_per_k(n, nc) : 0);
+    {
+        std::vector<double> all;
+        for (int i = 0; i < nk; ++i)
+            if (ks[i] > 0.0) { all.push_back(ks[i] * std::sqrt(c_i)); all.push_back(ks[i] * std::sqrt(c_o)); }
+        prepare_far(g, all);
+    }
     const int B = sweep_batch(per, usable_device_bytes(g->work_held()), nk);
     cdbl* dR = g->w_R.as<cdbl>((size_t)n * nc);
     HIPC(hipMemcpyAsync(dR, R, (size_t)n * nc * sizeof(cdbl), hipMemcpyHostToDevice, g->st));
