@@ -32,7 +32,8 @@ __all__ = ["LayerType", "ContinuousSpace1", "BuilderData", "GalerkinBuilder", "S
            "randomized_svd", "sigma_sweep", "sigma_sweep_der2", "sigma_sweep_mixed", "sigma_sweep_multi", "sigma_sweep_batched",
            "random_polygon_instances", "HbemError", "ParametrizedLine", "ParametrizedCircularArc",
            "ParametrizedTrigCurve", "ParametrizedMesh", "square_mesh", "star_mesh", "circle_mesh",
-           "polygon_mesh", "random_polygon_mesh", "gaussian_test_matrix", "Xoshiro256ss", "star_curve"]
+           "polygon_mesh", "random_polygon_mesh", "gaussian_test_matrix", "Xoshiro256ss", "star_curve",
+           "FLAG_QUIRK_SLP_COINC_DER", "FLAG_FAR_FIELD"]
 
 FLAG_QUIRK_SLP_COINC_DER = 1
 FLAG_FAR_FIELD = 2          # lower-order Gauss rules on separated panel pairs (HBEM_FLAG_FAR_FIELD)

@@ -39,6 +39,8 @@ def test_facade_driver_matches_oracle(tmp_path, H, O):
     assert abs(der - O.sv_der(T, Td, R, 30)[1]) <= 1e-8 * max(1.0, abs(der))
     K, W, V = O.assemble(mesh, 11, 2.0, 20.0, d=0)
     assert abs(v01 - abs(V[0][0, 1])) <= 1e-12 * abs(V[0][0, 1])
+    tag, err, nused = out[2].split()
+    assert tag == "mixed_far" and float(err) <= 1e-8 and int(nused) >= 0
 
 
 def test_facade_host_api(tmp_path):
