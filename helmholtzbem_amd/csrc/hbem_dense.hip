@@ -1297,8 +1297,9 @@ void lu_solve_thin_batched_f32(const cflt* LU, long long ms, const int* perm, in
 // ts; Z, Bv, R: n x NC with stride n NC).  op = T: a workgroup takes 256 rows
 // and a slice of kResidSlice columns (the slice's Z rows staged in LDS), one
 // thread per row streaming down the columns; the slices' partial sums are
-// added in slice order by k_resid_sum.  op = T^H: a wavefront takes one
-// column i of T, (T^H z)_i = sum_j conj(T(j, i)) z_j, lanes down the column.
+// added in slice order by k_resid_sum.  op = T^H: a wavefront takes four
+// columns i of T, (T^H z)_i = sum_j conj(T(j, i)) z_j, lanes down the columns
+// (k_resid_h).
 constexpr int kResidSlice = 256;
 
 template <int NC>
@@ -1351,41 +1352,77 @@ __global__ __launch_bounds__(256) void k_resid_sum(const cdbl* __restrict__ part
     R[id] = Bv[id] - acc;
 }
 
+// R = Bv - T^H Z: a wavefront per four columns i of T (column i gives row i
+// of T^H), lanes down the columns, two sweeps' loads in flight; every z value
+// loaded serves the four columns.  Each column's lane sums run over j in
+// sweep order and are reduced by shuffles in a fixed order.
 template <int NC>
 __global__ __launch_bounds__(256) void k_resid_h(const cdbl* __restrict__ T, long long ts, const cdbl* __restrict__ Z,
                                                  const cdbl* __restrict__ Bv, cdbl* __restrict__ R, int n) {
     const int b = blockIdx.y;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (i >= n) return;
-    const cdbl* col = T + (size_t)b * ts + (size_t)i * n;
+    const int i0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 4, lane = threadIdx.x & 63;
+    if (i0 >= n) return;                      // whole wavefront; no barrier in this kernel
+    const int qmax = min(4, n - i0) - 1;
+    const cdbl* T0 = T + (size_t)b * ts;
     const cdbl* z = Z + (size_t)b * n * NC;
-    cdbl acc[NC];
+    cdbl acc[4][NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] = mk(0, 0);
-    for (int j = lane; j < n; j += 64) {
-        const cdbl tv = cconj(col[j]);
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) cfma(acc[c], tv, z[j + (size_t)c * n]);
-    }
+        for (int c = 0; c < NC; ++c) acc[q][c] = mk(0, 0);
+    int j = lane;
+    for (; j + 64 < n; j += 128) {
+        cdbl tv[2][4], zv[2][NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c)
+        for (int u = 0; u < 2; ++u) {
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            acc[c].x += __shfl_xor(acc[c].x, o, 64);
-            acc[c].y += __shfl_xor(acc[c].y, o, 64);
+            for (int q = 0; q < 4; ++q) tv[u][q] = T0[j + 64 * u + (size_t)(i0 + min(q, qmax)) * n];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) zv[u][c] = z[j + 64 * u + (size_t)c * n];
         }
-    if (lane == 0) {
-        const size_t base = (size_t)b * n * NC + i;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) R[base + (size_t)c * n] = Bv[base + (size_t)c * n] - acc[c];
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const cdbl t = cconj(tv[u][q]);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) cfma(acc[q][c], t, zv[u][c]);
+            }
     }
+    for (; j < n; j += 64) {
+        cdbl zv[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) zv[c] = z[j + (size_t)c * n];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const cdbl t = cconj(T0[j + (size_t)(i0 + min(q, qmax)) * n]);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) cfma(acc[q][c], t, zv[c]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                acc[q][c].x += __shfl_xor(acc[q][c].x, o, 64);
+                acc[q][c].y += __shfl_xor(acc[q][c].y, o, 64);
+            }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (lane == q && q <= qmax) {
+            const size_t base = (size_t)b * n * NC + i0 + q;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) R[base + (size_t)c * n] = Bv[base + (size_t)c * n] - acc[q][c];
+        }
 }
 
 template <int NC>
 static void resid_impl(const cdbl* T, long long ts, const cdbl* Z, const cdbl* Bv, cdbl* R, int n, int batch, bool adj,
                        cdbl* part, hipStream_t st) {
     if (adj) {
-        hipLaunchKernelGGL(k_resid_h<NC>, dim3((n + 3) / 4, batch), dim3(256), 0, st, T, ts, Z, Bv, R, n);
+        hipLaunchKernelGGL(k_resid_h<NC>, dim3((n + 15) / 16, batch), dim3(256), 0, st, T, ts, Z, Bv, R, n);
         return;
     }
     const int nslice = (n + kResidSlice - 1) / kResidSlice, nrb = (n + 255) / 256;
