@@ -40,14 +40,6 @@ __device__ __forceinline__ void cfma(cdbl& acc, cdbl a, cdbl b) {   // acc += a 
 __device__ __forceinline__ cdbl shfl_c(cdbl v, int src) {
     return mk(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
 }
-// lane src's value to every lane when src is the same for the whole wavefront:
-// v_readlane through a scalar register, no LDS round trip (exact, as shfl_c)
-__device__ __forceinline__ double bcast_lane(double v, int src) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ cdbl bcast_lane(cdbl v, int src) { return mk(bcast_lane(v.x, src), bcast_lane(v.y, src)); }
 
 // The same operations on single-precision complex values (the factors of the
 // mixed-precision LU).
@@ -849,8 +841,7 @@ __device__ __forceinline__ void stage_diag(const F* M, int n, int r0, int bs, bo
 // Wavefront 0: solve the staged block for w[r0, r0 + bs) in place and leave
 // the solution in xb (zero beyond bs).  The reciprocal pivots are formed by
 // all lanes at once and the tile entries are read eight steps ahead, so a
-// step's dependence chain is one broadcast (v_readlane, the step's lane being
-// the same for the whole wavefront) and one complex multiply-add.
+// step's dependence chain is one broadcast and one complex multiply-add.
 template <int NC>
 __device__ __forceinline__ void diag_solve(int n, int r0, int bs, bool lower, bool unit, cdbl* w,
                                            const cdbl* __restrict__ tile, cdbl* xb) {
@@ -880,7 +871,7 @@ __device__ __forceinline__ void diag_solve(int n, int r0, int bs, bool lower, bo
             const bool other = lower ? (lane > j && lane < bs) : (lane < j);
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                const cdbl xj = bcast_lane(x[c], j);   // j is the same for every lane
+                const cdbl xj = shfl_c(x[c], j);
                 if (other) cfms(x[c], lv[u], xj);
             }
         }
