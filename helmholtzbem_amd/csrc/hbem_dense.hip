@@ -40,6 +40,14 @@ __device__ __forceinline__ void cfma(cdbl& acc, cdbl a, cdbl b) {   // acc += a 
 __device__ __forceinline__ cdbl shfl_c(cdbl v, int src) {
     return mk(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
 }
+// lane src's value in every lane, src the same for the whole wavefront:
+// v_readlane through a scalar register, no LDS round trip
+__device__ __forceinline__ double bcast_lane(double v, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ cdbl bcast_lane(cdbl v, int src) { return mk(bcast_lane(v.x, src), bcast_lane(v.y, src)); }
 
 // The same operations on single-precision complex values (the factors of the
 // mixed-precision LU).
@@ -839,9 +847,14 @@ __device__ __forceinline__ void stage_diag(const F* M, int n, int r0, int bs, bo
 }
 
 // Wavefront 0: solve the staged block for w[r0, r0 + bs) in place and leave
-// the solution in xb (zero beyond bs).  The reciprocal pivots are formed by
-// all lanes at once and the tile entries are read eight steps ahead, so a
-// step's dependence chain is one broadcast and one complex multiply-add.
+// the solution in xb (zero beyond bs).  With a non-unit diagonal the block is
+// solved as (M D^-1) y = w, x = D^-1 y: the reciprocal pivots are formed by
+// all lanes at once and scale the tile entries as they are read, eight steps
+// ahead, and x = D^-1 y is formed at the end, so a step's dependence chain is
+// one broadcast (v_readlane: the step's lane is the same for the whole
+// wavefront) and one complex multiply-add.  Every complex product is written
+// as explicit FMAs, so the bits do not depend on how the compiler contracts
+// the code around each call site.
 template <int NC>
 __device__ __forceinline__ void diag_solve(int n, int r0, int bs, bool lower, bool unit, cdbl* w,
                                            const cdbl* __restrict__ tile, cdbl* xb) {
@@ -858,26 +871,24 @@ __device__ __forceinline__ void diag_solve(int n, int r0, int bs, bool lower, bo
             const int step = min(s0 + u, bs - 1);
             const int j = lower ? step : bs - 1 - step;
             lv[u] = tile[lane * kTS + j];
+            if (!unit) lv[u] = cmulx(lv[u], bcast_lane(dinv, j));
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int step = s0 + u;
             if (step >= bs) break;
             const int j = lower ? step : bs - 1 - step;
-            if (!unit && lane == j) {
-#pragma unroll
-                for (int c = 0; c < NC; ++c) x[c] = x[c] * dinv;
-            }
             const bool other = lower ? (lane > j && lane < bs) : (lane < j);
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                const cdbl xj = shfl_c(x[c], j);
+                const cdbl xj = bcast_lane(x[c], j);
                 if (other) cfms(x[c], lv[u], xj);
             }
         }
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
+        if (!unit) x[c] = cmulx(x[c], dinv);
         if (lane < bs) w[r0 + lane + (size_t)c * n] = x[c];
         xb[lane * kMaxNc + c] = lane < bs ? x[c] : mk(0, 0);
     }
