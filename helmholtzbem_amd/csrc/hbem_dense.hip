@@ -769,9 +769,10 @@ static void lu_factor_impl(C* A, int n, int batch, int* ipiv, int* perm, hipStre
     auto panel = [&](int j, int jb) {
         const int rows = n - j;
         const dim3 g(batch), t(1024);
+        constexpr bool f32 = sizeof(C) == sizeof(cflt);   // half the registers per value: wider chunks
         if (rows <= 1024) hipLaunchKernelGGL((k_lu_panel_rg<C, 1, 8>), g, t, 0, st, A, n, j, jb, ipiv);
-        else if (rows <= 2048) hipLaunchKernelGGL((k_lu_panel_rg<C, 2, 6>), g, t, 0, st, A, n, j, jb, ipiv);
-        else if (rows <= 4096) hipLaunchKernelGGL((k_lu_panel_rg<C, 4, 4>), g, t, 0, st, A, n, j, jb, ipiv);
+        else if (rows <= 2048) hipLaunchKernelGGL((k_lu_panel_rg<C, 2, f32 ? 8 : 6>), g, t, 0, st, A, n, j, jb, ipiv);
+        else if (rows <= 4096) hipLaunchKernelGGL((k_lu_panel_rg<C, 4, f32 ? 8 : 4>), g, t, 0, st, A, n, j, jb, ipiv);
         else hipLaunchKernelGGL((k_lu_panel<C>), g, t, 0, st, A, n, j, jb, ipiv);
     };
     for (int j0 = 0; j0 < n; j0 += S * sb) {
