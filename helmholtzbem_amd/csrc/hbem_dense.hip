@@ -1125,45 +1125,26 @@ __global__ void k_thin_out(const int* __restrict__ perm, int n, int nc, cdbl* __
 
 // one workgroup of kSolveThreads threads per matrix: pass 0 (L, or U^H) or
 // pass 1 (U, or L^H) over the panel [pb, pe)
-template <class F, int NC>
-__global__ __launch_bounds__(kSolveThreads) void k_solve_panel(const F* __restrict__ LU, long long ms, int n, int pass,
-                                                               int adj, cdbl* __restrict__ work, int pb, int pe) {
-    __shared__ SolveLds S;
-    const int b = blockIdx.x;
-    cdbl* w = work + (size_t)b * n * NC;
-    const F* lu = LU + (size_t)b * ms;
-    // inside a panel the updates cover at most kSolvePanel - 64 rows: 16 rows
-    // per wavefront (more wavefronts busy, fewer load rounds each) and two
-    // sweeps' loads in flight in the left-looking sums
-    if (!adj) subst_right<NC, F, 16>(lu, n, pass == 0, w, S, pb, pe);
-    else subst_left<NC, F, NC <= 4 ? 2 : 1>(lu, n, pass == 0, w, S, pb, pe);
-}
-
-// Right-looking coupling of panel [pb, pe): every row i after it in the pass
-// (i >= pe for L, i < pb for U) takes w_i -= sum_{j in panel} M(i, j) w_j, the
-// sum in column order.  A thread per row (consecutive rows: the loads run down
-// the columns), LR column loads in flight, the panel's unknowns in LDS.
-template <class F, int NC>
-__global__ __launch_bounds__(256) void k_panel_update(const F* __restrict__ LU, long long ms, int n, int lower,
-                                                      cdbl* __restrict__ work, int pb, int pe) {
-    constexpr int LR = sizeof(F) == sizeof(cflt) ? 16 : 8;
-    __shared__ cdbl xs[kSolvePanel * NC];
-    const int b = blockIdx.y, tid = threadIdx.x;
-    const F* M = LU + (size_t)b * ms;
-    cdbl* w = work + (size_t)b * n * NC;
-    const int pw = pe - pb;
-    for (int e = tid; e < pw * NC; e += blockDim.x) {
+// Right-looking coupling of the unknowns [cb, ce) (one panel, staged in xs)
+// onto row i: w_i -= sum_j M(i, j) w_j, the sum in column order.  A thread per
+// row (consecutive rows: the loads run down the columns), LR column loads in
+// flight.
+template <int NC>
+__device__ __forceinline__ void stage_unknowns(const cdbl* w, int n, int cb, int ce, cdbl* xs) {
+    const int pw = ce - cb;
+    for (int e = threadIdx.x; e < pw * NC; e += blockDim.x) {
         const int j = e % pw, c = e / pw;
-        xs[j * NC + c] = w[pb + j + (size_t)c * n];
+        xs[j * NC + c] = w[cb + j + (size_t)c * n];
     }
-    __syncthreads();
-    const int r_lo = lower ? pe : 0, r_hi = lower ? n : pb;
-    const int i = r_lo + blockIdx.x * blockDim.x + tid;
-    if (i >= r_hi) return;
+}
+template <class F, int NC>
+__device__ __forceinline__ void couple_right_row(const F* M, int n, cdbl* w, int cb, int ce, int i, const cdbl* xs) {
+    constexpr int LR = sizeof(F) == sizeof(cflt) ? 16 : 8;
+    const int pw = ce - cb;
     cdbl acc[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) acc[c] = mk(0, 0);
-    const F* mr = M + i + (size_t)pb * n;
+    const F* mr = M + i + (size_t)cb * n;
     for (int j0 = 0; j0 < pw; j0 += LR) {
         F t[LR];
 #pragma unroll
@@ -1184,21 +1165,14 @@ __global__ __launch_bounds__(256) void k_panel_update(const F* __restrict__ LU, 
     }
 }
 
-// Left-looking coupling of panel [pb, pe): every row i of the panel takes
-// w_i -= sum_j conj(M(j, i)) w_j over the unknowns finished before it (j < pb
-// for U^H, j >= pe for L^H).  A wavefront per four columns, lanes down the
-// column, two sweeps' loads in flight; the lane sums are reduced by shuffles
-// in a fixed order.
+// Left-looking coupling of the unknowns [ja, jb) onto rows i0 .. i0 + 3 (those
+// below re): w_i -= sum_j conj(M(j, i)) w_j.  One wavefront, lanes down the
+// columns (lane j mod 64: ja is a multiple of 64), two sweeps' loads in
+// flight; the lane sums are reduced by shuffles in a fixed order.
 template <class F, int NC>
-__global__ __launch_bounds__(256) void k_panel_gather(const F* __restrict__ LU, long long ms, int n, int lower,
-                                                      cdbl* __restrict__ work, int pb, int pe) {
-    const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int i0 = pb + (blockIdx.x * 4 + wv) * 4;
-    if (i0 >= pe) return;                     // whole wavefront; no barrier in this kernel
-    const F* M = LU + (size_t)b * ms;
-    cdbl* w = work + (size_t)b * n * NC;
-    const int ja = lower ? 0 : pe, jb = lower ? pb : n;
-    const int qmax = min(4, pe - i0) - 1;
+__device__ __forceinline__ void couple_left_quad(const F* M, int n, cdbl* w, int ja, int jb, int i0, int re) {
+    const int lane = threadIdx.x & 63;
+    const int qmax = min(4, re - i0) - 1;
     cdbl acc[4][NC];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
@@ -1247,6 +1221,83 @@ __global__ __launch_bounds__(256) void k_panel_gather(const F* __restrict__ LU, 
         }
 }
 
+// One panel [pb, pe) of a pass, blocks 0 .. batch - 1 (one per matrix), and
+// beside it, in blocks batch + cwg b + k, a coupling that does not touch the
+// panel: the unknowns [cb, ce) onto the rows [rb, re) of matrix b,
+// right-looking (a thread per row, 1024 rows a block) or, for adj, left-looking
+// (a wavefront per four rows, 64 rows a block).  The coupling blocks take the
+// CUs the batch's panel solves leave idle.  Up to kFuseNc right-hand sides:
+// with more, the left-looking sums do not fit the registers of a
+// 1024-thread block, and the couplings run as launches of their own.
+constexpr int kFuseNc = 3;
+template <class F, int NC>
+__global__ __launch_bounds__(kSolveThreads) void k_solve_panel(const F* __restrict__ LU, long long ms, int n, int pass,
+                                                               int adj, cdbl* __restrict__ work, int pb, int pe,
+                                                               int batch, int cwg, int cb, int ce, int rb, int re) {
+    __shared__ SolveLds S;
+    if ((int)blockIdx.x < batch) {
+        const int b = blockIdx.x;
+        cdbl* w = work + (size_t)b * n * NC;
+        const F* lu = LU + (size_t)b * ms;
+        // inside a panel the updates cover at most kSolvePanel - 64 rows: 16 rows
+        // per wavefront (more wavefronts busy, fewer load rounds each) and two
+        // sweeps' loads in flight in the left-looking sums
+        if (!adj) subst_right<NC, F, 16>(lu, n, pass == 0, w, S, pb, pe);
+        else subst_left<NC, F, NC <= 4 ? 2 : 1>(lu, n, pass == 0, w, S, pb, pe);
+        return;
+    }
+    if constexpr (NC <= kFuseNc) {
+        const int q = blockIdx.x - batch, b = q / cwg, k = q - b * cwg;
+        cdbl* w = work + (size_t)b * n * NC;
+        const F* M = LU + (size_t)b * ms;
+        if (!adj) {
+            cdbl* xs = reinterpret_cast<cdbl*>(&S);   // the panel solve's LDS is not used by this block
+            stage_unknowns<NC>(w, n, cb, ce, xs);
+            __syncthreads();
+            const int i = rb + k * blockDim.x + threadIdx.x;
+            if (i < re) couple_right_row<F, NC>(M, n, w, cb, ce, i, xs);
+        } else {
+            const int i0 = rb + (k * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 4;
+            if (i0 < re) couple_left_quad<F, NC>(M, n, w, cb, ce, i0, re);
+        }
+    }
+}
+
+// The same couplings on their own: right-looking, 256 rows a block.
+template <class F, int NC>
+__global__ __launch_bounds__(256) void k_panel_update(const F* __restrict__ LU, long long ms, int n,
+                                                      cdbl* __restrict__ work, int cb, int ce, int rb, int re) {
+    __shared__ cdbl xs[kSolvePanel * NC];
+    const int b = blockIdx.y;
+    cdbl* w = work + (size_t)b * n * NC;
+    stage_unknowns<NC>(w, n, cb, ce, xs);
+    __syncthreads();
+    const int i = rb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < re) couple_right_row<F, NC>(LU + (size_t)b * ms, n, w, cb, ce, i, xs);
+}
+
+// Left-looking, 16 rows a block.
+template <class F, int NC>
+__global__ __launch_bounds__(256) void k_panel_gather(const F* __restrict__ LU, long long ms, int n,
+                                                      cdbl* __restrict__ work, int ja, int jb, int rb, int re) {
+    const int b = blockIdx.y;
+    const int i0 = rb + (blockIdx.x * 4 + (threadIdx.x >> 6)) * 4;
+    if (i0 >= re) return;                     // whole wavefront; no barrier in this kernel
+    couple_left_quad<F, NC>(LU + (size_t)b * ms, n, work + (size_t)b * n * NC, ja, jb, i0, re);
+}
+
+// Panelled substitution (n > 2 kSolvePanel).  A pass solves its panels in
+// order; every row must have taken the couplings of all panels solved before
+// its own, in that order.  After panel p's solve, the next panel's rows take
+// p's coupling (a small launch); the rows beyond the next panel take it in
+// the launch that solves the next panel, beside that solve.  Left-looking
+// passes (adj) gather instead: the next panel's rows take the unknowns solved
+// before p beside p's solve, then p's own unknowns.  Every right-looking row
+// update is the one-panel sum it is without the overlap, in the same order
+// (the same bits); a left-looking row takes its sum in two parts, split at a
+// panel boundary, which is the same for a matrix padded at the end.  Above
+// kFuseNc right-hand sides: each panel's coupling as a launch of its own,
+// all rows at once, after its solve (right-looking) or before it (left).
 template <class F, int NC>
 static void solve_thin_nc(const F* LU, long long ms, const int* perm, int n, int batch, cdbl* X, bool adj, cdbl* work,
                           hipStream_t st) {
@@ -1258,20 +1309,60 @@ static void solve_thin_nc(const F* LU, long long ms, const int* perm, int n, int
     const dim3 gio((unsigned)(((long long)n * NC + 255) / 256), (unsigned)batch);
     hipLaunchKernelGGL(k_thin_in, gio, dim3(256), 0, st, perm, n, NC, X, a, work);
     const int np = (n + kSolvePanel - 1) / kSolvePanel;
+    auto first = [&](int p) { return p * kSolvePanel; };
+    auto last = [&](int p) { return min(n, (p + 1) * kSolvePanel); };
+    if constexpr (NC > kFuseNc) {
+        for (int pass = 0; pass < 2; ++pass) {
+            const bool lower = pass == 0;
+            for (int t = 0; t < np; ++t) {
+                const int p = lower ? t : np - 1 - t;
+                const int pb = first(p), pe = last(p);
+                if (adj && (lower ? pb > 0 : pe < n))
+                    hipLaunchKernelGGL((k_panel_gather<F, NC>), dim3((unsigned)((pe - pb + 15) / 16), (unsigned)batch),
+                                       dim3(256), 0, st, LU, ms, n, work, lower ? 0 : pe, lower ? pb : n, pb, pe);
+                hipLaunchKernelGGL((k_solve_panel<F, NC>), dim3((unsigned)batch), dim3(kSolveThreads), 0, st, LU, ms, n,
+                                   pass, a, work, pb, pe, batch, 1, 0, 0, 0, 0);
+                const int rb = lower ? pe : 0, re = lower ? n : pb;
+                if (!adj && re > rb)
+                    hipLaunchKernelGGL((k_panel_update<F, NC>), dim3((unsigned)((re - rb + 255) / 256), (unsigned)batch),
+                                       dim3(256), 0, st, LU, ms, n, work, pb, pe, rb, re);
+            }
+        }
+        hipLaunchKernelGGL(k_thin_out, gio, dim3(256), 0, st, perm, n, NC, X, a, work);
+        return;
+    }
     for (int pass = 0; pass < 2; ++pass) {
         const bool lower = pass == 0;
         for (int t = 0; t < np; ++t) {
             const int p = lower ? t : np - 1 - t;
-            const int pb = p * kSolvePanel, pe = min(n, pb + kSolvePanel);
-            if (adj && (lower ? pb > 0 : pe < n))
-                hipLaunchKernelGGL((k_panel_gather<F, NC>), dim3((unsigned)((pe - pb + 15) / 16), (unsigned)batch),
-                                   dim3(256), 0, st, LU, ms, n, lower ? 1 : 0, work, pb, pe);
-            hipLaunchKernelGGL((k_solve_panel<F, NC>), dim3(batch), dim3(kSolveThreads), 0, st, LU, ms, n, pass, a,
-                               work, pb, pe);
-            const int rows = lower ? n - pe : pb;
-            if (!adj && rows > 0)
-                hipLaunchKernelGGL((k_panel_update<F, NC>), dim3((unsigned)((rows + 255) / 256), (unsigned)batch),
-                                   dim3(256), 0, st, LU, ms, n, lower ? 1 : 0, work, pb, pe);
+            const int pb = first(p), pe = last(p);
+            const bool next = t + 1 < np;
+            const int np_ = lower ? p + 1 : p - 1;             // the next panel of the pass
+            const int nb = next ? first(np_) : 0, ne = next ? last(np_) : 0;
+            int cb = 0, ce = 0, rb = 0, re = 0;                 // the coupling beside this solve
+            if (!adj && t > 0) {
+                const int pp = lower ? p - 1 : p + 1;          // the panel solved before
+                cb = first(pp);
+                ce = last(pp);
+                rb = lower ? pe : 0;                           // the rows beyond this panel
+                re = lower ? n : pb;
+            } else if (adj && next) {
+                cb = lower ? 0 : pe;
+                ce = lower ? pb : n;
+                rb = nb;
+                re = ne;
+            }
+            const bool cpl = ce > cb && re > rb;
+            const int cwg = !cpl ? 0 : (adj ? (re - rb + 63) / 64 : (re - rb + kSolveThreads - 1) / kSolveThreads);
+            hipLaunchKernelGGL((k_solve_panel<F, NC>), dim3((unsigned)(batch + batch * cwg)), dim3(kSolveThreads), 0, st,
+                               LU, ms, n, pass, a, work, pb, pe, batch, cwg > 0 ? cwg : 1, cb, ce, rb, re);
+            if (!next) continue;
+            if (!adj)
+                hipLaunchKernelGGL((k_panel_update<F, NC>), dim3((unsigned)((ne - nb + 255) / 256), (unsigned)batch),
+                                   dim3(256), 0, st, LU, ms, n, work, pb, pe, nb, ne);
+            else
+                hipLaunchKernelGGL((k_panel_gather<F, NC>), dim3((unsigned)((ne - nb + 15) / 16), (unsigned)batch),
+                                   dim3(256), 0, st, LU, ms, n, work, pb, pe, nb, ne);
         }
     }
     hipLaunchKernelGGL(k_thin_out, gio, dim3(256), 0, st, perm, n, NC, X, a, work);
