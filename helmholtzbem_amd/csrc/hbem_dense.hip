@@ -147,6 +147,41 @@ __global__ __launch_bounds__(1024) void k_lu_panel(C* __restrict__ A, int n, int
     }
 }
 
+// The interchanges of the panel [j0, j0 + jb) as a map over the rows they
+// touch, by one wavefront (lane k): the distinct pivot rows below the panel are
+// numbered in order of first appearance (erow), and the jb swaps are replayed
+// on the slot -> source index with wave-uniform readlanes, slot t < jb being
+// panel row j0 + t and slot jb + e pivot row erow[e]; src[t] is the slot whose
+// row the swaps move into slot t, nes the number of pivot rows below.
+__device__ __forceinline__ void row_map(const int* ip, int j0, int jb, int* src, int* erow, int* nes) {
+    const int k = threadIdx.x & 63;
+    const int p = k < jb ? ip[j0 + k] : 0;
+    const bool ex = k < jb && p >= j0 + jb;
+    int first = k;
+    if (ex)
+        for (int k2 = 0; k2 < k; ++k2)
+            if (ip[j0 + k2] == p) { first = k2; break; }
+    const unsigned long long firsts = __ballot(ex && first == k);
+    const int e = __popcll(firsts & ((1ull << first) - 1ull));
+    if (ex && first == k) erow[e] = p;
+    const int partner = k < jb ? (ex ? jb + e : p - j0) : k;   // slot swapped with slot k
+    int v = k;
+    for (int sw = 0; sw < jb; ++sw) {
+        const int b = __builtin_amdgcn_readlane(partner, sw);
+        if (b != sw) {
+            const int va = __builtin_amdgcn_readlane(v, sw);
+            const int vb = __builtin_amdgcn_readlane(v, b);
+            v = k == sw ? vb : (k == b ? va : v);
+        }
+    }
+    src[k] = v;
+    if (k == 0) *nes = __popcll(firsts);
+}
+
+// Map layout in memory (the caller's perm array doubles as its store while
+// the factorisation runs): src[64], erow[32], nes.
+constexpr int kRowMapInts = 97;
+
 // The same panel factorisation with the rows held in registers, CW panel
 // columns (a chunk) at a time, for panels of at most 1024 R rows: thread t owns
 // rows j0 + t + 1024 u, u < R.  Inside a chunk every column step -- pivot
@@ -160,7 +195,7 @@ __global__ __launch_bounds__(1024) void k_lu_panel(C* __restrict__ A, int n, int
 // 128 VGPRs a thread of a 1024-thread workgroup has).
 template <class C, int R, int CW>
 __global__ __launch_bounds__(1024) void k_lu_panel_rg(C* __restrict__ A, int n, int j0, int jb,
-                                                      int* __restrict__ ipiv) {
+                                                      int* __restrict__ ipiv, int* __restrict__ rmap) {
     constexpr int YB = R >= 2 ? 4 : 8;           // trailing columns updated per round
     C* a = A + (size_t)blockIdx.x * n * n;
     int* ip = ipiv + (size_t)blockIdx.x * n;
@@ -320,6 +355,15 @@ __global__ __launch_bounds__(1024) void k_lu_panel_rg(C* __restrict__ A, int n, 
         }
         __syncthreads();
     }
+    // the interchange map for the row kernels (the pivots are in ip; every
+    // thread has passed a barrier since the last was written)
+    if (rmap) {
+        __syncthreads();
+        if (wv == 0) {
+            int* m = rmap + (size_t)blockIdx.x * n;
+            row_map(ip, j0, jb, m, m + 64, m + 96);
+        }
+    }
 }
 
 // Row interchanges of the panel [j0, j0 + jb) (jb <= 32) and the U12 solve.
@@ -336,7 +380,7 @@ __global__ __launch_bounds__(1024) void k_lu_panel_rg(C* __restrict__ A, int n, 
 //     permute and updates the rows below it with L11 read from LDS.
 template <class C, int MODE>
 __global__ __launch_bounds__(256) void k_lu_rows(C* __restrict__ A, int n, int j0, int jb,
-                                                 const int* __restrict__ ipiv) {
+                                                 const int* __restrict__ ipiv, const int* __restrict__ rmap) {
     C* a = A + (size_t)blockIdx.y * n * n;
     const int* ip = ipiv + (size_t)blockIdx.y * n;
     __shared__ C rows[64][33];   // slot t < jb: panel row j0 + t; slot jb + e: pivot row erow[e]
@@ -354,29 +398,13 @@ __global__ __launch_bounds__(256) void k_lu_rows(C* __restrict__ A, int n, int j
     auto touched = [&](int cc) { return cc < ncol && (MODE != 2 || colof(cc) >= j0 + jb); };
     int ne = 0;
     if (MODE & 1) {
-        if (wv == 0) {
-            const int k = lane;
-            const int p = k < jb ? ip[j0 + k] : 0;
-            const bool ex = k < jb && p >= j0 + jb;
-            int first = k;
-            if (ex)
-                for (int k2 = 0; k2 < k; ++k2)
-                    if (ip[j0 + k2] == p) { first = k2; break; }
-            const unsigned long long firsts = __ballot(ex && first == k);
-            const int e = __popcll(firsts & ((1ull << first) - 1ull));
-            if (ex && first == k) erow[e] = p;
-            const int partner = k < jb ? (ex ? jb + e : p - j0) : k;   // slot swapped with slot k
-            int v = k;
-            for (int sw = 0; sw < jb; ++sw) {
-                const int b = __builtin_amdgcn_readlane(partner, sw);
-                if (b != sw) {
-                    const int va = __builtin_amdgcn_readlane(v, sw);
-                    const int vb = __builtin_amdgcn_readlane(v, b);
-                    v = k == sw ? vb : (k == b ? va : v);
-                }
-            }
-            src[k] = v;
-            if (k == 0) nes = __popcll(firsts);
+        if (rmap) {                               // left by the panel kernel
+            const int* m = rmap + (size_t)blockIdx.y * n;
+            if (tid < 64) src[tid] = m[tid];
+            else if (tid < 96) erow[tid - 64] = m[tid];
+            else if (tid == 96) nes = m[96];
+        } else if (wv == 0) {
+            row_map(ip, j0, jb, src, erow, &nes);
         }
         __syncthreads();
         ne = nes;
@@ -758,21 +786,27 @@ static void lu_factor_impl(C* A, int n, int batch, int* ipiv, int* perm, hipStre
         gemm_sub(m, nn, k, a, b, c, n, s, batch, st);
         if (tm) tm->end(st, 6.0 * m * nn * k * batch);
     };
+    // the register-resident panel kernels leave the interchange map of their
+    // panel in perm (free until the final permutation is written), so that the
+    // row kernels' workgroups do not each replay the swaps
+    const int* rmap = nullptr;
     auto rows = [&](int mode, int j0, int jb) {
         const int nout = n - jb;
         if (nout <= 0) return;
         const dim3 g((nout + 31) / 32, batch), t(256);
-        if (mode == 3) hipLaunchKernelGGL((k_lu_rows<C, 3>), g, t, 0, st, A, n, j0, jb, ipiv);
-        else if (mode == 1) hipLaunchKernelGGL((k_lu_rows<C, 1>), g, t, 0, st, A, n, j0, jb, ipiv);
-        else hipLaunchKernelGGL((k_lu_rows<C, 2>), g, t, 0, st, A, n, j0, jb, ipiv);
+        if (mode == 3) hipLaunchKernelGGL((k_lu_rows<C, 3>), g, t, 0, st, A, n, j0, jb, ipiv, rmap);
+        else if (mode == 1) hipLaunchKernelGGL((k_lu_rows<C, 1>), g, t, 0, st, A, n, j0, jb, ipiv, rmap);
+        else hipLaunchKernelGGL((k_lu_rows<C, 2>), g, t, 0, st, A, n, j0, jb, ipiv, nullptr);
     };
     auto panel = [&](int j, int jb) {
         const int rows = n - j;
         const dim3 g(batch), t(1024);
         constexpr bool f32 = sizeof(C) == sizeof(cflt);   // half the registers per value: wider chunks
-        if (rows <= 1024) hipLaunchKernelGGL((k_lu_panel_rg<C, 1, 8>), g, t, 0, st, A, n, j, jb, ipiv);
-        else if (rows <= 2048) hipLaunchKernelGGL((k_lu_panel_rg<C, 2, f32 ? 8 : 6>), g, t, 0, st, A, n, j, jb, ipiv);
-        else if (rows <= 4096) hipLaunchKernelGGL((k_lu_panel_rg<C, 4, f32 ? 8 : 4>), g, t, 0, st, A, n, j, jb, ipiv);
+        int* m = n >= kRowMapInts ? perm : nullptr;
+        rmap = rows <= 4096 ? m : nullptr;
+        if (rows <= 1024) hipLaunchKernelGGL((k_lu_panel_rg<C, 1, 8>), g, t, 0, st, A, n, j, jb, ipiv, m);
+        else if (rows <= 2048) hipLaunchKernelGGL((k_lu_panel_rg<C, 2, f32 ? 8 : 6>), g, t, 0, st, A, n, j, jb, ipiv, m);
+        else if (rows <= 4096) hipLaunchKernelGGL((k_lu_panel_rg<C, 4, f32 ? 8 : 4>), g, t, 0, st, A, n, j, jb, ipiv, m);
         else hipLaunchKernelGGL((k_lu_panel<C>), g, t, 0, st, A, n, j, jb, ipiv);
     };
     for (int j0 = 0; j0 < n; j0 += S * sb) {
