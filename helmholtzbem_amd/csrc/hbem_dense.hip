@@ -1297,17 +1297,56 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_panel(const F* __restri
     }
 }
 
-// The same couplings on their own: right-looking, 256 rows a block.
+// The same couplings on their own.  Right-looking, 64 rows a block: wavefront
+// v sums the unknowns cb + 128 v .. + 127 for the block's rows (lane = row,
+// so the loads still run down the columns in 64-row segments), and the four
+// partial sums are added in a fixed order, ((0 + 1) + (2 + 3)), by wavefront 0:
+// each thread's chain of dependent load rounds is a quarter as long as with
+// one thread per row over the whole panel.  The quarters are fixed column
+// ranges from the panel's start, so a panel padded with zero columns gives the
+// same sums.
 template <class F, int NC>
 __global__ __launch_bounds__(256) void k_panel_update(const F* __restrict__ LU, long long ms, int n,
                                                       cdbl* __restrict__ work, int cb, int ce, int rb, int re) {
+    constexpr int LR = sizeof(F) == sizeof(cflt) ? 16 : 8;
+    constexpr int QW = kSolvePanel / 4;
     __shared__ cdbl xs[kSolvePanel * NC];
-    const int b = blockIdx.y;
+    __shared__ cdbl part[4][64][NC];
+    const int b = blockIdx.y, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     cdbl* w = work + (size_t)b * n * NC;
     stage_unknowns<NC>(w, n, cb, ce, xs);
     __syncthreads();
-    const int i = rb + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < re) couple_right_row<F, NC>(LU + (size_t)b * ms, n, w, cb, ce, i, xs);
+    const int i = rb + blockIdx.x * 64 + lane;
+    const int q0 = wv * QW, q1 = min(ce - cb, q0 + QW);
+    cdbl acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = mk(0, 0);
+    if (i < re) {
+        const F* mr = LU + (size_t)b * ms + i + (size_t)cb * n;
+        for (int j0 = q0; j0 < q1; j0 += LR) {
+            F t[LR];
+#pragma unroll
+            for (int u = 0; u < LR; ++u) t[u] = mr[(size_t)min(j0 + u, q1 - 1) * n];
+#pragma unroll
+            for (int u = 0; u < LR; ++u) {
+                if (j0 + u < q1) {
+                    const cdbl tv = to_cdbl(t[u]);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) cfma(acc[c], tv, xs[(j0 + u) * NC + c]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) part[wv][lane][c] = acc[c];
+    __syncthreads();
+    if (wv == 0 && i < re) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            cdbl& y = w[i + (size_t)c * n];
+            y = y - ((part[0][lane][c] + part[1][lane][c]) + (part[2][lane][c] + part[3][lane][c]));
+        }
+    }
 }
 
 // Left-looking, 16 rows a block.
@@ -1326,10 +1365,10 @@ __global__ __launch_bounds__(256) void k_panel_gather(const F* __restrict__ LU, 
 // p's coupling (a small launch); the rows beyond the next panel take it in
 // the launch that solves the next panel, beside that solve.  Left-looking
 // passes (adj) gather instead: the next panel's rows take the unknowns solved
-// before p beside p's solve, then p's own unknowns.  Every right-looking row
-// update is the one-panel sum it is without the overlap, in the same order
-// (the same bits); a left-looking row takes its sum in two parts, split at a
-// panel boundary, which is the same for a matrix padded at the end.  Above
+// before p beside p's solve, then p's own unknowns.  A row's sums are split
+// only at panel boundaries and at fixed offsets inside a panel, and which
+// launch serves a row depends only on its panel, so a matrix padded at the
+// end gives the same bits.  Above
 // kFuseNc right-hand sides: each panel's coupling as a launch of its own,
 // all rows at once, after its solve (right-looking) or before it (left).
 template <class F, int NC>
@@ -1358,7 +1397,7 @@ static void solve_thin_nc(const F* LU, long long ms, const int* perm, int n, int
                                    pass, a, work, pb, pe, batch, 1, 0, 0, 0, 0);
                 const int rb = lower ? pe : 0, re = lower ? n : pb;
                 if (!adj && re > rb)
-                    hipLaunchKernelGGL((k_panel_update<F, NC>), dim3((unsigned)((re - rb + 255) / 256), (unsigned)batch),
+                    hipLaunchKernelGGL((k_panel_update<F, NC>), dim3((unsigned)((re - rb + 63) / 64), (unsigned)batch),
                                        dim3(256), 0, st, LU, ms, n, work, pb, pe, rb, re);
             }
         }
@@ -1392,7 +1431,7 @@ static void solve_thin_nc(const F* LU, long long ms, const int* perm, int n, int
                                LU, ms, n, pass, a, work, pb, pe, batch, cwg > 0 ? cwg : 1, cb, ce, rb, re);
             if (!next) continue;
             if (!adj)
-                hipLaunchKernelGGL((k_panel_update<F, NC>), dim3((unsigned)((ne - nb + 255) / 256), (unsigned)batch),
+                hipLaunchKernelGGL((k_panel_update<F, NC>), dim3((unsigned)((ne - nb + 63) / 64), (unsigned)batch),
                                    dim3(256), 0, st, LU, ms, n, work, pb, pe, nb, ne);
             else
                 hipLaunchKernelGGL((k_panel_gather<F, NC>), dim3((unsigned)((ne - nb + 15) / 16), (unsigned)batch),
