@@ -56,3 +56,21 @@ def test_mixed_polygon2000_grid(H):
     print("polygon2000 max rel err %.3e, fp64 fallbacks %d" % (err.max(), int(used.sum())), flush=True)
     assert err.max() <= MIXED_TOL
     assert used.sum() < ks.size // 4
+
+
+@pytest.mark.parametrize("nc", [3, 5])
+def test_mixed_panelled_solves_more_right_hand_sides(H, nc):
+    """A 700-panel polygon (T of order 1400: the solves run in panels), order
+    11, c_i = 20, 32 k in [5, 20], with nc = 3 (each panel's coupling beside
+    the next panel's solve) and nc = 5 (each coupling as a launch of its own):
+    mixed sigma per k within 1e-8 of FP64 with the same test matrix."""
+    mesh = H.random_polygon_mesh(7, 10, 700)
+    sp = H.ContinuousSpace1()
+    bd = H.BuilderData(mesh, sp, sp, 11)
+    R = H.gaussian_test_matrix(1400, nc, 9)
+    ks = np.linspace(5.0, 20.0, 32)
+    ref = H.sigma_sweep(bd, ks, 1.0, 20.0, R, 2)
+    sig, used = H.sigma_sweep_mixed(bd, ks, 1.0, 20.0, R, 2, refine=3)
+    err = np.abs(sig - ref) / ref
+    print("nc %d max rel err %.3e, fp64 fallbacks %d" % (nc, err.max(), int(used.sum())), flush=True)
+    assert err.max() <= MIXED_TOL
