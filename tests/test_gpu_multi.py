@@ -40,6 +40,24 @@ def test_batched_equals_each_instance_alone(workload, H):
         assert np.all(np.isfinite(s)) and np.all(s > 0)
 
 
+@pytest.mark.parametrize("nc", [3, 5])
+def test_batched_equals_alone_with_more_right_hand_sides(workload, H, nc):
+    """As above with nc = 3 (the panelled solves' couplings beside the panel
+    solves) and nc = 5 (couplings as launches of their own), on six instances:
+    the three largest, whose solves run in panels either way, and three of at
+    most 300 panels, which run the one-kernel solve alone and the panelled
+    solve when padded to the bucket's order."""
+    by_size = sorted(workload, key=lambda it: it[0].getNumPanels())
+    pick = [it for it in by_size if it[0].getNumPanels() <= 300][:3] + by_size[-3:]
+    sub = [(bd, ks, c_o, c_i, H.gaussian_test_matrix(2 * bd.getNumPanels(), nc, 200 + j))
+           for j, (bd, ks, c_o, c_i, _) in enumerate(pick)]
+    batched = H.sigma_sweep_batched(sub, Q)
+    for (bd, ks, c_o, c_i, R), s in zip(sub, batched):
+        alone = H.sigma_sweep(bd, ks, c_o, c_i, R, Q)
+        bd.release_work()
+        assert np.array_equal(s, alone), (nc, bd.getNumPanels(), np.abs(s - alone).max())
+
+
 def test_bucket_granularity_does_not_change_bits(workload, H):
     sub = workload[:16]
     a = H.sigma_sweep_batched(sub, Q, bucket=64)
